@@ -1,0 +1,152 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused viscoelastic velocity-stress step on one MI355X.
+
+Workload (default): the layered viscoelastic single shot -- 1024 x 512 cells, dx = 5 m,
+FP32, L = 3 SLS mechanisms, 20-cell CPML on the sides and bottom, free surface, 12 Hz
+Ricker force source, 201 receivers, dt = 0.4 ms. One "step" is one full time step (the
+velocity kernel then the stress kernel) over the whole grid, inputs resident in HBM.
+
+Prints ONE JSON line: value = grid-point updates per second (Nx*Nz*steps / wall time),
+``roofline`` for the dominant (stress) kernel from HIP events recorded around every launch
+on the library's stream, and ``cpu_baseline`` = the reference-structured CPU restatement
+(multi-pass, OpenMP) timed on this host on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gpt-updates/s & %HBM roofline; batched shots/s + p99 latency vs plain stepper"
+HBM_PEAK_GBS = 8000.0
+
+
+def host_threads():
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def pmc_traffic(kernel_name):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(kernel_name, {}).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def run_cpu_baseline(cfg, budget_s=12.0):
+    """Time the oracle's reference-structured multi-pass path (FP32) on host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import make_oracle
+    threads = host_threads()
+    o = make_oracle(cfg, np.float32)
+    o.run(5, mode=1, nthreads=threads)  # warm up page faults
+    t0 = time.perf_counter()
+    n_probe = 10
+    o.run(n_probe, mode=1, nthreads=threads)
+    dt_probe = (time.perf_counter() - t0) / n_probe
+    n = int(max(10, min(cfg["nt"], budget_s / max(dt_probe, 1e-9))))
+    t0 = time.perf_counter()
+    o.run(n, mode=1, nthreads=threads)
+    el = time.perf_counter() - t0
+    o.close()
+    cells = cfg["nx"] * cfg["nz"]
+    return {"value": cells * n / el / 1e9, "unit": "GPt/s", "cores": threads, "kind": "port",
+            "sample": f"{n} of {cfg['nt']} steps of the same {cfg['nx']}x{cfg['nz']} L=3 FP32 "
+                      f"workload, reference-structured multi-pass C restatement, "
+                      f"{el:.1f} s on {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--nz", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the stepper has no multi-GPU path in this build "
+              "(shots shard across GPUs without communication; see the design notes). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from viscoelasticinversion_amd.configs import build_simulation, config_layered_visco
+    nt_total = args.warmup + 2 * args.steps
+    cfg = config_layered_visco(nx=args.nx, nz=args.nz, nt=nt_total)
+    sim = build_simulation(cfg, "float32")
+    cells = cfg["nx"] * cfg["nz"]
+
+    sim.step(args.warmup)
+    sim.sync()
+    # timed region: K steps, HIP events around every launch on the library's stream
+    sim.profile(True)
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    ms_v, n_v = sim.profile_read(0)
+    ms_s, n_s = sim.profile_read(1)
+    sim.profile(False)
+    # same K steps again without events: launch-gap-free rate, reported beside it
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.sync()
+    wall_noev = time.perf_counter() - t0
+    tr = sim.traces(nt_total)
+    finite = bool(np.isfinite(tr).all())
+    bytes_v = sim.algorithmic_bytes(0)
+    bytes_s = sim.algorithmic_bytes(1)
+    sim.close()
+
+    value = cells * args.steps / wall / 1e9
+    avg_s = ms_s / max(n_s, 1) * 1e-3
+    avg_v = ms_v / max(n_v, 1) * 1e-3
+    ach_s = bytes_s / avg_s / 1e9
+    ach_v = bytes_v / avg_v / 1e9
+    roof = {"bound": "hbm", "achieved": round(ach_s, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach_s / HBM_PEAK_GBS, 4),
+            "traffic": pmc_traffic("k_stress"),
+            "kernel": "k_stress<float,3>", "avg_launch_us": round(avg_s * 1e6, 3),
+            "algorithmic_bytes_per_launch": bytes_s}
+    extra = {"k_velocity": {"avg_launch_us": round(avg_v * 1e6, 3), "achieved_GBs": round(ach_v, 1),
+                            "algorithmic_bytes_per_launch": bytes_v,
+                            "traffic": pmc_traffic("k_velocity")},
+             "step_kernel_us": round((avg_s + avg_v) * 1e6, 3),
+             "value_without_events": round(cells * args.steps / wall_noev / 1e9, 3),
+             "traces_finite": finite}
+    cpu = None
+    if not args.no_cpu:
+        cfg_cpu = config_layered_visco(nx=args.nx, nz=args.nz, nt=args.steps)
+        cpu = run_cpu_baseline(cfg_cpu, args.cpu_budget)
+    line = {"metric": METRIC, "value": round(value, 3), "unit": "GPt/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(wall / args.steps * 1e3, 5), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded layered model)",
+            "config": {"workload": f"layered viscoelastic single shot {cfg['nx']}x{cfg['nz']}, "
+                                   "L=3 SLS, CPML 20 + free surface, dt 0.4 ms",
+                       "nx": cfg["nx"], "nz": cfg["nz"], "nrelax": 3, "nrec": 201},
+            "roofline": roof, "cpu_baseline": cpu, "extra": extra}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,137 @@
+/*
+ * vi_mi355x.h -- C ABI of the MI355X (gfx950) viscoelastic velocity-stress stepper.
+ *
+ * The library replaces the time-stepping path of haralddd/ViscoelasticInversion:
+ *   Parameters(; Nx, Nz, dx, dz, dt, ..., model, cpml, free_surface, source)
+ *       src/sciml_solver/Parameters.jl:47-86                -> vi_create + vi_set_*
+ *   make_problem(p; s0, v0, tspan)  ViscoelasticProblem.jl:273-292
+ *                                                           -> vi_set_state
+ *   solve_problem(prob; ...)        ViscoelasticProblem.jl:307-309
+ *   leapfrog_step!(fields, parameters, dt)  src/visco-kernels.jl:87-100
+ *                                                           -> vi_step / vi_sync
+ *   _vel_1Q! / _stress_1Q!          src/visco-kernels.jl:3-20, 59-85   (fused kernels)
+ *   RickerSource / inject_source_kernel!  src/sciml_solver/Source.jl:7-58 -> vi_set_sources
+ *   CPMLConfig / CPMLCoefficients   src/sciml_solver/CPML.jl:28-221     -> vi_set_cpml
+ *   FreeSurfaceBC / apply_free_surface!   src/sciml_solver/CPML.jl:366-393 -> vi_set_free_surface
+ *   IsotropicModel / QModel         src/sciml_solver/Models.jl:3-25, 90-107 -> vi_set_model,
+ *                                                                              vi_set_relaxation
+ *
+ * Conventions
+ *   - Every 2D host array is [nz][nx] with x fastest (Julia's column-major (Nx, Nz)).
+ *   - Host arrays are in the handle's precision (float for VI_F32, double for VI_F64).
+ *     The library copies them during the call and never keeps host pointers.
+ *   - All functions return 0 on success or a negative VI_E* code; vi_last_error() then
+ *     returns a message (thread-local). No C++ exception crosses this boundary.
+ *   - A handle owns its device memory and one HIP stream. vi_step is asynchronous until
+ *     vi_sync or any vi_get_* call. A handle is used by one host thread at a time.
+ */
+#ifndef VI_MI355X_H
+#define VI_MI355X_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vi_sim_s* vi_handle;
+
+enum { VI_F32 = 0, VI_F64 = 1 };
+
+enum {
+  VI_OK = 0,
+  VI_EINVAL = -1,   /* bad argument */
+  VI_ESTATE = -2,   /* call out of order (e.g. step before set_model) */
+  VI_EDEVICE = -3,  /* HIP runtime error, no device, or kernel image missing */
+  VI_ENOMEM = -4    /* device allocation failed */
+};
+
+/* fields for vi_get_state / vi_set_state (values match the oracle's numbering) */
+enum {
+  VI_VX = 0, VI_VZ = 1, VI_SXX = 2, VI_SZZ = 3, VI_SXZ = 4,
+  VI_MXX = 5, VI_MZZ = 6, VI_MXZ = 7,                  /* [L][nz][nx] */
+  VI_PSI_VX_X = 8, VI_PSI_VZ_X = 9, VI_PSI_SXX_X = 10, VI_PSI_SXZ_X = 11, /* [nz][2T] */
+  VI_PSI_VX_Z = 12, VI_PSI_VZ_Z = 13, VI_PSI_SXZ_Z = 14, VI_PSI_SZZ_Z = 15, /* [2T][nx] */
+  VI_NFIELDS = 16
+};
+
+/* source kinds: force_z is the reference's RickerSource (added to dvz before buoyancy,
+ * ViscoelasticProblem.jl:162-171); explosive adds to the sxx and szz rates. */
+enum { VI_SRC_FORCE_Z = 0, VI_SRC_EXPLOSIVE = 1, VI_SRC_FORCE_X = 2 };
+
+/* receiver components */
+enum { VI_REC_VX = 0, VI_REC_VZ = 1, VI_REC_SXX = 2, VI_REC_SZZ = 3, VI_REC_SXZ = 4 };
+
+/* Grid and physics of one simulation (Parameters.jl:47-63: Nx, Nz, dx, dz, dt,
+ * fd_order_x, fd_order_z). Only 8th-order staggered operators are implemented. */
+typedef struct {
+  int nx, nz;
+  double dx, dz, dt;
+  int precision;       /* VI_F32 or VI_F64 */
+  int nrelax;          /* L standard linear solids, 0 = elastic, at most 8 */
+  int fd_order_x, fd_order_z; /* must be 8 */
+} vi_grid_desc;
+
+/* CPMLConfig (CPML.jl:28-63). d0 must be given explicitly; vi_cpml_auto_d0 computes the
+ * Collino & Tsogka value the reference uses when d0 === nothing (CPML.jl:55-61). */
+typedef struct {
+  int thickness;
+  double d0, alpha_max, kappa_max, power;
+  int left, right, top, bottom;
+} vi_cpml_desc;
+
+/* ---- host-only helpers (no device needed) ----------------------------------------- */
+const char* vi_last_error(void);
+const char* vi_version(void);
+/* staggered 8th-order weights / h, as _stencil (Stencil.jl:9-14) gives them */
+void vi_fd_coeffs(double h, double out[4]);
+/* d0 = -(power+1) vmax ln(R) / (2 thickness dx)   (CPML.jl:60) */
+double vi_cpml_auto_d0(int thickness, double vmax, double dx, double reflection_coef,
+                       double power);
+/* a, b, kappa of CPMLCoefficients along one axis (CPML.jl:109-189), in double */
+int vi_cpml_profile(int n, const vi_cpml_desc* c, double dt, int axis /*0=x,1=z*/,
+                    double* a, double* b, double* kappa);
+/* number of visible HIP devices (0 when none); never fails */
+int vi_device_count(void);
+
+/* ---- single-shot stepper ----------------------------------------------------------- */
+int vi_create(const vi_grid_desc* g, vi_handle* out);
+int vi_destroy(vi_handle h);
+/* rho, lam, mu (and tau when nrelax > 0) as [nz][nx]; tau may be NULL when nrelax == 0 */
+int vi_set_model(vi_handle h, const void* rho, const void* lam, const void* mu,
+                 const void* tau);
+/* taun[L] relaxation times of the SLS mechanisms (QModel.τns, Models.jl:56-62) */
+int vi_set_relaxation(vi_handle h, const double* taun);
+/* NULL disables the CPML; resets the psi memory to zero */
+int vi_set_cpml(vi_handle h, const vi_cpml_desc* c);
+/* 1: traction-free surface on row 4 with stress images in rows 0..3 */
+int vi_set_free_surface(vi_handle h, int enabled);
+/* n sources at interior cells (ix, iz); wavelets [n][nt] in the handle's precision,
+ * already scaled by the source coefficient; sample k is used by step k */
+int vi_set_sources(vi_handle h, int n, const int* ix, const int* iz, const int* kind, int nt,
+                   const void* wavelets);
+/* n receivers at interior cells; traces are recorded for up to nt_max steps after this call */
+int vi_set_receivers(vi_handle h, int n, const int* ix, const int* iz, const int* comp,
+                     int nt_max);
+int vi_set_state(vi_handle h, int field, const void* host);
+int vi_get_state(vi_handle h, int field, void* host);
+/* zero every state array (v, s, M, psi), the trace buffer and the step counter */
+int vi_reset(vi_handle h);
+/* advance nsteps full steps (velocity kernel, then stress kernel, per step) */
+int vi_step(vi_handle h, int nsteps);
+int vi_sync(vi_handle h);
+long vi_step_index(vi_handle h);
+/* traces recorded so far: out[nrec][nt] for steps [0, nt) since vi_set_receivers */
+int vi_get_traces(vi_handle h, void* out, int nt);
+
+/* ---- kernel timing (HIP events around each launch on the handle's stream) ---------- */
+int vi_profile_enable(vi_handle h, int enabled);
+/* kernel 0 = velocity, 1 = stress: summed milliseconds and launch count since enable */
+int vi_profile_read(vi_handle h, int kernel, double* ms_total, long* launches);
+/* algorithmic HBM bytes per launch of kernel 0/1 for this handle's configuration */
+int vi_algorithmic_bytes(vi_handle h, int kernel, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

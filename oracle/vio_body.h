@@ -1,0 +1,630 @@
+/*
+ * vio_body.h -- body of the CPU oracle, included by vio.c once per precision with
+ * REAL (float | double) and SUF (_f32 | _f64) defined. TEST INFRASTRUCTURE ONLY.
+ *
+ * Discrete scheme (0-based indices, every field [nz][nx] with x fastest):
+ *   positions (Virieux): sxx, szz, rho, lam, mu, tau at (i, j); vx at (i+1/2, j);
+ *   vz at (i, j+1/2); sxz at (i+1/2, j+1/2).
+ *   Interior I = [4, nx-4) x [4, nz-4) (visco-kernels.jl:95-96 pad = length(D) = 4);
+ *   fields outside I are never updated (Dirichlet ring, BoundaryConditions.jl:8-11).
+ *
+ *   Operators, accumulated from l = 1 in REAL without contraction (diff-funcs.jl:3-32):
+ *     DXP(u) = sum_l Dx[l] (u[i+l] - u[i-l+1])   DXM(u) = sum_l Dx[l] (u[i+l-1] - u[i-l])
+ *     DZP, DZM the same along j.
+ *
+ *   Velocity half step n (visco-kernels.jl:3-20 with the Virieux operator choice):
+ *     bx = 2/(rho[i]+rho[i+1]), bz = 2/(rho[j]+rho[j+1])          visco-kernels.jl:10-11
+ *     fx = DXP(sxx) + DZM(sxz);  fz = DZP(szz) + DXM(sxz) (+ force_z samples)
+ *     vx += dt*(bx*fx);  vz += dt*(bz*fz)
+ *     force sources add their sample to fz before buoyancy   ViscoelasticProblem.jl:162-171
+ *   Stress half step n (visco-kernels.jl:22-85):
+ *     exx = DXM(vx), ezz = DZM(vz), gxz = DZP(vx) + DXP(vz)
+ *     pi_r = lam + 2 mu; X1 = X*tau; X0 = X1 + X                  visco-kernels.jl:24-37
+ *     mu_xz = 0.25*(((mu[i,j]+mu[i+1,j])+mu[i,j+1])+mu[i+1,j+1])  visco-kernels.jl:25
+ *     dsxx = pi0 exx + lam0 ezz + pi1 SMxx + lam1 SMzz             visco-kernels.jl:42
+ *     dszz = lam0 exx + pi0 ezz + lam1 SMxx + pi1 SMzz             free-surface-kernels.jl:197
+ *     dsxz = mu0 gxz + mu1 SMxz
+ *     s += dt*ds (old M), then M_n += dt*(a1_n e + a2_n M_n),
+ *     a1_n = -1/(L taun_n), a2_n = -1/taun_n                        visco-kernels.jl:49-57
+ *     explosive sources add their sample to dsxx and dszz.
+ *   CPML (CPML.jl:91-189, ViscoelasticProblem.jl:208-216), per derivative d in a strip:
+ *     psi = b*psi + a*d;  d = d/kappa + psi   (profile index = array index of the cell)
+ *   Free surface at row js = 4 (stress imaging, CPML.jl:372-380, generalised to 4 image
+ *   rows for the 8th-order stencil): on row js, ezz is replaced by the value that makes
+ *   dszz = 0, i.e. ezz = -((lam0 exx + lam1 SMxx) + pi1 SMzz)/pi0 (free-surface-kernels.jl:
+ *   159-167 intent), and szz is set to 0 after the update; then
+ *     szz[js-m] = -szz[js+m],  sxz[js-m] = -sxz[js+m-1],  m = 1..4, columns i in [4, nx-4).
+ *   Receivers sample the stored value of their field at (ix, iz) after the stress half.
+ *
+ * Defects of the draft resolved here: _ddz := ddzp (visco-kernels.jl:67 undefined); the
+ * velocity kernel's operator choice is mirrored to match its own stress kernel
+ * (visco-kernels.jl:13-16 vs 65-67); sxz uses the full engineering shear rate gxz
+ * (the draft halves it, visco-kernels.jl:67 vs ViscoelasticProblem.jl:49); szz memory terms
+ * use the symmetric form of free-surface-kernels.jl:197 (visco-kernels.jl:43 swaps them);
+ * psi is updated (the loaded path never updates it, ViscoelasticProblem.jl:183-206).
+ * Literals are cast to REAL, so an FP32 run computes in FP32 throughout.
+ */
+
+#define CAT_(a, b) a##b
+#define CAT(a, b) CAT_(a, b)
+#define NAME(x) CAT(x, SUF)
+#define SIM NAME(vio_sim)
+
+#define VIO_MAXL 8
+
+typedef struct {
+  vio_desc d;
+  int L, T;
+  size_t ncell;
+  REAL *vx, *vz, *sxx, *szz, *sxz;
+  REAL *M; /* [3][L][nz][nx] */
+  REAL *rho, *lam, *mu, *tau;
+  REAL taun[VIO_MAXL], a1[VIO_MAXL], a2[VIO_MAXL];
+  REAL Dx[4], Dz[4];
+  REAL *ax, *bx, *kx, *az, *bz, *kz;
+  int xlo, xhi, zlo, zhi;
+  REAL* psi[8];
+  int nsrc, snt;
+  int *six, *siz, *skind;
+  REAL* sw;
+  unsigned char* srcmark;
+  int nrec;
+  int *rix, *riz, *rcomp;
+  long step;
+  REAL* scr[7];
+} SIM;
+
+static REAL* NAME(vio_alloc)(size_t n) {
+  REAL* p = (REAL*)calloc(n ? n : 1, sizeof(REAL));
+  return p;
+}
+
+void* NAME(vio_create)(const vio_desc* d, const REAL* rho, const REAL* lam, const REAL* mu,
+                       const REAL* tau, const REAL* taun) {
+  if (!d || d->nx < 16 || d->nz < 16 || d->nrelax < 0 || d->nrelax > VIO_MAXL) return NULL;
+  SIM* s = (SIM*)calloc(1, sizeof(SIM));
+  s->d = *d;
+  s->L = d->nrelax;
+  s->T = d->cpml_thickness > 0 ? d->cpml_thickness : 0;
+  if (s->T && (2 * s->T > d->nx || 2 * s->T > d->nz)) { free(s); return NULL; }
+  const int nx = d->nx, nz = d->nz;
+  s->ncell = (size_t)nx * nz;
+  s->vx = NAME(vio_alloc)(s->ncell); s->vz = NAME(vio_alloc)(s->ncell);
+  s->sxx = NAME(vio_alloc)(s->ncell); s->szz = NAME(vio_alloc)(s->ncell);
+  s->sxz = NAME(vio_alloc)(s->ncell);
+  s->M = NAME(vio_alloc)((size_t)3 * s->L * s->ncell);
+  s->rho = NAME(vio_alloc)(s->ncell); s->lam = NAME(vio_alloc)(s->ncell);
+  s->mu = NAME(vio_alloc)(s->ncell); s->tau = NAME(vio_alloc)(s->ncell);
+  memcpy(s->rho, rho, s->ncell * sizeof(REAL));
+  memcpy(s->lam, lam, s->ncell * sizeof(REAL));
+  memcpy(s->mu, mu, s->ncell * sizeof(REAL));
+  if (s->L > 0) {
+    memcpy(s->tau, tau, s->ncell * sizeof(REAL));
+    for (int n = 0; n < s->L; ++n) {
+      s->taun[n] = taun[n];
+      REAL one = 1, Lr = (REAL)s->L;
+      s->a1[n] = -one / (Lr * taun[n]);
+      s->a2[n] = -one / taun[n];
+    }
+  }
+  double c[4];
+  vio_fd_coeffs(d->dx, c);
+  for (int l = 0; l < 4; ++l) s->Dx[l] = (REAL)c[l];
+  vio_fd_coeffs(d->dz, c);
+  for (int l = 0; l < 4; ++l) s->Dz[l] = (REAL)c[l];
+
+  s->ax = NAME(vio_alloc)(nx); s->bx = NAME(vio_alloc)(nx); s->kx = NAME(vio_alloc)(nx);
+  s->az = NAME(vio_alloc)(nz); s->bz = NAME(vio_alloc)(nz); s->kz = NAME(vio_alloc)(nz);
+  for (int i = 0; i < nx; ++i) s->kx[i] = 1;
+  for (int j = 0; j < nz; ++j) s->kz[j] = 1;
+  if (s->T) {
+    s->xlo = d->cpml_left; s->xhi = d->cpml_right;
+    s->zlo = d->cpml_top; s->zhi = d->cpml_bottom;
+    double* a = (double*)calloc(nx > nz ? nx : nz, sizeof(double));
+    double* b = (double*)calloc(nx > nz ? nx : nz, sizeof(double));
+    double* k = (double*)calloc(nx > nz ? nx : nz, sizeof(double));
+    vio_cpml_axis(nx, s->T, d->cpml_d0, d->cpml_alpha_max, d->cpml_kappa_max, d->cpml_power,
+                  d->dt, s->xlo, s->xhi, a, b, k, NULL, NULL);
+    for (int i = 0; i < nx; ++i) { s->ax[i] = (REAL)a[i]; s->bx[i] = (REAL)b[i]; s->kx[i] = (REAL)k[i]; }
+    vio_cpml_axis(nz, s->T, d->cpml_d0, d->cpml_alpha_max, d->cpml_kappa_max, d->cpml_power,
+                  d->dt, s->zlo, s->zhi, a, b, k, NULL, NULL);
+    for (int j = 0; j < nz; ++j) { s->az[j] = (REAL)a[j]; s->bz[j] = (REAL)b[j]; s->kz[j] = (REAL)k[j]; }
+    free(a); free(b); free(k);
+    for (int q = 0; q < 4; ++q) s->psi[q] = NAME(vio_alloc)((size_t)nz * 2 * s->T);
+    for (int q = 4; q < 8; ++q) s->psi[q] = NAME(vio_alloc)((size_t)2 * s->T * nx);
+  }
+  s->srcmark = (unsigned char*)calloc(s->ncell, 1);
+  return s;
+}
+
+void NAME(vio_destroy)(void* h) {
+  SIM* s = (SIM*)h;
+  if (!s) return;
+  free(s->vx); free(s->vz); free(s->sxx); free(s->szz); free(s->sxz); free(s->M);
+  free(s->rho); free(s->lam); free(s->mu); free(s->tau);
+  free(s->ax); free(s->bx); free(s->kx); free(s->az); free(s->bz); free(s->kz);
+  for (int q = 0; q < 8; ++q) free(s->psi[q]);
+  free(s->six); free(s->siz); free(s->skind); free(s->sw); free(s->srcmark);
+  free(s->rix); free(s->riz); free(s->rcomp);
+  for (int q = 0; q < 7; ++q) free(s->scr[q]);
+  free(s);
+}
+
+long NAME(vio_step_index)(void* h) { return ((SIM*)h)->step; }
+
+static int NAME(in_interior)(const SIM* s, int i, int j) {
+  return i >= 4 && i < s->d.nx - 4 && j >= 4 && j < s->d.nz - 4;
+}
+
+int NAME(vio_set_sources)(void* h, int n, const int* ix, const int* iz, const int* kind, int nt,
+                          const REAL* w) {
+  SIM* s = (SIM*)h;
+  if (n < 0 || nt < 0) return -1;
+  for (int q = 0; q < n; ++q)
+    if (!NAME(in_interior)(s, ix[q], iz[q]) || kind[q] < 0 || kind[q] > 2) return -1;
+  free(s->six); free(s->siz); free(s->skind); free(s->sw);
+  memset(s->srcmark, 0, s->ncell);
+  s->nsrc = n; s->snt = nt;
+  s->six = (int*)malloc(sizeof(int) * (n ? n : 1));
+  s->siz = (int*)malloc(sizeof(int) * (n ? n : 1));
+  s->skind = (int*)malloc(sizeof(int) * (n ? n : 1));
+  s->sw = NAME(vio_alloc)((size_t)n * nt);
+  for (int q = 0; q < n; ++q) {
+    s->six[q] = ix[q]; s->siz[q] = iz[q]; s->skind[q] = kind[q];
+    s->srcmark[(size_t)iz[q] * s->d.nx + ix[q]] = 1;
+  }
+  memcpy(s->sw, w, sizeof(REAL) * (size_t)n * nt);
+  return 0;
+}
+
+int NAME(vio_set_receivers)(void* h, int n, const int* ix, const int* iz, const int* comp) {
+  SIM* s = (SIM*)h;
+  if (n < 0) return -1;
+  for (int q = 0; q < n; ++q)
+    if (ix[q] < 0 || ix[q] >= s->d.nx || iz[q] < 0 || iz[q] >= s->d.nz || comp[q] < 0 ||
+        comp[q] > 4)
+      return -1;
+  free(s->rix); free(s->riz); free(s->rcomp);
+  s->nrec = n;
+  s->rix = (int*)malloc(sizeof(int) * (n ? n : 1));
+  s->riz = (int*)malloc(sizeof(int) * (n ? n : 1));
+  s->rcomp = (int*)malloc(sizeof(int) * (n ? n : 1));
+  memcpy(s->rix, ix, sizeof(int) * n);
+  memcpy(s->riz, iz, sizeof(int) * n);
+  memcpy(s->rcomp, comp, sizeof(int) * n);
+  return 0;
+}
+
+static REAL* NAME(field_ptr)(SIM* s, int f, size_t* count) {
+  const size_t nc = s->ncell;
+  switch (f) {
+    case VIO_VX: *count = nc; return s->vx;
+    case VIO_VZ: *count = nc; return s->vz;
+    case VIO_SXX: *count = nc; return s->sxx;
+    case VIO_SZZ: *count = nc; return s->szz;
+    case VIO_SXZ: *count = nc; return s->sxz;
+    case VIO_MXX: case VIO_MZZ: case VIO_MXZ:
+      if (!s->L) return NULL;
+      *count = (size_t)s->L * nc;
+      return s->M + (size_t)(f - VIO_MXX) * s->L * nc;
+    default:
+      if (f >= VIO_PSI_VX_X && f <= VIO_PSI_SZZ_Z) {
+        if (!s->T) return NULL;
+        *count = f < VIO_PSI_VX_Z ? (size_t)s->d.nz * 2 * s->T : (size_t)2 * s->T * s->d.nx;
+        return s->psi[f - VIO_PSI_VX_X];
+      }
+      return NULL;
+  }
+}
+
+int NAME(vio_get)(void* h, int field, REAL* out) {
+  size_t n = 0;
+  REAL* p = NAME(field_ptr)((SIM*)h, field, &n);
+  if (!p) return -1;
+  memcpy(out, p, n * sizeof(REAL));
+  return 0;
+}
+
+int NAME(vio_set)(void* h, int field, const REAL* in) {
+  size_t n = 0;
+  REAL* p = NAME(field_ptr)((SIM*)h, field, &n);
+  if (!p) return -1;
+  memcpy(p, in, n * sizeof(REAL));
+  return 0;
+}
+
+/* ---- operators -------------------------------------------------------------------- */
+#define OP_P(u, c, st, D)                                                       \
+  ({                                                                           \
+    REAL v_ = (D)[0] * ((u)[(c) + (st)] - (u)[(c)]);                           \
+    v_ = v_ + (D)[1] * ((u)[(c) + 2 * (st)] - (u)[(c) - (st)]);                \
+    v_ = v_ + (D)[2] * ((u)[(c) + 3 * (st)] - (u)[(c) - 2 * (st)]);            \
+    v_ = v_ + (D)[3] * ((u)[(c) + 4 * (st)] - (u)[(c) - 3 * (st)]);            \
+    v_;                                                                        \
+  })
+#define OP_M(u, c, st, D)                                                       \
+  ({                                                                           \
+    REAL v_ = (D)[0] * ((u)[(c)] - (u)[(c) - (st)]);                           \
+    v_ = v_ + (D)[1] * ((u)[(c) + (st)] - (u)[(c) - 2 * (st)]);                \
+    v_ = v_ + (D)[2] * ((u)[(c) + 2 * (st)] - (u)[(c) - 3 * (st)]);            \
+    v_ = v_ + (D)[3] * ((u)[(c) + 3 * (st)] - (u)[(c) - 4 * (st)]);            \
+    v_;                                                                        \
+  })
+
+/* strip index of column i / row j, or -1 outside the enabled strips */
+static inline int NAME(xk)(const SIM* s, int i) {
+  if (!s->T) return -1;
+  if (s->xlo && i < s->T) return i;
+  if (s->xhi && i >= s->d.nx - s->T) return i - (s->d.nx - 2 * s->T);
+  return -1;
+}
+static inline int NAME(zk)(const SIM* s, int j) {
+  if (!s->T) return -1;
+  if (s->zlo && j < s->T) return j;
+  if (s->zhi && j >= s->d.nz - s->T) return j - (s->d.nz - 2 * s->T);
+  return -1;
+}
+
+static inline REAL NAME(cpml)(REAL* psi, REAL a, REAL b, REAL k, REAL d) {
+  REAL p = b * (*psi) + a * d;
+  *psi = p;
+  return d / k + p;
+}
+
+static inline REAL NAME(src_sample)(const SIM* s, int q, long n) {
+  return n < s->snt ? s->sw[(size_t)q * s->snt + n] : (REAL)0;
+}
+
+/* ---- fused path ------------------------------------------------------------------- */
+static void NAME(vel_fused)(SIM* s, long n) {
+  const int nx = s->d.nx, nz = s->d.nz, T = s->T;
+  const REAL dt = (REAL)s->d.dt, two = 2;
+  const REAL *Dx = s->Dx, *Dz = s->Dz;
+#pragma omp parallel for schedule(static)
+  for (int j = 4; j < nz - 4; ++j) {
+    const int zk = NAME(zk)(s, j);
+    for (int i = 4; i < nx - 4; ++i) {
+      const size_t c = (size_t)j * nx + i;
+      REAL dsxx_x = OP_P(s->sxx, c, 1, Dx);
+      REAL dsxz_z = OP_M(s->sxz, c, (size_t)nx, Dz);
+      REAL dsxz_x = OP_M(s->sxz, c, 1, Dx);
+      REAL dszz_z = OP_P(s->szz, c, (size_t)nx, Dz);
+      const int xk = NAME(xk)(s, i);
+      if (xk >= 0) {
+        size_t p = (size_t)j * 2 * T + xk;
+        dsxx_x = NAME(cpml)(&s->psi[2][p], s->ax[i], s->bx[i], s->kx[i], dsxx_x);
+        dsxz_x = NAME(cpml)(&s->psi[3][p], s->ax[i], s->bx[i], s->kx[i], dsxz_x);
+      }
+      if (zk >= 0) {
+        size_t p = (size_t)zk * nx + i;
+        dsxz_z = NAME(cpml)(&s->psi[6][p], s->az[j], s->bz[j], s->kz[j], dsxz_z);
+        dszz_z = NAME(cpml)(&s->psi[7][p], s->az[j], s->bz[j], s->kz[j], dszz_z);
+      }
+      REAL fx = dsxx_x + dsxz_z;
+      REAL fz = dszz_z + dsxz_x;
+      if (s->srcmark[c]) {
+        for (int q = 0; q < s->nsrc; ++q) {
+          if ((size_t)s->siz[q] * nx + s->six[q] != c) continue;
+          if (s->skind[q] == VIO_SRC_FORCE_Z) fz = fz + NAME(src_sample)(s, q, n);
+          else if (s->skind[q] == VIO_SRC_FORCE_X) fx = fx + NAME(src_sample)(s, q, n);
+        }
+      }
+      const REAL bxv = two / (s->rho[c] + s->rho[c + 1]);
+      const REAL bzv = two / (s->rho[c] + s->rho[c + nx]);
+      s->vx[c] = s->vx[c] + dt * (bxv * fx);
+      s->vz[c] = s->vz[c] + dt * (bzv * fz);
+    }
+  }
+}
+
+/* constitutive law + memory update for one cell; e* are CPML-corrected rates */
+static inline void NAME(stress_cell)(SIM* s, size_t c, long n, REAL exx, REAL ezz, REAL gxz,
+                                     int fs_row, REAL* ezz_used) {
+  const int nx = s->d.nx, L = s->L;
+  const REAL dt = (REAL)s->d.dt, two = 2, quarter = (REAL)0.25;
+  const size_t nc = s->ncell;
+  const REAL lr = s->lam[c], mr = s->mu[c];
+  const REAL mxz = quarter * (((s->mu[c] + s->mu[c + 1]) + s->mu[c + nx]) + s->mu[c + nx + 1]);
+  const REAL pr = lr + two * mr;
+  REAL dsxx, dszz, dsxz;
+  if (L > 0) {
+    const REAL t = s->tau[c];
+    const REAL p1 = pr * t, p0 = p1 + pr;
+    const REAL l1 = lr * t, l0 = l1 + lr;
+    const REAL m1 = mxz * t, m0 = m1 + mxz;
+    REAL* Mxx = s->M;
+    REAL* Mzz = s->M + (size_t)L * nc;
+    REAL* Mxz = s->M + (size_t)2 * L * nc;
+    REAL SMxx = Mxx[c], SMzz = Mzz[c], SMxz = Mxz[c];
+    for (int k = 1; k < L; ++k) {
+      SMxx = SMxx + Mxx[(size_t)k * nc + c];
+      SMzz = SMzz + Mzz[(size_t)k * nc + c];
+      SMxz = SMxz + Mxz[(size_t)k * nc + c];
+    }
+    if (fs_row) ezz = -(((l0 * exx) + l1 * SMxx) + p1 * SMzz) / p0;
+    dsxx = p0 * exx + l0 * ezz + p1 * SMxx + l1 * SMzz;
+    dszz = l0 * exx + p0 * ezz + l1 * SMxx + p1 * SMzz;
+    dsxz = m0 * gxz + m1 * SMxz;
+  } else {
+    if (fs_row) ezz = -(lr * exx) / pr;
+    dsxx = pr * exx + lr * ezz;
+    dszz = lr * exx + pr * ezz;
+    dsxz = mxz * gxz;
+  }
+  if (s->srcmark[c]) {
+    for (int q = 0; q < s->nsrc; ++q) {
+      if ((size_t)s->siz[q] * nx + s->six[q] != c || s->skind[q] != VIO_SRC_EXPLOSIVE) continue;
+      const REAL w = NAME(src_sample)(s, q, n);
+      dsxx = dsxx + w;
+      dszz = dszz + w;
+    }
+  }
+  s->sxx[c] = s->sxx[c] + dt * dsxx;
+  s->szz[c] = s->szz[c] + dt * dszz;
+  s->sxz[c] = s->sxz[c] + dt * dsxz;
+  if (fs_row) s->szz[c] = 0;
+  *ezz_used = ezz;
+}
+
+static inline void NAME(memory_cell)(SIM* s, size_t c, REAL exx, REAL ezz, REAL gxz) {
+  const int L = s->L;
+  const REAL dt = (REAL)s->d.dt;
+  const size_t nc = s->ncell;
+  REAL* Mxx = s->M;
+  REAL* Mzz = s->M + (size_t)L * nc;
+  REAL* Mxz = s->M + (size_t)2 * L * nc;
+  for (int k = 0; k < L; ++k) {
+    const size_t o = (size_t)k * nc + c;
+    const REAL a1 = s->a1[k], a2 = s->a2[k];
+    Mxx[o] = Mxx[o] + dt * (a1 * exx + a2 * Mxx[o]);
+    Mzz[o] = Mzz[o] + dt * (a1 * ezz + a2 * Mzz[o]);
+    Mxz[o] = Mxz[o] + dt * (a1 * gxz + a2 * Mxz[o]);
+  }
+}
+
+static void NAME(images)(SIM* s) {
+  const int nx = s->d.nx;
+  for (int m = 1; m <= 4; ++m)
+    for (int i = 4; i < nx - 4; ++i) {
+      s->szz[(size_t)(4 - m) * nx + i] = -s->szz[(size_t)(4 + m) * nx + i];
+      s->sxz[(size_t)(4 - m) * nx + i] = -s->sxz[(size_t)(4 + m - 1) * nx + i];
+    }
+}
+
+static void NAME(stress_fused)(SIM* s, long n) {
+  const int nx = s->d.nx, nz = s->d.nz, T = s->T;
+  const REAL *Dx = s->Dx, *Dz = s->Dz;
+  const int fs = s->d.free_surface;
+#pragma omp parallel for schedule(static)
+  for (int j = 4; j < nz - 4; ++j) {
+    const int zk = NAME(zk)(s, j);
+    for (int i = 4; i < nx - 4; ++i) {
+      const size_t c = (size_t)j * nx + i;
+      REAL exx = OP_M(s->vx, c, 1, Dx);
+      REAL ezz = OP_M(s->vz, c, (size_t)nx, Dz);
+      REAL ezx = OP_P(s->vx, c, (size_t)nx, Dz);
+      REAL exz = OP_P(s->vz, c, 1, Dx);
+      const int xk = NAME(xk)(s, i);
+      if (xk >= 0) {
+        size_t p = (size_t)j * 2 * T + xk;
+        exx = NAME(cpml)(&s->psi[0][p], s->ax[i], s->bx[i], s->kx[i], exx);
+        exz = NAME(cpml)(&s->psi[1][p], s->ax[i], s->bx[i], s->kx[i], exz);
+      }
+      if (zk >= 0) {
+        size_t p = (size_t)zk * nx + i;
+        ezx = NAME(cpml)(&s->psi[4][p], s->az[j], s->bz[j], s->kz[j], ezx);
+        ezz = NAME(cpml)(&s->psi[5][p], s->az[j], s->bz[j], s->kz[j], ezz);
+      }
+      const REAL gxz = ezx + exz;
+      REAL ezz_used;
+      NAME(stress_cell)(s, c, n, exx, ezz, gxz, fs && j == 4, &ezz_used);
+      if (s->L) NAME(memory_cell)(s, c, exx, ezz_used, gxz);
+    }
+  }
+  if (fs) NAME(images)(s);
+}
+
+/* ---- reference-structured multi-pass path (ViscoelasticProblem.jl:62-174) ----------- */
+static void NAME(ensure_scratch)(SIM* s) {
+  for (int q = 0; q < 7; ++q)
+    if (!s->scr[q]) s->scr[q] = NAME(vio_alloc)(s->ncell);
+}
+
+/* ddx!/ddz! analogue: one derivative into a scratch array over the padded interior */
+static void NAME(deriv_pass)(SIM* s, REAL* out, const REAL* u, int plus, int along_z) {
+  const int nx = s->d.nx, nz = s->d.nz;
+  const REAL* D = along_z ? s->Dz : s->Dx;
+  const size_t st = along_z ? (size_t)nx : 1;
+#pragma omp parallel for schedule(static)
+  for (int j = 4; j < nz - 4; ++j)
+    for (int i = 4; i < nx - 4; ++i) {
+      const size_t c = (size_t)j * nx + i;
+      out[c] = plus ? OP_P(u, c, st, D) : OP_M(u, c, st, D);
+    }
+}
+
+/* _cpml_scale_{x,z}! analogue with the psi recursion folded in */
+static void NAME(cpml_pass)(SIM* s, REAL* du, REAL* psi, int along_z) {
+  const int nx = s->d.nx, nz = s->d.nz, T = s->T;
+  if (!T) return;
+#pragma omp parallel for schedule(static)
+  for (int j = 4; j < nz - 4; ++j) {
+    const int zk = NAME(zk)(s, j);
+    if (along_z && zk < 0) continue;
+    for (int i = 4; i < nx - 4; ++i) {
+      const size_t c = (size_t)j * nx + i;
+      if (along_z) {
+        du[c] = NAME(cpml)(&psi[(size_t)zk * nx + i], s->az[j], s->bz[j], s->kz[j], du[c]);
+      } else {
+        const int xk = NAME(xk)(s, i);
+        if (xk < 0) continue;
+        du[c] = NAME(cpml)(&psi[(size_t)j * 2 * T + xk], s->ax[i], s->bx[i], s->kx[i], du[c]);
+      }
+    }
+  }
+}
+
+static void NAME(vel_multipass)(SIM* s, long n) {
+  const int nx = s->d.nx, nz = s->d.nz;
+  const REAL dt = (REAL)s->d.dt, two = 2;
+  NAME(ensure_scratch)(s);
+  REAL *d1 = s->scr[0], *d2 = s->scr[1], *d3 = s->scr[2], *d4 = s->scr[3];
+  NAME(deriv_pass)(s, d1, s->sxx, 1, 0); /* dx_sxx */
+  NAME(deriv_pass)(s, d2, s->sxz, 0, 1); /* dz_sxz */
+  NAME(deriv_pass)(s, d3, s->sxz, 0, 0); /* dx_szx */
+  NAME(deriv_pass)(s, d4, s->szz, 1, 1); /* dz_szz */
+  NAME(cpml_pass)(s, d1, s->psi[2], 0);
+  NAME(cpml_pass)(s, d3, s->psi[3], 0);
+  NAME(cpml_pass)(s, d2, s->psi[6], 1);
+  NAME(cpml_pass)(s, d4, s->psi[7], 1);
+#pragma omp parallel for schedule(static)
+  for (int j = 4; j < nz - 4; ++j)
+    for (int i = 4; i < nx - 4; ++i) {
+      const size_t c = (size_t)j * nx + i;
+      d1[c] = d1[c] + d2[c]; /* dvx = dx_sxx + dz_sxz */
+      d4[c] = d4[c] + d3[c]; /* dvz = dz_szz + dx_szx */
+    }
+  for (int q = 0; q < s->nsrc; ++q) { /* source!(dvz, t) before buoyancy */
+    const size_t c = (size_t)s->siz[q] * nx + s->six[q];
+    if (s->skind[q] == VIO_SRC_FORCE_Z) d4[c] = d4[c] + NAME(src_sample)(s, q, n);
+    else if (s->skind[q] == VIO_SRC_FORCE_X) d1[c] = d1[c] + NAME(src_sample)(s, q, n);
+  }
+#pragma omp parallel for schedule(static)
+  for (int j = 4; j < nz - 4; ++j)
+    for (int i = 4; i < nx - 4; ++i) {
+      const size_t c = (size_t)j * nx + i;
+      d1[c] = (two / (s->rho[c] + s->rho[c + 1])) * d1[c];
+      d4[c] = (two / (s->rho[c] + s->rho[c + nx])) * d4[c];
+    }
+#pragma omp parallel for schedule(static)
+  for (int j = 4; j < nz - 4; ++j)
+    for (int i = 4; i < nx - 4; ++i) {
+      const size_t c = (size_t)j * nx + i;
+      s->vx[c] = s->vx[c] + dt * d1[c];
+      s->vz[c] = s->vz[c] + dt * d4[c];
+    }
+}
+
+static void NAME(stress_multipass)(SIM* s, long n) {
+  const int nx = s->d.nx, nz = s->d.nz, L = s->L;
+  const REAL dt = (REAL)s->d.dt, two = 2, quarter = (REAL)0.25;
+  const size_t nc = s->ncell;
+  const int fs = s->d.free_surface;
+  NAME(ensure_scratch)(s);
+  REAL *e1 = s->scr[0], *e2 = s->scr[1], *e3 = s->scr[2], *e4 = s->scr[3];
+  REAL *ds1 = s->scr[4], *ds2 = s->scr[5], *ds3 = s->scr[6];
+  NAME(deriv_pass)(s, e1, s->vx, 0, 0); /* dxvx */
+  NAME(deriv_pass)(s, e2, s->vz, 0, 1); /* dzvz */
+  NAME(deriv_pass)(s, e3, s->vx, 1, 1); /* dzvx */
+  NAME(deriv_pass)(s, e4, s->vz, 1, 0); /* dxvz */
+  NAME(cpml_pass)(s, e1, s->psi[0], 0);
+  NAME(cpml_pass)(s, e4, s->psi[1], 0);
+  NAME(cpml_pass)(s, e3, s->psi[4], 1);
+  NAME(cpml_pass)(s, e2, s->psi[5], 1);
+  REAL* Mxx = s->M;
+  REAL* Mzz = s->M + (size_t)L * nc;
+  REAL* Mxz = s->M + (size_t)2 * L * nc;
+  /* _update_ds! analogue (ViscoelasticProblem.jl:38-50) extended with the SLS terms */
+#pragma omp parallel for schedule(static)
+  for (int j = 4; j < nz - 4; ++j)
+    for (int i = 4; i < nx - 4; ++i) {
+      const size_t c = (size_t)j * nx + i;
+      const REAL exx = e1[c];
+      REAL ezz = e2[c];
+      const REAL gxz = e3[c] + e4[c];
+      e3[c] = gxz;
+      const REAL lr = s->lam[c], mr = s->mu[c];
+      const REAL mxz = quarter * (((s->mu[c] + s->mu[c + 1]) + s->mu[c + nx]) + s->mu[c + nx + 1]);
+      const REAL pr = lr + two * mr;
+      const int fs_row = fs && j == 4;
+      if (L > 0) {
+        const REAL t = s->tau[c];
+        const REAL p1 = pr * t, p0 = p1 + pr, l1 = lr * t, l0 = l1 + lr, m1 = mxz * t, m0 = m1 + mxz;
+        REAL SMxx = Mxx[c], SMzz = Mzz[c], SMxz = Mxz[c];
+        for (int k = 1; k < L; ++k) {
+          SMxx = SMxx + Mxx[(size_t)k * nc + c];
+          SMzz = SMzz + Mzz[(size_t)k * nc + c];
+          SMxz = SMxz + Mxz[(size_t)k * nc + c];
+        }
+        if (fs_row) ezz = -(((l0 * exx) + l1 * SMxx) + p1 * SMzz) / p0;
+        ds1[c] = p0 * exx + l0 * ezz + p1 * SMxx + l1 * SMzz;
+        ds2[c] = l0 * exx + p0 * ezz + l1 * SMxx + p1 * SMzz;
+        ds3[c] = m0 * gxz + m1 * SMxz;
+      } else {
+        if (fs_row) ezz = -(lr * exx) / pr;
+        ds1[c] = pr * exx + lr * ezz;
+        ds2[c] = lr * exx + pr * ezz;
+        ds3[c] = mxz * gxz;
+      }
+      e2[c] = ezz;
+    }
+  for (int q = 0; q < s->nsrc; ++q) {
+    if (s->skind[q] != VIO_SRC_EXPLOSIVE) continue;
+    const size_t c = (size_t)s->siz[q] * nx + s->six[q];
+    const REAL w = NAME(src_sample)(s, q, n);
+    ds1[c] = ds1[c] + w;
+    ds2[c] = ds2[c] + w;
+  }
+#pragma omp parallel for schedule(static)
+  for (int j = 4; j < nz - 4; ++j)
+    for (int i = 4; i < nx - 4; ++i) {
+      const size_t c = (size_t)j * nx + i;
+      s->sxx[c] = s->sxx[c] + dt * ds1[c];
+      s->szz[c] = s->szz[c] + dt * ds2[c];
+      s->sxz[c] = s->sxz[c] + dt * ds3[c];
+      if (fs && j == 4) s->szz[c] = 0;
+    }
+  if (L > 0) {
+#pragma omp parallel for schedule(static)
+    for (int j = 4; j < nz - 4; ++j)
+      for (int i = 4; i < nx - 4; ++i) {
+        const size_t c = (size_t)j * nx + i;
+        NAME(memory_cell)(s, c, e1[c], e2[c], e3[c]);
+      }
+  }
+  if (fs) NAME(images)(s);
+}
+
+static void NAME(sample)(SIM* s, REAL* out) {
+  const int nx = s->d.nx;
+  for (int r = 0; r < s->nrec; ++r) {
+    const size_t c = (size_t)s->riz[r] * nx + s->rix[r];
+    const REAL* f = s->rcomp[r] == 0   ? s->vx
+                    : s->rcomp[r] == 1 ? s->vz
+                    : s->rcomp[r] == 2 ? s->sxx
+                    : s->rcomp[r] == 3 ? s->szz
+                                       : s->sxz;
+    out[r] = f[c];
+  }
+}
+
+int NAME(vio_run)(void* h, int nsteps, int mode, int nthreads, REAL* traces) {
+  SIM* s = (SIM*)h;
+  if (!s || nsteps < 0) return -1;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  for (int k = 0; k < nsteps; ++k) {
+    const long n = s->step;
+    if (mode == VIO_MULTIPASS) {
+      NAME(vel_multipass)(s, n);
+      NAME(stress_multipass)(s, n);
+    } else {
+      NAME(vel_fused)(s, n);
+      NAME(stress_fused)(s, n);
+    }
+    if (traces && s->nrec) NAME(sample)(s, traces + (size_t)k * s->nrec);
+    s->step = n + 1;
+  }
+  return 0;
+}
+
+#undef SIM
+#undef NAME
+#undef CAT
+#undef CAT_
+#undef OP_P
+#undef OP_M
+#undef VIO_MAXL

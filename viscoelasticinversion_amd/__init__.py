@@ -1,0 +1,17 @@
+"""MI355X-native (gfx950) 2D staggered-grid viscoelastic velocity-stress stepper.
+
+The compute path is the HIP library ``libvi_mi355x.so`` behind the C ABI of
+``include/vi_mi355x.h``; this package is its host-side mirror of the reference solver API.
+"""
+from .api import (CPMLConfig, DirichletBC, Fields, FreeSurfaceBC, IsotropicModel, K,
+                  Parameters, Problem, QModel, RickerSource, Solution, _ricker, dK_dw,
+                  find_optimal_taun, get_bc, get_Q, get_tau, leapfrog_step, make_problem,
+                  solve_problem)
+from .stepper import COMPONENTS, FIELDS, SOURCE_KINDS, Simulation
+
+__all__ = [
+    "CPMLConfig", "DirichletBC", "Fields", "FreeSurfaceBC", "IsotropicModel", "K",
+    "Parameters", "Problem", "QModel", "RickerSource", "Solution", "_ricker", "dK_dw",
+    "find_optimal_taun", "get_bc", "get_Q", "get_tau", "leapfrog_step", "make_problem",
+    "solve_problem", "Simulation", "FIELDS", "COMPONENTS", "SOURCE_KINDS",
+]

@@ -1,0 +1,359 @@
+"""Host-side mirror of ViscoelasticInversion's solver API, backed by the HIP library.
+
+Names, keyword arguments, defaults and error behaviour follow the reference:
+  _ricker, RickerSource            src/sciml_solver/Source.jl:3-58
+  CPMLConfig                       src/sciml_solver/CPML.jl:28-63
+  FreeSurfaceBC                    src/sciml_solver/CPML.jl:366-370
+  DirichletBC / get_bc             src/sciml_solver/BoundaryConditions.jl:3-11, 88-100
+  IsotropicModel                   src/sciml_solver/Models.jl:3-25
+  K, dK_dω, get_τ, get_Q, QModel   src/sciml_solver/Models.jl:65-107
+  Parameters                       src/sciml_solver/Parameters.jl:31-86
+  make_problem, solve_problem      src/sciml_solver/ViscoelasticProblem.jl:273-309
+  Fields, leapfrog_step!           src/visco-kernels.jl:87-113
+The time integrator is the fused staggered leapfrog of visco-kernels.jl (the adaptive
+SciML Tsit5 path of the reference is replaced by it; ``solver`` only accepts "leapfrog").
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from .stepper import Simulation
+
+
+# ---------------------------------------------------------------------------- source
+def _ricker(f, t):
+    """Ricker wavelet (1 - 2(pi f t)^2) exp(-(pi f t)^2)   (Source.jl:3-5)."""
+    a = (np.pi * f * np.asarray(t, dtype=np.float64)) ** 2
+    return (1.0 - 2.0 * a) * np.exp(-a)
+
+
+class RickerSource:
+    """Point Ricker source (Source.jl:7-33).
+
+    ``kind`` is "force_z" (the reference: added to dvz before buoyancy), "force_x" or
+    "explosive" (added to the sxx and szz rates). The active window ``tspan`` is the open
+    interval (tc - 2/f, tc + 2/f) unless given; |r| there is below 6e-16 at the ends.
+    Grid indices are 0-based (ix, iz).
+    """
+
+    def __init__(self, freq, tc, nx, nz, coef=1.0, kind="force_z", tspan=None, rtol=1e-6):
+        if freq <= 0:
+            raise ValueError("freq must be positive")
+        self.freq, self.tc = float(freq), float(tc)
+        self.grid = (int(nx), int(nz))
+        self.coefs = float(coef)
+        self.kind = kind
+        self.tspan = tuple(tspan) if tspan is not None else (tc - 2.0 / freq, tc + 2.0 / freq)
+
+    def value(self, t):
+        """Source sample c * r(t - tc) inside the open window, else 0 (Source.jl:42-58)."""
+        t = np.asarray(t, dtype=np.float64)
+        inside = (self.tspan[0] < t) & (t < self.tspan[1])
+        return np.where(inside, self.coefs * _ricker(self.freq, t - self.tc), 0.0)
+
+    def __call__(self, du, t):
+        """Add the sample at time t to du[nz, nx] (inject_source_kernel!, Source.jl:35-40)."""
+        du[self.grid[1], self.grid[0]] += self.value(t)
+        return None
+
+    def samples(self, nt, dt, t0=0.0):
+        """Samples used by step k: force sources at t0 + k dt, explosive at t0 + (k+1/2) dt."""
+        off = 0.5 if self.kind == "explosive" else 0.0
+        return self.value(t0 + (np.arange(nt) + off) * dt)
+
+
+# ---------------------------------------------------------------------------- CPML
+@dataclass
+class CPMLConfig:
+    """CPML configuration (CPML.jl:28-63)."""
+    thickness: int = 10
+    d0: Optional[float] = None
+    alpha_max: float = 0.0
+    kappa_max: float = 1.0
+    power: float = 2.0
+    left: bool = True
+    right: bool = True
+    top: bool = False
+    bottom: bool = True
+    vmax: Optional[float] = None
+    dx: Optional[float] = None
+    reflection_coef: float = 1e-5
+
+    def __post_init__(self):
+        if self.d0 is None:
+            if self.vmax is None or self.dx is None:
+                raise ValueError("Must provide either d0 or (vmax and dx) for automatic "
+                                 "damping calculation")
+            # Collino & Tsogka (2001), CPML.jl:60
+            self.d0 = -(self.power + 1) * self.vmax * math.log(self.reflection_coef) / (
+                2 * self.thickness * self.dx)
+
+
+@dataclass
+class FreeSurfaceBC:
+    """Stress-imaging free surface on row 4 (CPML.jl:366-393). ``depth`` is kept for API
+    compatibility; the 8th-order stencil always uses 4 image rows."""
+    depth: int = 1
+
+
+class DirichletBC:
+    """Fields outside the padded interior stay at their initial value (BoundaryConditions.jl:3-11)."""
+
+    def __call__(self, du):
+        return None
+
+
+def get_bc(bc_type):
+    """BoundaryConditions.jl:88-100. Only :dirichlet is implemented; periodic and Neumann are
+    outside this library's scope."""
+    key = str(bc_type).lstrip(":")
+    if key == "dirichlet":
+        return DirichletBC()
+    if key in ("neumann", "periodic", "absorbing"):
+        raise NotImplementedError(f"boundary condition {key!r} is not implemented; use "
+                                  "'dirichlet' with an optional CPMLConfig")
+    raise ValueError(f"Unknown boundary condition type: {bc_type}")
+
+
+# ---------------------------------------------------------------------------- models
+class IsotropicModel:
+    """Isotropic model from density and Lamé parameters, scalars or (Nz, Nx) arrays
+    (Models.jl:3-25; fields b, λ, μ)."""
+
+    def __init__(self, rho, lam, mu, Nx, Nz):
+        shp = (int(Nz), int(Nx))
+        self.rho = np.array(np.broadcast_to(rho, shp), dtype=np.float64)
+        self.lam = np.array(np.broadcast_to(lam, shp), dtype=np.float64)
+        self.mu = np.array(np.broadcast_to(mu, shp), dtype=np.float64)
+
+    @property
+    def b(self):
+        return 1.0 / self.rho
+
+    @classmethod
+    def from_velocities(cls, vp, vs, rho, Nx, Nz):
+        rho = np.broadcast_to(rho, (Nz, Nx)).astype(np.float64)
+        mu = rho * np.asarray(vs, dtype=np.float64) ** 2
+        lam = rho * np.asarray(vp, dtype=np.float64) ** 2 - 2.0 * mu
+        return cls(rho, lam, mu, Nx, Nz)
+
+    def vmax(self):
+        return float(np.sqrt((self.lam + 2 * self.mu) / self.rho).max())
+
+
+def K(w, taun):
+    """K(ω) = Σ ωτₙ/(1+τₙ²ω²)   (Models.jl:65)."""
+    w = np.asarray(w, dtype=np.float64)[..., None]
+    t = np.asarray(taun, dtype=np.float64)
+    return np.sum(w * t / (1.0 + t ** 2 * w ** 2), axis=-1)
+
+
+def dK_dw(w, taun):
+    """dK/dω   (Models.jl:66)."""
+    w = np.asarray(w, dtype=np.float64)[..., None]
+    t = np.asarray(taun, dtype=np.float64)
+    return np.sum(t * (1.0 - t ** 2 * w ** 2) / (1.0 + t ** 2 * w ** 2) ** 2, axis=-1)
+
+
+def get_tau(Q0, Kbar, N):
+    """τ = N/(K̄ Q₀)   (Models.jl:67)."""
+    return N / (Kbar * Q0)
+
+
+def get_Q(w, tau, taun):
+    """Q(ω) = N/(τ K(ω))   (Models.jl:68)."""
+    return len(np.atleast_1d(taun)) / (tau * K(w, taun))
+
+
+def Q_model_J2(taun, w1, w2):
+    """∫_{ω1}^{ω2} (dK/dω)² dω   (Models.jl:70-74)."""
+    from scipy.integrate import quad
+    val, _ = quad(lambda w: float(dK_dw(w, taun)) ** 2, w1, w2, limit=200)
+    return val
+
+
+def find_optimal_taun(w1, w2, N):
+    """Nelder-Mead minimisation of Q_model_J2 from τₙ = 1/linspace(ω1, ω2, N)
+    (Models.jl:77-87). The objective is scale-degenerate, so the result depends on the
+    optimiser's path; callers that need reproducible τₙ pass them explicitly."""
+    from scipy.optimize import minimize
+    t0 = 1.0 / np.linspace(w1, w2, N)
+    res = minimize(lambda t: Q_model_J2(t, w1, w2), t0, method="Nelder-Mead",
+                   options=dict(maxiter=10 ** 6, maxfev=10 ** 6))
+    return np.sort(res.x)
+
+
+class QModel:
+    """Nearly-constant-Q SLS model (Models.jl:56-107): τₙ (N mechanisms) and the field τ."""
+
+    def __init__(self, f1, f2, Nx, Nz, Q0=100, N=5, taun=None):
+        w1, w2 = 2 * np.pi * f1, 2 * np.pi * f2
+        self.f1, self.f2, self.N = float(f1), float(f2), int(N)
+        self.taun = np.sort(np.asarray(taun, dtype=np.float64)) if taun is not None else \
+            find_optimal_taun(w1, w2, N)
+        self.K = float(np.mean(K(np.linspace(w1, w2, 100), self.taun)))
+        self.tau = np.full((int(Nz), int(Nx)), get_tau(Q0, self.K, self.N))
+
+    def update(self, Q):
+        """τ = get_τ.(Q, K̄, N)   (Models.jl:104-107)."""
+        self.tau = get_tau(np.asarray(Q, dtype=np.float64), self.K, self.N)
+
+    def Q(self, w):
+        return get_Q(w, float(np.mean(self.tau)), self.taun)
+
+
+# ---------------------------------------------------------------------------- parameters
+@dataclass
+class Parameters:
+    """Solver configuration (Parameters.jl:31-86), keyword-constructed like the reference.
+
+    Differences: ``bc`` defaults to "dirichlet" (the reference's periodic BC is defective
+    and out of scope), ``device`` must be "mi355x", ``precision`` selects FP32 (default,
+    the reference's GPU precision) or FP64, ``qmodel`` adds SLS attenuation, ``receivers``
+    is an optional (ix, iz, comp) triple.
+    """
+    Nx: int = 64
+    Nz: int = 64
+    dx: float = 10.0
+    dz: float = 10.0
+    dt: float = 0.001
+    fd_order_x: int = 8
+    fd_order_z: int = 8
+    bc: str = "dirichlet"
+    source: Optional[object] = None
+    device: str = "mi355x"
+    model: Optional[IsotropicModel] = None
+    cpml: Optional[CPMLConfig] = None
+    free_surface: bool = False
+    precision: str = "float32"
+    qmodel: Optional[QModel] = None
+    receivers: Optional[tuple] = None
+
+    def __post_init__(self):
+        if self.fd_order_x != 8 or self.fd_order_z != 8:
+            raise NotImplementedError("only 8th-order staggered operators are implemented")
+        if str(self.device).lower() not in ("mi355x", "gpu", "hip"):
+            raise ValueError(f"unsupported device {self.device!r}: this library runs on MI355X")
+        self.bc_obj = get_bc(self.bc)
+        if self.source is None:
+            self.source = RickerSource(40.0, 0.2, self.Nx // 2, self.Nz // 2)  # Parameters.jl:59
+        if self.model is None:
+            self.model = IsotropicModel(2000.0, 1e9, 1e9, self.Nx, self.Nz)  # Parameters.jl:63
+        self.free_surface_bc = FreeSurfaceBC() if self.free_surface else None
+        self.sources = self.source if isinstance(self.source, (list, tuple)) else [self.source]
+
+    @property
+    def nrelax(self):
+        return self.qmodel.N if self.qmodel is not None else 0
+
+    def build(self) -> Simulation:
+        """Create and configure a device simulation for these parameters."""
+        sim = Simulation(self.Nx, self.Nz, self.dx, self.dz, self.dt, self.precision,
+                         self.nrelax)
+        m = self.model
+        sim.set_model(m.rho, m.lam, m.mu, self.qmodel.tau if self.qmodel is not None else None)
+        if self.qmodel is not None:
+            sim.set_relaxation(self.qmodel.taun)
+        if self.cpml is not None:
+            c = self.cpml
+            sim.set_cpml(c.thickness, c.d0, c.alpha_max, c.kappa_max, c.power, c.left, c.right,
+                         c.top, c.bottom)
+        sim.set_free_surface(self.free_surface)
+        return sim
+
+
+# ---------------------------------------------------------------------------- problem
+@dataclass
+class Problem:
+    """What make_problem returns: initial state u0 = (s0, v0) with s0 (3, Nz, Nx) holding
+    [σxx, σzz, σxz] and v0 (2, Nz, Nx) holding [vx, vz], the time span and the parameters."""
+    u0: tuple
+    tspan: tuple
+    p: Parameters
+
+
+def make_problem(parameters, s0=None, v0=None, tspan=(0.0, 2.0)):
+    """ViscoelasticProblem.jl:273-292 (zero initial state by default)."""
+    shp = (parameters.Nz, parameters.Nx)
+    s0 = np.zeros((3,) + shp) if s0 is None else np.asarray(s0, dtype=np.float64)
+    v0 = np.zeros((2,) + shp) if v0 is None else np.asarray(v0, dtype=np.float64)
+    if s0.shape != (3,) + shp or v0.shape != (2,) + shp:
+        raise ValueError("s0 must be (3, Nz, Nx) and v0 (2, Nz, Nx)")
+    return Problem((s0, v0), tuple(tspan), parameters)
+
+
+@dataclass
+class Solution:
+    t: np.ndarray
+    s: np.ndarray            # (3, Nz, Nx) final stress
+    v: np.ndarray            # (2, Nz, Nx) final velocity
+    traces: Optional[np.ndarray] = None   # (nrec, nt)
+    retcode: str = "Success"
+    saved: list = field(default_factory=list)
+
+
+def solve_problem(problem, solver="leapfrog", saveat=None, **kwargs):
+    """Integrate from tspan[0] to tspan[1] with nt = round((t1 - t0)/dt) fused leapfrog
+    steps on the GPU (ViscoelasticProblem.jl:307-309). ``saveat`` (a step interval) stores
+    (t, s, v) snapshots."""
+    if solver not in ("leapfrog", None):
+        raise NotImplementedError("only the fused leapfrog integrator is available")
+    p = problem.p
+    t0, t1 = problem.tspan
+    nt = int(round((t1 - t0) / p.dt))
+    sim = p.build()
+    try:
+        s0, v0 = problem.u0
+        for k, name in enumerate(("sxx", "szz", "sxz")):
+            sim.set(name, s0[k])
+        for k, name in enumerate(("vx", "vz")):
+            sim.set(name, v0[k])
+        if p.sources:
+            ix = [s.grid[0] for s in p.sources]
+            iz = [s.grid[1] for s in p.sources]
+            kinds = [getattr(s, "kind", "force_z") for s in p.sources]
+            w = np.stack([s.samples(nt, p.dt, t0) for s in p.sources])
+            sim.set_sources(ix, iz, kinds, w)
+        if p.receivers is not None:
+            rix, riz, rc = p.receivers
+            sim.set_receivers(rix, riz, rc, nt)
+        saved = []
+        every = int(saveat) if saveat else 0
+        done = 0
+        while done < nt:
+            n = nt - done if not every else min(every, nt - done)
+            sim.step(n)
+            done += n
+            if every:
+                saved.append((t0 + done * p.dt,
+                              np.stack([sim.get(f) for f in ("sxx", "szz", "sxz")]),
+                              np.stack([sim.get(f) for f in ("vx", "vz")])))
+        s = np.stack([sim.get(f) for f in ("sxx", "szz", "sxz")])
+        v = np.stack([sim.get(f) for f in ("vx", "vz")])
+        tr = sim.traces(nt) if p.receivers is not None else None
+        return Solution(t=t0 + np.arange(1, nt + 1) * p.dt, s=s, v=v, traces=tr, saved=saved)
+    finally:
+        sim.close()
+
+
+# ---------------------------------------------------------------------------- fused step
+@dataclass
+class Fields:
+    """Fields(s, M, v) of visco-kernels.jl:102-106, held on the device by ``sim``."""
+    sim: Simulation
+
+
+def leapfrog_step(fields, parameters=None, dt=None, nsteps=1):
+    """leapfrog_step!(fields, parameters, dt) (visco-kernels.jl:87-100): advance the device
+    state by full steps (velocity kernel, then stress kernel). ``dt`` must equal the
+    simulation's dt when given."""
+    sim = fields.sim if isinstance(fields, Fields) else fields
+    if dt is not None and not math.isclose(dt, sim.dt, rel_tol=0, abs_tol=0):
+        raise ValueError("dt differs from the simulation's dt")
+    sim.step(nsteps)
+    sim.sync()
+    return None

@@ -1,0 +1,187 @@
+"""Plain single-shot stepper: a thin object over one C-ABI handle (vi_create ... vi_destroy).
+
+Arrays cross the boundary as numpy arrays of shape (nz, nx) -- x fastest, which is the
+memory order of the reference's column-major (Nx, Nz) Julia arrays -- in the handle's
+precision. Every call goes to the HIP library; there is no CPU execution path.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+FIELDS = {"vx": L.VI_VX, "vz": L.VI_VZ, "sxx": L.VI_SXX, "szz": L.VI_SZZ, "sxz": L.VI_SXZ,
+          "mxx": L.VI_MXX, "mzz": L.VI_MZZ, "mxz": L.VI_MXZ,
+          "psi_vx_x": L.VI_PSI_VX_X, "psi_vz_x": L.VI_PSI_VZ_X, "psi_sxx_x": L.VI_PSI_SXX_X,
+          "psi_sxz_x": L.VI_PSI_SXZ_X, "psi_vx_z": L.VI_PSI_VX_Z, "psi_vz_z": L.VI_PSI_VZ_Z,
+          "psi_sxz_z": L.VI_PSI_SXZ_Z, "psi_szz_z": L.VI_PSI_SZZ_Z}
+SOURCE_KINDS = {"force_z": L.VI_SRC_FORCE_Z, "explosive": L.VI_SRC_EXPLOSIVE,
+                "force_x": L.VI_SRC_FORCE_X}
+COMPONENTS = {"vx": 0, "vz": 1, "sxx": 2, "szz": 3, "sxz": 4}
+
+
+def _as_int32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32).ravel())
+
+
+class Simulation:
+    """One simulation on the GPU.
+
+    ``precision`` is "float32" (the reference's GPU precision, utils.jl:4) or "float64".
+    """
+
+    def __init__(self, nx, nz, dx, dz, dt, precision="float32", nrelax=0):
+        self.nx, self.nz, self.dx, self.dz, self.dt = int(nx), int(nz), float(dx), float(dz), float(dt)
+        self.dtype = np.dtype(np.float32 if precision in ("float32", np.float32, "f32") else np.float64)
+        self.nrelax = int(nrelax)
+        g = L.GridDesc(self.nx, self.nz, self.dx, self.dz, self.dt,
+                       L.VI_F32 if self.dtype == np.float32 else L.VI_F64, self.nrelax, 8, 8)
+        h = ctypes.c_void_p()
+        L.check(L.lib().vi_create(ctypes.byref(g), ctypes.byref(h)))
+        self._h = h
+        self.cpml_thickness = 0
+        self.nrec = 0
+        self.rec_nt = 0
+
+    # -- helpers ---------------------------------------------------------------------
+    def _arr(self, a, shape=None):
+        a = np.ascontiguousarray(np.asarray(a, dtype=self.dtype))
+        if shape is not None and a.shape != shape:
+            if a.size != int(np.prod(shape)):
+                raise ValueError(f"expected shape {shape}, got {a.shape}")
+            a = a.reshape(shape)
+        return a
+
+    def _ptr(self, a):
+        return a.ctypes.data_as(ctypes.c_void_p)
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- configuration ---------------------------------------------------------------
+    def set_model(self, rho, lam, mu, tau=None):
+        shp = (self.nz, self.nx)
+        arrs = [self._arr(np.broadcast_to(x, shp), shp) for x in (rho, lam, mu)]
+        t = None
+        if self.nrelax:
+            if tau is None:
+                raise ValueError("tau is required when nrelax > 0")
+            t = self._arr(np.broadcast_to(tau, shp), shp)
+        L.check(L.lib().vi_set_model(self._h, *(self._ptr(a) for a in arrs),
+                                     self._ptr(t) if t is not None else None))
+
+    def set_relaxation(self, taun):
+        t = np.ascontiguousarray(np.asarray(taun, dtype=np.float64).ravel())
+        if t.size != self.nrelax:
+            raise ValueError(f"expected {self.nrelax} relaxation times")
+        L.check(L.lib().vi_set_relaxation(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def set_cpml(self, thickness=0, d0=0.0, alpha_max=0.0, kappa_max=1.0, power=2.0,
+                 left=True, right=True, top=False, bottom=True):
+        if not thickness:
+            L.check(L.lib().vi_set_cpml(self._h, None))
+            self.cpml_thickness = 0
+            return
+        c = L.CpmlDesc(int(thickness), float(d0), float(alpha_max), float(kappa_max),
+                       float(power), int(left), int(right), int(top), int(bottom))
+        L.check(L.lib().vi_set_cpml(self._h, ctypes.byref(c)))
+        self.cpml_thickness = int(thickness)
+
+    def set_free_surface(self, enabled=True):
+        L.check(L.lib().vi_set_free_surface(self._h, int(bool(enabled))))
+
+    def set_sources(self, ix, iz, kind, wavelets):
+        ix, iz = _as_int32(ix), _as_int32(iz)
+        kinds = _as_int32([SOURCE_KINDS.get(k, k) if isinstance(k, str) else k
+                           for k in np.atleast_1d(kind)])
+        n = ix.size
+        w = self._arr(wavelets).reshape(n, -1) if n else self._arr(np.zeros((0, 0)))
+        IP = ctypes.POINTER(ctypes.c_int)
+        L.check(L.lib().vi_set_sources(self._h, n, ix.ctypes.data_as(IP), iz.ctypes.data_as(IP),
+                                       kinds.ctypes.data_as(IP), w.shape[1] if n else 0,
+                                       self._ptr(w)))
+
+    def set_receivers(self, ix, iz, comp, nt_max):
+        ix, iz = _as_int32(ix), _as_int32(iz)
+        comps = _as_int32([COMPONENTS.get(c, c) if isinstance(c, str) else c
+                           for c in np.broadcast_to(np.asarray(comp, dtype=object), ix.shape)])
+        IP = ctypes.POINTER(ctypes.c_int)
+        L.check(L.lib().vi_set_receivers(self._h, ix.size, ix.ctypes.data_as(IP),
+                                         iz.ctypes.data_as(IP), comps.ctypes.data_as(IP),
+                                         int(nt_max)))
+        self.nrec, self.rec_nt = ix.size, int(nt_max)
+
+    # -- state -------------------------------------------------------------------------
+    def _shape(self, fid):
+        if fid <= L.VI_SXZ:
+            return (self.nz, self.nx)
+        if fid <= L.VI_MXZ:
+            return (self.nrelax, self.nz, self.nx)
+        T2 = 2 * self.cpml_thickness
+        return (self.nz, T2) if fid < L.VI_PSI_VX_Z else (T2, self.nx)
+
+    def get(self, field):
+        fid = FIELDS[field] if isinstance(field, str) else int(field)
+        out = np.empty(self._shape(fid), dtype=self.dtype)
+        L.check(L.lib().vi_get_state(self._h, fid, self._ptr(out)))
+        return out
+
+    def set(self, field, values):
+        fid = FIELDS[field] if isinstance(field, str) else int(field)
+        a = self._arr(values, self._shape(fid))
+        L.check(L.lib().vi_set_state(self._h, fid, self._ptr(a)))
+
+    def reset(self):
+        L.check(L.lib().vi_reset(self._h))
+
+    # -- stepping ----------------------------------------------------------------------
+    def step(self, nsteps=1):
+        L.check(L.lib().vi_step(self._h, int(nsteps)))
+
+    def sync(self):
+        L.check(L.lib().vi_sync(self._h))
+
+    @property
+    def step_index(self):
+        return int(L.lib().vi_step_index(self._h))
+
+    def traces(self, nt=None):
+        nt = min(self.step_index, self.rec_nt) if nt is None else int(nt)
+        out = np.empty((self.nrec, nt), dtype=self.dtype)
+        L.check(L.lib().vi_get_traces(self._h, self._ptr(out), nt))
+        return out
+
+    # -- timing ------------------------------------------------------------------------
+    def profile(self, enabled=True):
+        L.check(L.lib().vi_profile_enable(self._h, int(bool(enabled))))
+
+    def profile_read(self, kernel):
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        L.check(L.lib().vi_profile_read(self._h, int(kernel), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def algorithmic_bytes(self, kernel):
+        b = ctypes.c_double()
+        L.check(L.lib().vi_algorithmic_bytes(self._h, int(kernel), ctypes.byref(b)))
+        return b.value
+
+    # -- lifetime ----------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().vi_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
