@@ -93,6 +93,7 @@ def main():
     cfg = config_layered_visco(nx=args.nx, nz=args.nz, nt=nt_total)
     sim = build_simulation(cfg, "float32")
     cells = cfg["nx"] * cfg["nz"]
+    rows, tune_table = sim.autotune(trial_steps=50)
 
     sim.step(args.warmup)
     sim.sync()
@@ -131,7 +132,8 @@ def main():
                             "traffic": pmc_traffic("k_velocity")},
              "step_kernel_us": round((avg_s + avg_v) * 1e6, 3),
              "value_without_events": round(cells * args.steps / wall_noev / 1e9, 3),
-             "traces_finite": finite}
+             "traces_finite": finite, "rows_per_wave": rows,
+             "autotune_us_per_step": {str(k): round(v, 2) for k, v in tune_table.items()}}
     cpu = None
     if not args.no_cpu:
         cfg_cpu = config_layered_visco(nx=args.nx, nz=args.nz, nt=args.steps)

@@ -68,7 +68,7 @@ typedef struct {
   int nx, nz;
   double dx, dz, dt;
   int precision;       /* VI_F32 or VI_F64 */
-  int nrelax;          /* L standard linear solids, 0 = elastic, at most 8 */
+  int nrelax;          /* L standard linear solids, 0 = elastic, at most 6 */
   int fd_order_x, fd_order_z; /* must be 8 */
 } vi_grid_desc;
 
@@ -123,6 +123,14 @@ int vi_sync(vi_handle h);
 long vi_step_index(vi_handle h);
 /* traces recorded so far: out[nrec][nt] for steps [0, nt) since vi_set_receivers */
 int vi_get_traces(vi_handle h, void* out, int nt);
+
+/* ---- tiling (no counterpart in the reference: KA picks workgroup shapes itself) ------ */
+/* rows per wavefront of the kernel variant: 1, 2, 4 or 8 (results are identical) */
+int vi_set_tiling(vi_handle h, int rows_per_wave);
+int vi_get_tiling(vi_handle h);
+/* time every variant for trial_steps steps on a snapshot of the state, restore the state,
+ * keep the fastest; us_per_step (optional, 4 entries) receives the timings of 1, 2, 4, 8 */
+int vi_autotune(vi_handle h, int trial_steps, int* chosen, double* us_per_step);
 
 /* ---- kernel timing (HIP events around each launch on the handle's stream) ---------- */
 int vi_profile_enable(vi_handle h, int enabled);

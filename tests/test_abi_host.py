@@ -1,0 +1,188 @@
+"""CPU tests of the C ABI library (no compute calls) and of the host-side API logic.
+
+Ported known-answer tests from the reference's own suites:
+  tests/test_source.jl:8-64, 142-166   Ricker values and source windows
+  tests/test_cpml.jl:7-94              CPMLConfig / CPMLCoefficients / FreeSurfaceBC
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "vi_mi355x.h")
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vi_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    from viscoelasticinversion_amd import _lib
+    lib = _lib.lib()
+    names = _declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding covers the whole header
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+def test_fd_coeffs_match_oracle(oracle_mod):
+    from viscoelasticinversion_amd import _lib
+    for h in (1.0, 5.0, 2.5, 0.1):
+        out = (ctypes.c_double * 4)()
+        _lib.lib().vi_fd_coeffs(h, out)
+        assert np.array_equal(np.array(out[:]), oracle_mod.fd_coeffs(h))
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_cpml_profile_matches_oracle_bitwise(oracle_mod, axis):
+    from viscoelasticinversion_amd import _lib
+    n = 120
+    c = _lib.CpmlDesc(20, 1234.5, 31.4, 1.7, 2.0, 1, 0, 1, 1)
+    a, b, k = (np.zeros(n) for _ in range(3))
+    P = ctypes.POINTER(ctypes.c_double)
+    _lib.check(_lib.lib().vi_cpml_profile(n, ctypes.byref(c), 4e-4, axis, a.ctypes.data_as(P),
+                                          b.ctypes.data_as(P), k.ctypes.data_as(P)))
+    lo, hi = (1, 0) if axis == 0 else (1, 1)
+    ref = oracle_mod.cpml_axis(n, 20, 1234.5, 31.4, 1.7, 2.0, 4e-4, lo, hi)
+    assert np.array_equal(a, ref["a"]) and np.array_equal(b, ref["b"])
+    assert np.array_equal(k, ref["kappa"])
+
+
+def test_cpml_auto_d0_matches_config():
+    from viscoelasticinversion_amd import CPMLConfig, _lib
+    lib = _lib.lib()
+    d0 = lib.vi_cpml_auto_d0(10, 3000.0, 10.0, 1e-5, 2.0)
+    cfg = CPMLConfig(thickness=10, vmax=3000.0, dx=10.0, reflection_coef=1e-5)
+    assert d0 == pytest.approx(cfg.d0, rel=1e-15)
+    assert d0 == pytest.approx(-(3.0) * 3000.0 * np.log(1e-5) / (2 * 10 * 10.0))
+
+
+def test_create_without_device_or_with_bad_args_fails_loudly():
+    from viscoelasticinversion_amd import Simulation, _lib
+    lib = _lib.lib()
+    ndev = lib.vi_device_count()
+    h = ctypes.c_void_p()
+    bad = _lib.GridDesc(8, 8, 1.0, 1.0, 1e-3, 0, 0, 8, 8)
+    assert lib.vi_create(ctypes.byref(bad), ctypes.byref(h)) == _lib.VI_EINVAL
+    assert b"nx and nz" in lib.vi_last_error()
+    bad = _lib.GridDesc(32, 32, 1.0, 1.0, 1e-3, 0, 0, 4, 8)
+    assert lib.vi_create(ctypes.byref(bad), ctypes.byref(h)) == _lib.VI_EINVAL
+    bad = _lib.GridDesc(32, 32, 1.0, 1.0, 1e-3, 0, 7, 8, 8)
+    assert lib.vi_create(ctypes.byref(bad), ctypes.byref(h)) == _lib.VI_EINVAL
+    if ndev == 0:
+        good = _lib.GridDesc(32, 32, 1.0, 1.0, 1e-3, 0, 0, 8, 8)
+        assert lib.vi_create(ctypes.byref(good), ctypes.byref(h)) == _lib.VI_EDEVICE
+        with pytest.raises(_lib.ViError):
+            Simulation(32, 32, 1.0, 1.0, 1e-3)
+    assert lib.vi_step(None, 1) == _lib.VI_EINVAL
+
+
+# ---------------------------------------------------------------- reference KATs (source)
+def test_ricker_values():
+    """tests/test_source.jl:8-31."""
+    from viscoelasticinversion_amd import _ricker
+    assert _ricker(25.0, 0.0) == pytest.approx(1.0)
+    assert _ricker(25.0, 0.01) == pytest.approx(_ricker(25.0, -0.01))
+    assert abs(_ricker(25.0, 0.1)) < 0.01
+    assert _ricker(50.0, 0.0) == pytest.approx(1.0)
+
+
+def test_ricker_source_windows():
+    """tests/test_source.jl:33-64, 142-166 and test_viscoelastic_problem.jl:106-122."""
+    from viscoelasticinversion_amd import RickerSource
+    s = RickerSource(25.0, 0.1, 10, 15)
+    assert s.grid == (10, 15) and s.coefs == 1.0
+    assert s.tspan[0] < 0.1 < s.tspan[1]
+    assert abs((s.tspan[1] - s.tspan[0]) - 2.0 / 25.0) < 0.1
+    assert RickerSource(1.0, 0.1, 4, 6).tspan[1] - RickerSource(1.0, 0.1, 4, 6).tspan[0] > 2.0
+    hi = RickerSource(100.0, 0.1, 4, 6)
+    assert hi.tspan[1] - hi.tspan[0] < 0.1
+    assert s.value(0.1 + 0.1) == 0.0  # closed after tc + 0.1 at 25 Hz
+    assert s.value(s.tspan[0] - 0.01) == 0.0 and s.value(s.tspan[1] + 0.01) == 0.0
+    assert s.value(0.1) == pytest.approx(1.0)
+    assert s.value(0.11) == pytest.approx(float(np.squeeze((1 - 2 * (np.pi * 25 * 0.01) ** 2)
+                                                           * np.exp(-(np.pi * 25 * 0.01) ** 2))))
+    du = np.zeros((15 + 1, 10 + 1), np.float32)
+    s(du, 0.1)
+    s(du, 0.1)
+    assert np.count_nonzero(du) == 1 and du[15, 10] == pytest.approx(2.0)
+
+
+def test_source_samples_times():
+    from viscoelasticinversion_amd import RickerSource
+    f = RickerSource(10.0, 0.1, 5, 5, kind="force_z")
+    e = RickerSource(10.0, 0.1, 5, 5, kind="explosive")
+    dt = 1e-3
+    assert f.samples(300, dt)[100] == pytest.approx(1.0)  # t = 100 dt = tc
+    np.testing.assert_allclose(e.samples(300, dt), e.value((np.arange(300) + 0.5) * dt))
+
+
+# ---------------------------------------------------------------- reference KATs (CPML)
+def test_cpml_config():
+    """tests/test_cpml.jl:7-26."""
+    from viscoelasticinversion_amd import CPMLConfig
+    c = CPMLConfig(thickness=10, d0=100.0)
+    assert (c.thickness, c.d0, c.left, c.right, c.top, c.bottom) == (10, 100.0, True, True, False, True)
+    c = CPMLConfig(thickness=10, vmax=3000.0, dx=10.0, reflection_coef=1e-5)
+    assert c.d0 > 0 and c.thickness == 10
+    with pytest.raises(ValueError, match="Must provide either d0"):
+        CPMLConfig(thickness=10)
+
+
+def test_free_surface_bc():
+    """tests/test_cpml.jl:71-77."""
+    from viscoelasticinversion_amd import FreeSurfaceBC
+    assert FreeSurfaceBC().depth == 1
+    assert FreeSurfaceBC(2).depth == 2
+
+
+def test_parameters_defaults_and_errors():
+    """Parameters.jl:47-86 keyword defaults; unsupported options raise."""
+    from viscoelasticinversion_amd import CPMLConfig, Parameters
+    p = Parameters()
+    assert (p.Nx, p.Nz, p.dx, p.dz, p.dt, p.fd_order_x) == (64, 64, 10.0, 10.0, 0.001, 8)
+    assert p.source.freq == 40.0 and p.source.tc == 0.2 and p.source.grid == (32, 32)
+    assert np.all(p.model.b == 1 / 2000.0) and np.all(p.model.lam == 1e9)
+    p = Parameters(Nx=32, Nz=32, dt=0.001, cpml=CPMLConfig(thickness=5, d0=100.0),
+                   free_surface=True)
+    assert p.cpml is not None and p.free_surface_bc.depth == 1
+    with pytest.raises(NotImplementedError):
+        Parameters(bc="periodic")
+    with pytest.raises(ValueError):
+        Parameters(bc="bogus")
+    with pytest.raises(NotImplementedError):
+        Parameters(fd_order_x=4)
+    with pytest.raises(ValueError):
+        Parameters(device="cpu")
+
+
+def test_make_problem_shapes():
+    """test_viscoelastic_problem.jl:44-61: zero initial state, source kept on the problem."""
+    from viscoelasticinversion_amd import Parameters, RickerSource, make_problem
+    src = RickerSource(25.0, 0.1, 16, 16)
+    p = Parameters(Nx=32, Nz=32, source=src)
+    prob = make_problem(p, tspan=(0.0, 0.5))
+    assert prob.u0[0].shape == (3, 32, 32) and prob.u0[1].shape == (2, 32, 32)
+    assert np.all(prob.u0[0] == 0) and np.all(prob.u0[1] == 0)
+    assert prob.p.source is src and prob.tspan == (0.0, 0.5)
+
+
+def test_qmodel_fixture_band():
+    """Models.jl:65-107: tau = N/(Kbar Q0) and Q(w) = N/(tau K(w)); with the stored 2-40 Hz,
+    L=3 relaxation times Q stays within 8 % of Q0 = 50 over the band."""
+    from viscoelasticinversion_amd import QModel
+    from viscoelasticinversion_amd.configs import TAUN_2_40HZ_L3
+    q = QModel(2.0, 40.0, 8, 8, Q0=50, N=3, taun=TAUN_2_40HZ_L3)
+    w = 2 * np.pi * np.linspace(2.0, 40.0, 100)
+    Q = q.Q(w)
+    assert np.all(np.abs(Q / 50 - 1) < 0.08)
+    assert np.mean(1.0 / Q) == pytest.approx(1.0 / 50, rel=0.01)
+    q.update(np.full((8, 8), 100.0))
+    assert np.allclose(q.tau, 3 / (q.K * 100.0))

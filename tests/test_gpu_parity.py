@@ -128,3 +128,33 @@ def test_reference_injection_magnitude():
     vz[16, 16] = 0
     assert np.all(vz == 0) and np.all(sim.get("vx") == 0)
     sim.close()
+
+
+@pytest.mark.parametrize("rows", [1, 2, 4, 8])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_every_tiling_variant(rows, dtype):
+    """All rows-per-wave kernel variants give the oracle's result."""
+    cfg = small_cfg(nx=150, nz=90, nrelax=3, nt=30)
+    prec = "float64" if dtype == np.float64 else "float32"
+    print(f"[case] tiling rows={rows} {prec}", flush=True)
+    ref = run_oracle(cfg, dtype)
+    gpu = run_gpu(cfg, prec, tiling=rows)
+    compare(gpu, ref, dtype, f"tiling {rows}")
+
+
+def test_autotune_leaves_state_unchanged():
+    from viscoelasticinversion_amd.configs import build_simulation
+    cfg = small_cfg(nx=300, nz=200, nrelax=3, nt=40)
+    print("[case] autotune", flush=True)
+    ref = run_oracle(cfg, np.float32, nt=40)
+    sim = build_simulation(cfg, "float32")
+    sim.step(15)
+    chosen, table = sim.autotune(trial_steps=5)
+    assert chosen in (1, 2, 4, 8) and sim.tiling == chosen
+    assert all(v > 0 for v in table.values())
+    assert sim.step_index == 15
+    sim.step(25)
+    out = {k: sim.get(k) for k in ("vx", "vz", "sxx", "szz", "sxz", "mxx", "mzz", "mxz")}
+    out["traces"] = sim.traces(40)
+    sim.close()
+    compare(out, {k: ref[k] for k in out}, np.float32, "autotune")

@@ -1,0 +1,309 @@
+"""Pins of the CPU oracle against what the mathematics and physics fix.
+
+Each check is chosen so that a plausible mistake in the oracle (a wrong operator offset,
+sign, dropped term, swapped memory term, wrong a/b formula) fails it. None of them
+retypes the oracle's formulas: they use exact rational algebra, polynomial exactness,
+conservation laws, solved recursions, and wave speeds from closed-form physics.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from helpers import rel_l2, small_cfg
+
+F64 = np.float64
+
+
+def _vandermonde_weights(x, x0, m):
+    """Exact rational solve of the system of _stencil (src/common/Stencil.jl:9-14):
+    sum_k w_k (x_k - x0)^l / l! = [l == m], l = 0..n-1."""
+    n = len(x)
+    A = [[Fraction(xk - x0) ** l / Fraction(math.factorial(l)) for xk in x] for l in range(n)]
+    b = [Fraction(int(l == m)) for l in range(n)]
+    # Gauss-Jordan on rationals
+    M = [row[:] + [b[r]] for r, row in enumerate(A)]
+    for c in range(n):
+        p = next(r for r in range(c, n) if M[r][c] != 0)
+        M[c], M[p] = M[p], M[c]
+        piv = M[c][c]
+        M[c] = [v / piv for v in M[c]]
+        for r in range(n):
+            if r != c and M[r][c] != 0:
+                f = M[r][c]
+                M[r] = [a - f * bb for a, bb in zip(M[r], M[c])]
+    return [M[r][n] for r in range(n)]
+
+
+def test_fd_weights_are_the_exact_staggered_8th_order_solution(oracle_mod):
+    x = [Fraction(2 * k - 1, 2) for k in range(-3, 5)]  # -7/2 .. 7/2
+    w = _vandermonde_weights(x, Fraction(0), 1)
+    # antisymmetric: weight at +(2l-1)/2 is D[l], at -(2l-1)/2 is -D[l]
+    half = {x[k]: w[k] for k in range(8)}
+    D = [half[Fraction(2 * l - 1, 2)] for l in range(1, 5)]
+    for l in range(1, 5):
+        assert half[-Fraction(2 * l - 1, 2)] == -D[l - 1]
+    assert D == [Fraction(1225, 1024), Fraction(-245, 3072), Fraction(49, 5120), Fraction(-5, 7168)]
+    for h in (1.0, 5.0, 0.37):
+        got = oracle_mod.fd_coeffs(h)
+        np.testing.assert_allclose(got, [float(d) / h for d in D], rtol=2e-16, atol=0)
+
+
+def _uniform_model(nx, nz, rho=2000.0, vp=2000.0, vs=1150.0):
+    mu = rho * vs ** 2
+    lam = rho * vp ** 2 - 2 * mu
+    return dict(rho=np.full((nz, nx), rho), lam=np.full((nz, nx), lam), mu=np.full((nz, nx), mu))
+
+
+@pytest.mark.parametrize("k", [1, 3, 5, 7])
+def test_operators_exact_on_polynomials(oracle_mod, k):
+    """One velocity half step from sigma = polynomial: vx = dt*b*d(sxx)/dx at x_{i+1/2}, and
+    vz = dt*b*d(szz)/dz at z_{j+1/2}, exactly for degree <= 7 (8th-order staggered)."""
+    nx, nz, h, dt = 40, 36, 1.0, 1e-3
+    m = _uniform_model(nx, nz)
+    o = oracle_mod.Oracle(nx, nz, h, h, dt, m, dtype=F64)
+    x = np.arange(nx) * h / 10.0
+    z = np.arange(nz) * h / 10.0
+    X, Z = np.meshgrid(x, z)
+    o.set(oracle_mod.SXX, X ** k)
+    o.set(oracle_mod.SZZ, Z ** k)
+    o.run(1)
+    b = 1.0 / 2000.0
+    vx = o.get(oracle_mod.VX)[4:-4, 4:-4]
+    vz = o.get(oracle_mod.VZ)[4:-4, 4:-4]
+    exp_vx = dt * b * k * ((X + 0.05) ** (k - 1)) / 10.0
+    exp_vz = dt * b * k * ((Z + 0.05) ** (k - 1)) / 10.0
+    assert rel_l2(vx, exp_vx[4:-4, 4:-4]) < 1e-12
+    assert rel_l2(vz, exp_vz[4:-4, 4:-4]) < 1e-12
+
+
+def test_operators_8th_order_convergence(oracle_mod):
+    """Error of the staggered derivative of sin(2 pi x / lam) falls by ~2^8 per halving of h."""
+    errs = []
+    for n in (24, 48):
+        nx, nz, dt = n + 8, 16, 1e-3
+        h = 1.0 / n
+        m = _uniform_model(nx, nz)
+        o = oracle_mod.Oracle(nx, nz, h, h, dt, m, dtype=F64)
+        x = np.arange(nx) * h
+        o.set(oracle_mod.SXX, np.tile(np.sin(2 * np.pi * x), (nz, 1)))
+        o.run(1)
+        vx = o.get(oracle_mod.VX)[8, 4:-4] / (dt / 2000.0)
+        exact = 2 * np.pi * np.cos(2 * np.pi * (x[4:-4] + 0.5 * h))
+        errs.append(np.abs(vx - exact).max())
+    ratio = errs[0] / errs[1]
+    assert 200 < ratio < 320, ratio
+
+
+def _energy(o, mod, v_prev, v_now, model):
+    """Discrete energy of the staggered leapfrog: 1/2 sum m v^{n-1/2} v^{n+1/2} + 1/2 s C^-1 s,
+    with m = 1/b (the averaged buoyancies) and C the cell-local stiffness."""
+    rho, lam, mu = model["rho"], model["lam"], model["mu"]
+    mx = np.zeros_like(rho)
+    mz = np.zeros_like(rho)
+    mx[:, :-1] = 0.5 * (rho[:, :-1] + rho[:, 1:])
+    mz[:-1, :] = 0.5 * (rho[:-1, :] + rho[1:, :])
+    sl = (slice(4, -4), slice(4, -4))
+    kin = 0.5 * np.sum((mx * v_prev[0] * v_now[0])[sl] + (mz * v_prev[1] * v_now[1])[sl])
+    sxx, szz, sxz = (o.get(mod.SXX), o.get(mod.SZZ), o.get(mod.SXZ))
+    pi = lam + 2 * mu
+    det = pi ** 2 - lam ** 2
+    muxz = np.zeros_like(mu)
+    muxz[:-1, :-1] = 0.25 * (mu[:-1, :-1] + mu[:-1, 1:] + mu[1:, :-1] + mu[1:, 1:])
+    pot = 0.5 * np.sum(((pi * sxx ** 2 - 2 * lam * sxx * szz + pi * szz ** 2) / det)[sl]
+                       + (sxz[sl] ** 2 / muxz[sl]))
+    return kin + pot
+
+
+def test_elastic_energy_conserved(oracle_mod):
+    """Heterogeneous elastic model, Dirichlet ring, random interior state: the leapfrog's
+    discrete energy is constant (the D+ / D- operators are exact negative transposes)."""
+    nx, nz, h, dt = 48, 40, 5.0, 4e-4
+    rng = np.random.default_rng(3)
+    cfg = small_cfg(nx=nx, nz=nz, nrelax=0, cpml=False, fs=False)
+    model = cfg["model"]
+    o = oracle_mod.Oracle(nx, nz, h, h, dt, model, dtype=F64)
+    ring = np.zeros((nz, nx), bool)
+    ring[4:-4, 4:-4] = True
+    for f in range(5):
+        a = rng.standard_normal((nz, nx)) * (1e3 if f >= 2 else 1e-3)
+        o.set(f, np.where(ring, a, 0.0))
+    v_prev = (o.get(0), o.get(1))
+    es = []
+    for k in range(300):
+        # energy pairs v^{k-1/2} (before the step) with s^k and v^{k+1/2} (after the velocity
+        # half): evaluate with s taken before the step, so run the velocity half through a
+        # full step and read s^k first
+        s_before = [o.get(f) for f in (2, 3, 4)]
+        o.run(1)
+        v_now = (o.get(0), o.get(1))
+        s_after = [o.get(f) for f in (2, 3, 4)]
+        for f, arr in zip((2, 3, 4), s_before):
+            o.set(f, arr)
+        es.append(_energy(o, oracle_mod, v_prev, v_now, model))
+        for f, arr in zip((2, 3, 4), s_after):
+            o.set(f, arr)
+        v_prev = v_now
+    es = np.array(es)
+    assert np.abs(es - es[0]).max() / abs(es[0]) < 1e-11
+
+
+def test_sls_constant_strain_rate_closed_form(oracle_mod):
+    """With buoyancy ~0 the velocities stay vx = a x (and a z for the shear test), so every
+    cell integrates the SLS law under a constant strain rate. The solved forward-Euler
+    recursion gives sxx_k = pi a t_k + pi tau (a/L) sum_n taun (1 - (1 - dt/taun)^k) and
+    the same with lam for szz and mu for sxz (full engineering shear rate)."""
+    nx = nz = 24
+    h, dt, L = 1.0, 1e-4, 3
+    taun = np.array([1e-3, 5e-3, 2e-2])
+    rho, lam, mu, tau = 1e30, 3e9, 2e9, 0.07
+    m = dict(rho=np.full((nz, nx), rho), lam=np.full((nz, nx), lam), mu=np.full((nz, nx), mu),
+             tau=np.full((nz, nx), tau))
+    a = 2.0
+    x = (np.arange(nx) + 0.5) * h
+    z = np.arange(nz) * h
+    X, Z = np.meshgrid(x, z)
+    for test in ("normal", "shear"):
+        o = oracle_mod.Oracle(nx, nz, h, h, dt, m, dtype=F64, nrelax=L, taun=taun)
+        o.set(oracle_mod.VX, a * X if test == "normal" else a * Z)
+        nsteps = 400
+        o.run(nsteps)
+        t = nsteps * dt
+        relax = np.sum(taun * (1.0 - (1.0 - dt / taun) ** nsteps))
+        c = (12, 12)
+        if test == "normal":
+            pi = lam + 2 * mu
+            assert o.get(oracle_mod.SXX)[c] == pytest.approx(pi * a * t + pi * tau * a / L * relax, rel=1e-10)
+            assert o.get(oracle_mod.SZZ)[c] == pytest.approx(lam * a * t + lam * tau * a / L * relax, rel=1e-10)
+            assert abs(o.get(oracle_mod.SXZ)[c]) < 1e-6 * pi * a * t
+        else:
+            assert o.get(oracle_mod.SXZ)[c] == pytest.approx(mu * a * t + mu * tau * a / L * relax, rel=1e-10)
+            assert abs(o.get(oracle_mod.SXX)[c]) < 1e-6 * mu * a * t
+
+
+def _ricker(f, t):
+    a = (np.pi * f * t) ** 2
+    return (1 - 2 * a) * np.exp(-a)
+
+
+def test_cpml_absorbs_outgoing_waves(oracle_mod):
+    """A pulse in a box with CPML on all four sides leaves: the energy left after the waves
+    have crossed the boundary is < 1e-3 of the peak; with a rigid box it stays."""
+    n, h, dt, f = 160, 5.0, 5e-4, 15.0
+    m = _uniform_model(n, n)
+    nt = 900
+    w = _ricker(f, (np.arange(nt) + 0.5) * dt - 1 / f)[None, :]
+    energies = {}
+    for T in (0, 20):
+        cp = None
+        if T:
+            d0 = -(2 + 1) * 2000.0 * np.log(1e-5) / (2 * T * h)
+            cp = dict(thickness=T, d0=d0, alpha_max=np.pi * f, kappa_max=1.0, power=2.0,
+                      left=True, right=True, top=True, bottom=True)
+        o = oracle_mod.Oracle(n, n, h, h, dt, m, dtype=F64, cpml=cp)
+        o.set_sources([n // 2], [n // 2], [1], w)
+        e = []
+        for k in range(nt // 50):
+            o.run(50)
+            e.append(sum(np.sum(o.get(f_) ** 2) * (2000.0 if f_ < 2 else 1.0 / (2000.0 * 2000 ** 2))
+                         for f_ in range(5)))
+        energies[T] = np.array(e)
+    rigid, pml = energies[0], energies[20]
+    assert pml[-1] < 1e-3 * pml.max()
+    assert rigid[-1] > 100 * pml[-1]
+
+
+def test_cpml_coefficients_match_cfs_stretching(oracle_mod):
+    """psi_inf = a/(1-b) for a constant derivative makes d/kappa + psi equal 1/s(0) with the
+    CFS stretching s(w) = kappa + d/(alpha + i w)   (Komatitsch & Martin 2007)."""
+    c = oracle_mod.cpml_axis(100, 20, 500.0, 30.0, 2.0, 2.0, 4e-4, 1, 1)
+    strip = c["d"] > 0
+    a, b, k, d, al = (c[q][strip] for q in ("a", "b", "kappa", "d", "alpha"))
+    psi_inf = a / (1 - b)
+    np.testing.assert_allclose(1.0 / k + psi_inf, 1.0 / (k + d / al), rtol=1e-12)
+    # test_cpml.jl:28-47: sizes, zero damping in the interior, positive in both strips
+    assert len(c["d"]) == 100
+    assert np.all(c["d"][20:80] == 0) and np.all(c["d"][:20] > 0) and np.all(c["d"][80:] > 0)
+    assert np.all(c["kappa"][20:80] == 1) and np.all(c["b"][20:80] == 0)
+    # profiles grow towards the outer edge
+    assert np.all(np.diff(c["d"][:20]) < 0) and np.all(np.diff(c["d"][80:]) > 0)
+
+
+def _rayleigh_speed(vp, vs):
+    """Root of the Rayleigh equation (2 - c^2/vs^2)^2 = 4 sqrt(1-c^2/vp^2) sqrt(1-c^2/vs^2)."""
+    from scipy.optimize import brentq
+    f = lambda c: (2 - c ** 2 / vs ** 2) ** 2 - 4 * np.sqrt(1 - c ** 2 / vp ** 2) * np.sqrt(1 - c ** 2 / vs ** 2)
+    return brentq(f, 0.5 * vs, 0.9999 * vs)
+
+
+def test_free_surface_rayleigh_wave_speed(oracle_mod):
+    """A vertical force just below the free surface launches a Rayleigh wave; its speed along
+    the surface (from the lag between two surface receivers) matches the Rayleigh equation."""
+    nx, nz, h, dt, f = 700, 110, 2.0, 2.5e-4, 25.0
+    vp, vs = 2000.0, 2000.0 / np.sqrt(3.0)
+    m = _uniform_model(nx, nz, vp=vp, vs=vs)
+    nt = 3000
+    w = _ricker(f, np.arange(nt) * dt - 1.2 / f)[None, :]
+    T = 30
+    d0 = -(3) * vp * np.log(1e-5) / (2 * T * h)
+    cp = dict(thickness=T, d0=d0, alpha_max=np.pi * f, kappa_max=1.0, power=2.0, left=True,
+              right=True, top=False, bottom=True)
+    o = oracle_mod.Oracle(nx, nz, h, h, dt, m, dtype=F64, free_surface=True, cpml=cp)
+    o.set_sources([60], [5], [0], w)
+    r1, r2 = 160, 400
+    o.set_receivers([r1, r2], [4, 4], [1, 1])
+    tr = o.run(nt)
+    a, b = tr[:, 0], tr[:, 1]
+    # lag by cross-correlation with parabolic refinement
+    cc = np.correlate(b, a, mode="full")
+    k = int(np.argmax(cc))
+    y0, y1, y2 = cc[k - 1], cc[k], cc[k + 1]
+    lag = (k - (len(a) - 1)) + 0.5 * (y0 - y2) / (y0 - 2 * y1 + y2)
+    c_meas = (r2 - r1) * h / (lag * dt)
+    c_r = _rayleigh_speed(vp, vs)
+    assert abs(c_meas / c_r - 1) < 0.02, (c_meas, c_r)
+    # the traction condition holds exactly on the surface row
+    assert np.all(o.get(oracle_mod.SZZ)[4, 4:-4] == 0)
+
+
+def test_p_wave_speed(oracle_mod):
+    """An explosive source radiates a P wave whose arrival moves out at vp."""
+    n, h, dt, f = 260, 5.0, 5e-4, 15.0
+    vp = 2000.0
+    m = _uniform_model(n, n, vp=vp)
+    nt = 700
+    w = _ricker(f, (np.arange(nt) + 0.5) * dt - 1 / f)[None, :]
+    o = oracle_mod.Oracle(n, n, h, h, dt, m, dtype=F64)
+    o.set_sources([130], [130], [1], w)
+    o.set_receivers([150, 230], [130, 130], [2, 2])
+    tr = o.run(nt)
+    a, b = tr[:, 0], tr[:, 1]
+    cc = np.correlate(b, a, mode="full")
+    k = int(np.argmax(cc))
+    y0, y1, y2 = cc[k - 1], cc[k], cc[k + 1]
+    lag = (k - (len(a) - 1)) + 0.5 * (y0 - y2) / (y0 - 2 * y1 + y2)
+    c = 80 * h / (lag * dt)
+    assert abs(c / vp - 1) < 0.02, c
+
+
+@pytest.mark.parametrize("nrelax", [0, 3])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_fused_and_multipass_paths_agree_bitwise(oracle_mod, nrelax, dtype):
+    """The fused restatement and the reference-structured multi-pass one are independent
+    code paths with the same operation order: their results are identical."""
+    import helpers
+    cfg = small_cfg(nrelax=nrelax, nt=50)
+    a = helpers.run_oracle(cfg, dtype, mode=0)
+    b = helpers.run_oracle(cfg, dtype, mode=1)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_fp32_oracle_close_to_fp64(oracle_mod):
+    import helpers
+    cfg = small_cfg(nrelax=3, nt=60)
+    a = helpers.run_oracle(cfg, np.float32)
+    b = helpers.run_oracle(cfg, np.float64)
+    for k in ("vx", "vz", "sxx", "szz", "sxz", "traces"):
+        assert rel_l2(a[k], b[k]) < 1e-4, k

@@ -69,6 +69,9 @@ SIGNATURES = {
     "vi_sync": (_I, [_H]),
     "vi_step_index": (_L, [_H]),
     "vi_get_traces": (_I, [_H, _V, _I]),
+    "vi_set_tiling": (_I, [_H, _I]),
+    "vi_get_tiling": (_I, [_H]),
+    "vi_autotune": (_I, [_H, _I, _IP, _DP]),
     "vi_profile_enable": (_I, [_H, _I]),
     "vi_profile_read": (_I, [_H, _I, _DP, ctypes.POINTER(_L)]),
     "vi_algorithmic_bytes": (_I, [_H, _I, _DP]),

@@ -90,6 +90,7 @@ struct vi_sim_s {
   int ldx = 0, nzp = 0;
   size_t plane = 0;
   int ntx = 0, ntz = 0, ntiles = 0;
+  int rz = 8;  // rows per wavefront of the current kernel variant
   int device = 0;
   hipStream_t stream = nullptr;
   size_t esz = 4;
@@ -107,6 +108,7 @@ struct vi_sim_s {
   int fs = 0;
   int nsrc = 0, snt = 0;
   int nrec = 0, rec_nt = 0;
+  std::vector<int> src_ix, src_iz, src_kind, rec_ix, rec_iz, rec_comp;  // host copies
   long step = 0;
   // profiling
   bool prof = false;
@@ -203,7 +205,7 @@ vi::StepParams<R> make_params(vi_sim_s* h) {
 }
 
 int tile_of(const vi_sim_s* h, int ix, int iz) {
-  return ((iz - 4) / vi::TZ) * h->ntx + (ix - 4) / vi::TX;
+  return ((iz - 4) / (vi::NW * h->rz)) * h->ntx + (ix - 4) / vi::TX;
 }
 
 // Build a per-tile CSR table [off(ntiles+1) | cell(n) | a(n) | b(n)] and upload it.
@@ -238,11 +240,31 @@ bool interior(const vi_sim_s* h, int ix, int iz) {
   return ix >= 4 && ix < h->g.nx - 4 && iz >= 4 && iz < h->g.nz - 4;
 }
 
+template <typename R, int RZ>
+int launch_kernel(vi_sim_s* h, int k, const vi::StepParams<R>& p) {
+  const dim3 block(vi::TX, vi::NW);
+  const dim3 grid(h->ntiles);
+  if (k == 0) {
+    hipLaunchKernelGGL((vi::k_velocity<R, RZ>), grid, block, 0, h->stream, p);
+  } else {
+    switch (h->L) {
+      case 0: hipLaunchKernelGGL((vi::k_stress<R, 0, RZ>), grid, block, 0, h->stream, p); break;
+      case 1: hipLaunchKernelGGL((vi::k_stress<R, 1, RZ>), grid, block, 0, h->stream, p); break;
+      case 2: hipLaunchKernelGGL((vi::k_stress<R, 2, RZ>), grid, block, 0, h->stream, p); break;
+      case 3: hipLaunchKernelGGL((vi::k_stress<R, 3, RZ>), grid, block, 0, h->stream, p); break;
+      case 4: hipLaunchKernelGGL((vi::k_stress<R, 4, RZ>), grid, block, 0, h->stream, p); break;
+      case 5: hipLaunchKernelGGL((vi::k_stress<R, 5, RZ>), grid, block, 0, h->stream, p); break;
+      case 6: hipLaunchKernelGGL((vi::k_stress<R, 6, RZ>), grid, block, 0, h->stream, p); break;
+      default: return fail(VI_EINVAL, "nrelax out of range");
+    }
+  }
+  VI_HIP(hipGetLastError());
+  return 0;
+}
+
 template <typename R>
 int launch_step(vi_sim_s* h) {
   vi::StepParams<R> p = make_params<R>(h);
-  const dim3 block(vi::TX, vi::NW);
-  const dim3 grid(h->ntiles);
   for (int k = 0; k < 2; ++k) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (h->prof) {
@@ -260,24 +282,43 @@ int launch_step(vi_sim_s* h) {
       h->ev_used += 2;
       VI_HIP(hipEventRecord(e0, h->stream));
     }
-    if (k == 0) {
-      hipLaunchKernelGGL(vi::k_velocity<R>, grid, block, 0, h->stream, p);
-    } else {
-      switch (h->L) {
-        case 0: hipLaunchKernelGGL((vi::k_stress<R, 0>), grid, block, 0, h->stream, p); break;
-        case 1: hipLaunchKernelGGL((vi::k_stress<R, 1>), grid, block, 0, h->stream, p); break;
-        case 2: hipLaunchKernelGGL((vi::k_stress<R, 2>), grid, block, 0, h->stream, p); break;
-        case 3: hipLaunchKernelGGL((vi::k_stress<R, 3>), grid, block, 0, h->stream, p); break;
-        case 4: hipLaunchKernelGGL((vi::k_stress<R, 4>), grid, block, 0, h->stream, p); break;
-        case 5: hipLaunchKernelGGL((vi::k_stress<R, 5>), grid, block, 0, h->stream, p); break;
-        case 6: hipLaunchKernelGGL((vi::k_stress<R, 6>), grid, block, 0, h->stream, p); break;
-        default: return fail(VI_EINVAL, "nrelax out of range");
-      }
+    int rc = 0;
+    switch (h->rz) {
+      case 1: rc = launch_kernel<R, 1>(h, k, p); break;
+      case 2: rc = launch_kernel<R, 2>(h, k, p); break;
+      case 4: rc = launch_kernel<R, 4>(h, k, p); break;
+      case 8: rc = launch_kernel<R, 8>(h, k, p); break;
+      default: return fail(VI_EINVAL, "bad tiling");
     }
-    VI_HIP(hipGetLastError());
+    if (rc) return rc;
     if (h->prof) VI_HIP(hipEventRecord(e1, h->stream));
   }
   return 0;
+}
+
+// Re-tile the interior for rows-per-wave rz and rebuild the per-tile tables.
+int apply_tiling(vi_sim_s* h, int rz) {
+  h->rz = rz;
+  h->ntx = (h->g.nx - 8 + vi::TX - 1) / vi::TX;
+  h->ntz = (h->g.nz - 8 + vi::NW * rz - 1) / (vi::NW * rz);
+  h->ntiles = h->ntx * h->ntz;
+  if (int rc = upload_table(h, h->src_tab, h->nsrc, h->src_ix.data(), h->src_iz.data(),
+                            h->src_kind.data(), nullptr))
+    return rc;
+  if (int rc = upload_table(h, h->rec_tab, h->nrec, h->rec_ix.data(), h->rec_iz.data(),
+                            h->rec_comp.data(), nullptr))
+    return rc;
+  return 0;
+}
+
+// Default rows per wave: the largest variant that still gives about 16 waves per CU.
+int default_rz(const vi_grid_desc& g) {
+  const long cols = (g.nx - 8 + vi::TX - 1) / vi::TX;
+  for (int rz : {8, 4, 2}) {
+    const long waves = cols * ((g.nz - 8 + rz - 1) / rz);
+    if (waves >= 256L * 16) return rz;
+  }
+  return 1;
 }
 
 int collect_profile(vi_sim_s* h) {
@@ -364,10 +405,7 @@ int vi_create(const vi_grid_desc* g, vi_handle* out) {
   VI_HIP(hipGetDevice(&h->device));
   VI_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   h->ldx = (int)round_up((size_t)g->nx + 72, 64);
-  h->ntx = (g->nx - 8 + vi::TX - 1) / vi::TX;
-  h->ntz = (g->nz - 8 + vi::TZ - 1) / vi::TZ;
-  h->ntiles = h->ntx * h->ntz;
-  h->nzp = g->nz + vi::TZ + 16;
+  h->nzp = g->nz + vi::NW * vi::RZ_MAX + 16;
   h->plane = (size_t)h->ldx * h->nzp;
   const size_t pb = h->plane * h->esz;
   for (auto& b : h->state)
@@ -376,11 +414,8 @@ int vi_create(const vi_grid_desc* g, vi_handle* out) {
     if (int rc = h->M.alloc(3 * (size_t)h->L * pb, h->stream)) return rc;
   for (int q = 0; q < 4; ++q)
     if (int rc = h->model[q].alloc(pb, h->stream)) return rc;
-  // empty source / receiver tables (all offsets zero)
-  if (int rc = upload_table(h.get(), h->src_tab, 0, nullptr, nullptr, nullptr, nullptr))
-    return rc;
-  if (int rc = upload_table(h.get(), h->rec_tab, 0, nullptr, nullptr, nullptr, nullptr))
-    return rc;
+  // tiling and the (empty) source / receiver tables
+  if (int rc = apply_tiling(h.get(), default_rz(*g))) return rc;
   VI_HIP(hipStreamSynchronize(h->stream));
   *out = h.release();
   return 0;
@@ -486,6 +521,10 @@ int vi_set_sources(vi_handle h, int n, const int* ix, const int* iz, const int* 
                                  " is outside the interior [4, nx-4) x [4, nz-4)");
     if (kind[q] < 0 || kind[q] > 2) return fail(VI_EINVAL, "vi_set_sources: bad kind");
   }
+  h->src_ix.assign(ix, ix + n);
+  h->src_iz.assign(iz, iz + n);
+  h->src_kind.assign(kind, kind + n);
+  h->nsrc = n;
   if (int rc = upload_table(h, h->src_tab, n, ix, iz, kind, nullptr)) return rc;
   if (int rc = h->src_w.alloc((size_t)n * nt * h->esz, h->stream)) return rc;
   if (n && nt)
@@ -508,6 +547,10 @@ int vi_set_receivers(vi_handle h, int n, const int* ix, const int* iz, const int
                                  " is outside the interior [4, nx-4) x [4, nz-4)");
     if (comp[q] < 0 || comp[q] > 4) return fail(VI_EINVAL, "vi_set_receivers: bad component");
   }
+  h->rec_ix.assign(ix, ix + n);
+  h->rec_iz.assign(iz, iz + n);
+  h->rec_comp.assign(comp, comp + n);
+  h->nrec = n;
   if (int rc = upload_table(h, h->rec_tab, n, ix, iz, comp, nullptr)) return rc;
   if (int rc = h->traces.alloc((size_t)n * nt_max * h->esz, h->stream)) return rc;
   VI_HIP(hipStreamSynchronize(h->stream));
@@ -662,4 +705,89 @@ int vi_algorithmic_bytes(vi_handle h, int kernel, double* bytes) {
   return 0;
 }
 
+int vi_set_tiling(vi_handle h, int rz) {
+  if (int rc = check_handle(h)) return rc;
+  if (rz != 1 && rz != 2 && rz != 4 && rz != 8)
+    return fail(VI_EINVAL, "vi_set_tiling: rows per wavefront must be 1, 2, 4 or 8");
+  VI_HIP(hipStreamSynchronize(h->stream));
+  return apply_tiling(h, rz);
+}
+
+int vi_get_tiling(vi_handle h) { return h ? h->rz : -1; }
+
+int vi_autotune(vi_handle h, int trial_steps, int* chosen, double* us_per_step) {
+  if (int rc = check_handle(h)) return rc;
+  if (trial_steps < 1) return fail(VI_EINVAL, "vi_autotune: trial_steps must be >= 1");
+  if (!h->model_set || (h->L && !h->relax_set))
+    return fail(VI_ESTATE, "vi_autotune: configure the model first");
+  VI_HIP(hipStreamSynchronize(h->stream));
+  // snapshot every mutable array so that the trials leave the state untouched
+  std::vector<DevBuf*> live;
+  for (auto& b : h->state) live.push_back(&b);
+  live.push_back(&h->M);
+  for (auto& b : h->psi) live.push_back(&b);
+  live.push_back(&h->traces);
+  std::vector<DevBuf> snap(live.size());
+  for (size_t q = 0; q < live.size(); ++q) {
+    if (!live[q]->p) continue;
+    if (int rc = snap[q].alloc(live[q]->bytes, h->stream)) return rc;
+    VI_HIP(hipMemcpyAsync(snap[q].p, live[q]->p, live[q]->bytes, hipMemcpyDeviceToDevice,
+                          h->stream));
+  }
+  auto restore = [&]() -> int {
+    for (size_t q = 0; q < live.size(); ++q)
+      if (live[q]->p)
+        VI_HIP(hipMemcpyAsync(live[q]->p, snap[q].p, live[q]->bytes, hipMemcpyDeviceToDevice,
+                              h->stream));
+    return 0;
+  };
+  const long step0 = h->step;
+  const bool prof0 = h->prof;
+  h->prof = false;
+  hipEvent_t e0, e1;
+  VI_HIP(hipEventCreate(&e0));
+  VI_HIP(hipEventCreate(&e1));
+  const int cands[4] = {1, 2, 4, 8};
+  int best = h->rz;
+  double best_us = 1e300;
+  int rc = 0;
+  for (int c = 0; c < 4 && !rc; ++c) {
+    if ((rc = apply_tiling(h, cands[c]))) break;
+    for (int k = 0; k < 2 && !rc; ++k) rc = h->esz == 4 ? launch_step<float>(h) : launch_step<double>(h);
+    if (rc) break;
+    if ((rc = hipEventRecord(e0, h->stream) == hipSuccess ? 0 : VI_EDEVICE)) break;
+    for (int k = 0; k < trial_steps && !rc; ++k) {
+      rc = h->esz == 4 ? launch_step<float>(h) : launch_step<double>(h);
+      h->step++;
+    }
+    if (rc) break;
+    if ((rc = hipEventRecord(e1, h->stream) == hipSuccess ? 0 : VI_EDEVICE)) break;
+    if ((rc = hipEventSynchronize(e1) == hipSuccess ? 0 : VI_EDEVICE)) break;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const double us = 1e3 * ms / trial_steps;
+    if (us_per_step) us_per_step[c] = us;
+    if (us < best_us) {
+      best_us = us;
+      best = cands[c];
+    }
+    h->step = step0;
+    if ((rc = restore())) break;
+  }
+  h->step = step0;
+  h->prof = prof0;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) {
+    (void)restore();
+    (void)hipStreamSynchronize(h->stream);
+    return rc == VI_EDEVICE && g_err.empty() ? fail(rc, "vi_autotune: HIP error") : rc;
+  }
+  VI_HIP(hipStreamSynchronize(h->stream));
+  if (int r2 = apply_tiling(h, best)) return r2;
+  if (chosen) *chosen = best;
+  return 0;
+}
+
 }  // extern "C"
+

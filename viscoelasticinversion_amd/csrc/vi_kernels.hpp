@@ -7,10 +7,12 @@
 //   - fields differentiated along x are staged row by row in LDS with a 4-column halo;
 //   - fields differentiated along z are held as a register window of RZ+7 rows per lane
 //     (z register blocking), so every global load is a coalesced 64-lane row segment;
+//   - RZ (rows per wave) is a template parameter chosen per grid by the autotuner: small
+//     grids need small RZ for enough waves in flight, large grids amortise the z halo;
 //   - the workgroup -> tile map is XCD-aware: consecutive tiles of a row band go to the
 //     same XCD so vertical halos are re-read from that XCD's L2.
-// Arrays are padded (pitch ldx >= nx + 72, plane rows >= nz + NW*RZ + 16) so that halo
-// and window loads of edge tiles stay inside the allocation without bounds checks; only
+// Arrays are padded (pitch ldx >= nx + 72, plane rows >= nz + NW*8 + 16) so that halo and
+// window loads of edge tiles stay inside the allocation without bounds checks; only
 // stores are guarded. The padding is zero and is never written.
 //
 // Arithmetic follows the oracle's operation order exactly (no contraction: the library is
@@ -24,10 +26,9 @@
 
 namespace vi {
 
-constexpr int TX = 64;   // tile width: one wavefront along x
-constexpr int NW = 4;    // wavefronts per workgroup
-constexpr int RZ = 8;    // rows per wavefront
-constexpr int TZ = NW * RZ;
+constexpr int TX = 64;     // tile width: one wavefront along x
+constexpr int NW = 4;      // wavefronts per workgroup
+constexpr int RZ_MAX = 8;  // largest rows-per-wave variant (sets the padding)
 constexpr int MAXL = 6;
 
 template <typename R>
@@ -88,23 +89,6 @@ __device__ __forceinline__ R op_m(const R* q, int st, const R* D) {
   v = v + D[3] * (q[3 * st] - q[-4 * st]);
   return v;
 }
-// the same on a register window w[] with the centre at compile-time index C
-template <typename R, int C, int N>
-__device__ __forceinline__ R win_p(const R (&w)[N], const R* D) {
-  R v = D[0] * (w[C + 1] - w[C]);
-  v = v + D[1] * (w[C + 2] - w[C - 1]);
-  v = v + D[2] * (w[C + 3] - w[C - 2]);
-  v = v + D[3] * (w[C + 4] - w[C - 3]);
-  return v;
-}
-template <typename R, int C, int N>
-__device__ __forceinline__ R win_m(const R (&w)[N], const R* D) {
-  R v = D[0] * (w[C] - w[C - 1]);
-  v = v + D[1] * (w[C + 1] - w[C - 2]);
-  v = v + D[2] * (w[C + 2] - w[C - 3]);
-  v = v + D[3] * (w[C + 3] - w[C - 4]);
-  return v;
-}
 
 template <typename R>
 __device__ __forceinline__ R cpml_apply(R* psi, R a, R b, R k, R d) {
@@ -119,11 +103,12 @@ __device__ __forceinline__ int strip_x(int i, int nx, int T, int lo, int hi) {
   return -1;
 }
 
-// Stage RZ rows of one field (own column from the register window, halo columns from
-// global memory) into this wave's LDS rows.
-template <typename R, int OFF, int N>
-__device__ __forceinline__ void stage_rows(R (*sh)[TX + 8], const R (&win)[N], const R* g,
-                                           size_t base, int ldx, int lane) {
+// Stage RZ rows of one field (own column from the register window starting at OFF, halo
+// columns from global memory) into this wave's LDS rows.
+template <typename R, int RZ, int OFF, int N>
+__device__ __forceinline__ void stage_rows(R (*sh)[TX + 8], const R (&win)[N],
+                                           const R* __restrict__ g, size_t base, int ldx,
+                                           int lane) {
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     sh[r][lane + 4] = win[OFF + r];
@@ -132,7 +117,7 @@ __device__ __forceinline__ void stage_rows(R (*sh)[TX + 8], const R (&win)[N], c
   }
 }
 
-template <typename R>
+template <typename R, int RZ>
 __global__ __launch_bounds__(TX* NW) void k_velocity(const StepParams<R> p) {
   __shared__ R sh_xx[NW][RZ][TX + 8];
   __shared__ R sh_xz[NW][RZ][TX + 8];
@@ -140,30 +125,43 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const StepParams<R> p) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = tile % p.ntx, tz = tile / p.ntx;
   const int i = 4 + tx * TX + lane;
-  const int j0 = 4 + tz * TZ + w * RZ;
+  const int j0 = 4 + (tz * NW + w) * RZ;
   const int ldx = p.ldx;
   const size_t base = (size_t)j0 * ldx + i;
+  const R* __restrict__ sxx = p.sxx;
+  const R* __restrict__ sxz = p.sxz;
+  const R* __restrict__ szz = p.szz;
+  const R* __restrict__ rho = p.rho;
+  R* __restrict__ vx = p.vx;
+  R* __restrict__ vz = p.vz;
 
   R wxx[RZ];      // sxx rows j0 .. j0+RZ-1
   R wxz[RZ + 7];  // sxz rows j0-4 .. j0+RZ+2   (D- along z, centre at r+4)
   R wzz[RZ + 7];  // szz rows j0-3 .. j0+RZ+3   (D+ along z, centre at r+3)
-  R wrho[RZ + 1]; // rho rows j0 .. j0+RZ
+  R wrho[RZ + 1], wrho1[RZ];  // rho rows j0 .. j0+RZ at column i; rows j0.. at column i+1
+  R ovx[RZ], ovz[RZ];
 #pragma unroll
   for (int k = 0; k < RZ + 7; ++k) {
-    wxz[k] = p.sxz[base + (ptrdiff_t)(k - 4) * ldx];
-    wzz[k] = p.szz[base + (ptrdiff_t)(k - 3) * ldx];
+    wxz[k] = sxz[base + (ptrdiff_t)(k - 4) * ldx];
+    wzz[k] = szz[base + (ptrdiff_t)(k - 3) * ldx];
   }
 #pragma unroll
-  for (int r = 0; r < RZ; ++r) wxx[r] = p.sxx[base + (size_t)r * ldx];
+  for (int r = 0; r < RZ; ++r) {
+    wxx[r] = sxx[base + (size_t)r * ldx];
+    wrho1[r] = rho[base + (size_t)r * ldx + 1];
+    ovx[r] = vx[base + (size_t)r * ldx];
+    ovz[r] = vz[base + (size_t)r * ldx];
+  }
 #pragma unroll
-  for (int r = 0; r <= RZ; ++r) wrho[r] = p.rho[base + (size_t)r * ldx];
+  for (int r = 0; r <= RZ; ++r) wrho[r] = rho[base + (size_t)r * ldx];
 
-  stage_rows<R, 0>(sh_xx[w], wxx, p.sxx, base, ldx, lane);
-  stage_rows<R, 4>(sh_xz[w], wxz, p.sxz, base, ldx, lane);
+  stage_rows<R, RZ, 0>(sh_xx[w], wxx, sxx, base, ldx, lane);
+  stage_rows<R, RZ, 4>(sh_xz[w], wxz, sxz, base, ldx, lane);
   __syncthreads();
 
   const bool col_ok = i < p.nx - 4;
   const R dt = p.dt, two = R(2);
+  const R Dz0 = p.Dz[0], Dz1 = p.Dz[1], Dz2 = p.Dz[2], Dz3 = p.Dz[3];
   const int xk = p.T ? strip_x(i, p.nx, p.T, p.xlo, p.xhi) : -1;
   const int s0 = p.src_off[tile], s1 = p.src_off[tile + 1];
 #pragma unroll
@@ -172,17 +170,15 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const StepParams<R> p) {
     const size_t c = base + (size_t)r * ldx;
     R dsxx_x = op_p(&sh_xx[w][r][lane + 4], 1, p.Dx);
     R dsxz_x = op_m(&sh_xz[w][r][lane + 4], 1, p.Dx);
-    R dsxz_z, dszz_z;
-    switch (r) {  // compile-time window centres
-#define VI_VEL_Z(RR)                                   \
-  case RR:                                             \
-    dsxz_z = win_m<R, RR + 4>(wxz, p.Dz);              \
-    dszz_z = win_p<R, RR + 3>(wzz, p.Dz);              \
-    break;
-      VI_VEL_Z(0) VI_VEL_Z(1) VI_VEL_Z(2) VI_VEL_Z(3) VI_VEL_Z(4) VI_VEL_Z(5) VI_VEL_Z(6)
-      VI_VEL_Z(7)
-#undef VI_VEL_Z
-    }
+    // D- of sxz along z (centre r+4) and D+ of szz along z (centre r+3)
+    R dsxz_z = Dz0 * (wxz[r + 4] - wxz[r + 3]);
+    dsxz_z = dsxz_z + Dz1 * (wxz[r + 5] - wxz[r + 2]);
+    dsxz_z = dsxz_z + Dz2 * (wxz[r + 6] - wxz[r + 1]);
+    dsxz_z = dsxz_z + Dz3 * (wxz[r + 7] - wxz[r + 0]);
+    R dszz_z = Dz0 * (wzz[r + 4] - wzz[r + 3]);
+    dszz_z = dszz_z + Dz1 * (wzz[r + 5] - wzz[r + 2]);
+    dszz_z = dszz_z + Dz2 * (wzz[r + 6] - wzz[r + 1]);
+    dszz_z = dszz_z + Dz3 * (wzz[r + 7] - wzz[r + 0]);
     const bool ok = col_ok && j < p.nz - 4;
     if (p.T && ok) {
       if (xk >= 0) {
@@ -206,16 +202,16 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const StepParams<R> p) {
       if (kind == 0) fz = fz + val;
       else if (kind == 2) fx = fx + val;
     }
-    const R bxv = two / (wrho[r] + p.rho[c + 1]);
+    const R bxv = two / (wrho[r] + wrho1[r]);
     const R bzv = two / (wrho[r] + wrho[r + 1]);
     if (ok) {
-      p.vx[c] = p.vx[c] + dt * (bxv * fx);
-      p.vz[c] = p.vz[c] + dt * (bzv * fz);
+      vx[c] = ovx[r] + dt * (bxv * fx);
+      vz[c] = ovz[r] + dt * (bzv * fz);
     }
   }
 }
 
-template <typename R, int L>
+template <typename R, int L, int RZ>
 __global__ __launch_bounds__(TX* NW) void k_stress(const StepParams<R> p) {
   __shared__ R sh_vx[NW][RZ][TX + 8];
   __shared__ R sh_vz[NW][RZ][TX + 8];
@@ -223,29 +219,40 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const StepParams<R> p) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = tile % p.ntx, tz = tile / p.ntx;
   const int i = 4 + tx * TX + lane;
-  const int j0 = 4 + tz * TZ + w * RZ;
+  const int j0 = 4 + (tz * NW + w) * RZ;
   const int ldx = p.ldx;
   const size_t base = (size_t)j0 * ldx + i;
+  const R* __restrict__ vx = p.vx;
+  const R* __restrict__ vz = p.vz;
+  const R* __restrict__ mu = p.mu;
+  const R* __restrict__ lam = p.lam;
+  const R* __restrict__ tau = p.tau;
+  R* __restrict__ sxx = p.sxx;
+  R* __restrict__ szz = p.szz;
+  R* __restrict__ sxz = p.sxz;
+  R* __restrict__ M = p.M;
+  const size_t plane = p.plane;
 
   R wvx[RZ + 7];  // vx rows j0-3 .. j0+RZ+3  (D+ along z, centre r+3)
   R wvz[RZ + 7];  // vz rows j0-4 .. j0+RZ+2  (D- along z, centre r+4)
   R wmu[RZ + 1], wmu1[RZ + 1];  // mu at columns i and i+1, rows j0 .. j0+RZ
 #pragma unroll
   for (int k = 0; k < RZ + 7; ++k) {
-    wvx[k] = p.vx[base + (ptrdiff_t)(k - 3) * ldx];
-    wvz[k] = p.vz[base + (ptrdiff_t)(k - 4) * ldx];
+    wvx[k] = vx[base + (ptrdiff_t)(k - 3) * ldx];
+    wvz[k] = vz[base + (ptrdiff_t)(k - 4) * ldx];
   }
 #pragma unroll
   for (int r = 0; r <= RZ; ++r) {
-    wmu[r] = p.mu[base + (size_t)r * ldx];
-    wmu1[r] = p.mu[base + (size_t)r * ldx + 1];
+    wmu[r] = mu[base + (size_t)r * ldx];
+    wmu1[r] = mu[base + (size_t)r * ldx + 1];
   }
-  stage_rows<R, 3>(sh_vx[w], wvx, p.vx, base, ldx, lane);
-  stage_rows<R, 4>(sh_vz[w], wvz, p.vz, base, ldx, lane);
+  stage_rows<R, RZ, 3>(sh_vx[w], wvx, vx, base, ldx, lane);
+  stage_rows<R, RZ, 4>(sh_vz[w], wvz, vz, base, ldx, lane);
   __syncthreads();
 
   const bool col_ok = i < p.nx - 4;
   const R dt = p.dt, two = R(2), quarter = R(0.25);
+  const R Dz0 = p.Dz[0], Dz1 = p.Dz[1], Dz2 = p.Dz[2], Dz3 = p.Dz[3];
   const int xk = p.T ? strip_x(i, p.nx, p.T, p.xlo, p.xhi) : -1;
   const int s0 = p.src_off[tile], s1 = p.src_off[tile + 1];
   const int r0 = p.rec_off[tile], r1 = p.rec_off[tile + 1];
@@ -255,17 +262,14 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const StepParams<R> p) {
     const size_t c = base + (size_t)r * ldx;
     R exx = op_m(&sh_vx[w][r][lane + 4], 1, p.Dx);
     R exz = op_p(&sh_vz[w][r][lane + 4], 1, p.Dx);
-    R ezz, ezx;
-    switch (r) {
-#define VI_STR_Z(RR)                                   \
-  case RR:                                             \
-    ezz = win_m<R, RR + 4>(wvz, p.Dz);                 \
-    ezx = win_p<R, RR + 3>(wvx, p.Dz);                 \
-    break;
-      VI_STR_Z(0) VI_STR_Z(1) VI_STR_Z(2) VI_STR_Z(3) VI_STR_Z(4) VI_STR_Z(5) VI_STR_Z(6)
-      VI_STR_Z(7)
-#undef VI_STR_Z
-    }
+    R ezz = Dz0 * (wvz[r + 4] - wvz[r + 3]);
+    ezz = ezz + Dz1 * (wvz[r + 5] - wvz[r + 2]);
+    ezz = ezz + Dz2 * (wvz[r + 6] - wvz[r + 1]);
+    ezz = ezz + Dz3 * (wvz[r + 7] - wvz[r + 0]);
+    R ezx = Dz0 * (wvx[r + 4] - wvx[r + 3]);
+    ezx = ezx + Dz1 * (wvx[r + 5] - wvx[r + 2]);
+    ezx = ezx + Dz2 * (wvx[r + 6] - wvx[r + 1]);
+    ezx = ezx + Dz3 * (wvx[r + 7] - wvx[r + 0]);
     const bool ok = col_ok && j < p.nz - 4;
     if (!ok) continue;
     if (p.T) {
@@ -282,22 +286,22 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const StepParams<R> p) {
       }
     }
     const R gxz = ezx + exz;
-    const R lr = p.lam[c], mr = wmu[r];
+    const R lr = lam[c], mr = wmu[r];
     const R mxz = quarter * (((wmu[r] + wmu1[r]) + wmu[r + 1]) + wmu1[r + 1]);
     const R pr = lr + two * mr;
     const bool fs_row = p.fs && j == 4;
     R dsxx, dszz, dsxz;
     R mxx[L > 0 ? L : 1], mzz[L > 0 ? L : 1], mxzv[L > 0 ? L : 1];
     if constexpr (L > 0) {
-      const R t = p.tau[c];
+      const R t = tau[c];
       const R p1 = pr * t, p0 = p1 + pr;
       const R l1 = lr * t, l0 = l1 + lr;
       const R m1 = mxz * t, m0 = m1 + mxz;
 #pragma unroll
       for (int k = 0; k < L; ++k) {
-        mxx[k] = p.M[(size_t)k * p.plane + c];
-        mzz[k] = p.M[(size_t)(L + k) * p.plane + c];
-        mxzv[k] = p.M[(size_t)(2 * L + k) * p.plane + c];
+        mxx[k] = M[(size_t)k * plane + c];
+        mzz[k] = M[(size_t)(L + k) * plane + c];
+        mxzv[k] = M[(size_t)(2 * L + k) * plane + c];
       }
       R SMxx = mxx[0], SMzz = mzz[0], SMxz = mxzv[0];
 #pragma unroll
@@ -322,25 +326,25 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const StepParams<R> p) {
       dsxx = dsxx + val;
       dszz = dszz + val;
     }
-    const R nsxx = p.sxx[c] + dt * dsxx;
-    R nszz = p.szz[c] + dt * dszz;
-    const R nsxz = p.sxz[c] + dt * dsxz;
+    const R nsxx = sxx[c] + dt * dsxx;
+    R nszz = szz[c] + dt * dszz;
+    const R nsxz = sxz[c] + dt * dsxz;
     if (fs_row) nszz = R(0);
-    p.sxx[c] = nsxx;
-    p.szz[c] = nszz;
-    p.sxz[c] = nsxz;
+    sxx[c] = nsxx;
+    szz[c] = nszz;
+    sxz[c] = nsxz;
     if constexpr (L > 0) {
 #pragma unroll
       for (int k = 0; k < L; ++k) {
         const R a1 = p.a1[k], a2 = p.a2[k];
-        p.M[(size_t)k * p.plane + c] = mxx[k] + dt * (a1 * exx + a2 * mxx[k]);
-        p.M[(size_t)(L + k) * p.plane + c] = mzz[k] + dt * (a1 * ezz + a2 * mzz[k]);
-        p.M[(size_t)(2 * L + k) * p.plane + c] = mxzv[k] + dt * (a1 * gxz + a2 * mxzv[k]);
+        M[(size_t)k * plane + c] = mxx[k] + dt * (a1 * exx + a2 * mxx[k]);
+        M[(size_t)(L + k) * plane + c] = mzz[k] + dt * (a1 * ezz + a2 * mzz[k]);
+        M[(size_t)(2 * L + k) * plane + c] = mxzv[k] + dt * (a1 * gxz + a2 * mxzv[k]);
       }
     }
     if (p.fs) {  // stress images above the surface row 4
-      if (j >= 5 && j <= 8) p.szz[(size_t)(8 - j) * ldx + i] = -nszz;
-      if (j >= 4 && j <= 7) p.sxz[(size_t)(7 - j) * ldx + i] = -nsxz;
+      if (j >= 5 && j <= 8) szz[(size_t)(8 - j) * ldx + i] = -nszz;
+      if (j >= 4 && j <= 7) sxz[(size_t)(7 - j) * ldx + i] = -nsxz;
     }
     for (int q = r0; q < r1; ++q) {
       if ((size_t)p.rec_cell[q] != c || p.step >= p.rec_nt) continue;

@@ -154,6 +154,22 @@ class Simulation:
         L.check(L.lib().vi_get_traces(self._h, self._ptr(out), nt))
         return out
 
+    # -- tiling ------------------------------------------------------------------------
+    def set_tiling(self, rows_per_wave):
+        L.check(L.lib().vi_set_tiling(self._h, int(rows_per_wave)))
+
+    @property
+    def tiling(self):
+        return int(L.lib().vi_get_tiling(self._h))
+
+    def autotune(self, trial_steps=20):
+        """Pick the fastest kernel variant for this configuration; the state is unchanged.
+        Returns (chosen rows per wave, {rows: us per step})."""
+        chosen = ctypes.c_int()
+        us = (ctypes.c_double * 4)()
+        L.check(L.lib().vi_autotune(self._h, int(trial_steps), ctypes.byref(chosen), us))
+        return chosen.value, dict(zip((1, 2, 4, 8), list(us)))
+
     # -- timing ------------------------------------------------------------------------
     def profile(self, enabled=True):
         L.check(L.lib().vi_profile_enable(self._h, int(bool(enabled))))
