@@ -35,6 +35,8 @@ extern "C" {
 #endif
 
 typedef struct vi_sim_s* vi_handle;
+typedef struct vi_model_s* vi_model;
+typedef struct vi_batch_s* vi_batch;
 
 enum { VI_F32 = 0, VI_F64 = 1 };
 
@@ -93,6 +95,16 @@ int vi_cpml_profile(int n, const vi_cpml_desc* c, double dt, int axis /*0=x,1=z*
                     double* a, double* b, double* kappa);
 /* number of visible HIP devices (0 when none); never fails */
 int vi_device_count(void);
+/* free and total device memory of the current device */
+int vi_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+/* ---- device-resident models (no counterpart in the reference: IsotropicModel arrays are
+ * re-created per Parameters). A model is uploaded once and shared by any number of shots,
+ * each reading a column window of it (multi-shot acquisition on one large model). ------ */
+int vi_model_create(int nx, int nz, int precision, const void* rho, const void* lam,
+                    const void* mu, const void* tau /* NULL for elastic */, vi_model* out);
+/* releases the caller's reference; shots that use the model keep it alive */
+int vi_model_destroy(vi_model m);
 
 /* ---- single-shot stepper ----------------------------------------------------------- */
 int vi_create(const vi_grid_desc* g, vi_handle* out);
@@ -100,6 +112,8 @@ int vi_destroy(vi_handle h);
 /* rho, lam, mu (and tau when nrelax > 0) as [nz][nx]; tau may be NULL when nrelax == 0 */
 int vi_set_model(vi_handle h, const void* rho, const void* lam, const void* mu,
                  const void* tau);
+/* use columns [x0, x0 + nx) of a shared model (same nz and precision) */
+int vi_set_model_window(vi_handle h, vi_model m, int x0);
 /* taun[L] relaxation times of the SLS mechanisms (QModel.τns, Models.jl:56-62) */
 int vi_set_relaxation(vi_handle h, const double* taun);
 /* NULL disables the CPML; resets the psi memory to zero */
@@ -123,6 +137,8 @@ int vi_sync(vi_handle h);
 long vi_step_index(vi_handle h);
 /* traces recorded so far: out[nrec][nt] for steps [0, nt) since vi_set_receivers */
 int vi_get_traces(vi_handle h, void* out, int nt);
+/* device bytes owned by the handle (state, CPML memory, traces, tables, private model) */
+int vi_device_bytes(vi_handle h, size_t* bytes);
 
 /* ---- tiling (no counterpart in the reference: KA picks workgroup shapes itself) ------ */
 /* rows per wavefront of the kernel variant: 1, 2, 4 or 8 (results are identical) */
@@ -131,6 +147,25 @@ int vi_get_tiling(vi_handle h);
 /* time every variant for trial_steps steps on a snapshot of the state, restore the state,
  * keep the fastest; us_per_step (optional, 4 entries) receives the timings of 1, 2, 4, 8 */
 int vi_autotune(vi_handle h, int trial_steps, int* chosen, double* us_per_step);
+
+/* ---- batches: several shots advanced by one launch per half step -------------------
+ * (no counterpart in the reference, which has one source per Parameters, Parameters.jl:59)
+ * Members keep their own state; a batch launch covers all their tiles (grouped by precision
+ * and nrelax). Every member's result is identical to stepping it alone. */
+int vi_batch_create(vi_batch* out);
+int vi_batch_destroy(vi_batch b);
+/* set the members (replaces the previous set); handles must stay alive while members */
+int vi_batch_set(vi_batch b, const vi_handle* hs, int n);
+int vi_batch_set_tiling(vi_batch b, int rows_per_wave);
+int vi_batch_get_tiling(vi_batch b);
+int vi_batch_autotune(vi_batch b, int trial_steps, int* chosen, double* us_per_step);
+/* replay a captured hipGraph of steps_per_graph steps (0 = direct launches) */
+int vi_batch_set_graph(vi_batch b, int steps_per_graph);
+/* advance every member by nsteps; asynchronous until vi_batch_sync or a member's vi_get_* */
+int vi_batch_step(vi_batch b, int nsteps);
+int vi_batch_sync(vi_batch b);
+int vi_batch_profile_enable(vi_batch b, int enabled);
+int vi_batch_profile_read(vi_batch b, int kernel, double* ms_total, long* launches);
 
 /* ---- kernel timing (HIP events around each launch on the handle's stream) ---------- */
 int vi_profile_enable(vi_handle h, int enabled);

@@ -7,11 +7,13 @@ from .api import (CPMLConfig, DirichletBC, Fields, FreeSurfaceBC, IsotropicModel
                   Parameters, Problem, QModel, RickerSource, Solution, _ricker, dK_dw,
                   find_optimal_taun, get_bc, get_Q, get_tau, leapfrog_step, make_problem,
                   solve_problem)
+from .batch import Batch, Model, mem_info
 from .stepper import COMPONENTS, FIELDS, SOURCE_KINDS, Simulation
 
 __all__ = [
     "CPMLConfig", "DirichletBC", "Fields", "FreeSurfaceBC", "IsotropicModel", "K",
     "Parameters", "Problem", "QModel", "RickerSource", "Solution", "_ricker", "dK_dw",
     "find_optimal_taun", "get_bc", "get_Q", "get_tau", "leapfrog_step", "make_problem",
-    "solve_problem", "Simulation", "FIELDS", "COMPONENTS", "SOURCE_KINDS",
+    "solve_problem", "Simulation", "FIELDS", "COMPONENTS", "SOURCE_KINDS", "Batch", "Model",
+    "mem_info",
 ]

@@ -54,6 +54,22 @@ SIGNATURES = {
     "vi_cpml_auto_d0": (_D, [_I, _D, _D, _D, _D]),
     "vi_cpml_profile": (_I, [_I, ctypes.POINTER(CpmlDesc), _D, _I, _DP, _DP, _DP]),
     "vi_device_count": (_I, []),
+    "vi_mem_info": (_I, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "vi_model_create": (_I, [_I, _I, _I, _V, _V, _V, _V, ctypes.POINTER(_H)]),
+    "vi_model_destroy": (_I, [_H]),
+    "vi_set_model_window": (_I, [_H, _H, _I]),
+    "vi_device_bytes": (_I, [_H, ctypes.POINTER(ctypes.c_size_t)]),
+    "vi_batch_create": (_I, [ctypes.POINTER(_H)]),
+    "vi_batch_destroy": (_I, [_H]),
+    "vi_batch_set": (_I, [_H, ctypes.POINTER(_H), _I]),
+    "vi_batch_set_tiling": (_I, [_H, _I]),
+    "vi_batch_get_tiling": (_I, [_H]),
+    "vi_batch_autotune": (_I, [_H, _I, _IP, _DP]),
+    "vi_batch_set_graph": (_I, [_H, _I]),
+    "vi_batch_step": (_I, [_H, _I]),
+    "vi_batch_sync": (_I, [_H]),
+    "vi_batch_profile_enable": (_I, [_H, _I]),
+    "vi_batch_profile_read": (_I, [_H, _I, _DP, ctypes.POINTER(_L)]),
     "vi_create": (_I, [ctypes.POINTER(GridDesc), ctypes.POINTER(_H)]),
     "vi_destroy": (_I, [_H]),
     "vi_set_model": (_I, [_H, _V, _V, _V, _V]),

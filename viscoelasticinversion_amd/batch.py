@@ -1,0 +1,122 @@
+"""Device-resident models and batches of shots (the C ABI's vi_model / vi_batch).
+
+A ``Model`` is uploaded once and shared by many shots, each reading a column window of it.
+A ``Batch`` advances a set of ``Simulation`` objects with one kernel launch per half step
+covering all their tiles; optionally it replays a captured hipGraph of a block of steps.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Model:
+    """Device-resident model arrays (rho, lam, mu[, tau]) of shape (nz, nx)."""
+
+    def __init__(self, rho, lam, mu, tau=None, precision="float32"):
+        self.dtype = np.dtype(np.float32 if precision in ("float32", np.float32, "f32") else np.float64)
+        arrs = [np.ascontiguousarray(np.asarray(a, dtype=self.dtype)) for a in (rho, lam, mu)]
+        self.nz, self.nx = arrs[0].shape
+        for a in arrs[1:]:
+            if a.shape != arrs[0].shape:
+                raise ValueError("model arrays must share one shape")
+        t = None if tau is None else np.ascontiguousarray(np.asarray(tau, dtype=self.dtype))
+        if t is not None and t.shape != arrs[0].shape:
+            raise ValueError("tau must have the model's shape")
+        h = ctypes.c_void_p()
+        P = ctypes.c_void_p
+        L.check(L.lib().vi_model_create(
+            self.nx, self.nz, L.VI_F32 if self.dtype == np.float32 else L.VI_F64,
+            *(a.ctypes.data_as(P) for a in arrs), t.ctypes.data_as(P) if t is not None else None,
+            ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().vi_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Advance several simulations together (same results as stepping each alone)."""
+
+    def __init__(self, sims=()):
+        h = ctypes.c_void_p()
+        L.check(L.lib().vi_batch_create(ctypes.byref(h)))
+        self._h = h
+        self.sims = []
+        if sims:
+            self.set(sims)
+
+    def set(self, sims):
+        self.sims = list(sims)
+        arr = (ctypes.c_void_p * max(len(self.sims), 1))(*[s.handle for s in self.sims])
+        L.check(L.lib().vi_batch_set(self._h, arr, len(self.sims)))
+
+    def set_tiling(self, rows):
+        L.check(L.lib().vi_batch_set_tiling(self._h, int(rows)))
+
+    @property
+    def tiling(self):
+        return int(L.lib().vi_batch_get_tiling(self._h))
+
+    def autotune(self, trial_steps=10):
+        chosen = ctypes.c_int()
+        us = (ctypes.c_double * 4)()
+        L.check(L.lib().vi_batch_autotune(self._h, int(trial_steps), ctypes.byref(chosen), us))
+        return chosen.value, dict(zip((1, 2, 4, 8), list(us)))
+
+    def set_graph(self, steps_per_graph):
+        L.check(L.lib().vi_batch_set_graph(self._h, int(steps_per_graph)))
+
+    def step(self, nsteps=1):
+        L.check(L.lib().vi_batch_step(self._h, int(nsteps)))
+
+    def sync(self):
+        L.check(L.lib().vi_batch_sync(self._h))
+
+    def profile(self, enabled=True):
+        L.check(L.lib().vi_batch_profile_enable(self._h, int(bool(enabled))))
+
+    def profile_read(self, kernel):
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        L.check(L.lib().vi_batch_profile_read(self._h, int(kernel), ctypes.byref(ms),
+                                              ctypes.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().vi_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mem_info():
+    """(free, total) device bytes."""
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    L.check(L.lib().vi_mem_info(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value

@@ -113,3 +113,75 @@ def build_simulation(cfg, precision="float32"):
     sim.set_sources(cfg["src_ix"], cfg["src_iz"], cfg["src_kind"], cfg["wavelets"])
     sim.set_receivers(cfg["rec_ix"], cfg["rec_iz"], cfg["rec_comp"], cfg["nt"])
     return sim
+
+
+def config_multishot(nshots=64, nz=512, wmin=512, wmax=1536, nt=3000, seed=0, model_nx=None,
+                     rec_every=8):
+    """Batched multi-shot forward: ``nshots`` windows of width U{wmin..wmax} x nz cut from
+    one layered viscoelastic model (the single-shot model type), each with its own Ricker
+    force source at a seeded position along the surface and receivers every ``rec_every``
+    cells 10 m below the surface; L = 3, CPML + free surface, FP32 by default."""
+    rng = np.random.default_rng(seed)
+    widths = rng.integers(wmin, wmax + 1, nshots)
+    W = int(model_nx) if model_nx else max(4 * wmax, 4096)
+    model, vmax = layered_model(W, nz, seed=seed)
+    dx, dt = 5.0, 4e-4
+    T = 20
+    power, R = 2.0, 1e-5
+    d0 = -(power + 1) * vmax * np.log(R) / (2 * T * dx)
+    f = 12.0
+    depth = JS + int(round(10.0 / dx))
+    wav = ricker_samples(f, 1.0 / f, nt, dt)
+    shots = []
+    for s in range(nshots):
+        w = int(widths[s])
+        sx = int(rng.integers(T + 8, W - T - 8))
+        x0 = int(np.clip(sx - w // 2, 0, W - w))
+        rix = np.arange(T, w - T, rec_every, dtype=np.int32)
+        shots.append(dict(nx=w, x0=x0, src_ix=np.array([sx - x0], np.int32),
+                          src_iz=np.array([depth], np.int32), src_kind=np.array([0], np.int32),
+                          wavelets=wav[None, :], rec_ix=rix,
+                          rec_iz=np.full(len(rix), depth, np.int32),
+                          rec_comp=np.full(len(rix), 1, np.int32)))
+    return dict(name=f"multishot_{nshots}x{nz}", nz=nz, model_nx=W, dx=dx, dz=dx, dt=dt, nt=nt,
+                nrelax=3, taun=np.array(TAUN_2_40HZ_L3), model=model, free_surface=True,
+                cpml=dict(thickness=T, d0=d0, alpha_max=0.0, kappa_max=1.0, power=power,
+                          left=True, right=True, top=False, bottom=True),
+                shots=shots)
+
+
+def shot_config(mcfg, s):
+    """The single-shot config dict of shot ``s`` of a multi-shot config (model window
+    copied out), for running it alone or on the oracle."""
+    sh = mcfg["shots"][s]
+    x0, w = sh["x0"], sh["nx"]
+    model = {k: np.ascontiguousarray(v[:, x0:x0 + w]) for k, v in mcfg["model"].items()}
+    return dict(name=f"shot{s}", nx=w, nz=mcfg["nz"], dx=mcfg["dx"], dz=mcfg["dz"],
+                dt=mcfg["dt"], nt=mcfg["nt"], nrelax=mcfg["nrelax"], taun=mcfg["taun"],
+                model=model, free_surface=mcfg["free_surface"], cpml=mcfg["cpml"],
+                src_ix=sh["src_ix"], src_iz=sh["src_iz"], src_kind=sh["src_kind"],
+                wavelets=sh["wavelets"], rec_ix=sh["rec_ix"], rec_iz=sh["rec_iz"],
+                rec_comp=sh["rec_comp"])
+
+
+def build_multishot(mcfg, precision="float32"):
+    """Upload the shared model once and create one Simulation per shot reading its window.
+    Returns (model, sims)."""
+    from .batch import Model
+    from .stepper import Simulation
+    m = mcfg["model"]
+    model = Model(m["rho"], m["lam"], m["mu"], m.get("tau"), precision)
+    sims = []
+    for sh in mcfg["shots"]:
+        sim = Simulation(sh["nx"], mcfg["nz"], mcfg["dx"], mcfg["dz"], mcfg["dt"], precision,
+                         mcfg["nrelax"])
+        sim.set_model_window(model, sh["x0"])
+        if mcfg["nrelax"]:
+            sim.set_relaxation(mcfg["taun"])
+        if mcfg["cpml"]:
+            sim.set_cpml(**mcfg["cpml"])
+        sim.set_free_surface(mcfg["free_surface"])
+        sim.set_sources(sh["src_ix"], sh["src_iz"], sh["src_kind"], sh["wavelets"])
+        sim.set_receivers(sh["rec_ix"], sh["rec_iz"], sh["rec_comp"], mcfg["nt"])
+        sims.append(sim)
+    return model, sims

@@ -1,8 +1,10 @@
 // vi_api.hip -- host side of the C ABI declared in include/vi_mi355x.h.
 //
-// A handle (vi_sim_s) owns the device arrays of one simulation in the padded layout the
-// kernels expect, one HIP stream, and the per-tile source/receiver tables. The precision
-// is a run-time property of the handle; the implementation is the template Sim<R>.
+// vi_sim_s (a handle) owns the device state of one shot in the padded layout the kernels
+// expect, a reference to a device-resident model (its own or a column window of a shared
+// vi_model), one HIP stream, the per-tile source/receiver tables and a device ShotDesc.
+// vi_batch_s advances a set of handles with one launch per half step (per precision and L
+// group), optionally replaying hipGraphs of a block of steps.
 #include "vi_mi355x.h"
 #include "vi_kernels.hpp"
 
@@ -12,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -33,6 +36,12 @@ int fail(int code, const std::string& msg) {
                   std::string(#call) + ": " + hipGetErrorString(e_));              \
   } while (0)
 
+#define VI_TRY(expr)     \
+  do {                   \
+    int rc_ = (expr);    \
+    if (rc_) return rc_; \
+  } while (0)
+
 size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
 
 // CPMLCoefficients along one axis (CPML.jl:109-189), in double.
@@ -50,7 +59,7 @@ void cpml_axis(int n, int T, double d0, double alpha_max, double kappa_max, doub
     }
     double bb = 0.0, aa = 0.0;
     if (d > 0.0 || alpha > 0.0) {
-      bb = std::exp(-(d / k + alpha) * dt);                           // CPML.jl:174
+      bb = std::exp(-(d / k + alpha) * dt);                                 // CPML.jl:174
       if (std::fabs(d) > 0.0) aa = d / (k * (d + k * alpha)) * (bb - 1.0);  // CPML.jl:176
     }
     a[i] = aa;
@@ -62,27 +71,91 @@ void cpml_axis(int n, int T, double d0, double alpha_max, double kappa_max, doub
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
-  ~DevBuf() {
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
     if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
   }
   int alloc(size_t nbytes, hipStream_t s) {
-    if (p) {
-      (void)hipFree(p);
-      p = nullptr;
-    }
+    release();
     bytes = nbytes;
     if (!nbytes) return 0;
     VI_HIP(hipMalloc(&p, nbytes));
     VI_HIP(hipMemsetAsync(p, 0, nbytes, s));
     return 0;
   }
+  // grow-only allocation without clearing (tables that are fully rewritten)
+  int reserve(size_t nbytes) {
+    if (nbytes <= bytes && p) return 0;
+    release();
+    bytes = nbytes;
+    if (!nbytes) return 0;
+    VI_HIP(hipMalloc(&p, nbytes));
+    return 0;
+  }
 };
 
-struct SimBase {
-  virtual ~SimBase() = default;
+// HIP events around every launch of one stream, summed per kernel kind.
+struct Prof {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> kind;
+  size_t used = 0;
+  double ms[2] = {0, 0};
+  long n[2] = {0, 0};
+  ~Prof() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+  int begin(hipStream_t s, int k, hipEvent_t* e1) {
+    if (used + 2 > ev.size()) {
+      for (int q = 0; q < 2; ++q) {
+        hipEvent_t e;
+        VI_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+        kind.push_back(0);
+      }
+    }
+    kind[used] = k;
+    VI_HIP(hipEventRecord(ev[used], s));
+    *e1 = ev[used + 1];
+    used += 2;
+    return 0;
+  }
+  int collect(hipStream_t s) {
+    if (!used) return 0;
+    VI_HIP(hipStreamSynchronize(s));
+    for (size_t q = 0; q + 1 < used; q += 2) {
+      float t = 0.f;
+      VI_HIP(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+      ms[kind[q]] += t;
+      n[kind[q]] += 1;
+    }
+    used = 0;
+    return 0;
+  }
+  void reset() {
+    ms[0] = ms[1] = 0;
+    n[0] = n[1] = 0;
+  }
 };
 
 }  // namespace
+
+// A device-resident model: rho, lam, mu, tau planes with padded pitch.
+struct ModelStore {
+  int nx = 0, nz = 0, ldm = 0, nzp = 0;
+  size_t esz = 4;
+  bool has_tau = false;
+  DevBuf arr[4];
+};
+
+struct vi_model_s {
+  std::shared_ptr<ModelStore> m;
+};
 
 struct vi_sim_s {
   vi_grid_desc g{};
@@ -90,17 +163,17 @@ struct vi_sim_s {
   int ldx = 0, nzp = 0;
   size_t plane = 0;
   int ntx = 0, ntz = 0, ntiles = 0;
-  int rz = 8;  // rows per wavefront of the current kernel variant
+  int rz = 8;
   int device = 0;
   hipStream_t stream = nullptr;
   size_t esz = 4;
   DevBuf state[5];  // vx vz sxx szz sxz
   DevBuf M;
-  DevBuf model[4];  // rho lam mu tau
+  std::shared_ptr<ModelStore> model;
+  int model_x0 = 0;
   DevBuf psi[8];
-  DevBuf coef;      // ax bx kx (nx each), az bz kz (nz each)
+  DevBuf coef;  // ax bx kx (nx each), az bz kz (nz each)
   DevBuf src_tab, src_w, rec_tab, traces;
-  bool model_set = false;
   double taun[vi::MAXL] = {};
   bool relax_set = false;
   bool cpml_on = false;
@@ -108,15 +181,38 @@ struct vi_sim_s {
   int fs = 0;
   int nsrc = 0, snt = 0;
   int nrec = 0, rec_nt = 0;
-  std::vector<int> src_ix, src_iz, src_kind, rec_ix, rec_iz, rec_comp;  // host copies
+  std::vector<int> src_ix, src_iz, src_kind, rec_ix, rec_iz, rec_comp;
   long step = 0;
-  // profiling
-  bool prof = false;
-  std::vector<hipEvent_t> ev;  // pairs
-  std::vector<int> ev_kind;
-  size_t ev_used = 0;
-  double prof_ms[2] = {0, 0};
-  long prof_n[2] = {0, 0};
+  // device descriptor for launches of this handle alone
+  std::vector<char> desc_host;
+  DevBuf desc_dev, tile_shot_dev;
+  bool desc_dirty = true;
+  unsigned long version = 1;  // bumped on every change that invalidates a descriptor
+  // ordering with batches
+  hipEvent_t own_ev = nullptr, ext_ev = nullptr;
+  bool own_pending = false, ext_pending = false;
+  Prof prof;
+};
+
+struct BatchGroup {
+  int esz = 4, L = 0;
+  std::vector<vi_sim_s*> members;
+  int ntiles = 0;
+  std::vector<char> desc_host;
+  DevBuf desc_dev, tile_shot_dev;
+};
+
+struct vi_batch_s {
+  std::vector<vi_sim_s*> members;
+  std::vector<unsigned long> versions;
+  std::vector<std::unique_ptr<BatchGroup>> groups;
+  bool built = false;
+  int rz = 8;
+  int graph_steps = 0;
+  hipGraphExec_t graph = nullptr;
+  hipGraph_t graph_def = nullptr;
+  hipStream_t stream = nullptr;
+  Prof prof;
 };
 
 namespace {
@@ -126,91 +222,39 @@ int check_handle(vi_handle h) {
   return 0;
 }
 
-// Upload a host [nz][nx] array into a padded device plane.
-int upload_plane(vi_sim_s* h, void* dst, const void* src) {
-  VI_HIP(hipMemcpy2DAsync(dst, h->ldx * h->esz, src, h->g.nx * h->esz, h->g.nx * h->esz,
-                          h->g.nz, hipMemcpyHostToDevice, h->stream));
-  return 0;
+void touch(vi_sim_s* h) {
+  h->desc_dirty = true;
+  h->version++;
 }
-int download_plane(vi_sim_s* h, void* dst, const void* src) {
-  VI_HIP(hipMemcpy2DAsync(dst, h->g.nx * h->esz, src, h->ldx * h->esz, h->g.nx * h->esz,
-                          h->g.nz, hipMemcpyDeviceToHost, h->stream));
+
+// Make this handle's stream wait for batch work that touched its arrays.
+int wait_external(vi_sim_s* h) {
+  if (h->ext_pending) {
+    VI_HIP(hipStreamWaitEvent(h->stream, h->ext_ev, 0));
+    h->ext_pending = false;
+  }
   return 0;
 }
 
-template <typename R>
-vi::StepParams<R> make_params(vi_sim_s* h) {
-  vi::StepParams<R> p{};
-  p.nx = h->g.nx;
-  p.nz = h->g.nz;
-  p.ldx = h->ldx;
-  p.ntx = h->ntx;
-  p.ntiles = h->ntiles;
-  p.plane = h->plane;
-  p.dt = (R)h->g.dt;
-  double c[4];
-  vi_fd_coeffs(h->g.dx, c);
-  for (int l = 0; l < 4; ++l) p.Dx[l] = (R)c[l];
-  vi_fd_coeffs(h->g.dz, c);
-  for (int l = 0; l < 4; ++l) p.Dz[l] = (R)c[l];
-  p.vx = (R*)h->state[0].p;
-  p.vz = (R*)h->state[1].p;
-  p.sxx = (R*)h->state[2].p;
-  p.szz = (R*)h->state[3].p;
-  p.sxz = (R*)h->state[4].p;
-  p.M = (R*)h->M.p;
-  p.rho = (const R*)h->model[0].p;
-  p.lam = (const R*)h->model[1].p;
-  p.mu = (const R*)h->model[2].p;
-  p.tau = (const R*)h->model[3].p;
-  for (int k = 0; k < h->L; ++k) {
-    const R one = R(1), Lr = (R)h->L, tn = (R)h->taun[k];
-    p.a1[k] = -one / (Lr * tn);  // visco-kernels.jl:50
-    p.a2[k] = -one / tn;         // visco-kernels.jl:51
-  }
-  if (h->cpml_on) {
-    p.T = h->cpml.thickness;
-    p.xlo = h->cpml.left;
-    p.xhi = h->cpml.right;
-    p.zlo = h->cpml.top;
-    p.zhi = h->cpml.bottom;
-    R* cb = (R*)h->coef.p;
-    const int nx = h->g.nx, nz = h->g.nz;
-    p.ax = cb;
-    p.bx = cb + nx;
-    p.kx = cb + 2 * nx;
-    p.az = cb + 3 * nx;
-    p.bz = cb + 3 * nx + nz;
-    p.kz = cb + 3 * nx + 2 * nz;
-    for (int q = 0; q < 8; ++q) p.psi[q] = (R*)h->psi[q].p;
-  }
-  p.fs = h->fs;
-  const int* st = (const int*)h->src_tab.p;
-  p.src_off = st;
-  p.src_cell = st + (h->ntiles + 1);
-  p.src_kind = st + (h->ntiles + 1) + h->nsrc;
-  p.src_row = st + (h->ntiles + 1) + 2 * h->nsrc;
-  p.src_w = (const R*)h->src_w.p;
-  p.snt = h->snt;
-  const int* rt = (const int*)h->rec_tab.p;
-  p.rec_off = rt;
-  p.rec_cell = rt + (h->ntiles + 1);
-  p.rec_comp = rt + (h->ntiles + 1) + h->nrec;
-  p.rec_id = rt + (h->ntiles + 1) + 2 * h->nrec;
-  p.traces = (R*)h->traces.p;
-  p.nrec = h->nrec;
-  p.rec_nt = h->rec_nt;
-  p.step = h->step;
-  return p;
+int upload_plane(hipStream_t s, void* dst, size_t ld, const void* src, int nx, int nz,
+                 size_t esz) {
+  VI_HIP(hipMemcpy2DAsync(dst, ld * esz, src, nx * esz, nx * esz, nz, hipMemcpyHostToDevice,
+                          s));
+  return 0;
+}
+int download_plane(hipStream_t s, void* dst, const void* src, size_t ld, int nx, int nz,
+                   size_t esz) {
+  VI_HIP(hipMemcpy2DAsync(dst, nx * esz, src, ld * esz, nx * esz, nz, hipMemcpyDeviceToHost,
+                          s));
+  return 0;
 }
 
 int tile_of(const vi_sim_s* h, int ix, int iz) {
   return ((iz - 4) / (vi::NW * h->rz)) * h->ntx + (ix - 4) / vi::TX;
 }
 
-// Build a per-tile CSR table [off(ntiles+1) | cell(n) | a(n) | b(n)] and upload it.
-int upload_table(vi_sim_s* h, DevBuf& buf, int n, const int* ix, const int* iz,
-                 const int* a, const int* b) {
+// Per-tile CSR table [off(ntiles+1) | cell(n) | a(n) | original index(n)], uploaded.
+int upload_table(vi_sim_s* h, DevBuf& buf, int n, const int* ix, const int* iz, const int* a) {
   std::vector<int> order(n);
   for (int q = 0; q < n; ++q) order[q] = q;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -227,9 +271,9 @@ int upload_table(vi_sim_s* h, DevBuf& buf, int n, const int* ix, const int* iz,
     const int q = order[k];
     cell[k] = iz[q] * h->ldx + ix[q];
     ca[k] = a[q];
-    cb[k] = b ? b[q] : q;
+    cb[k] = q;
   }
-  if (int rc = buf.alloc(tab.size() * sizeof(int), h->stream)) return rc;
+  VI_TRY(buf.alloc(tab.size() * sizeof(int), h->stream));
   VI_HIP(hipMemcpyAsync(buf.p, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice,
                         h->stream));
   VI_HIP(hipStreamSynchronize(h->stream));
@@ -240,21 +284,148 @@ bool interior(const vi_sim_s* h, int ix, int iz) {
   return ix >= 4 && ix < h->g.nx - 4 && iz >= 4 && iz < h->g.nz - 4;
 }
 
+// Re-tile the interior for rows-per-wave rz and rebuild the per-tile tables.
+int apply_tiling(vi_sim_s* h, int rz) {
+  h->rz = rz;
+  h->ntx = (h->g.nx - 8 + vi::TX - 1) / vi::TX;
+  h->ntz = (h->g.nz - 8 + vi::NW * rz - 1) / (vi::NW * rz);
+  h->ntiles = h->ntx * h->ntz;
+  VI_TRY(upload_table(h, h->src_tab, h->nsrc, h->src_ix.data(), h->src_iz.data(),
+                      h->src_kind.data()));
+  VI_TRY(upload_table(h, h->rec_tab, h->nrec, h->rec_ix.data(), h->rec_iz.data(),
+                      h->rec_comp.data()));
+  touch(h);
+  return 0;
+}
+
+// Default rows per wave: the largest variant that still gives about 16 waves per CU.
+int default_rz(int nx, int nz) {
+  const long cols = (nx - 8 + vi::TX - 1) / vi::TX;
+  for (int rz : {8, 4, 2}) {
+    const long waves = cols * ((nz - 8 + rz - 1) / rz);
+    if (waves >= 256L * 16) return rz;
+  }
+  return 1;
+}
+
+template <typename R>
+void build_desc(const vi_sim_s* h, int tile_base, vi::ShotDesc<R>* d) {
+  std::memset((void*)d, 0, sizeof(*d));
+  d->nx = h->g.nx;
+  d->nz = h->g.nz;
+  d->ldx = h->ldx;
+  d->ldm = h->model->ldm;
+  d->ntx = h->ntx;
+  d->tile_base = tile_base;
+  d->plane = h->plane;
+  d->step = h->step;
+  d->dt = (R)h->g.dt;
+  double c[4];
+  vi_fd_coeffs(h->g.dx, c);
+  for (int l = 0; l < 4; ++l) d->Dx[l] = (R)c[l];
+  vi_fd_coeffs(h->g.dz, c);
+  for (int l = 0; l < 4; ++l) d->Dz[l] = (R)c[l];
+  for (int k = 0; k < h->L; ++k) {
+    const R one = R(1), Lr = (R)h->L, tn = (R)h->taun[k];
+    d->a1[k] = -one / (Lr * tn);  // visco-kernels.jl:50
+    d->a2[k] = -one / tn;         // visco-kernels.jl:51
+  }
+  d->vx = (R*)h->state[0].p;
+  d->vz = (R*)h->state[1].p;
+  d->sxx = (R*)h->state[2].p;
+  d->szz = (R*)h->state[3].p;
+  d->sxz = (R*)h->state[4].p;
+  d->M = (R*)h->M.p;
+  const ModelStore& m = *h->model;
+  d->rho = (const R*)m.arr[0].p + h->model_x0;
+  d->lam = (const R*)m.arr[1].p + h->model_x0;
+  d->mu = (const R*)m.arr[2].p + h->model_x0;
+  d->tau = m.arr[3].p ? (const R*)m.arr[3].p + h->model_x0 : nullptr;
+  if (h->cpml_on) {
+    d->T = h->cpml.thickness;
+    d->xlo = h->cpml.left;
+    d->xhi = h->cpml.right;
+    d->zlo = h->cpml.top;
+    d->zhi = h->cpml.bottom;
+    const R* cb = (const R*)h->coef.p;
+    const int nx = h->g.nx, nz = h->g.nz;
+    d->ax = cb;
+    d->bx = cb + nx;
+    d->kx = cb + 2 * nx;
+    d->az = cb + 3 * nx;
+    d->bz = cb + 3 * nx + nz;
+    d->kz = cb + 3 * nx + 2 * nz;
+    for (int q = 0; q < 8; ++q) d->psi[q] = (R*)h->psi[q].p;
+  }
+  d->fs = h->fs;
+  const int* st = (const int*)h->src_tab.p;
+  d->src_off = st;
+  d->src_cell = st + (h->ntiles + 1);
+  d->src_kind = st + (h->ntiles + 1) + h->nsrc;
+  d->src_row = st + (h->ntiles + 1) + 2 * h->nsrc;
+  d->src_w = (const R*)h->src_w.p;
+  d->snt = h->snt;
+  const int* rt = (const int*)h->rec_tab.p;
+  d->rec_off = rt;
+  d->rec_cell = rt + (h->ntiles + 1);
+  d->rec_comp = rt + (h->ntiles + 1) + h->nrec;
+  d->rec_id = rt + (h->ntiles + 1) + 2 * h->nrec;
+  d->traces = (R*)h->traces.p;
+  d->nrec = h->nrec;
+  d->rec_nt = h->rec_nt;
+}
+
+size_t desc_size(size_t esz) {
+  return esz == 4 ? sizeof(vi::ShotDesc<float>) : sizeof(vi::ShotDesc<double>);
+}
+
+void build_desc_bytes(const vi_sim_s* h, int tile_base, char* out) {
+  if (h->esz == 4) build_desc<float>(h, tile_base, (vi::ShotDesc<float>*)out);
+  else build_desc<double>(h, tile_base, (vi::ShotDesc<double>*)out);
+}
+
+int check_ready(const vi_sim_s* h) {
+  if (!h->model) return fail(VI_ESTATE, "vi_step: no model set (vi_set_model)");
+  if (h->L && !h->relax_set)
+    return fail(VI_ESTATE, "vi_step: vi_set_relaxation has not been called");
+  return 0;
+}
+
+// Upload the handle's own descriptor and tile->shot table if anything changed.
+int refresh_own_desc(vi_sim_s* h) {
+  if (!h->desc_dirty) return 0;
+  VI_HIP(hipStreamSynchronize(h->stream));
+  const size_t ds = desc_size(h->esz);
+  h->desc_host.assign(ds, 0);
+  build_desc_bytes(h, 0, h->desc_host.data());
+  VI_TRY(h->desc_dev.reserve(ds));
+  VI_HIP(hipMemcpyAsync(h->desc_dev.p, h->desc_host.data(), ds, hipMemcpyHostToDevice,
+                        h->stream));
+  const size_t tb = (size_t)h->ntiles * sizeof(int);
+  if (h->tile_shot_dev.bytes < tb || !h->tile_shot_dev.p)
+    VI_TRY(h->tile_shot_dev.alloc(tb, h->stream));  // zero = shot 0
+  VI_HIP(hipStreamSynchronize(h->stream));
+  h->desc_dirty = false;
+  return 0;
+}
+
 template <typename R, int RZ>
-int launch_kernel(vi_sim_s* h, int k, const vi::StepParams<R>& p) {
+int launch_kernel(hipStream_t s, int L, int k, const void* descs, const int* tile_shot,
+                  int ntiles, int kstep) {
   const dim3 block(vi::TX, vi::NW);
-  const dim3 grid(h->ntiles);
+  const dim3 grid(ntiles);
+  const vi::ShotDesc<R>* d = (const vi::ShotDesc<R>*)descs;
   if (k == 0) {
-    hipLaunchKernelGGL((vi::k_velocity<R, RZ>), grid, block, 0, h->stream, p);
+    hipLaunchKernelGGL((vi::k_velocity<R, RZ>), grid, block, 0, s, d, tile_shot, kstep);
   } else {
-    switch (h->L) {
-      case 0: hipLaunchKernelGGL((vi::k_stress<R, 0, RZ>), grid, block, 0, h->stream, p); break;
-      case 1: hipLaunchKernelGGL((vi::k_stress<R, 1, RZ>), grid, block, 0, h->stream, p); break;
-      case 2: hipLaunchKernelGGL((vi::k_stress<R, 2, RZ>), grid, block, 0, h->stream, p); break;
-      case 3: hipLaunchKernelGGL((vi::k_stress<R, 3, RZ>), grid, block, 0, h->stream, p); break;
-      case 4: hipLaunchKernelGGL((vi::k_stress<R, 4, RZ>), grid, block, 0, h->stream, p); break;
-      case 5: hipLaunchKernelGGL((vi::k_stress<R, 5, RZ>), grid, block, 0, h->stream, p); break;
-      case 6: hipLaunchKernelGGL((vi::k_stress<R, 6, RZ>), grid, block, 0, h->stream, p); break;
+    switch (L) {
+      case 0: hipLaunchKernelGGL((vi::k_stress<R, 0, RZ>), grid, block, 0, s, d, tile_shot, kstep); break;
+      case 1: hipLaunchKernelGGL((vi::k_stress<R, 1, RZ>), grid, block, 0, s, d, tile_shot, kstep); break;
+      case 2: hipLaunchKernelGGL((vi::k_stress<R, 2, RZ>), grid, block, 0, s, d, tile_shot, kstep); break;
+      case 3: hipLaunchKernelGGL((vi::k_stress<R, 3, RZ>), grid, block, 0, s, d, tile_shot, kstep); break;
+      case 4: hipLaunchKernelGGL((vi::k_stress<R, 4, RZ>), grid, block, 0, s, d, tile_shot, kstep); break;
+      case 5: hipLaunchKernelGGL((vi::k_stress<R, 5, RZ>), grid, block, 0, s, d, tile_shot, kstep); break;
+      case 6: hipLaunchKernelGGL((vi::k_stress<R, 6, RZ>), grid, block, 0, s, d, tile_shot, kstep); break;
       default: return fail(VI_EINVAL, "nrelax out of range");
     }
   }
@@ -263,74 +434,51 @@ int launch_kernel(vi_sim_s* h, int k, const vi::StepParams<R>& p) {
 }
 
 template <typename R>
-int launch_step(vi_sim_s* h) {
-  vi::StepParams<R> p = make_params<R>(h);
-  for (int k = 0; k < 2; ++k) {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (h->prof) {
-      if (h->ev_used + 2 > h->ev.size()) {
-        for (int q = 0; q < 2; ++q) {
-          hipEvent_t e;
-          VI_HIP(hipEventCreate(&e));
-          h->ev.push_back(e);
-          h->ev_kind.push_back(0);
-        }
-      }
-      e0 = h->ev[h->ev_used];
-      e1 = h->ev[h->ev_used + 1];
-      h->ev_kind[h->ev_used] = k;
-      h->ev_used += 2;
-      VI_HIP(hipEventRecord(e0, h->stream));
+int launch_one(hipStream_t s, int rz, int L, int k, const void* descs, const int* tile_shot,
+               int ntiles, int kstep) {
+  switch (rz) {
+    case 1: return launch_kernel<R, 1>(s, L, k, descs, tile_shot, ntiles, kstep);
+    case 2: return launch_kernel<R, 2>(s, L, k, descs, tile_shot, ntiles, kstep);
+    case 4: return launch_kernel<R, 4>(s, L, k, descs, tile_shot, ntiles, kstep);
+    case 8: return launch_kernel<R, 8>(s, L, k, descs, tile_shot, ntiles, kstep);
+  }
+  return fail(VI_EINVAL, "bad tiling");
+}
+
+// nsteps full steps of one group of shots (same precision and L), then advance their step
+// indices on the device.
+int launch_steps(hipStream_t s, size_t esz, int rz, int L, void* descs, const int* tile_shot,
+                 int ntiles, int nshots, int nsteps, Prof* prof) {
+  for (int q = 0; q < nsteps; ++q) {
+    for (int k = 0; k < 2; ++k) {
+      hipEvent_t e1 = nullptr;
+      if (prof && prof->on) VI_TRY(prof->begin(s, k, &e1));
+      VI_TRY(esz == 4 ? launch_one<float>(s, rz, L, k, descs, tile_shot, ntiles, q)
+                      : launch_one<double>(s, rz, L, k, descs, tile_shot, ntiles, q));
+      if (e1) VI_HIP(hipEventRecord(e1, s));
     }
-    int rc = 0;
-    switch (h->rz) {
-      case 1: rc = launch_kernel<R, 1>(h, k, p); break;
-      case 2: rc = launch_kernel<R, 2>(h, k, p); break;
-      case 4: rc = launch_kernel<R, 4>(h, k, p); break;
-      case 8: rc = launch_kernel<R, 8>(h, k, p); break;
-      default: return fail(VI_EINVAL, "bad tiling");
-    }
-    if (rc) return rc;
-    if (h->prof) VI_HIP(hipEventRecord(e1, h->stream));
+  }
+  if (nsteps > 0) {
+    const int bs = 64, nb = (nshots + bs - 1) / bs;
+    if (esz == 4)
+      hipLaunchKernelGGL(vi::k_advance<float>, dim3(nb), dim3(bs), 0, s,
+                         (vi::ShotDesc<float>*)descs, nshots, nsteps);
+    else
+      hipLaunchKernelGGL(vi::k_advance<double>, dim3(nb), dim3(bs), 0, s,
+                         (vi::ShotDesc<double>*)descs, nshots, nsteps);
+    VI_HIP(hipGetLastError());
   }
   return 0;
 }
 
-// Re-tile the interior for rows-per-wave rz and rebuild the per-tile tables.
-int apply_tiling(vi_sim_s* h, int rz) {
-  h->rz = rz;
-  h->ntx = (h->g.nx - 8 + vi::TX - 1) / vi::TX;
-  h->ntz = (h->g.nz - 8 + vi::NW * rz - 1) / (vi::NW * rz);
-  h->ntiles = h->ntx * h->ntz;
-  if (int rc = upload_table(h, h->src_tab, h->nsrc, h->src_ix.data(), h->src_iz.data(),
-                            h->src_kind.data(), nullptr))
-    return rc;
-  if (int rc = upload_table(h, h->rec_tab, h->nrec, h->rec_ix.data(), h->rec_iz.data(),
-                            h->rec_comp.data(), nullptr))
-    return rc;
-  return 0;
-}
-
-// Default rows per wave: the largest variant that still gives about 16 waves per CU.
-int default_rz(const vi_grid_desc& g) {
-  const long cols = (g.nx - 8 + vi::TX - 1) / vi::TX;
-  for (int rz : {8, 4, 2}) {
-    const long waves = cols * ((g.nz - 8 + rz - 1) / rz);
-    if (waves >= 256L * 16) return rz;
-  }
-  return 1;
-}
-
-int collect_profile(vi_sim_s* h) {
-  if (!h->ev_used) return 0;
-  VI_HIP(hipStreamSynchronize(h->stream));
-  for (size_t q = 0; q + 1 < h->ev_used; q += 2) {
-    float ms = 0.f;
-    VI_HIP(hipEventElapsedTime(&ms, h->ev[q], h->ev[q + 1]));
-    h->prof_ms[h->ev_kind[q]] += ms;
-    h->prof_n[h->ev_kind[q]] += 1;
-  }
-  h->ev_used = 0;
+int step_single(vi_sim_s* h, int nsteps) {
+  VI_TRY(check_ready(h));
+  VI_TRY(wait_external(h));
+  VI_TRY(refresh_own_desc(h));
+  VI_TRY(launch_steps(h->stream, h->esz, h->rz, h->L, h->desc_dev.p,
+                      (const int*)h->tile_shot_dev.p, h->ntiles, 1, nsteps, &h->prof));
+  h->step += nsteps;
+  h->own_pending = true;
   return 0;
 }
 
@@ -346,12 +494,135 @@ size_t field_count(const vi_sim_s* h, int field) {
   return 0;
 }
 
+std::vector<DevBuf*> mutable_arrays(vi_sim_s* h) {
+  std::vector<DevBuf*> v;
+  for (auto& b : h->state) v.push_back(&b);
+  v.push_back(&h->M);
+  for (auto& b : h->psi) v.push_back(&b);
+  v.push_back(&h->traces);
+  return v;
+}
+
+int make_model_store(int nx, int nz, size_t esz, bool with_tau, const void* const src[4],
+                     hipStream_t s, std::shared_ptr<ModelStore>* out) {
+  auto m = std::make_shared<ModelStore>();
+  m->nx = nx;
+  m->nz = nz;
+  m->esz = esz;
+  m->ldm = (int)round_up((size_t)nx + 72, 64);
+  m->nzp = nz + vi::NW * vi::RZ_MAX + 16;
+  m->has_tau = with_tau;
+  const size_t pb = (size_t)m->ldm * m->nzp * esz;
+  for (int q = 0; q < (with_tau ? 4 : 3); ++q) {
+    VI_TRY(m->arr[q].alloc(pb, s));
+    VI_TRY(upload_plane(s, m->arr[q].p, m->ldm, src[q], nx, nz, esz));
+  }
+  VI_HIP(hipStreamSynchronize(s));
+  *out = m;
+  return 0;
+}
+
+// ---------------------------------------------------------------- batches
+void invalidate_graph(vi_batch_s* b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (b->graph_def) (void)hipGraphDestroy(b->graph_def);
+  b->graph = nullptr;
+  b->graph_def = nullptr;
+}
+
+int batch_build(vi_batch_s* b) {
+  invalidate_graph(b);
+  b->groups.clear();
+  std::map<std::pair<int, int>, size_t> gi;
+  for (auto* h : b->members) {
+    VI_TRY(check_ready(h));
+    if (h->rz != b->rz) VI_TRY(apply_tiling(h, b->rz));
+    auto key = std::make_pair((int)h->esz, h->L);
+    if (!gi.count(key)) {
+      gi[key] = b->groups.size();
+      b->groups.emplace_back(new BatchGroup());
+      b->groups.back()->esz = (int)h->esz;
+      b->groups.back()->L = h->L;
+    }
+    b->groups[gi[key]]->members.push_back(h);
+  }
+  for (auto& gp : b->groups) {
+    BatchGroup& g = *gp;
+    const size_t ds = desc_size(g.esz);
+    g.desc_host.assign(ds * g.members.size(), 0);
+    std::vector<int> ts;
+    int base = 0;
+    for (size_t q = 0; q < g.members.size(); ++q) {
+      build_desc_bytes(g.members[q], base, g.desc_host.data() + q * ds);
+      ts.insert(ts.end(), g.members[q]->ntiles, (int)q);
+      base += g.members[q]->ntiles;
+    }
+    g.ntiles = base;
+    VI_TRY(g.desc_dev.reserve(g.desc_host.size()));
+    VI_TRY(g.tile_shot_dev.reserve(ts.size() * sizeof(int)));
+    VI_HIP(hipMemcpyAsync(g.desc_dev.p, g.desc_host.data(), g.desc_host.size(),
+                          hipMemcpyHostToDevice, b->stream));
+    VI_HIP(hipMemcpyAsync(g.tile_shot_dev.p, ts.data(), ts.size() * sizeof(int),
+                          hipMemcpyHostToDevice, b->stream));
+    VI_HIP(hipStreamSynchronize(b->stream));
+  }
+  b->versions.resize(b->members.size());
+  for (size_t q = 0; q < b->members.size(); ++q) b->versions[q] = b->members[q]->version;
+  b->built = true;
+  return 0;
+}
+
+int batch_launch(vi_batch_s* b, int nsteps) {
+  for (auto& gp : b->groups) {
+    BatchGroup& g = *gp;
+    VI_TRY(launch_steps(b->stream, g.esz, b->rz, g.L, g.desc_dev.p,
+                        (const int*)g.tile_shot_dev.p, g.ntiles, (int)g.members.size(), nsteps,
+                        &b->prof));
+  }
+  return 0;
+}
+
+int batch_prepare(vi_batch_s* b) {
+  bool stale = !b->built;
+  for (size_t q = 0; !stale && q < b->members.size(); ++q)
+    stale = b->versions[q] != b->members[q]->version;
+  if (stale) {
+    for (auto* h : b->members) {
+      VI_TRY(wait_external(h));
+      VI_HIP(hipStreamSynchronize(h->stream));
+      h->own_pending = false;
+    }
+    VI_HIP(hipStreamSynchronize(b->stream));
+    VI_TRY(batch_build(b));
+  }
+  // order after work queued on the members' own streams
+  for (auto* h : b->members) {
+    if (h->own_pending) {
+      VI_HIP(hipEventRecord(h->own_ev, h->stream));
+      VI_HIP(hipStreamWaitEvent(b->stream, h->own_ev, 0));
+      h->own_pending = false;
+    }
+  }
+  return 0;
+}
+
+int batch_finish(vi_batch_s* b, int nsteps) {
+  for (size_t q = 0; q < b->members.size(); ++q) {
+    vi_sim_s* h = b->members[q];
+    h->step += nsteps;
+    h->desc_dirty = true;  // its own descriptor holds the old step
+    VI_HIP(hipEventRecord(h->ext_ev, b->stream));
+    h->ext_pending = true;
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
 
 const char* vi_last_error(void) { return g_err.c_str(); }
-const char* vi_version(void) { return "vi_mi355x 0.1 (gfx950)"; }
+const char* vi_version(void) { return "vi_mi355x 0.2 (gfx950)"; }
 
 void vi_fd_coeffs(double h, double out[4]) {
   // _stencil(x, 0, 1) (Stencil.jl:9-14) at x = +-1/2 .. +-7/2, folded to the half stencil
@@ -383,6 +654,38 @@ int vi_device_count(void) {
   return n;
 }
 
+int vi_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return fail(VI_EINVAL, "vi_mem_info: null argument");
+  VI_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return 0;
+}
+
+// ---------------------------------------------------------------- models
+int vi_model_create(int nx, int nz, int precision, const void* rho, const void* lam,
+                    const void* mu, const void* tau, vi_model* out) {
+  if (!out || !rho || !lam || !mu) return fail(VI_EINVAL, "vi_model_create: null argument");
+  *out = nullptr;
+  if (nx < 16 || nz < 16) return fail(VI_EINVAL, "vi_model_create: nx and nz must be >= 16");
+  if (precision != VI_F32 && precision != VI_F64)
+    return fail(VI_EINVAL, "vi_model_create: bad precision");
+  if (vi_device_count() < 1) return fail(VI_EDEVICE, "vi_model_create: no HIP device visible");
+  hipStream_t s;
+  VI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  const void* src[4] = {rho, lam, mu, tau};
+  std::shared_ptr<ModelStore> m;
+  int rc = make_model_store(nx, nz, precision == VI_F32 ? 4 : 8, tau != nullptr, src, s, &m);
+  (void)hipStreamDestroy(s);
+  if (rc) return rc;
+  *out = new vi_model_s{m};
+  return 0;
+}
+
+int vi_model_destroy(vi_model m) {
+  delete m;  // handles that use the model keep its arrays alive
+  return 0;
+}
+
+// ---------------------------------------------------------------- single shot
 int vi_create(const vi_grid_desc* g, vi_handle* out) {
   if (!g || !out) return fail(VI_EINVAL, "vi_create: null argument");
   *out = nullptr;
@@ -395,27 +698,22 @@ int vi_create(const vi_grid_desc* g, vi_handle* out) {
     return fail(VI_EINVAL, "vi_create: only 8th-order staggered operators are implemented");
   if (!(g->dx > 0) || !(g->dz > 0) || !(g->dt > 0))
     return fail(VI_EINVAL, "vi_create: dx, dz, dt must be positive");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-    return fail(VI_EDEVICE, "vi_create: no HIP device visible");
+  if (vi_device_count() < 1) return fail(VI_EDEVICE, "vi_create: no HIP device visible");
   std::unique_ptr<vi_sim_s> h(new vi_sim_s());
   h->g = *g;
   h->L = g->nrelax;
   h->esz = g->precision == VI_F32 ? 4 : 8;
   VI_HIP(hipGetDevice(&h->device));
   VI_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  VI_HIP(hipEventCreateWithFlags(&h->own_ev, hipEventDisableTiming));
+  VI_HIP(hipEventCreateWithFlags(&h->ext_ev, hipEventDisableTiming));
   h->ldx = (int)round_up((size_t)g->nx + 72, 64);
   h->nzp = g->nz + vi::NW * vi::RZ_MAX + 16;
   h->plane = (size_t)h->ldx * h->nzp;
   const size_t pb = h->plane * h->esz;
-  for (auto& b : h->state)
-    if (int rc = b.alloc(pb, h->stream)) return rc;
-  if (h->L)
-    if (int rc = h->M.alloc(3 * (size_t)h->L * pb, h->stream)) return rc;
-  for (int q = 0; q < 4; ++q)
-    if (int rc = h->model[q].alloc(pb, h->stream)) return rc;
-  // tiling and the (empty) source / receiver tables
-  if (int rc = apply_tiling(h.get(), default_rz(*g))) return rc;
+  for (auto& b : h->state) VI_TRY(b.alloc(pb, h->stream));
+  if (h->L) VI_TRY(h->M.alloc(3 * (size_t)h->L * pb, h->stream));
+  VI_TRY(apply_tiling(h.get(), default_rz(g->nx, g->nz)));
   VI_HIP(hipStreamSynchronize(h->stream));
   *out = h.release();
   return 0;
@@ -423,44 +721,68 @@ int vi_create(const vi_grid_desc* g, vi_handle* out) {
 
 int vi_destroy(vi_handle h) {
   if (!h) return 0;
+  (void)wait_external(h);
   (void)hipStreamSynchronize(h->stream);
-  for (auto e : h->ev) (void)hipEventDestroy(e);
   hipStream_t s = h->stream;
+  hipEvent_t e0 = h->own_ev, e1 = h->ext_ev;
   delete h;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   if (s) (void)hipStreamDestroy(s);
   return 0;
 }
 
 int vi_set_model(vi_handle h, const void* rho, const void* lam, const void* mu,
                  const void* tau) {
-  if (int rc = check_handle(h)) return rc;
+  VI_TRY(check_handle(h));
   if (!rho || !lam || !mu) return fail(VI_EINVAL, "vi_set_model: rho, lam, mu required");
   if (h->L && !tau) return fail(VI_EINVAL, "vi_set_model: tau required when nrelax > 0");
-  const void* src[4] = {rho, lam, mu, tau};
-  for (int q = 0; q < 4; ++q)
-    if (src[q])
-      if (int rc = upload_plane(h, h->model[q].p, src[q])) return rc;
+  VI_TRY(wait_external(h));
+  const void* src[4] = {rho, lam, mu, h->L ? tau : nullptr};
   VI_HIP(hipStreamSynchronize(h->stream));
-  h->model_set = true;
+  std::shared_ptr<ModelStore> m;
+  VI_TRY(make_model_store(h->g.nx, h->g.nz, h->esz, h->L > 0, src, h->stream, &m));
+  h->model = m;
+  h->model_x0 = 0;
+  touch(h);
+  return 0;
+}
+
+int vi_set_model_window(vi_handle h, vi_model m, int x0) {
+  VI_TRY(check_handle(h));
+  if (!m || !m->m) return fail(VI_EINVAL, "vi_set_model_window: null model");
+  const ModelStore& s = *m->m;
+  if (s.esz != h->esz) return fail(VI_EINVAL, "vi_set_model_window: precision mismatch");
+  if (s.nz != h->g.nz) return fail(VI_EINVAL, "vi_set_model_window: nz mismatch");
+  if (x0 < 0 || x0 + h->g.nx > s.nx)
+    return fail(VI_EINVAL, "vi_set_model_window: window [x0, x0+nx) outside the model");
+  if (h->L && !s.has_tau) return fail(VI_EINVAL, "vi_set_model_window: model has no tau");
+  VI_TRY(wait_external(h));
+  VI_HIP(hipStreamSynchronize(h->stream));
+  h->model = m->m;
+  h->model_x0 = x0;
+  touch(h);
   return 0;
 }
 
 int vi_set_relaxation(vi_handle h, const double* taun) {
-  if (int rc = check_handle(h)) return rc;
+  VI_TRY(check_handle(h));
   if (h->L == 0) return 0;
   if (!taun) return fail(VI_EINVAL, "vi_set_relaxation: null taun");
-  for (int k = 0; k < h->L; ++k) {
+  for (int k = 0; k < h->L; ++k)
     if (!(taun[k] > 0)) return fail(VI_EINVAL, "vi_set_relaxation: taun must be positive");
-    h->taun[k] = taun[k];
-  }
+  for (int k = 0; k < h->L; ++k) h->taun[k] = taun[k];
   h->relax_set = true;
+  touch(h);
   return 0;
 }
 
 int vi_set_cpml(vi_handle h, const vi_cpml_desc* c) {
-  if (int rc = check_handle(h)) return rc;
+  VI_TRY(check_handle(h));
+  VI_TRY(wait_external(h));
   if (!c) {
     h->cpml_on = false;
+    touch(h);
     return 0;
   }
   if (c->thickness < 1 || 2 * c->thickness > h->g.nx || 2 * c->thickness > h->g.nz)
@@ -488,31 +810,32 @@ int vi_set_cpml(vi_handle h, const vi_cpml_desc* c) {
   put(3 * nx, a, nz);
   put(3 * nx + nz, b, nz);
   put(3 * nx + 2 * nz, k, nz);
-  if (int rc = h->coef.alloc(hostc.size(), h->stream)) return rc;
+  VI_HIP(hipStreamSynchronize(h->stream));
+  VI_TRY(h->coef.alloc(hostc.size(), h->stream));
   VI_HIP(hipMemcpyAsync(h->coef.p, hostc.data(), hostc.size(), hipMemcpyHostToDevice,
                         h->stream));
-  for (int q = 0; q < 4; ++q)
-    if (int rc = h->psi[q].alloc((size_t)nz * 2 * T * h->esz, h->stream)) return rc;
-  for (int q = 4; q < 8; ++q)
-    if (int rc = h->psi[q].alloc((size_t)2 * T * h->ldx * h->esz, h->stream)) return rc;
+  for (int q = 0; q < 4; ++q) VI_TRY(h->psi[q].alloc((size_t)nz * 2 * T * h->esz, h->stream));
+  for (int q = 4; q < 8; ++q) VI_TRY(h->psi[q].alloc((size_t)2 * T * h->ldx * h->esz, h->stream));
   VI_HIP(hipStreamSynchronize(h->stream));
   h->cpml = *c;
   h->cpml_on = true;
+  touch(h);
   return 0;
 }
 
 int vi_set_free_surface(vi_handle h, int enabled) {
-  if (int rc = check_handle(h)) return rc;
+  VI_TRY(check_handle(h));
   if (enabled && h->cpml_on && h->cpml.top)
     return fail(VI_EINVAL, "vi_set_free_surface: conflicts with a top CPML strip");
   if (enabled && h->g.nz < 17) return fail(VI_EINVAL, "vi_set_free_surface: nz must be >= 17");
   h->fs = enabled ? 1 : 0;
+  touch(h);
   return 0;
 }
 
 int vi_set_sources(vi_handle h, int n, const int* ix, const int* iz, const int* kind, int nt,
                    const void* wavelets) {
-  if (int rc = check_handle(h)) return rc;
+  VI_TRY(check_handle(h));
   if (n < 0 || nt < 0 || (n > 0 && (!ix || !iz || !kind || (nt > 0 && !wavelets))))
     return fail(VI_EINVAL, "vi_set_sources: bad arguments");
   for (int q = 0; q < n; ++q) {
@@ -521,24 +844,26 @@ int vi_set_sources(vi_handle h, int n, const int* ix, const int* iz, const int* 
                                  " is outside the interior [4, nx-4) x [4, nz-4)");
     if (kind[q] < 0 || kind[q] > 2) return fail(VI_EINVAL, "vi_set_sources: bad kind");
   }
+  VI_TRY(wait_external(h));
+  VI_HIP(hipStreamSynchronize(h->stream));
   h->src_ix.assign(ix, ix + n);
   h->src_iz.assign(iz, iz + n);
   h->src_kind.assign(kind, kind + n);
   h->nsrc = n;
-  if (int rc = upload_table(h, h->src_tab, n, ix, iz, kind, nullptr)) return rc;
-  if (int rc = h->src_w.alloc((size_t)n * nt * h->esz, h->stream)) return rc;
+  VI_TRY(upload_table(h, h->src_tab, n, ix, iz, kind));
+  VI_TRY(h->src_w.alloc((size_t)n * nt * h->esz, h->stream));
   if (n && nt)
     VI_HIP(hipMemcpyAsync(h->src_w.p, wavelets, (size_t)n * nt * h->esz,
                           hipMemcpyHostToDevice, h->stream));
   VI_HIP(hipStreamSynchronize(h->stream));
-  h->nsrc = n;
   h->snt = nt;
+  touch(h);
   return 0;
 }
 
 int vi_set_receivers(vi_handle h, int n, const int* ix, const int* iz, const int* comp,
                      int nt_max) {
-  if (int rc = check_handle(h)) return rc;
+  VI_TRY(check_handle(h));
   if (n < 0 || nt_max < 0 || (n > 0 && (!ix || !iz || !comp)))
     return fail(VI_EINVAL, "vi_set_receivers: bad arguments");
   for (int q = 0; q < n; ++q) {
@@ -547,145 +872,227 @@ int vi_set_receivers(vi_handle h, int n, const int* ix, const int* iz, const int
                                  " is outside the interior [4, nx-4) x [4, nz-4)");
     if (comp[q] < 0 || comp[q] > 4) return fail(VI_EINVAL, "vi_set_receivers: bad component");
   }
+  VI_TRY(wait_external(h));
+  VI_HIP(hipStreamSynchronize(h->stream));
   h->rec_ix.assign(ix, ix + n);
   h->rec_iz.assign(iz, iz + n);
   h->rec_comp.assign(comp, comp + n);
   h->nrec = n;
-  if (int rc = upload_table(h, h->rec_tab, n, ix, iz, comp, nullptr)) return rc;
-  if (int rc = h->traces.alloc((size_t)n * nt_max * h->esz, h->stream)) return rc;
+  VI_TRY(upload_table(h, h->rec_tab, n, ix, iz, comp));
+  VI_TRY(h->traces.alloc((size_t)n * nt_max * h->esz, h->stream));
   VI_HIP(hipStreamSynchronize(h->stream));
-  h->nrec = n;
   h->rec_nt = nt_max;
+  touch(h);
   return 0;
 }
 
 int vi_set_state(vi_handle h, int field, const void* host) {
-  if (int rc = check_handle(h)) return rc;
+  VI_TRY(check_handle(h));
   if (!host) return fail(VI_EINVAL, "vi_set_state: null buffer");
   const size_t n = field_count(h, field);
   if (!n) return fail(VI_EINVAL, "vi_set_state: field not present in this configuration");
+  VI_TRY(wait_external(h));
   if (field <= VI_SXZ) {
-    if (int rc = upload_plane(h, h->state[field].p, host)) return rc;
+    VI_TRY(upload_plane(h->stream, h->state[field].p, h->ldx, host, h->g.nx, h->g.nz, h->esz));
   } else if (field <= VI_MXZ) {
     const size_t nc = (size_t)h->g.nx * h->g.nz;
     for (int k = 0; k < h->L; ++k) {
       char* dst = (char*)h->M.p + ((size_t)(field - VI_MXX) * h->L + k) * h->plane * h->esz;
-      if (int rc = upload_plane(h, dst, (const char*)host + k * nc * h->esz)) return rc;
+      VI_TRY(upload_plane(h->stream, dst, h->ldx, (const char*)host + k * nc * h->esz, h->g.nx,
+                          h->g.nz, h->esz));
     }
   } else {
     const int q = field - VI_PSI_VX_X;
     if (q < 4) {
       VI_HIP(hipMemcpyAsync(h->psi[q].p, host, n * h->esz, hipMemcpyHostToDevice, h->stream));
     } else {
-      const int T2 = 2 * h->cpml.thickness;
-      VI_HIP(hipMemcpy2DAsync(h->psi[q].p, h->ldx * h->esz, host, h->g.nx * h->esz,
-                              h->g.nx * h->esz, T2, hipMemcpyHostToDevice, h->stream));
+      VI_TRY(upload_plane(h->stream, h->psi[q].p, h->ldx, host, h->g.nx,
+                          2 * h->cpml.thickness, h->esz));
     }
   }
   VI_HIP(hipStreamSynchronize(h->stream));
+  h->own_pending = false;
   return 0;
 }
 
 int vi_get_state(vi_handle h, int field, void* host) {
-  if (int rc = check_handle(h)) return rc;
+  VI_TRY(check_handle(h));
   if (!host) return fail(VI_EINVAL, "vi_get_state: null buffer");
   const size_t n = field_count(h, field);
   if (!n) return fail(VI_EINVAL, "vi_get_state: field not present in this configuration");
+  VI_TRY(wait_external(h));
   if (field <= VI_SXZ) {
-    if (int rc = download_plane(h, host, h->state[field].p)) return rc;
+    VI_TRY(download_plane(h->stream, host, h->state[field].p, h->ldx, h->g.nx, h->g.nz, h->esz));
   } else if (field <= VI_MXZ) {
     const size_t nc = (size_t)h->g.nx * h->g.nz;
     for (int k = 0; k < h->L; ++k) {
       const char* src =
           (const char*)h->M.p + ((size_t)(field - VI_MXX) * h->L + k) * h->plane * h->esz;
-      if (int rc = download_plane(h, (char*)host + k * nc * h->esz, src)) return rc;
+      VI_TRY(download_plane(h->stream, (char*)host + k * nc * h->esz, src, h->ldx, h->g.nx,
+                            h->g.nz, h->esz));
     }
   } else {
     const int q = field - VI_PSI_VX_X;
     if (q < 4) {
       VI_HIP(hipMemcpyAsync(host, h->psi[q].p, n * h->esz, hipMemcpyDeviceToHost, h->stream));
     } else {
-      const int T2 = 2 * h->cpml.thickness;
-      VI_HIP(hipMemcpy2DAsync(host, h->g.nx * h->esz, h->psi[q].p, h->ldx * h->esz,
-                              h->g.nx * h->esz, T2, hipMemcpyDeviceToHost, h->stream));
+      VI_TRY(download_plane(h->stream, host, h->psi[q].p, h->ldx, h->g.nx,
+                            2 * h->cpml.thickness, h->esz));
     }
   }
   VI_HIP(hipStreamSynchronize(h->stream));
+  h->own_pending = false;
   return 0;
 }
 
 int vi_reset(vi_handle h) {
-  if (int rc = check_handle(h)) return rc;
-  for (auto& b : h->state)
-    if (b.p) VI_HIP(hipMemsetAsync(b.p, 0, b.bytes, h->stream));
-  if (h->M.p) VI_HIP(hipMemsetAsync(h->M.p, 0, h->M.bytes, h->stream));
-  for (auto& b : h->psi)
-    if (b.p) VI_HIP(hipMemsetAsync(b.p, 0, b.bytes, h->stream));
-  if (h->traces.p) VI_HIP(hipMemsetAsync(h->traces.p, 0, h->traces.bytes, h->stream));
+  VI_TRY(check_handle(h));
+  VI_TRY(wait_external(h));
+  for (auto* b : mutable_arrays(h))
+    if (b->p) VI_HIP(hipMemsetAsync(b->p, 0, b->bytes, h->stream));
   VI_HIP(hipStreamSynchronize(h->stream));
+  h->own_pending = false;
   h->step = 0;
+  touch(h);
   return 0;
 }
 
 int vi_step(vi_handle h, int nsteps) {
-  if (int rc = check_handle(h)) return rc;
+  VI_TRY(check_handle(h));
   if (nsteps < 0) return fail(VI_EINVAL, "vi_step: nsteps < 0");
-  if (!h->model_set) return fail(VI_ESTATE, "vi_step: vi_set_model has not been called");
-  if (h->L && !h->relax_set)
-    return fail(VI_ESTATE, "vi_step: vi_set_relaxation has not been called");
-  for (int k = 0; k < nsteps; ++k) {
-    int rc = h->esz == 4 ? launch_step<float>(h) : launch_step<double>(h);
-    if (rc) return rc;
-    h->step++;
-  }
-  return 0;
+  if (nsteps == 0) return check_ready(h);
+  return step_single(h, nsteps);
 }
 
 int vi_sync(vi_handle h) {
-  if (int rc = check_handle(h)) return rc;
+  VI_TRY(check_handle(h));
+  VI_TRY(wait_external(h));
   VI_HIP(hipStreamSynchronize(h->stream));
+  h->own_pending = false;
   return 0;
 }
 
 long vi_step_index(vi_handle h) { return h ? h->step : -1; }
 
 int vi_get_traces(vi_handle h, void* out, int nt) {
-  if (int rc = check_handle(h)) return rc;
+  VI_TRY(check_handle(h));
   if (nt < 0 || nt > h->rec_nt || (!out && nt && h->nrec))
     return fail(VI_EINVAL, "vi_get_traces: nt exceeds the recorded capacity");
   if (!h->nrec || !nt) return 0;
+  VI_TRY(wait_external(h));
   std::vector<char> tmp((size_t)nt * h->nrec * h->esz);
   VI_HIP(hipMemcpyAsync(tmp.data(), h->traces.p, tmp.size(), hipMemcpyDeviceToHost, h->stream));
   VI_HIP(hipStreamSynchronize(h->stream));
-  // device layout [t][rec] -> host [rec][t]
-  for (int r = 0; r < h->nrec; ++r)
+  h->own_pending = false;
+  for (int r = 0; r < h->nrec; ++r)  // device [t][rec] -> host [rec][t]
     for (int t = 0; t < nt; ++t)
       std::memcpy((char*)out + ((size_t)r * nt + t) * h->esz,
                   tmp.data() + ((size_t)t * h->nrec + r) * h->esz, h->esz);
   return 0;
 }
 
+int vi_device_bytes(vi_handle h, size_t* bytes) {
+  VI_TRY(check_handle(h));
+  if (!bytes) return fail(VI_EINVAL, "vi_device_bytes: null argument");
+  size_t t = 0;
+  for (auto* b : mutable_arrays(h)) t += b->bytes;
+  t += h->coef.bytes + h->src_tab.bytes + h->src_w.bytes + h->rec_tab.bytes;
+  if (h->model && h->model.use_count() == 1)
+    for (auto& b : h->model->arr) t += b.bytes;
+  *bytes = t;
+  return 0;
+}
+
+int vi_set_tiling(vi_handle h, int rz) {
+  VI_TRY(check_handle(h));
+  if (rz != 1 && rz != 2 && rz != 4 && rz != 8)
+    return fail(VI_EINVAL, "vi_set_tiling: rows per wavefront must be 1, 2, 4 or 8");
+  VI_TRY(wait_external(h));
+  VI_HIP(hipStreamSynchronize(h->stream));
+  return apply_tiling(h, rz);
+}
+
+int vi_get_tiling(vi_handle h) { return h ? h->rz : -1; }
+
+int vi_autotune(vi_handle h, int trial_steps, int* chosen, double* us_per_step) {
+  VI_TRY(check_handle(h));
+  if (trial_steps < 1) return fail(VI_EINVAL, "vi_autotune: trial_steps must be >= 1");
+  VI_TRY(check_ready(h));
+  VI_TRY(wait_external(h));
+  VI_HIP(hipStreamSynchronize(h->stream));
+  // snapshot every mutable array so that the trials leave the state untouched
+  std::vector<DevBuf*> live = mutable_arrays(h);
+  std::vector<DevBuf> snap(live.size());
+  for (size_t q = 0; q < live.size(); ++q) {
+    if (!live[q]->p) continue;
+    VI_TRY(snap[q].alloc(live[q]->bytes, h->stream));
+    VI_HIP(hipMemcpyAsync(snap[q].p, live[q]->p, live[q]->bytes, hipMemcpyDeviceToDevice,
+                          h->stream));
+  }
+  const long step0 = h->step;
+  const bool prof0 = h->prof.on;
+  h->prof.on = false;
+  hipEvent_t e0, e1;
+  VI_HIP(hipEventCreate(&e0));
+  VI_HIP(hipEventCreate(&e1));
+  const int cands[4] = {1, 2, 4, 8};
+  int best = h->rz;
+  double best_us = 1e300;
+  int rc = 0;
+  for (int c = 0; c < 4 && !rc; ++c) {
+    if ((rc = apply_tiling(h, cands[c]))) break;
+    if ((rc = step_single(h, 2))) break;
+    if ((rc = hipEventRecord(e0, h->stream) == hipSuccess ? 0 : VI_EDEVICE)) break;
+    if ((rc = step_single(h, trial_steps))) break;
+    if ((rc = hipEventRecord(e1, h->stream) == hipSuccess ? 0 : VI_EDEVICE)) break;
+    if ((rc = hipEventSynchronize(e1) == hipSuccess ? 0 : VI_EDEVICE)) break;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const double us = 1e3 * ms / trial_steps;
+    if (us_per_step) us_per_step[c] = us;
+    if (us < best_us) {
+      best_us = us;
+      best = cands[c];
+    }
+    for (size_t q = 0; q < live.size(); ++q)
+      if (live[q]->p && hipMemcpyAsync(live[q]->p, snap[q].p, live[q]->bytes,
+                                       hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+        rc = VI_EDEVICE;
+    h->step = step0;
+    touch(h);
+  }
+  (void)hipStreamSynchronize(h->stream);
+  h->step = step0;
+  h->prof.on = prof0;
+  h->own_pending = false;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return g_err.empty() ? fail(rc, "vi_autotune: HIP error") : rc;
+  VI_TRY(apply_tiling(h, best));
+  if (chosen) *chosen = best;
+  return 0;
+}
+
 int vi_profile_enable(vi_handle h, int enabled) {
-  if (int rc = check_handle(h)) return rc;
-  if (int rc = collect_profile(h)) return rc;
-  h->prof = enabled != 0;
-  h->prof_ms[0] = h->prof_ms[1] = 0;
-  h->prof_n[0] = h->prof_n[1] = 0;
+  VI_TRY(check_handle(h));
+  VI_TRY(h->prof.collect(h->stream));
+  h->prof.on = enabled != 0;
+  h->prof.reset();
   return 0;
 }
 
 int vi_profile_read(vi_handle h, int kernel, double* ms_total, long* launches) {
-  if (int rc = check_handle(h)) return rc;
+  VI_TRY(check_handle(h));
   if (kernel < 0 || kernel > 1 || !ms_total || !launches)
     return fail(VI_EINVAL, "vi_profile_read: bad arguments");
-  if (int rc = collect_profile(h)) return rc;
-  *ms_total = h->prof_ms[kernel];
-  *launches = h->prof_n[kernel];
+  VI_TRY(h->prof.collect(h->stream));
+  *ms_total = h->prof.ms[kernel];
+  *launches = h->prof.n[kernel];
   return 0;
 }
 
 int vi_algorithmic_bytes(vi_handle h, int kernel, double* bytes) {
-  if (int rc = check_handle(h)) return rc;
+  VI_TRY(check_handle(h));
   if (kernel < 0 || kernel > 1 || !bytes) return fail(VI_EINVAL, "vi_algorithmic_bytes");
   const double ni = (double)(h->g.nx - 8) * (h->g.nz - 8);
   // velocity: v R+W (4) + s R (3) + rho (1); stress: v R (2) + s R+W (6) + lam, mu (2)
@@ -705,63 +1112,149 @@ int vi_algorithmic_bytes(vi_handle h, int kernel, double* bytes) {
   return 0;
 }
 
-int vi_set_tiling(vi_handle h, int rz) {
-  if (int rc = check_handle(h)) return rc;
-  if (rz != 1 && rz != 2 && rz != 4 && rz != 8)
-    return fail(VI_EINVAL, "vi_set_tiling: rows per wavefront must be 1, 2, 4 or 8");
-  VI_HIP(hipStreamSynchronize(h->stream));
-  return apply_tiling(h, rz);
+// ---------------------------------------------------------------- batches
+int vi_batch_create(vi_batch* out) {
+  if (!out) return fail(VI_EINVAL, "vi_batch_create: null argument");
+  *out = nullptr;
+  if (vi_device_count() < 1) return fail(VI_EDEVICE, "vi_batch_create: no HIP device visible");
+  std::unique_ptr<vi_batch_s> b(new vi_batch_s());
+  VI_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  *out = b.release();
+  return 0;
 }
 
-int vi_get_tiling(vi_handle h) { return h ? h->rz : -1; }
+int vi_batch_destroy(vi_batch b) {
+  if (!b) return 0;
+  (void)hipStreamSynchronize(b->stream);
+  invalidate_graph(b);
+  hipStream_t s = b->stream;
+  delete b;
+  if (s) (void)hipStreamDestroy(s);
+  return 0;
+}
 
-int vi_autotune(vi_handle h, int trial_steps, int* chosen, double* us_per_step) {
-  if (int rc = check_handle(h)) return rc;
-  if (trial_steps < 1) return fail(VI_EINVAL, "vi_autotune: trial_steps must be >= 1");
-  if (!h->model_set || (h->L && !h->relax_set))
-    return fail(VI_ESTATE, "vi_autotune: configure the model first");
-  VI_HIP(hipStreamSynchronize(h->stream));
-  // snapshot every mutable array so that the trials leave the state untouched
-  std::vector<DevBuf*> live;
-  for (auto& b : h->state) live.push_back(&b);
-  live.push_back(&h->M);
-  for (auto& b : h->psi) live.push_back(&b);
-  live.push_back(&h->traces);
-  std::vector<DevBuf> snap(live.size());
-  for (size_t q = 0; q < live.size(); ++q) {
-    if (!live[q]->p) continue;
-    if (int rc = snap[q].alloc(live[q]->bytes, h->stream)) return rc;
-    VI_HIP(hipMemcpyAsync(snap[q].p, live[q]->p, live[q]->bytes, hipMemcpyDeviceToDevice,
-                          h->stream));
+int vi_batch_set(vi_batch b, const vi_handle* hs, int n) {
+  if (!b || n < 0 || (n && !hs)) return fail(VI_EINVAL, "vi_batch_set: bad arguments");
+  for (int q = 0; q < n; ++q) {
+    if (!hs[q]) return fail(VI_EINVAL, "vi_batch_set: null handle");
+    for (int p = 0; p < q; ++p)
+      if (hs[p] == hs[q]) return fail(VI_EINVAL, "vi_batch_set: handle listed twice");
+  }
+  VI_HIP(hipStreamSynchronize(b->stream));
+  b->members.assign(hs, hs + n);
+  b->built = false;
+  invalidate_graph(b);
+  return 0;
+}
+
+int vi_batch_set_tiling(vi_batch b, int rz) {
+  if (!b) return fail(VI_EINVAL, "null batch");
+  if (rz != 1 && rz != 2 && rz != 4 && rz != 8)
+    return fail(VI_EINVAL, "vi_batch_set_tiling: rows per wavefront must be 1, 2, 4 or 8");
+  VI_HIP(hipStreamSynchronize(b->stream));
+  b->rz = rz;
+  b->built = false;
+  invalidate_graph(b);
+  return 0;
+}
+
+int vi_batch_get_tiling(vi_batch b) { return b ? b->rz : -1; }
+
+int vi_batch_set_graph(vi_batch b, int steps_per_graph) {
+  if (!b || steps_per_graph < 0) return fail(VI_EINVAL, "vi_batch_set_graph: bad arguments");
+  VI_HIP(hipStreamSynchronize(b->stream));
+  invalidate_graph(b);
+  b->graph_steps = steps_per_graph;
+  return 0;
+}
+
+int vi_batch_step(vi_batch b, int nsteps) {
+  if (!b || nsteps < 0) return fail(VI_EINVAL, "vi_batch_step: bad arguments");
+  if (b->members.empty() || nsteps == 0) return 0;
+  VI_TRY(batch_prepare(b));
+  int left = nsteps;
+  if (b->graph_steps > 0 && !b->prof.on) {
+    const int K = b->graph_steps;
+    if (left >= K && !b->graph) {
+      VI_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      int rc = batch_launch(b, K);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(VI_EDEVICE, "vi_batch_step: graph capture failed");
+      b->graph_def = g;
+      VI_HIP(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+    }
+    while (left >= K) {
+      VI_HIP(hipGraphLaunch(b->graph, b->stream));
+      left -= K;
+    }
+  }
+  if (left) VI_TRY(batch_launch(b, left));
+  return batch_finish(b, nsteps);
+}
+
+int vi_batch_sync(vi_batch b) {
+  if (!b) return fail(VI_EINVAL, "null batch");
+  VI_HIP(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int vi_batch_autotune(vi_batch b, int trial_steps, int* chosen, double* us_per_step) {
+  if (!b || trial_steps < 1) return fail(VI_EINVAL, "vi_batch_autotune: bad arguments");
+  if (b->members.empty()) return fail(VI_ESTATE, "vi_batch_autotune: empty batch");
+  VI_TRY(batch_prepare(b));
+  VI_HIP(hipStreamSynchronize(b->stream));
+  for (auto* h : b->members) VI_TRY(wait_external(h));
+  std::vector<std::vector<DevBuf>> snaps(b->members.size());
+  std::vector<long> step0(b->members.size());
+  for (size_t m = 0; m < b->members.size(); ++m) {
+    vi_sim_s* h = b->members[m];
+    step0[m] = h->step;
+    auto live = mutable_arrays(h);
+    snaps[m] = std::vector<DevBuf>(live.size());
+    for (size_t q = 0; q < live.size(); ++q) {
+      if (!live[q]->p) continue;
+      VI_TRY(snaps[m][q].alloc(live[q]->bytes, b->stream));
+      VI_HIP(hipMemcpyAsync(snaps[m][q].p, live[q]->p, live[q]->bytes,
+                            hipMemcpyDeviceToDevice, b->stream));
+    }
   }
   auto restore = [&]() -> int {
-    for (size_t q = 0; q < live.size(); ++q)
-      if (live[q]->p)
-        VI_HIP(hipMemcpyAsync(live[q]->p, snap[q].p, live[q]->bytes, hipMemcpyDeviceToDevice,
-                              h->stream));
+    for (size_t m = 0; m < b->members.size(); ++m) {
+      vi_sim_s* h = b->members[m];
+      auto live = mutable_arrays(h);
+      for (size_t q = 0; q < live.size(); ++q)
+        if (live[q]->p)
+          VI_HIP(hipMemcpyAsync(live[q]->p, snaps[m][q].p, live[q]->bytes,
+                                hipMemcpyDeviceToDevice, b->stream));
+      h->step = step0[m];
+      touch(h);
+    }
     return 0;
   };
-  const long step0 = h->step;
-  const bool prof0 = h->prof;
-  h->prof = false;
+  const bool prof0 = b->prof.on;
+  b->prof.on = false;
+  const int graph0 = b->graph_steps;
+  b->graph_steps = 0;
   hipEvent_t e0, e1;
   VI_HIP(hipEventCreate(&e0));
   VI_HIP(hipEventCreate(&e1));
   const int cands[4] = {1, 2, 4, 8};
-  int best = h->rz;
+  int best = b->rz;
   double best_us = 1e300;
   int rc = 0;
   for (int c = 0; c < 4 && !rc; ++c) {
-    if ((rc = apply_tiling(h, cands[c]))) break;
-    for (int k = 0; k < 2 && !rc; ++k) rc = h->esz == 4 ? launch_step<float>(h) : launch_step<double>(h);
-    if (rc) break;
-    if ((rc = hipEventRecord(e0, h->stream) == hipSuccess ? 0 : VI_EDEVICE)) break;
-    for (int k = 0; k < trial_steps && !rc; ++k) {
-      rc = h->esz == 4 ? launch_step<float>(h) : launch_step<double>(h);
-      h->step++;
-    }
-    if (rc) break;
-    if ((rc = hipEventRecord(e1, h->stream) == hipSuccess ? 0 : VI_EDEVICE)) break;
+    b->rz = cands[c];
+    b->built = false;
+    if ((rc = batch_prepare(b))) break;
+    if ((rc = batch_launch(b, 2))) break;
+    if ((rc = hipEventRecord(e0, b->stream) == hipSuccess ? 0 : VI_EDEVICE)) break;
+    if ((rc = batch_launch(b, trial_steps))) break;
+    if ((rc = hipEventRecord(e1, b->stream) == hipSuccess ? 0 : VI_EDEVICE)) break;
     if ((rc = hipEventSynchronize(e1) == hipSuccess ? 0 : VI_EDEVICE)) break;
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
@@ -771,23 +1264,38 @@ int vi_autotune(vi_handle h, int trial_steps, int* chosen, double* us_per_step) 
       best_us = us;
       best = cands[c];
     }
-    h->step = step0;
     if ((rc = restore())) break;
   }
-  h->step = step0;
-  h->prof = prof0;
+  (void)hipStreamSynchronize(b->stream);
+  if (rc) (void)restore();
+  (void)hipStreamSynchronize(b->stream);
+  b->prof.on = prof0;
+  b->graph_steps = graph0;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (rc) {
-    (void)restore();
-    (void)hipStreamSynchronize(h->stream);
-    return rc == VI_EDEVICE && g_err.empty() ? fail(rc, "vi_autotune: HIP error") : rc;
-  }
-  VI_HIP(hipStreamSynchronize(h->stream));
-  if (int r2 = apply_tiling(h, best)) return r2;
+  if (rc) return g_err.empty() ? fail(rc, "vi_batch_autotune: HIP error") : rc;
+  b->rz = best;
+  b->built = false;
+  invalidate_graph(b);
   if (chosen) *chosen = best;
   return 0;
 }
 
-}  // extern "C"
+int vi_batch_profile_enable(vi_batch b, int enabled) {
+  if (!b) return fail(VI_EINVAL, "null batch");
+  VI_TRY(b->prof.collect(b->stream));
+  b->prof.on = enabled != 0;
+  b->prof.reset();
+  return 0;
+}
 
+int vi_batch_profile_read(vi_batch b, int kernel, double* ms_total, long* launches) {
+  if (!b || kernel < 0 || kernel > 1 || !ms_total || !launches)
+    return fail(VI_EINVAL, "vi_batch_profile_read: bad arguments");
+  VI_TRY(b->prof.collect(b->stream));
+  *ms_total = b->prof.ms[kernel];
+  *launches = b->prof.n[kernel];
+  return 0;
+}
+
+}  // extern "C"

@@ -73,6 +73,16 @@ class Simulation:
         L.check(L.lib().vi_set_model(self._h, *(self._ptr(a) for a in arrs),
                                      self._ptr(t) if t is not None else None))
 
+    def set_model_window(self, model, x0):
+        """Read the model from columns [x0, x0 + nx) of a shared device Model."""
+        L.check(L.lib().vi_set_model_window(self._h, model.handle, int(x0)))
+        self._model_ref = model
+
+    def device_bytes(self):
+        b = ctypes.c_size_t()
+        L.check(L.lib().vi_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
     def set_relaxation(self, taun):
         t = np.ascontiguousarray(np.asarray(taun, dtype=np.float64).ravel())
         if t.size != self.nrelax:
