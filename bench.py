@@ -72,6 +72,72 @@ def run_cpu_baseline(cfg, budget_s=12.0):
                       f"{el:.1f} s on {threads} threads"}
 
 
+def bench_multishot(nshots, nt, graph_steps=100):
+    """Batched multi-shot forward (uneven windows, one launch per half step for all shots)
+    against the plain single-shot stepper run serially on the same GPU, same work."""
+    from viscoelasticinversion_amd import Batch
+    from viscoelasticinversion_amd.configs import build_multishot, config_multishot
+    mcfg = config_multishot(nshots=nshots, nt=nt)
+    model, sims = build_multishot(mcfg, "float32")
+    cells = sum(sh["nx"] for sh in mcfg["shots"]) * mcfg["nz"]
+    b = Batch(sims)
+    rows, table = b.autotune(trial_steps=20)
+    b.set_graph(graph_steps)
+    b.step(graph_steps)  # capture + warm up
+    b.sync()
+    for sim in sims:
+        sim.reset()
+    t0 = time.perf_counter()
+    b.step(nt)
+    b.sync()
+    wall_b = time.perf_counter() - t0
+    finite = all(bool(np.isfinite(sim.traces(nt)).all()) for sim in sims[:4])
+    # per-kernel durations of the batched launch (events recorded by the launches)
+    for sim in sims:
+        sim.reset()
+    b.profile(True)
+    b.step(50)
+    ms_v, n_v = b.profile_read(0)
+    ms_s, n_s = b.profile_read(1)
+    b.profile(False)
+    bytes_s = sum(sim.algorithmic_bytes(1) for sim in sims)
+    bytes_v = sum(sim.algorithmic_bytes(0) for sim in sims)
+    b.close()
+    # plain stepper, one shot at a time
+    done = []
+    for sim in sims:
+        sim.reset()
+        sim.set_tiling(rows)
+    t0 = time.perf_counter()
+    for sim in sims:
+        sim.step(nt)
+        sim.sync()
+        done.append(time.perf_counter() - t0)
+    wall_s = time.perf_counter() - t0
+    for sim in sims:
+        sim.close()
+    model.close()
+    avg_s = ms_s / max(n_s, 1) * 1e-3
+    avg_v = ms_v / max(n_v, 1) * 1e-3
+    return {"workload": f"{nshots} shots, widths U{{512..1536}} x 512, L=3, CPML + free surface, "
+                        f"{nt} steps, FP32", "cells_per_step": cells,
+            "batched": {"wall_s": round(wall_b, 4), "GPt_per_s": round(cells * nt / wall_b / 1e9, 3),
+                        "shots_per_s": round(nshots / wall_b, 3),
+                        "latency_p50_s": round(wall_b, 4), "latency_p99_s": round(wall_b, 4),
+                        "rows_per_wave": rows, "graph_steps": graph_steps,
+                        "k_stress_avg_us": round(avg_s * 1e6, 2),
+                        "k_stress_hbm_frac": round(bytes_s / avg_s / 1e9 / HBM_PEAK_GBS, 4),
+                        "k_velocity_avg_us": round(avg_v * 1e6, 2),
+                        "k_velocity_hbm_frac": round(bytes_v / avg_v / 1e9 / HBM_PEAK_GBS, 4),
+                        "autotune_us_per_step": {str(k): round(v, 1) for k, v in table.items()},
+                        "traces_finite": finite},
+            "serial_plain_stepper": {"wall_s": round(wall_s, 4),
+                                     "GPt_per_s": round(cells * nt / wall_s / 1e9, 3),
+                                     "shots_per_s": round(nshots / wall_s, 3),
+                                     "latency_p50_s": round(float(np.percentile(done, 50)), 4),
+                                     "latency_p99_s": round(float(np.percentile(done, 99)), 4)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +147,9 @@ def main():
     ap.add_argument("--nz", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--multishot-shots", type=int, default=64)
+    ap.add_argument("--multishot-steps", type=int, default=3000)
+    ap.add_argument("--no-multishot", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the stepper has no multi-GPU path in this build "
@@ -134,6 +203,8 @@ def main():
              "value_without_events": round(cells * args.steps / wall_noev / 1e9, 3),
              "traces_finite": finite, "rows_per_wave": rows,
              "autotune_us_per_step": {str(k): round(v, 2) for k, v in tune_table.items()}}
+    if not args.no_multishot:
+        extra["multishot"] = bench_multishot(args.multishot_shots, args.multishot_steps)
     cpu = None
     if not args.no_cpu:
         cfg_cpu = config_layered_visco(nx=args.nx, nz=args.nz, nt=args.steps)

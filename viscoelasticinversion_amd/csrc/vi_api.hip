@@ -8,6 +8,7 @@
 #include "vi_mi355x.h"
 #include "vi_kernels.hpp"
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -110,7 +111,9 @@ struct Prof {
   ~Prof() {
     for (auto e : ev) (void)hipEventDestroy(e);
   }
-  int begin(hipStream_t s, int k, hipEvent_t* e1) {
+  // An event pair that the launch itself records (hipExtLaunchKernelGGL), so the timing is
+  // the kernel's execution without extra marker packets in the stream.
+  int begin(int k, hipEvent_t* e0, hipEvent_t* e1) {
     if (used + 2 > ev.size()) {
       for (int q = 0; q < 2; ++q) {
         hipEvent_t e;
@@ -120,7 +123,7 @@ struct Prof {
       }
     }
     kind[used] = k;
-    VI_HIP(hipEventRecord(ev[used], s));
+    *e0 = ev[used];
     *e1 = ev[used + 1];
     used += 2;
     return 0;
@@ -409,23 +412,30 @@ int refresh_own_desc(vi_sim_s* h) {
   return 0;
 }
 
+template <typename K, typename... Args>
+void launch(K kern, dim3 grid, dim3 block, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+            Args... args) {
+  if (e0) hipExtLaunchKernelGGL(kern, grid, block, 0, s, e0, e1, 0, args...);
+  else hipLaunchKernelGGL(kern, grid, block, 0, s, args...);
+}
+
 template <typename R, int RZ>
 int launch_kernel(hipStream_t s, int L, int k, const void* descs, const int* tile_shot,
-                  int ntiles, int kstep) {
+                  int ntiles, int kstep, hipEvent_t e0, hipEvent_t e1) {
   const dim3 block(vi::TX, vi::NW);
   const dim3 grid(ntiles);
   const vi::ShotDesc<R>* d = (const vi::ShotDesc<R>*)descs;
   if (k == 0) {
-    hipLaunchKernelGGL((vi::k_velocity<R, RZ>), grid, block, 0, s, d, tile_shot, kstep);
+    launch(vi::k_velocity<R, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep);
   } else {
     switch (L) {
-      case 0: hipLaunchKernelGGL((vi::k_stress<R, 0, RZ>), grid, block, 0, s, d, tile_shot, kstep); break;
-      case 1: hipLaunchKernelGGL((vi::k_stress<R, 1, RZ>), grid, block, 0, s, d, tile_shot, kstep); break;
-      case 2: hipLaunchKernelGGL((vi::k_stress<R, 2, RZ>), grid, block, 0, s, d, tile_shot, kstep); break;
-      case 3: hipLaunchKernelGGL((vi::k_stress<R, 3, RZ>), grid, block, 0, s, d, tile_shot, kstep); break;
-      case 4: hipLaunchKernelGGL((vi::k_stress<R, 4, RZ>), grid, block, 0, s, d, tile_shot, kstep); break;
-      case 5: hipLaunchKernelGGL((vi::k_stress<R, 5, RZ>), grid, block, 0, s, d, tile_shot, kstep); break;
-      case 6: hipLaunchKernelGGL((vi::k_stress<R, 6, RZ>), grid, block, 0, s, d, tile_shot, kstep); break;
+      case 0: launch(vi::k_stress<R, 0, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep); break;
+      case 1: launch(vi::k_stress<R, 1, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep); break;
+      case 2: launch(vi::k_stress<R, 2, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep); break;
+      case 3: launch(vi::k_stress<R, 3, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep); break;
+      case 4: launch(vi::k_stress<R, 4, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep); break;
+      case 5: launch(vi::k_stress<R, 5, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep); break;
+      case 6: launch(vi::k_stress<R, 6, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep); break;
       default: return fail(VI_EINVAL, "nrelax out of range");
     }
   }
@@ -435,12 +445,12 @@ int launch_kernel(hipStream_t s, int L, int k, const void* descs, const int* til
 
 template <typename R>
 int launch_one(hipStream_t s, int rz, int L, int k, const void* descs, const int* tile_shot,
-               int ntiles, int kstep) {
+               int ntiles, int kstep, hipEvent_t e0, hipEvent_t e1) {
   switch (rz) {
-    case 1: return launch_kernel<R, 1>(s, L, k, descs, tile_shot, ntiles, kstep);
-    case 2: return launch_kernel<R, 2>(s, L, k, descs, tile_shot, ntiles, kstep);
-    case 4: return launch_kernel<R, 4>(s, L, k, descs, tile_shot, ntiles, kstep);
-    case 8: return launch_kernel<R, 8>(s, L, k, descs, tile_shot, ntiles, kstep);
+    case 1: return launch_kernel<R, 1>(s, L, k, descs, tile_shot, ntiles, kstep, e0, e1);
+    case 2: return launch_kernel<R, 2>(s, L, k, descs, tile_shot, ntiles, kstep, e0, e1);
+    case 4: return launch_kernel<R, 4>(s, L, k, descs, tile_shot, ntiles, kstep, e0, e1);
+    case 8: return launch_kernel<R, 8>(s, L, k, descs, tile_shot, ntiles, kstep, e0, e1);
   }
   return fail(VI_EINVAL, "bad tiling");
 }
@@ -451,11 +461,10 @@ int launch_steps(hipStream_t s, size_t esz, int rz, int L, void* descs, const in
                  int ntiles, int nshots, int nsteps, Prof* prof) {
   for (int q = 0; q < nsteps; ++q) {
     for (int k = 0; k < 2; ++k) {
-      hipEvent_t e1 = nullptr;
-      if (prof && prof->on) VI_TRY(prof->begin(s, k, &e1));
-      VI_TRY(esz == 4 ? launch_one<float>(s, rz, L, k, descs, tile_shot, ntiles, q)
-                      : launch_one<double>(s, rz, L, k, descs, tile_shot, ntiles, q));
-      if (e1) VI_HIP(hipEventRecord(e1, s));
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (prof && prof->on) VI_TRY(prof->begin(k, &e0, &e1));
+      VI_TRY(esz == 4 ? launch_one<float>(s, rz, L, k, descs, tile_shot, ntiles, q, e0, e1)
+                      : launch_one<double>(s, rz, L, k, descs, tile_shot, ntiles, q, e0, e1));
     }
   }
   if (nsteps > 0) {

@@ -7,11 +7,13 @@
 // form one grid (tile_shot[] maps a global tile to its shot). A single simulation is a
 // batch of one.
 //   - a workgroup is NW wavefronts of 64 lanes; lane = x offset, wave = band of RZ rows;
-//   - fields differentiated along x are staged in LDS with a 4-column halo;
-//   - fields differentiated along z are held as a register window of RZ+7 rows per lane
-//     (z register blocking), so every global load is a coalesced 64-lane row segment;
-//   - RZ (rows per wave) is a template parameter chosen by the autotuner: small grids need
-//     small RZ for enough waves in flight, large grids amortise the z halo;
+//   - the differentiated fields of the tile are staged once in LDS with a 4-cell halo in x
+//     and z (the halo is shared by the NW wavefronts, so each global row segment is a
+//     coalesced 64-lane load issued once per workgroup); cell-local fields (state being
+//     updated, model, memory variables) stream straight from HBM to registers;
+//   - RZ (rows per wave, so NW*RZ rows per tile) is a template parameter chosen by the
+//     autotuner: small grids need small tiles for enough waves in flight, large grids
+//     amortise the 8-row z halo over taller tiles;
 //   - the workgroup -> tile map is XCD-aware: each of the 8 XCDs gets one contiguous range
 //     of tiles, so vertical halos are re-read from that XCD's L2.
 // State planes are padded (pitch ldx >= nx + 72, rows >= nz + NW*RZ_MAX + 16) and so are
@@ -97,17 +99,22 @@ __device__ __forceinline__ int strip_x(int i, int nx, int T, int lo, int hi) {
   return -1;
 }
 
-// Stage RZ rows of one field (own column from the register window starting at OFF, halo
-// columns from global memory) into this wave's LDS rows.
-template <typename R, int RZ, int OFF, int N>
-__device__ __forceinline__ void stage_rows(R (*sh)[TX + 8], const R (&win)[N],
-                                           const R* __restrict__ g, size_t base, int ldx,
-                                           int lane) {
+// Cooperative load of NROWS rows (starting at global row jr0) of one field into an LDS
+// tile of width TX + 2*HALO (HALO = 4 or 0): lane = main column, lanes 0..7 also fetch the
+// 4 + 4 halo columns. Rows are spread over the NW wavefronts.
+template <typename R, int NROWS, int HALO>
+__device__ __forceinline__ void load_tile(R* sh, const R* __restrict__ g, int jr0, int i0,
+                                          int ldx, int lane, int w) {
+  constexpr int W = TX + 2 * HALO;
 #pragma unroll
-  for (int r = 0; r < RZ; ++r) {
-    sh[r][lane + 4] = win[OFF + r];
-    if (lane < 4) sh[r][lane] = g[base + (size_t)r * ldx - 4];
-    if (lane >= TX - 4) sh[r][lane + 8] = g[base + (size_t)r * ldx + 4];
+  for (int r = w; r < NROWS; r += NW) {
+    const size_t row = (size_t)(jr0 + r) * ldx;
+    sh[r * W + lane + HALO] = g[row + i0 + lane];
+    if (HALO && lane < 8) {
+      const int c = lane < 4 ? lane : lane + TX;           // LDS column of the halo
+      const int gi = lane < 4 ? i0 - 4 + lane : i0 + TX + lane - 4;
+      sh[r * W + c] = g[row + gi];
+    }
   }
 }
 
@@ -115,73 +122,58 @@ template <typename R, int RZ>
 __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restrict__ shots,
                                                      const int* __restrict__ tile_shot,
                                                      int kstep) {
-  __shared__ R sh_xx[NW][RZ][TX + 8];
-  __shared__ R sh_xz[NW][RZ][TX + 8];
+  constexpr int TZ = NW * RZ;
+  constexpr int WX = TX + 8;
+  __shared__ R s_xz[(TZ + 8) * WX];  // sxz rows j0-4 .. j0+TZ+3, columns i0-4 .. i0+67
+  __shared__ R s_zz[(TZ + 8) * TX];  // szz rows j0-4 .. j0+TZ+3, columns i0 .. i0+63
+  __shared__ R s_xx[TZ * WX];        // sxx rows j0 .. j0+TZ-1, columns i0-4 .. i0+67
   const int lane = threadIdx.x, w = threadIdx.y;
   const int gtile = xcd_tile(blockIdx.x, gridDim.x);
   const ShotDesc<R>& d = shots[tile_shot[gtile]];
   const int tile = gtile - d.tile_base;
   const int ntx = d.ntx;
   const int tx = tile % ntx, tz = tile / ntx;
-  const int i = 4 + tx * TX + lane;
-  const int j0 = 4 + (tz * NW + w) * RZ;
+  const int i0 = 4 + tx * TX, j0 = 4 + tz * TZ;
+  const int i = i0 + lane;
   const int ldx = d.ldx, ldm = d.ldm;
-  const size_t base = (size_t)j0 * ldx + i;
-  const size_t mbase = (size_t)j0 * ldm + i;
-  const R* __restrict__ sxx = d.sxx;
-  const R* __restrict__ sxz = d.sxz;
-  const R* __restrict__ szz = d.szz;
+  load_tile<R, TZ + 8, 4>(s_xz, d.sxz, j0 - 4, i0, ldx, lane, w);
+  load_tile<R, TZ + 8, 0>(s_zz, d.szz, j0 - 4, i0, ldx, lane, w);
+  load_tile<R, TZ, 4>(s_xx, d.sxx, j0, i0, ldx, lane, w);
+
+  const int jw = j0 + w * RZ;  // first row of this wave
+  const size_t base = (size_t)jw * ldx + i;
+  const size_t mbase = (size_t)jw * ldm + i;
   const R* __restrict__ rho = d.rho;
   R* __restrict__ vx = d.vx;
   R* __restrict__ vz = d.vz;
-
-  R wxx[RZ];      // sxx rows j0 .. j0+RZ-1
-  R wxz[RZ + 7];  // sxz rows j0-4 .. j0+RZ+2   (D- along z, centre at r+4)
-  R wzz[RZ + 7];  // szz rows j0-3 .. j0+RZ+3   (D+ along z, centre at r+3)
-  R wrho[RZ + 1], wrho1[RZ];  // rho at column i rows j0 .. j0+RZ; column i+1 rows j0 ..
-  R ovx[RZ], ovz[RZ];
-#pragma unroll
-  for (int k = 0; k < RZ + 7; ++k) {
-    wxz[k] = sxz[base + (ptrdiff_t)(k - 4) * ldx];
-    wzz[k] = szz[base + (ptrdiff_t)(k - 3) * ldx];
-  }
+  R wrho[RZ + 1], wrho1[RZ], ovx[RZ], ovz[RZ];
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
-    wxx[r] = sxx[base + (size_t)r * ldx];
     wrho1[r] = rho[mbase + (size_t)r * ldm + 1];
     ovx[r] = vx[base + (size_t)r * ldx];
     ovz[r] = vz[base + (size_t)r * ldx];
   }
 #pragma unroll
   for (int r = 0; r <= RZ; ++r) wrho[r] = rho[mbase + (size_t)r * ldm];
-
-  stage_rows<R, RZ, 0>(sh_xx[w], wxx, sxx, base, ldx, lane);
-  stage_rows<R, RZ, 4>(sh_xz[w], wxz, sxz, base, ldx, lane);
   __syncthreads();
 
   const int nx = d.nx, nz = d.nz, T = d.T;
   const bool col_ok = i < nx - 4;
   const R dt = d.dt, two = R(2);
   const R Dx[4] = {d.Dx[0], d.Dx[1], d.Dx[2], d.Dx[3]};
-  const R Dz0 = d.Dz[0], Dz1 = d.Dz[1], Dz2 = d.Dz[2], Dz3 = d.Dz[3];
+  const R Dz[4] = {d.Dz[0], d.Dz[1], d.Dz[2], d.Dz[3]};
   const int xk = T ? strip_x(i, nx, T, d.xlo, d.xhi) : -1;
   const int s0 = d.src_off[tile], s1 = d.src_off[tile + 1];
   const long step = d.step + kstep;
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
-    const int j = j0 + r;
+    const int j = jw + r;
+    const int tr = w * RZ + r;  // row within the tile
     const size_t c = base + (size_t)r * ldx;
-    R dsxx_x = op_p(&sh_xx[w][r][lane + 4], 1, Dx);
-    R dsxz_x = op_m(&sh_xz[w][r][lane + 4], 1, Dx);
-    // D- of sxz along z (centre r+4) and D+ of szz along z (centre r+3)
-    R dsxz_z = Dz0 * (wxz[r + 4] - wxz[r + 3]);
-    dsxz_z = dsxz_z + Dz1 * (wxz[r + 5] - wxz[r + 2]);
-    dsxz_z = dsxz_z + Dz2 * (wxz[r + 6] - wxz[r + 1]);
-    dsxz_z = dsxz_z + Dz3 * (wxz[r + 7] - wxz[r + 0]);
-    R dszz_z = Dz0 * (wzz[r + 4] - wzz[r + 3]);
-    dszz_z = dszz_z + Dz1 * (wzz[r + 5] - wzz[r + 2]);
-    dszz_z = dszz_z + Dz2 * (wzz[r + 6] - wzz[r + 1]);
-    dszz_z = dszz_z + Dz3 * (wzz[r + 7] - wzz[r + 0]);
+    R dsxx_x = op_p(&s_xx[tr * WX + lane + 4], 1, Dx);
+    R dsxz_x = op_m(&s_xz[(tr + 4) * WX + lane + 4], 1, Dx);
+    R dsxz_z = op_m(&s_xz[(tr + 4) * WX + lane + 4], WX, Dz);
+    R dszz_z = op_p(&s_zz[(tr + 4) * TX + lane], TX, Dz);
     const bool ok = col_ok && j < nz - 4;
     if (T && ok) {
       if (xk >= 0) {
@@ -218,21 +210,25 @@ template <typename R, int L, int RZ>
 __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict__ shots,
                                                    const int* __restrict__ tile_shot,
                                                    int kstep) {
-  __shared__ R sh_vx[NW][RZ][TX + 8];
-  __shared__ R sh_vz[NW][RZ][TX + 8];
+  constexpr int TZ = NW * RZ;
+  constexpr int WX = TX + 8;
+  __shared__ R s_vx[(TZ + 8) * WX];  // vx rows j0-4 .. j0+TZ+3, columns i0-4 .. i0+67
+  __shared__ R s_vz[(TZ + 8) * WX];  // vz, same window
   const int lane = threadIdx.x, w = threadIdx.y;
   const int gtile = xcd_tile(blockIdx.x, gridDim.x);
   const ShotDesc<R>& d = shots[tile_shot[gtile]];
   const int tile = gtile - d.tile_base;
   const int ntx = d.ntx;
   const int tx = tile % ntx, tz = tile / ntx;
-  const int i = 4 + tx * TX + lane;
-  const int j0 = 4 + (tz * NW + w) * RZ;
+  const int i0 = 4 + tx * TX, j0 = 4 + tz * TZ;
+  const int i = i0 + lane;
   const int ldx = d.ldx, ldm = d.ldm;
-  const size_t base = (size_t)j0 * ldx + i;
-  const size_t mbase = (size_t)j0 * ldm + i;
-  const R* __restrict__ vx = d.vx;
-  const R* __restrict__ vz = d.vz;
+  load_tile<R, TZ + 8, 4>(s_vx, d.vx, j0 - 4, i0, ldx, lane, w);
+  load_tile<R, TZ + 8, 4>(s_vz, d.vz, j0 - 4, i0, ldx, lane, w);
+
+  const int jw = j0 + w * RZ;
+  const size_t base = (size_t)jw * ldx + i;
+  const size_t mbase = (size_t)jw * ldm + i;
   const R* __restrict__ mu = d.mu;
   const R* __restrict__ lam = d.lam;
   const R* __restrict__ tau = d.tau;
@@ -241,29 +237,19 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   R* __restrict__ sxz = d.sxz;
   R* __restrict__ M = d.M;
   const size_t plane = d.plane;
-
-  R wvx[RZ + 7];  // vx rows j0-3 .. j0+RZ+3  (D+ along z, centre r+3)
-  R wvz[RZ + 7];  // vz rows j0-4 .. j0+RZ+2  (D- along z, centre r+4)
-  R wmu[RZ + 1], wmu1[RZ + 1];  // mu at columns i and i+1, rows j0 .. j0+RZ
-#pragma unroll
-  for (int k = 0; k < RZ + 7; ++k) {
-    wvx[k] = vx[base + (ptrdiff_t)(k - 3) * ldx];
-    wvz[k] = vz[base + (ptrdiff_t)(k - 4) * ldx];
-  }
+  R wmu[RZ + 1], wmu1[RZ + 1];  // mu at columns i and i+1, rows jw .. jw+RZ
 #pragma unroll
   for (int r = 0; r <= RZ; ++r) {
     wmu[r] = mu[mbase + (size_t)r * ldm];
     wmu1[r] = mu[mbase + (size_t)r * ldm + 1];
   }
-  stage_rows<R, RZ, 3>(sh_vx[w], wvx, vx, base, ldx, lane);
-  stage_rows<R, RZ, 4>(sh_vz[w], wvz, vz, base, ldx, lane);
   __syncthreads();
 
   const int nx = d.nx, nz = d.nz, T = d.T;
   const bool col_ok = i < nx - 4;
   const R dt = d.dt, two = R(2), quarter = R(0.25);
   const R Dx[4] = {d.Dx[0], d.Dx[1], d.Dx[2], d.Dx[3]};
-  const R Dz0 = d.Dz[0], Dz1 = d.Dz[1], Dz2 = d.Dz[2], Dz3 = d.Dz[3];
+  const R Dz[4] = {d.Dz[0], d.Dz[1], d.Dz[2], d.Dz[3]};
   const int xk = T ? strip_x(i, nx, T, d.xlo, d.xhi) : -1;
   const int s0 = d.src_off[tile], s1 = d.src_off[tile + 1];
   const int r0 = d.rec_off[tile], r1 = d.rec_off[tile + 1];
@@ -277,19 +263,16 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   }
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
-    const int j = j0 + r;
+    const int j = jw + r;
+    const int tr = w * RZ + r + 4;  // LDS row of this output
     const size_t c = base + (size_t)r * ldx;
     const size_t mc = mbase + (size_t)r * ldm;
-    R exx = op_m(&sh_vx[w][r][lane + 4], 1, Dx);
-    R exz = op_p(&sh_vz[w][r][lane + 4], 1, Dx);
-    R ezz = Dz0 * (wvz[r + 4] - wvz[r + 3]);
-    ezz = ezz + Dz1 * (wvz[r + 5] - wvz[r + 2]);
-    ezz = ezz + Dz2 * (wvz[r + 6] - wvz[r + 1]);
-    ezz = ezz + Dz3 * (wvz[r + 7] - wvz[r + 0]);
-    R ezx = Dz0 * (wvx[r + 4] - wvx[r + 3]);
-    ezx = ezx + Dz1 * (wvx[r + 5] - wvx[r + 2]);
-    ezx = ezx + Dz2 * (wvx[r + 6] - wvx[r + 1]);
-    ezx = ezx + Dz3 * (wvx[r + 7] - wvx[r + 0]);
+    const R* pvx = &s_vx[tr * WX + lane + 4];
+    const R* pvz = &s_vz[tr * WX + lane + 4];
+    R exx = op_m(pvx, 1, Dx);
+    R exz = op_p(pvz, 1, Dx);
+    R ezz = op_m(pvz, WX, Dz);
+    R ezx = op_p(pvx, WX, Dz);
     const bool ok = col_ok && j < nz - 4;
     if (!ok) continue;
     if (T) {
@@ -368,8 +351,8 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
     for (int q = r0; q < r1; ++q) {
       if ((size_t)d.rec_cell[q] != c || step >= d.rec_nt) continue;
       const int comp = d.rec_comp[q];
-      const R val = comp == 0   ? sh_vx[w][r][lane + 4]
-                    : comp == 1 ? sh_vz[w][r][lane + 4]
+      const R val = comp == 0   ? pvx[0]
+                    : comp == 1 ? pvz[0]
                     : comp == 2 ? nsxx
                     : comp == 3 ? nszz
                                 : nsxz;
