@@ -95,11 +95,11 @@ def bench_multishot(nshots, nt, graph_steps=100):
     # per-kernel durations of the batched launch (events recorded by the launches)
     for sim in sims:
         sim.reset()
-    b.profile(True)
+    b.profile(1)
     b.step(50)
     ms_v, n_v = b.profile_read(0)
     ms_s, n_s = b.profile_read(1)
-    b.profile(False)
+    b.profile(0)
     bytes_s = sum(sim.algorithmic_bytes(1) for sim in sims)
     bytes_v = sum(sim.algorithmic_bytes(0) for sim in sims)
     b.close()
@@ -138,6 +138,37 @@ def bench_multishot(nshots, nt, graph_steps=100):
                                      "latency_p99_s": round(float(np.percentile(done, 99)), 4)}}
 
 
+def pmc_probe(nshots, steps, copy_bytes=1 << 30):
+    """Short run for rocprofv3 --pmc passes: the batched multi-shot kernels for a few steps
+    plus one device-to-device copy of ``copy_bytes`` (a plain copy kernel of known size,
+    used to scale FETCH_SIZE / WRITE_SIZE to bytes)."""
+    import ctypes
+    from viscoelasticinversion_amd import Batch
+    from viscoelasticinversion_amd.configs import build_multishot, config_multishot
+    mcfg = config_multishot(nshots=nshots, nt=steps)
+    model, sims = build_multishot(mcfg, "float32")
+    b = Batch(sims)
+    b.set_tiling(int(os.environ.get("VI_ROWS", "4")))
+    b.step(steps)
+    b.sync()
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(copy_bytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(copy_bytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(copy_bytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(dst, src, ctypes.c_size_t(copy_bytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    per_launch = {k: sum(sim.algorithmic_bytes(k) for sim in sims) for k in (0, 1)}
+    print(json.dumps({"pmc_probe": True, "shots": nshots, "steps": steps, "rows": b.tiling,
+                      "copy_bytes": copy_bytes,
+                      "algorithmic_bytes_per_launch": {"k_velocity": per_launch[0],
+                                                       "k_stress": per_launch[1]}}), flush=True)
+    b.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +181,10 @@ def main():
     ap.add_argument("--multishot-shots", type=int, default=64)
     ap.add_argument("--multishot-steps", type=int, default=3000)
     ap.add_argument("--no-multishot", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="short batched run + 1 GiB copy for rocprofv3 --pmc passes")
+    ap.add_argument("--profile-every", type=int, default=20,
+                    help="time the launches of every N-th step with dispatch events")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the stepper has no multi-GPU path in this build "
@@ -157,6 +192,9 @@ def main():
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.pmc_probe:
+        pmc_probe(args.multishot_shots, 20)
+        return
     from viscoelasticinversion_amd.configs import build_simulation, config_layered_visco
     nt_total = args.warmup + 2 * args.steps
     cfg = config_layered_visco(nx=args.nx, nz=args.nz, nt=nt_total)
@@ -166,16 +204,17 @@ def main():
 
     sim.step(args.warmup)
     sim.sync()
-    # timed region: K steps, HIP events around every launch on the library's stream
-    sim.profile(True)
+    # timed region: K steps; the launches of every N-th step carry HIP events recorded by
+    # the dispatch itself on the library's stream (per-kernel durations)
+    sim.profile(args.profile_every)
     t0 = time.perf_counter()
     sim.step(args.steps)
     sim.sync()
     wall = time.perf_counter() - t0
     ms_v, n_v = sim.profile_read(0)
     ms_s, n_s = sim.profile_read(1)
-    sim.profile(False)
-    # same K steps again without events: launch-gap-free rate, reported beside it
+    sim.profile(0)
+    # same K steps again without events, reported beside it
     t0 = time.perf_counter()
     sim.step(args.steps)
     sim.sync()

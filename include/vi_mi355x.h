@@ -164,11 +164,13 @@ int vi_batch_set_graph(vi_batch b, int steps_per_graph);
 /* advance every member by nsteps; asynchronous until vi_batch_sync or a member's vi_get_* */
 int vi_batch_step(vi_batch b, int nsteps);
 int vi_batch_sync(vi_batch b);
-int vi_batch_profile_enable(vi_batch b, int enabled);
+int vi_batch_profile_enable(vi_batch b, int every);
 int vi_batch_profile_read(vi_batch b, int kernel, double* ms_total, long* launches);
 
-/* ---- kernel timing (HIP events around each launch on the handle's stream) ---------- */
-int vi_profile_enable(vi_handle h, int enabled);
+/* ---- kernel timing: HIP events recorded by the launches themselves
+ * (hipExtLaunchKernelGGL) on the handle's stream, for the launches of every `every`-th
+ * step (1 = all); 0 disables. Enabling resets the sums. ----------------------------- */
+int vi_profile_enable(vi_handle h, int every);
 /* kernel 0 = velocity, 1 = stress: summed milliseconds and launch count since enable */
 int vi_profile_read(vi_handle h, int kernel, double* ms_total, long* launches);
 /* algorithmic HBM bytes per launch of kernel 0/1 for this handle's configuration */

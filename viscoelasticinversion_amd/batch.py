@@ -88,8 +88,9 @@ class Batch:
     def sync(self):
         L.check(L.lib().vi_batch_sync(self._h))
 
-    def profile(self, enabled=True):
-        L.check(L.lib().vi_batch_profile_enable(self._h, int(bool(enabled))))
+    def profile(self, every=1):
+        """Time the launches of every ``every``-th step with dispatch events (0 = off)."""
+        L.check(L.lib().vi_batch_profile_enable(self._h, int(every)))
 
     def profile_read(self, kernel):
         ms, n = ctypes.c_double(), ctypes.c_long()

@@ -103,6 +103,7 @@ struct DevBuf {
 // HIP events around every launch of one stream, summed per kernel kind.
 struct Prof {
   bool on = false;
+  int every = 1;  // time launches of every `every`-th step of a launch sequence
   std::vector<hipEvent_t> ev;
   std::vector<int> kind;
   size_t used = 0;
@@ -462,7 +463,7 @@ int launch_steps(hipStream_t s, size_t esz, int rz, int L, void* descs, const in
   for (int q = 0; q < nsteps; ++q) {
     for (int k = 0; k < 2; ++k) {
       hipEvent_t e0 = nullptr, e1 = nullptr;
-      if (prof && prof->on) VI_TRY(prof->begin(k, &e0, &e1));
+      if (prof && prof->on && q % prof->every == 0) VI_TRY(prof->begin(k, &e0, &e1));
       VI_TRY(esz == 4 ? launch_one<float>(s, rz, L, k, descs, tile_shot, ntiles, q, e0, e1)
                       : launch_one<double>(s, rz, L, k, descs, tile_shot, ntiles, q, e0, e1));
     }
@@ -1082,10 +1083,12 @@ int vi_autotune(vi_handle h, int trial_steps, int* chosen, double* us_per_step) 
   return 0;
 }
 
-int vi_profile_enable(vi_handle h, int enabled) {
+int vi_profile_enable(vi_handle h, int every) {
   VI_TRY(check_handle(h));
+  if (every < 0) return fail(VI_EINVAL, "vi_profile_enable: every must be >= 0");
   VI_TRY(h->prof.collect(h->stream));
-  h->prof.on = enabled != 0;
+  h->prof.on = every != 0;
+  h->prof.every = every ? every : 1;
   h->prof.reset();
   return 0;
 }
@@ -1290,10 +1293,11 @@ int vi_batch_autotune(vi_batch b, int trial_steps, int* chosen, double* us_per_s
   return 0;
 }
 
-int vi_batch_profile_enable(vi_batch b, int enabled) {
-  if (!b) return fail(VI_EINVAL, "null batch");
+int vi_batch_profile_enable(vi_batch b, int every) {
+  if (!b || every < 0) return fail(VI_EINVAL, "vi_batch_profile_enable: bad arguments");
   VI_TRY(b->prof.collect(b->stream));
-  b->prof.on = enabled != 0;
+  b->prof.on = every != 0;
+  b->prof.every = every ? every : 1;
   b->prof.reset();
   return 0;
 }

@@ -237,11 +237,32 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   R* __restrict__ sxz = d.sxz;
   R* __restrict__ M = d.M;
   const size_t plane = d.plane;
+  // Cell-local operands of all RZ rows are requested before the barrier so that they are
+  // in flight together with the tile loads (every address is inside the padded planes).
+  constexpr int LM = L > 0 ? L : 1;
   R wmu[RZ + 1], wmu1[RZ + 1];  // mu at columns i and i+1, rows jw .. jw+RZ
+  R olam[RZ], otau[RZ], osxx[RZ], oszz[RZ], osxz[RZ];
+  R om[3 * LM][RZ];
 #pragma unroll
   for (int r = 0; r <= RZ; ++r) {
     wmu[r] = mu[mbase + (size_t)r * ldm];
     wmu1[r] = mu[mbase + (size_t)r * ldm + 1];
+  }
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const size_t c = base + (size_t)r * ldx;
+    const size_t mc = mbase + (size_t)r * ldm;
+    olam[r] = lam[mc];
+    osxx[r] = sxx[c];
+    oszz[r] = szz[c];
+    osxz[r] = sxz[c];
+    if constexpr (L > 0) {
+      otau[r] = tau[mc];
+#pragma unroll
+      for (int k = 0; k < 3 * L; ++k) om[k][r] = M[(size_t)k * plane + c];
+    } else {
+      otau[r] = R(0);
+    }
   }
   __syncthreads();
 
@@ -266,7 +287,6 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
     const int j = jw + r;
     const int tr = w * RZ + r + 4;  // LDS row of this output
     const size_t c = base + (size_t)r * ldx;
-    const size_t mc = mbase + (size_t)r * ldm;
     const R* pvx = &s_vx[tr * WX + lane + 4];
     const R* pvz = &s_vz[tr * WX + lane + 4];
     R exx = op_m(pvx, 1, Dx);
@@ -289,22 +309,22 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
       }
     }
     const R gxz = ezx + exz;
-    const R lr = lam[mc], mr = wmu[r];
+    const R lr = olam[r], mr = wmu[r];
     const R mxz = quarter * (((wmu[r] + wmu1[r]) + wmu[r + 1]) + wmu1[r + 1]);
     const R pr = lr + two * mr;
     const bool fs_row = fs && j == 4;
     R dsxx, dszz, dsxz;
-    R mxx[L > 0 ? L : 1], mzz[L > 0 ? L : 1], mxzv[L > 0 ? L : 1];
+    R mxx[LM], mzz[LM], mxzv[LM];
     if constexpr (L > 0) {
-      const R t = tau[mc];
+      const R t = otau[r];
       const R p1 = pr * t, p0 = p1 + pr;
       const R l1 = lr * t, l0 = l1 + lr;
       const R m1 = mxz * t, m0 = m1 + mxz;
 #pragma unroll
       for (int k = 0; k < L; ++k) {
-        mxx[k] = M[(size_t)k * plane + c];
-        mzz[k] = M[(size_t)(L + k) * plane + c];
-        mxzv[k] = M[(size_t)(2 * L + k) * plane + c];
+        mxx[k] = om[k][r];
+        mzz[k] = om[L + k][r];
+        mxzv[k] = om[2 * L + k][r];
       }
       R SMxx = mxx[0], SMzz = mzz[0], SMxz = mxzv[0];
 #pragma unroll
@@ -329,9 +349,9 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
       dsxx = dsxx + val;
       dszz = dszz + val;
     }
-    const R nsxx = sxx[c] + dt * dsxx;
-    R nszz = szz[c] + dt * dszz;
-    const R nsxz = sxz[c] + dt * dsxz;
+    const R nsxx = osxx[r] + dt * dsxx;
+    R nszz = oszz[r] + dt * dszz;
+    const R nsxz = osxz[r] + dt * dsxz;
     if (fs_row) nszz = R(0);
     sxx[c] = nsxx;
     szz[c] = nszz;

@@ -181,8 +181,9 @@ class Simulation:
         return chosen.value, dict(zip((1, 2, 4, 8), list(us)))
 
     # -- timing ------------------------------------------------------------------------
-    def profile(self, enabled=True):
-        L.check(L.lib().vi_profile_enable(self._h, int(bool(enabled))))
+    def profile(self, every=1):
+        """Time the launches of every ``every``-th step with dispatch events (0 = off)."""
+        L.check(L.lib().vi_profile_enable(self._h, int(every)))
 
     def profile_read(self, kernel):
         ms, n = ctypes.c_double(), ctypes.c_long()
