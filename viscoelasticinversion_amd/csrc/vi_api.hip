@@ -523,6 +523,7 @@ int make_model_store(int nx, int nz, size_t esz, bool with_tau, const void* cons
   m->nzp = nz + vi::NW * vi::RZ_MAX + 16;
   m->has_tau = with_tau;
   const size_t pb = (size_t)m->ldm * m->nzp * esz;
+  if (pb >= (size_t(1) << 31)) return fail(VI_EINVAL, "model plane must stay below 2 GiB");
   for (int q = 0; q < (with_tau ? 4 : 3); ++q) {
     VI_TRY(m->arr[q].alloc(pb, s));
     VI_TRY(upload_plane(s, m->arr[q].p, m->ldm, src[q], nx, nz, esz));
@@ -721,6 +722,8 @@ int vi_create(const vi_grid_desc* g, vi_handle* out) {
   h->nzp = g->nz + vi::NW * vi::RZ_MAX + 16;
   h->plane = (size_t)h->ldx * h->nzp;
   const size_t pb = h->plane * h->esz;
+  if (pb >= (size_t(1) << 31))
+    return fail(VI_EINVAL, "vi_create: one padded field plane must stay below 2 GiB");
   for (auto& b : h->state) VI_TRY(b.alloc(pb, h->stream));
   if (h->L) VI_TRY(h->M.alloc(3 * (size_t)h->L * pb, h->stream));
   VI_TRY(apply_tiling(h.get(), default_rz(g->nx, g->nz)));

@@ -18,7 +18,8 @@
 //     of tiles, so vertical halos are re-read from that XCD's L2.
 // State planes are padded (pitch ldx >= nx + 72, rows >= nz + NW*RZ_MAX + 16) and so are
 // model planes, so halo and window loads of edge tiles stay inside the allocation without
-// bounds checks; only stores are guarded. Padding is zero and never written.
+// bounds checks; only stores are guarded. Padding is zero and never written. Per-lane
+// addresses are 32-bit byte offsets from uniform plane pointers.
 //
 // Arithmetic follows the oracle's operation order exactly (no contraction: the library is
 // built with -ffp-contract=off), so FP32 and FP64 results match the CPU oracle bit for
@@ -86,10 +87,25 @@ __device__ __forceinline__ R op_m(const R* q, int st, const R* D) {
   return v;
 }
 
+// Global-memory access through a uniform base pointer (SGPR pair) plus a 32-bit per-lane
+// byte offset: compiles to global_load/store in saddr mode (no 64-bit per-lane address
+// arithmetic, no FLAT instructions that would also count against the LDS wait counter).
+// Every plane is smaller than 2 GiB (checked when a handle is created).
+template <typename T>
+__device__ __forceinline__ T gld(const T* base, unsigned boff) {
+  return *(const __attribute__((address_space(1))) T*)(
+      (const __attribute__((address_space(1))) char*)base + boff);
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* base, unsigned boff, T v) {
+  *(__attribute__((address_space(1))) T*)((__attribute__((address_space(1))) char*)base + boff) = v;
+}
+
+// CPML on one derivative d of a strip cell: psi = b psi + a d; return d/kappa + psi.
 template <typename R>
-__device__ __forceinline__ R cpml_apply(R* psi, R a, R b, R k, R d) {
-  const R q = b * (*psi) + a * d;
-  *psi = q;
+__device__ __forceinline__ R cpml_apply(R* psi, unsigned pb, R a, R b, R k, R d) {
+  const R q = b * gld(psi, pb) + a * d;
+  gst(psi, pb, q);
   return d / k + q;
 }
 
@@ -100,23 +116,40 @@ __device__ __forceinline__ int strip_x(int i, int nx, int T, int lo, int hi) {
 }
 
 // Cooperative load of NROWS rows (starting at global row jr0) of one field into an LDS
-// tile of width TX + 2*HALO (HALO = 4 or 0): lane = main column, lanes 0..7 also fetch the
-// 4 + 4 halo columns. Rows are spread over the NW wavefronts.
+// tile of width TX + 2*HALO (HALO = 4 or 0). Wave w owns rows w, w+NW, ...; lane = main
+// column, lanes 0..7 also fetch the 4 + 4 halo columns. fetch() only issues the global
+// loads into registers and store() writes them to LDS, so a kernel can put every load of
+// all its tiles in flight before the first LDS write (one wait instead of a round trip
+// per tile). Rows past NROWS re-load the last row and are not written.
 template <typename R, int NROWS, int HALO>
-__device__ __forceinline__ void load_tile(R* sh, const R* __restrict__ g, int jr0, int i0,
-                                          int ldx, int lane, int w) {
-  constexpr int W = TX + 2 * HALO;
+struct TileLoad {
+  static constexpr int W = TX + 2 * HALO;
+  static constexpr int K = (NROWS + NW - 1) / NW;
+  R v[K], hv[K];
+  __device__ __forceinline__ void fetch(const R* __restrict__ g, int jr0, int i0, int ldx,
+                                        int lane, int w) {
+    constexpr unsigned E = sizeof(R);
+    const int hg = lane < 4 ? lane - 4 : TX + lane - 4;  // halo column relative to i0
 #pragma unroll
-  for (int r = w; r < NROWS; r += NW) {
-    const size_t row = (size_t)(jr0 + r) * ldx;
-    sh[r * W + lane + HALO] = g[row + i0 + lane];
-    if (HALO && lane < 8) {
-      const int c = lane < 4 ? lane : lane + TX;           // LDS column of the halo
-      const int gi = lane < 4 ? i0 - 4 + lane : i0 + TX + lane - 4;
-      sh[r * W + c] = g[row + gi];
+    for (int k = 0; k < K; ++k) {
+      const int r = min(w + k * NW, NROWS - 1);
+      const unsigned rb = (unsigned)((jr0 + r) * ldx + i0) * E;
+      v[k] = gld(g, rb + lane * E);
+      if (HALO) hv[k] = lane < 8 ? gld(g, rb + hg * (int)E) : R(0);
     }
   }
-}
+  __device__ __forceinline__ void store(R* sh, int lane, int w) const {
+    const int hc = lane < 4 ? lane : lane + TX;  // LDS column of a halo value
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int r = w + k * NW;
+      if (r < NROWS) {
+        sh[r * W + lane + HALO] = v[k];
+        if (HALO && lane < 8) sh[r * W + hc] = hv[k];
+      }
+    }
+  }
+};
 
 template <typename R, int RZ>
 __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restrict__ shots,
@@ -124,6 +157,7 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
                                                      int kstep) {
   constexpr int TZ = NW * RZ;
   constexpr int WX = TX + 8;
+  constexpr unsigned E = sizeof(R);
   __shared__ R s_xz[(TZ + 8) * WX];  // sxz rows j0-4 .. j0+TZ+3, columns i0-4 .. i0+67
   __shared__ R s_zz[(TZ + 8) * TX];  // szz rows j0-4 .. j0+TZ+3, columns i0 .. i0+63
   __shared__ R s_xx[TZ * WX];        // sxx rows j0 .. j0+TZ-1, columns i0-4 .. i0+67
@@ -136,25 +170,32 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
   const int i0 = 4 + tx * TX, j0 = 4 + tz * TZ;
   const int i = i0 + lane;
   const int ldx = d.ldx, ldm = d.ldm;
-  load_tile<R, TZ + 8, 4>(s_xz, d.sxz, j0 - 4, i0, ldx, lane, w);
-  load_tile<R, TZ + 8, 0>(s_zz, d.szz, j0 - 4, i0, ldx, lane, w);
-  load_tile<R, TZ, 4>(s_xx, d.sxx, j0, i0, ldx, lane, w);
+  TileLoad<R, TZ + 8, 4> t_xz;
+  TileLoad<R, TZ + 8, 0> t_zz;
+  TileLoad<R, TZ, 4> t_xx;
+  t_xz.fetch(d.sxz, j0 - 4, i0, ldx, lane, w);
+  t_zz.fetch(d.szz, j0 - 4, i0, ldx, lane, w);
+  t_xx.fetch(d.sxx, j0, i0, ldx, lane, w);
 
   const int jw = j0 + w * RZ;  // first row of this wave
-  const size_t base = (size_t)jw * ldx + i;
-  const size_t mbase = (size_t)jw * ldm + i;
+  const unsigned rowb = (unsigned)ldx * E, mrowb = (unsigned)ldm * E;
+  const unsigned cb0 = (unsigned)(jw * ldx + i) * E;   // byte offset of (i, jw) in a state plane
+  const unsigned mb0 = (unsigned)(jw * ldm + i) * E;   // ... and in a model plane
   const R* __restrict__ rho = d.rho;
   R* __restrict__ vx = d.vx;
   R* __restrict__ vz = d.vz;
   R wrho[RZ + 1], wrho1[RZ], ovx[RZ], ovz[RZ];
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
-    wrho1[r] = rho[mbase + (size_t)r * ldm + 1];
-    ovx[r] = vx[base + (size_t)r * ldx];
-    ovz[r] = vz[base + (size_t)r * ldx];
+    wrho1[r] = gld(rho, mb0 + r * mrowb + E);
+    ovx[r] = gld((const R*)vx, cb0 + r * rowb);
+    ovz[r] = gld((const R*)vz, cb0 + r * rowb);
   }
 #pragma unroll
-  for (int r = 0; r <= RZ; ++r) wrho[r] = rho[mbase + (size_t)r * ldm];
+  for (int r = 0; r <= RZ; ++r) wrho[r] = gld(rho, mb0 + r * mrowb);
+  t_xz.store(s_xz, lane, w);
+  t_zz.store(s_zz, lane, w);
+  t_xx.store(s_xx, lane, w);
   __syncthreads();
 
   const int nx = d.nx, nz = d.nz, T = d.T;
@@ -169,7 +210,7 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
   for (int r = 0; r < RZ; ++r) {
     const int j = jw + r;
     const int tr = w * RZ + r;  // row within the tile
-    const size_t c = base + (size_t)r * ldx;
+    const unsigned cb = cb0 + r * rowb;
     R dsxx_x = op_p(&s_xx[tr * WX + lane + 4], 1, Dx);
     R dsxz_x = op_m(&s_xz[(tr + 4) * WX + lane + 4], 1, Dx);
     R dsxz_z = op_m(&s_xz[(tr + 4) * WX + lane + 4], WX, Dz);
@@ -177,21 +218,23 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
     const bool ok = col_ok && j < nz - 4;
     if (T && ok) {
       if (xk >= 0) {
-        const size_t q = (size_t)j * 2 * T + xk;
-        dsxx_x = cpml_apply(&d.psi[2][q], d.ax[i], d.bx[i], d.kx[i], dsxx_x);
-        dsxz_x = cpml_apply(&d.psi[3][q], d.ax[i], d.bx[i], d.kx[i], dsxz_x);
+        const unsigned q = (unsigned)(j * 2 * T + xk) * E;
+        const R a = gld(d.ax, i * E), b = gld(d.bx, i * E), k = gld(d.kx, i * E);
+        dsxx_x = cpml_apply(d.psi[2], q, a, b, k, dsxx_x);
+        dsxz_x = cpml_apply(d.psi[3], q, a, b, k, dsxz_x);
       }
       const int zk = strip_x(j, nz, T, d.zlo, d.zhi);
       if (zk >= 0) {
-        const size_t q = (size_t)zk * ldx + i;
-        dsxz_z = cpml_apply(&d.psi[6][q], d.az[j], d.bz[j], d.kz[j], dsxz_z);
-        dszz_z = cpml_apply(&d.psi[7][q], d.az[j], d.bz[j], d.kz[j], dszz_z);
+        const unsigned q = (unsigned)(zk * ldx + i) * E;
+        const R a = d.az[j], b = d.bz[j], k = d.kz[j];
+        dsxz_z = cpml_apply(d.psi[6], q, a, b, k, dsxz_z);
+        dszz_z = cpml_apply(d.psi[7], q, a, b, k, dszz_z);
       }
     }
     R fx = dsxx_x + dsxz_z;
     R fz = dszz_z + dsxz_x;
     for (int s = s0; s < s1; ++s) {
-      if ((size_t)d.src_cell[s] != c) continue;
+      if ((unsigned)d.src_cell[s] * E != cb) continue;
       const int kind = d.src_kind[s];
       const R val = step < d.snt ? d.src_w[(size_t)d.src_row[s] * d.snt + step] : R(0);
       if (kind == 0) fz = fz + val;
@@ -200,8 +243,8 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
     const R bxv = two / (wrho[r] + wrho1[r]);
     const R bzv = two / (wrho[r] + wrho[r + 1]);
     if (ok) {
-      vx[c] = ovx[r] + dt * (bxv * fx);
-      vz[c] = ovz[r] + dt * (bzv * fz);
+      gst(vx, cb, ovx[r] + dt * (bxv * fx));
+      gst(vz, cb, ovz[r] + dt * (bzv * fz));
     }
   }
 }
@@ -212,6 +255,12 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
                                                    int kstep) {
   constexpr int TZ = NW * RZ;
   constexpr int WX = TX + 8;
+  constexpr unsigned E = sizeof(R);
+  // Cell-local operands of all RZ rows are requested before the barrier (in flight with
+  // the tile loads) for short tiles; taller tiles load them row by row to keep VGPRs low.
+  constexpr bool HOIST = RZ <= 2;
+  constexpr int LM = L > 0 ? L : 1;
+  constexpr int RH = HOIST ? RZ : 1;
   __shared__ R s_vx[(TZ + 8) * WX];  // vx rows j0-4 .. j0+TZ+3, columns i0-4 .. i0+67
   __shared__ R s_vz[(TZ + 8) * WX];  // vz, same window
   const int lane = threadIdx.x, w = threadIdx.y;
@@ -223,47 +272,50 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   const int i0 = 4 + tx * TX, j0 = 4 + tz * TZ;
   const int i = i0 + lane;
   const int ldx = d.ldx, ldm = d.ldm;
-  load_tile<R, TZ + 8, 4>(s_vx, d.vx, j0 - 4, i0, ldx, lane, w);
-  load_tile<R, TZ + 8, 4>(s_vz, d.vz, j0 - 4, i0, ldx, lane, w);
+  TileLoad<R, TZ + 8, 4> t_vx, t_vz;
+  t_vx.fetch(d.vx, j0 - 4, i0, ldx, lane, w);
+  t_vz.fetch(d.vz, j0 - 4, i0, ldx, lane, w);
 
   const int jw = j0 + w * RZ;
-  const size_t base = (size_t)jw * ldx + i;
-  const size_t mbase = (size_t)jw * ldm + i;
+  const unsigned rowb = (unsigned)ldx * E, mrowb = (unsigned)ldm * E;
+  const unsigned cb0 = (unsigned)(jw * ldx + i) * E;
+  const unsigned mb0 = (unsigned)(jw * ldm + i) * E;
   const R* __restrict__ mu = d.mu;
   const R* __restrict__ lam = d.lam;
   const R* __restrict__ tau = d.tau;
   R* __restrict__ sxx = d.sxx;
   R* __restrict__ szz = d.szz;
   R* __restrict__ sxz = d.sxz;
-  R* __restrict__ M = d.M;
-  const size_t plane = d.plane;
-  // Cell-local operands of all RZ rows are requested before the barrier so that they are
-  // in flight together with the tile loads (every address is inside the padded planes).
-  constexpr int LM = L > 0 ? L : 1;
+  R* Mp[3 * LM];
+#pragma unroll
+  for (int k = 0; k < 3 * LM; ++k) Mp[k] = d.M + (size_t)k * d.plane;
   R wmu[RZ + 1], wmu1[RZ + 1];  // mu at columns i and i+1, rows jw .. jw+RZ
-  R olam[RZ], otau[RZ], osxx[RZ], oszz[RZ], osxz[RZ];
-  R om[3 * LM][RZ];
+  R olam[RH], otau[RH], osxx[RH], oszz[RH], osxz[RH], om[3 * LM][RH];
+  auto load_cell = [&](int r, int h) {
+    const unsigned cb = cb0 + r * rowb, mb = mb0 + r * mrowb;
+    olam[h] = gld(lam, mb);
+    osxx[h] = gld((const R*)sxx, cb);
+    oszz[h] = gld((const R*)szz, cb);
+    osxz[h] = gld((const R*)sxz, cb);
+    if constexpr (L > 0) {
+      otau[h] = gld(tau, mb);
+#pragma unroll
+      for (int k = 0; k < 3 * L; ++k) om[k][h] = gld((const R*)Mp[k], cb);
+    } else {
+      otau[h] = R(0);
+    }
+  };
 #pragma unroll
   for (int r = 0; r <= RZ; ++r) {
-    wmu[r] = mu[mbase + (size_t)r * ldm];
-    wmu1[r] = mu[mbase + (size_t)r * ldm + 1];
+    wmu[r] = gld(mu, mb0 + r * mrowb);
+    wmu1[r] = gld(mu, mb0 + r * mrowb + E);
   }
+  if constexpr (HOIST) {
 #pragma unroll
-  for (int r = 0; r < RZ; ++r) {
-    const size_t c = base + (size_t)r * ldx;
-    const size_t mc = mbase + (size_t)r * ldm;
-    olam[r] = lam[mc];
-    osxx[r] = sxx[c];
-    oszz[r] = szz[c];
-    osxz[r] = sxz[c];
-    if constexpr (L > 0) {
-      otau[r] = tau[mc];
-#pragma unroll
-      for (int k = 0; k < 3 * L; ++k) om[k][r] = M[(size_t)k * plane + c];
-    } else {
-      otau[r] = R(0);
-    }
+    for (int r = 0; r < RZ; ++r) load_cell(r, r);
   }
+  t_vx.store(s_vx, lane, w);
+  t_vz.store(s_vz, lane, w);
   __syncthreads();
 
   const int nx = d.nx, nz = d.nz, T = d.T;
@@ -276,7 +328,7 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   const int r0 = d.rec_off[tile], r1 = d.rec_off[tile + 1];
   const long step = d.step + kstep;
   const int fs = d.fs;
-  R a1[L > 0 ? L : 1], a2[L > 0 ? L : 1];
+  R a1[LM], a2[LM];
 #pragma unroll
   for (int k = 0; k < L; ++k) {
     a1[k] = d.a1[k];
@@ -286,7 +338,8 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   for (int r = 0; r < RZ; ++r) {
     const int j = jw + r;
     const int tr = w * RZ + r + 4;  // LDS row of this output
-    const size_t c = base + (size_t)r * ldx;
+    const unsigned cb = cb0 + r * rowb;
+    const int h = HOIST ? r : 0;
     const R* pvx = &s_vx[tr * WX + lane + 4];
     const R* pvz = &s_vz[tr * WX + lane + 4];
     R exx = op_m(pvx, 1, Dx);
@@ -295,43 +348,39 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
     R ezx = op_p(pvx, WX, Dz);
     const bool ok = col_ok && j < nz - 4;
     if (!ok) continue;
+    if constexpr (!HOIST) load_cell(r, 0);
     if (T) {
       if (xk >= 0) {
-        const size_t q = (size_t)j * 2 * T + xk;
-        exx = cpml_apply(&d.psi[0][q], d.ax[i], d.bx[i], d.kx[i], exx);
-        exz = cpml_apply(&d.psi[1][q], d.ax[i], d.bx[i], d.kx[i], exz);
+        const unsigned q = (unsigned)(j * 2 * T + xk) * E;
+        const R a = gld(d.ax, i * E), b = gld(d.bx, i * E), k = gld(d.kx, i * E);
+        exx = cpml_apply(d.psi[0], q, a, b, k, exx);
+        exz = cpml_apply(d.psi[1], q, a, b, k, exz);
       }
       const int zk = strip_x(j, nz, T, d.zlo, d.zhi);
       if (zk >= 0) {
-        const size_t q = (size_t)zk * ldx + i;
-        ezx = cpml_apply(&d.psi[4][q], d.az[j], d.bz[j], d.kz[j], ezx);
-        ezz = cpml_apply(&d.psi[5][q], d.az[j], d.bz[j], d.kz[j], ezz);
+        const unsigned q = (unsigned)(zk * ldx + i) * E;
+        const R a = d.az[j], b = d.bz[j], k = d.kz[j];
+        ezx = cpml_apply(d.psi[4], q, a, b, k, ezx);
+        ezz = cpml_apply(d.psi[5], q, a, b, k, ezz);
       }
     }
     const R gxz = ezx + exz;
-    const R lr = olam[r], mr = wmu[r];
+    const R lr = olam[h], mr = wmu[r];
     const R mxz = quarter * (((wmu[r] + wmu1[r]) + wmu[r + 1]) + wmu1[r + 1]);
     const R pr = lr + two * mr;
     const bool fs_row = fs && j == 4;
     R dsxx, dszz, dsxz;
-    R mxx[LM], mzz[LM], mxzv[LM];
     if constexpr (L > 0) {
-      const R t = otau[r];
+      const R t = otau[h];
       const R p1 = pr * t, p0 = p1 + pr;
       const R l1 = lr * t, l0 = l1 + lr;
       const R m1 = mxz * t, m0 = m1 + mxz;
-#pragma unroll
-      for (int k = 0; k < L; ++k) {
-        mxx[k] = om[k][r];
-        mzz[k] = om[L + k][r];
-        mxzv[k] = om[2 * L + k][r];
-      }
-      R SMxx = mxx[0], SMzz = mzz[0], SMxz = mxzv[0];
+      R SMxx = om[0][h], SMzz = om[L][h], SMxz = om[2 * L][h];
 #pragma unroll
       for (int k = 1; k < L; ++k) {
-        SMxx = SMxx + mxx[k];
-        SMzz = SMzz + mzz[k];
-        SMxz = SMxz + mxzv[k];
+        SMxx = SMxx + om[k][h];
+        SMzz = SMzz + om[L + k][h];
+        SMxz = SMxz + om[2 * L + k][h];
       }
       if (fs_row) ezz = -(((l0 * exx) + l1 * SMxx) + p1 * SMzz) / p0;
       dsxx = p0 * exx + l0 * ezz + p1 * SMxx + l1 * SMzz;
@@ -344,32 +393,32 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
       dsxz = mxz * gxz;
     }
     for (int s = s0; s < s1; ++s) {
-      if ((size_t)d.src_cell[s] != c || d.src_kind[s] != 1) continue;
+      if ((unsigned)d.src_cell[s] * E != cb || d.src_kind[s] != 1) continue;
       const R val = step < d.snt ? d.src_w[(size_t)d.src_row[s] * d.snt + step] : R(0);
       dsxx = dsxx + val;
       dszz = dszz + val;
     }
-    const R nsxx = osxx[r] + dt * dsxx;
-    R nszz = oszz[r] + dt * dszz;
-    const R nsxz = osxz[r] + dt * dsxz;
+    const R nsxx = osxx[h] + dt * dsxx;
+    R nszz = oszz[h] + dt * dszz;
+    const R nsxz = osxz[h] + dt * dsxz;
     if (fs_row) nszz = R(0);
-    sxx[c] = nsxx;
-    szz[c] = nszz;
-    sxz[c] = nsxz;
+    gst(sxx, cb, nsxx);
+    gst(szz, cb, nszz);
+    gst(sxz, cb, nsxz);
     if constexpr (L > 0) {
 #pragma unroll
       for (int k = 0; k < L; ++k) {
-        M[(size_t)k * plane + c] = mxx[k] + dt * (a1[k] * exx + a2[k] * mxx[k]);
-        M[(size_t)(L + k) * plane + c] = mzz[k] + dt * (a1[k] * ezz + a2[k] * mzz[k]);
-        M[(size_t)(2 * L + k) * plane + c] = mxzv[k] + dt * (a1[k] * gxz + a2[k] * mxzv[k]);
+        gst(Mp[k], cb, om[k][h] + dt * (a1[k] * exx + a2[k] * om[k][h]));
+        gst(Mp[L + k], cb, om[L + k][h] + dt * (a1[k] * ezz + a2[k] * om[L + k][h]));
+        gst(Mp[2 * L + k], cb, om[2 * L + k][h] + dt * (a1[k] * gxz + a2[k] * om[2 * L + k][h]));
       }
     }
     if (fs) {  // stress images above the surface row 4
-      if (j >= 5 && j <= 8) szz[(size_t)(8 - j) * ldx + i] = -nszz;
-      if (j >= 4 && j <= 7) sxz[(size_t)(7 - j) * ldx + i] = -nsxz;
+      if (j >= 5 && j <= 8) gst(szz, (unsigned)((8 - j) * ldx + i) * E, -nszz);
+      if (j >= 4 && j <= 7) gst(sxz, (unsigned)((7 - j) * ldx + i) * E, -nsxz);
     }
     for (int q = r0; q < r1; ++q) {
-      if ((size_t)d.rec_cell[q] != c || step >= d.rec_nt) continue;
+      if ((unsigned)d.rec_cell[q] * E != cb || step >= d.rec_nt) continue;
       const int comp = d.rec_comp[q];
       const R val = comp == 0   ? pvx[0]
                     : comp == 1 ? pvz[0]
