@@ -72,7 +72,7 @@ def run_cpu_baseline(cfg, budget_s=12.0):
                       f"{el:.1f} s on {threads} threads"}
 
 
-def bench_multishot(nshots, nt, graph_steps=100):
+def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True):
     """Batched multi-shot forward (uneven windows, one launch per half step for all shots)
     against the plain single-shot stepper run serially on the same GPU, same work."""
     from viscoelasticinversion_amd import Batch
@@ -81,7 +81,11 @@ def bench_multishot(nshots, nt, graph_steps=100):
     model, sims = build_multishot(mcfg, "float32")
     cells = sum(sh["nx"] for sh in mcfg["shots"]) * mcfg["nz"]
     b = Batch(sims)
-    rows, table = b.autotune(trial_steps=20)
+    if rows:
+        b.set_tiling(rows)
+        table = {}
+    else:
+        rows, table = b.autotune(trial_steps=20)
     b.set_graph(graph_steps)
     b.step(graph_steps)  # capture + warm up
     b.sync()
@@ -109,11 +113,13 @@ def bench_multishot(nshots, nt, graph_steps=100):
         sim.reset()
         sim.set_tiling(rows)
     t0 = time.perf_counter()
-    for sim in sims:
+    for sim in (sims if serial else []):
         sim.step(nt)
         sim.sync()
         done.append(time.perf_counter() - t0)
-    wall_s = time.perf_counter() - t0
+    wall_s = max(time.perf_counter() - t0, 1e-9)
+    if not done:
+        done = [float("nan")]
     for sim in sims:
         sim.close()
     model.close()
@@ -139,16 +145,26 @@ def bench_multishot(nshots, nt, graph_steps=100):
 
 
 def pmc_probe(nshots, steps, copy_bytes=1 << 30):
-    """Short run for rocprofv3 --pmc passes: the batched multi-shot kernels for a few steps
-    plus one device-to-device copy of ``copy_bytes`` (a plain copy kernel of known size,
-    used to scale FETCH_SIZE / WRITE_SIZE to bytes)."""
+    """Short run for rocprofv3 --pmc passes, three phases in order:
+    1. the single-shot 1024 x 512 L=3 workload, 20 steps, rows per wave from VI_ROWS1 (1);
+    2. the batched 64-shot workload, ``steps`` steps, rows per wave from VI_ROWS (2);
+    3. one device-to-device copy of ``copy_bytes`` (a plain copy kernel of known size, used
+       to scale FETCH_SIZE / WRITE_SIZE to bytes)."""
     import ctypes
     from viscoelasticinversion_amd import Batch
-    from viscoelasticinversion_amd.configs import build_multishot, config_multishot
+    from viscoelasticinversion_amd.configs import (build_multishot, build_simulation,
+                                                   config_layered_visco, config_multishot)
+    cfg = config_layered_visco(nt=20)
+    sim = build_simulation(cfg, "float32")
+    sim.set_tiling(int(os.environ.get("VI_ROWS1", "1")))
+    sim.step(20)
+    sim.sync()
+    single = {k: sim.algorithmic_bytes(k) for k in (0, 1)}
+    sim.close()
     mcfg = config_multishot(nshots=nshots, nt=steps)
     model, sims = build_multishot(mcfg, "float32")
     b = Batch(sims)
-    b.set_tiling(int(os.environ.get("VI_ROWS", "4")))
+    b.set_tiling(int(os.environ.get("VI_ROWS", "2")))
     b.step(steps)
     b.sync()
     hip = ctypes.CDLL("libamdhip64.so")
@@ -161,11 +177,12 @@ def pmc_probe(nshots, steps, copy_bytes=1 << 30):
     assert hip.hipDeviceSynchronize() == 0
     hip.hipFree(src)
     hip.hipFree(dst)
-    per_launch = {k: sum(sim.algorithmic_bytes(k) for sim in sims) for k in (0, 1)}
+    batched = {k: sum(s.algorithmic_bytes(k) for s in sims) for k in (0, 1)}
     print(json.dumps({"pmc_probe": True, "shots": nshots, "steps": steps, "rows": b.tiling,
                       "copy_bytes": copy_bytes,
-                      "algorithmic_bytes_per_launch": {"k_velocity": per_launch[0],
-                                                       "k_stress": per_launch[1]}}), flush=True)
+                      "single_shot_algorithmic_bytes": {"k_velocity": single[0], "k_stress": single[1]},
+                      "batched_algorithmic_bytes": {"k_velocity": batched[0], "k_stress": batched[1]}}),
+          flush=True)
     b.close()
 
 
@@ -181,6 +198,12 @@ def main():
     ap.add_argument("--multishot-shots", type=int, default=64)
     ap.add_argument("--multishot-steps", type=int, default=3000)
     ap.add_argument("--no-multishot", action="store_true")
+    ap.add_argument("--rows", type=int, default=0,
+                    help="fix the kernel variant (rows per wavefront) instead of autotuning")
+    ap.add_argument("--no-serial", action="store_true",
+                    help="multi-shot leg: skip the serial plain-stepper comparison")
+    ap.add_argument("--only-multishot", action="store_true",
+                    help="run only the batched multi-shot leg and print its JSON")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short batched run + 1 GiB copy for rocprofv3 --pmc passes")
     ap.add_argument("--profile-every", type=int, default=20,
@@ -195,12 +218,21 @@ def main():
     if args.pmc_probe:
         pmc_probe(args.multishot_shots, 20)
         return
+    if args.only_multishot:
+        print(json.dumps({"multishot": bench_multishot(
+            args.multishot_shots, args.multishot_steps, rows=args.rows,
+            serial=not args.no_serial)}), flush=True)
+        return
     from viscoelasticinversion_amd.configs import build_simulation, config_layered_visco
     nt_total = args.warmup + 2 * args.steps
     cfg = config_layered_visco(nx=args.nx, nz=args.nz, nt=nt_total)
     sim = build_simulation(cfg, "float32")
     cells = cfg["nx"] * cfg["nz"]
-    rows, tune_table = sim.autotune(trial_steps=50)
+    if args.rows:
+        sim.set_tiling(args.rows)
+        rows, tune_table = args.rows, {}
+    else:
+        rows, tune_table = sim.autotune(trial_steps=50)
 
     sim.step(args.warmup)
     sim.sync()
@@ -243,7 +275,8 @@ def main():
              "traces_finite": finite, "rows_per_wave": rows,
              "autotune_us_per_step": {str(k): round(v, 2) for k, v in tune_table.items()}}
     if not args.no_multishot:
-        extra["multishot"] = bench_multishot(args.multishot_shots, args.multishot_steps)
+        extra["multishot"] = bench_multishot(args.multishot_shots, args.multishot_steps,
+                                             serial=not args.no_serial)
     cpu = None
     if not args.no_cpu:
         cfg_cpu = config_layered_visco(nx=args.nx, nz=args.nz, nt=args.steps)
