@@ -140,6 +140,17 @@ int vi_get_traces(vi_handle h, void* out, int nt);
 /* device bytes owned by the handle (state, CPML memory, traces, tables, private model) */
 int vi_device_bytes(vi_handle h, size_t* bytes);
 
+/* ---- gradient by the discrete adjoint (no counterpart in the reference: Enzyme is a
+ * dependency but never called; the ENGD optimiser's grad_fn, EnergyNaturalGradient.jl:246,
+ * is the intended caller). J = 1/2 sum_t sum_r (d_r(t) - dobs_r(t))^2 over nt steps from a
+ * zero initial state with the handle's sources and receivers (capacity >= nt); gradients of J
+ * with respect to the handle's rho, lam, mu, tau arrays ([nz][nx], handle precision; any may
+ * be NULL). Checkpoints every checkpoint_every steps (0 = ceil(sqrt(nt))) and recomputes each
+ * segment, so device memory is about 2 sqrt(nt) states. The handle is left reset (zero state,
+ * step 0). */
+int vi_gradient(vi_handle h, int nt, const void* dobs /*[nrec][nt]*/, int checkpoint_every,
+                double* misfit, void* g_rho, void* g_lam, void* g_mu, void* g_tau);
+
 /* ---- tiling (no counterpart in the reference: KA picks workgroup shapes itself) ------ */
 /* rows per wavefront of the kernel variant: 1, 2, 4 or 8 (results are identical) */
 int vi_set_tiling(vi_handle h, int rows_per_wave);

@@ -5,8 +5,8 @@ The compute path is the HIP library ``libvi_mi355x.so`` behind the C ABI of
 """
 from .api import (CPMLConfig, DirichletBC, Fields, FreeSurfaceBC, IsotropicModel, K,
                   Parameters, Problem, QModel, RickerSource, Solution, _ricker, dK_dw,
-                  find_optimal_taun, get_bc, get_Q, get_tau, leapfrog_step, make_problem,
-                  solve_problem)
+                  find_optimal_taun, get_bc, get_Q, get_tau, gradient_to_velocity_params,
+                  leapfrog_step, make_problem, misfit_and_pullback, solve_problem)
 from .batch import Batch, Model, mem_info
 from .stepper import COMPONENTS, FIELDS, SOURCE_KINDS, Simulation
 
@@ -15,5 +15,5 @@ __all__ = [
     "Parameters", "Problem", "QModel", "RickerSource", "Solution", "_ricker", "dK_dw",
     "find_optimal_taun", "get_bc", "get_Q", "get_tau", "leapfrog_step", "make_problem",
     "solve_problem", "Simulation", "FIELDS", "COMPONENTS", "SOURCE_KINDS", "Batch", "Model",
-    "mem_info",
+    "mem_info", "gradient_to_velocity_params", "misfit_and_pullback",
 ]

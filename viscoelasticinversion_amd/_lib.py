@@ -85,6 +85,7 @@ SIGNATURES = {
     "vi_sync": (_I, [_H]),
     "vi_step_index": (_L, [_H]),
     "vi_get_traces": (_I, [_H, _V, _I]),
+    "vi_gradient": (_I, [_H, _I, _V, _I, ctypes.POINTER(ctypes.c_double), _V, _V, _V, _V]),
     "vi_set_tiling": (_I, [_H, _I]),
     "vi_get_tiling": (_I, [_H]),
     "vi_autotune": (_I, [_H, _I, _IP, _DP]),

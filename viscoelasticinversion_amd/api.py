@@ -357,3 +357,53 @@ def leapfrog_step(fields, parameters=None, dt=None, nsteps=1):
     sim.step(nsteps)
     sim.sync()
     return None
+
+
+# ---------------------------------------------------------------------------- gradients
+def gradient_to_velocity_params(grads, rho, lam, mu, tau=None, Q=None):
+    """Chain rule from the gradient with respect to (rho, lam, mu, tau) to (vp, vs, rho, Q)
+    with lam = rho (vp^2 - 2 vs^2), mu = rho vs^2 and tau = L/(Kbar Q) (Models.jl:67), i.e.
+    d tau / d Q = -tau / Q."""
+    rho = np.asarray(rho, np.float64)
+    vp = np.sqrt((np.asarray(lam) + 2 * np.asarray(mu)) / rho)
+    vs = np.sqrt(np.asarray(mu) / rho)
+    gl, gm, gr = (np.asarray(grads[k], np.float64) for k in ("lam", "mu", "rho"))
+    out = {"vp": gl * 2 * rho * vp,
+           "vs": gl * (-4 * rho * vs) + gm * 2 * rho * vs,
+           "rho": gr + gl * (vp ** 2 - 2 * vs ** 2) + gm * vs ** 2}
+    if tau is not None and Q is not None and "tau" in grads:
+        out["Q"] = np.asarray(grads["tau"], np.float64) * (-np.asarray(tau) / np.asarray(Q))
+    return out
+
+
+def misfit_and_pullback(parameters, dobs, tspan=(0.0, 1.0), checkpoint_every=0):
+    """Forward solve and L2 trace misfit with a pullback for the model, the analogue of a
+    ChainRulesCore rrule for J(model): returns (J, pullback) where pullback(dJ) gives
+    dJ * dJ/d(vp, vs, rho[, Q]) as arrays (Nz, Nx). The gradient comes from the discrete
+    adjoint on the GPU with sqrt(nt) checkpointing."""
+    p = parameters
+    nt = int(round((tspan[1] - tspan[0]) / p.dt))
+    if p.receivers is None:
+        raise ValueError("parameters.receivers is required for a trace misfit")
+    sim = p.build()
+    try:
+        ix = [s.grid[0] for s in p.sources]
+        iz = [s.grid[1] for s in p.sources]
+        kinds = [getattr(s, "kind", "force_z") for s in p.sources]
+        sim.set_sources(ix, iz, kinds, np.stack([s.samples(nt, p.dt, tspan[0]) for s in p.sources]))
+        rix, riz, rc = p.receivers
+        sim.set_receivers(rix, riz, rc, nt)
+        J, g = sim.gradient(dobs, nt, checkpoint_every)
+    finally:
+        sim.close()
+    m = p.model
+    q = p.qmodel
+    Qarr = None
+    if q is not None:
+        Qarr = q.N / (q.K * q.tau)
+    gv = gradient_to_velocity_params(g, m.rho, m.lam, m.mu, q.tau if q is not None else None, Qarr)
+
+    def pullback(dJ=1.0):
+        return {k: dJ * v for k, v in gv.items()}
+
+    return J, pullback

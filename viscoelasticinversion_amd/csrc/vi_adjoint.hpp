@@ -1,0 +1,389 @@
+// vi_adjoint.hpp -- discrete adjoint of the fused velocity-stress step.
+//
+// The forward step n is u^{n+1} = S(V(u^n)) with V the velocity half step and S the stress
+// half step (k_velocity, k_stress). For a fixed model both are affine in the state
+// u = (v, s, M, psi), so the gradient of J = 1/2 sum_n |R u^{n+1} - d_obs^n|^2 follows from
+// the transposed half steps applied backwards in time:
+//   for n = nt-1 .. 0:  u_hat += R^T r^n;  u_hat = S^T u_hat;  u_hat = V^T u_hat
+// with the model gradient accumulated as u_hat_post . d(half step)/d(model) . u_pre.
+// Each transposed half step is a pointwise kernel (adjoint of the constitutive / buoyancy /
+// CPML / memory-variable arithmetic of one cell, which also writes the adjoint of every raw
+// derivative) followed by a gather kernel that applies the transposed 8th-order operators:
+// (D+)^T = -D- and (D-)^T = -D+ on the padded interior, so s_hat += -D-(d_hat) etc.
+// The forward quantities each adjoint kernel needs (strain rates, stress divergence, CPML
+// state, memory variables) are recomputed from the stored forward state of that step.
+// The forward scheme is the one of src/visco-kernels.jl:3-85 with the CPML of
+// src/sciml_solver/CPML.jl:291-328 and the stress imaging of CPML.jl:372-380.
+#pragma once
+#include "vi_kernels.hpp"
+
+namespace vi {
+
+// Everything the adjoint kernels of one shot need.
+template <typename R>
+struct AdjDesc {
+  int nx, nz, ldx, ldm;
+  R dt, Dx[4], Dz[4];
+  R a1[MAXL], a2[MAXL];
+  int L, T, xlo, xhi, zlo, zhi, fs;
+  size_t plane;
+  const R *rho, *lam, *mu, *tau;  // model (pitch ldm)
+  const R *ax, *bx, *kx, *az, *bz, *kz;
+  // forward state of the current step (pitch ldx): stress s^n (pre-step) with images, the
+  // velocity v^{n+1/2} after the velocity half, memory variables M^n, CPML memory psi^n
+  const R *fsxx, *fszz, *fsxz, *fvx, *fvz, *fM;
+  const R* fpsi[8];
+  // sources of the shot (flat list) and the sample index of this step
+  int nsrc, snt;
+  const int *src_cell, *src_kind;
+  const R* src_w;
+  long step;
+  // adjoint state (pitch ldx) and per-step work arrays
+  R *avx, *avz, *asxx, *aszz, *asxz, *aM;
+  R* apsi[8];
+  R* ehat[4];  // adjoint of raw D-x vx, D+z vx, D-z vz, D+x vz (stress half)
+  R* dhat[4];  // adjoint of raw D+x sxx, D-z sxz, D-x sxz, D+z szz (velocity half)
+  // gradient accumulators (pitch ldx): lam, mu (through pi), mu_xz, tau, bx, bz
+  R *g_lam, *g_mu, *g_muxz, *g_tau, *g_bx, *g_bz;
+};
+
+template <typename R>
+__device__ __forceinline__ R gp(const R* g, unsigned cb, int stb, const R* D) {
+  R v = D[0] * (gld(g, cb + stb) - gld(g, cb));
+  v = v + D[1] * (gld(g, cb + 2 * stb) - gld(g, cb - stb));
+  v = v + D[2] * (gld(g, cb + 3 * stb) - gld(g, cb - 2 * stb));
+  v = v + D[3] * (gld(g, cb + 4 * stb) - gld(g, cb - 3 * stb));
+  return v;
+}
+template <typename R>
+__device__ __forceinline__ R gm(const R* g, unsigned cb, int stb, const R* D) {
+  R v = D[0] * (gld(g, cb) - gld(g, cb - stb));
+  v = v + D[1] * (gld(g, cb + stb) - gld(g, cb - 2 * stb));
+  v = v + D[2] * (gld(g, cb + 2 * stb) - gld(g, cb - 3 * stb));
+  v = v + D[3] * (gld(g, cb + 3 * stb) - gld(g, cb - 4 * stb));
+  return v;
+}
+
+// forward CPML on a raw derivative d with memory p (pre-step): returns the stretched value
+__device__ __forceinline__ float cpml_fwd(float d, float p, float a, float b, float k) {
+  return d / k + (b * p + a * d);
+}
+__device__ __forceinline__ double cpml_fwd(double d, double p, double a, double b, double k) {
+  return d / k + (b * p + a * d);
+}
+// adjoint of (p' = b p + a d; dt = d/k + p'): given the adjoint of the stretched value (h)
+// and of p' (ph), returns the adjoint of d and writes the adjoint of p into *ph
+template <typename R>
+__device__ __forceinline__ R cpml_adj(R h, R* ph, R a, R b, R k) {
+  const R tot = *ph + h;
+  *ph = b * tot;
+  return h / k + a * tot;
+}
+
+// ---- adjoint of the stress half step (pointwise part) --------------------------------
+template <typename R, int L>
+__global__ __launch_bounds__(256) void k_adj_stress_point(const AdjDesc<R> d) {
+  constexpr unsigned E = sizeof(R);
+  constexpr int LM = L > 0 ? L : 1;
+  const int i = 4 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 4 + blockIdx.y * 4 + threadIdx.y;
+  if (i >= d.nx - 4 || j >= d.nz - 4) return;
+  const int ldx = d.ldx;
+  const unsigned cb = (unsigned)(j * ldx + i) * E;
+  const unsigned mb = (unsigned)(j * d.ldm + i) * E;
+  const int rb = ldx * (int)E;
+  const R dt = d.dt, two = R(2), quarter = R(0.25);
+  // forward strain rates of this step (CPML-stretched)
+  R exx = gm(d.fvx, cb, (int)E, d.Dx), ezx = gp(d.fvx, cb, rb, d.Dz);
+  R ezz = gm(d.fvz, cb, rb, d.Dz), exz = gp(d.fvz, cb, (int)E, d.Dx);
+  const int xk = d.T ? strip_x(i, d.nx, d.T, d.xlo, d.xhi) : -1;
+  const int zk = d.T ? strip_x(j, d.nz, d.T, d.zlo, d.zhi) : -1;
+  const unsigned qx = (unsigned)(j * 2 * d.T + xk) * E, qz = (unsigned)(zk * ldx + i) * E;
+  R axv = 0, bxv = 0, kxv = 1, azv = 0, bzv = 0, kzv = 1;
+  if (xk >= 0) {
+    axv = gld(d.ax, i * E); bxv = gld(d.bx, i * E); kxv = gld(d.kx, i * E);
+    exx = cpml_fwd(exx, gld(d.fpsi[0], qx), axv, bxv, kxv);
+    exz = cpml_fwd(exz, gld(d.fpsi[1], qx), axv, bxv, kxv);
+  }
+  if (zk >= 0) {
+    azv = d.az[j]; bzv = d.bz[j]; kzv = d.kz[j];
+    ezx = cpml_fwd(ezx, gld(d.fpsi[4], qz), azv, bzv, kzv);
+    ezz = cpml_fwd(ezz, gld(d.fpsi[5], qz), azv, bzv, kzv);
+  }
+  const R gxz = ezx + exz;
+  const R lr = gld(d.lam, mb), mr = gld(d.mu, mb);
+  const R mxz = quarter * (((mr + gld(d.mu, mb + E)) + gld(d.mu, mb + d.ldm * E)) +
+                           gld(d.mu, mb + d.ldm * E + E));
+  const R pr = lr + two * mr;
+  const R t = L > 0 ? gld(d.tau, mb) : R(0);
+  const R p1 = pr * t, p0 = p1 + pr, l1 = lr * t, l0 = l1 + lr, m1 = mxz * t, m0 = m1 + mxz;
+  R SMxx = 0, SMzz = 0, SMxz = 0;
+  R* Mf[3 * LM];
+  for (int k = 0; k < 3 * L; ++k) Mf[k] = (R*)d.fM + (size_t)k * d.plane;
+  for (int k = 0; k < L; ++k) {
+    SMxx = SMxx + gld((const R*)Mf[k], cb);
+    SMzz = SMzz + gld((const R*)Mf[L + k], cb);
+    SMxz = SMxz + gld((const R*)Mf[2 * L + k], cb);
+  }
+  const bool fs_row = d.fs && j == 4;
+  const R N = l0 * exx + l1 * SMxx + p1 * SMzz;  // free-surface row: ezz* = -N/p0
+  const R ezs = fs_row ? -N / p0 : ezz;
+  // stress images: their adjoint goes to the interior rows they copy, then they are cleared
+  if (d.fs) {
+    if (j >= 5 && j <= 8) {
+      const unsigned ib = (unsigned)((8 - j) * ldx + i) * E;
+      gst(d.aszz, cb, gld((const R*)d.aszz, cb) - gld((const R*)d.aszz, ib));
+      gst(d.aszz, ib, R(0));
+    }
+    if (j >= 4 && j <= 7) {
+      const unsigned ib = (unsigned)((7 - j) * ldx + i) * E;
+      gst(d.asxz, cb, gld((const R*)d.asxz, cb) - gld((const R*)d.asxz, ib));
+      gst(d.asxz, ib, R(0));
+    }
+  }
+  const R sxxh = gld((const R*)d.asxx, cb);
+  const R szzh = fs_row ? R(0) : gld((const R*)d.aszz, cb);
+  const R sxzh = gld((const R*)d.asxz, cb);
+  if (fs_row) gst(d.aszz, cb, R(0));
+  // adjoints of the (stretched) rates and of the memory sums
+  R exh = dt * (p0 * sxxh + l0 * szzh);
+  R ezh = dt * (l0 * sxxh + p0 * szzh);
+  R gh = dt * (m0 * sxzh);
+  const R SMxxh = dt * (p1 * sxxh + l1 * szzh);
+  const R SMzzh = dt * (l1 * sxxh + p1 * szzh);
+  const R SMxzh = dt * (m1 * sxzh);
+  R Mzzh_sum = 0;  // sum_n a1_n * M_hat'_zz,n (free-surface gradient term)
+  R* Ma[3 * LM];
+  for (int k = 0; k < 3 * L; ++k) Ma[k] = d.aM + (size_t)k * d.plane;
+  R fsMxx = 0, fsMzz = 0;  // extra adjoint of each M_n through ezz* on the free-surface row
+  R ezh_from_M = 0;
+  for (int k = 0; k < L; ++k) {
+    const R hxx = gld((const R*)Ma[k], cb), hzz = gld((const R*)Ma[L + k], cb),
+            hxz = gld((const R*)Ma[2 * L + k], cb);
+    exh = exh + dt * d.a1[k] * hxx;
+    ezh_from_M = ezh_from_M + dt * d.a1[k] * hzz;
+    gh = gh + dt * d.a1[k] * hxz;
+    Mzzh_sum = Mzzh_sum + d.a1[k] * hzz;
+    gst(Ma[k], cb, (R(1) + dt * d.a2[k]) * hxx + SMxxh);
+    gst(Ma[L + k], cb, (R(1) + dt * d.a2[k]) * hzz + SMzzh);
+    gst(Ma[2 * L + k], cb, (R(1) + dt * d.a2[k]) * hxz + SMxzh);
+  }
+  ezh = ezh + ezh_from_M;
+  R ezz_raw_h = ezh;
+  if (fs_row) {  // ezz* = -(l0 exx + l1 SMxx + p1 SMzz)/p0 replaces the raw rate
+    exh = exh - l0 / p0 * ezh;
+    fsMxx = -l1 / p0 * ezh;
+    fsMzz = -p1 / p0 * ezh;
+    for (int k = 0; k < L; ++k) {
+      gst(Ma[k], cb, gld((const R*)Ma[k], cb) + fsMxx);
+      gst(Ma[L + k], cb, gld((const R*)Ma[L + k], cb) + fsMzz);
+    }
+    ezz_raw_h = 0;
+  }
+  // gradient with respect to lam, mu (through pi), mu_xz and tau of this cell
+  {
+    const R tt = R(1) + t;
+    R glam, gmu, gtau;
+    if (!fs_row) {
+      const R cl = dt * (tt * (exx + ezz) + t * (SMxx + SMzz));
+      glam = sxxh * cl + szzh * cl;
+      gmu = sxxh * dt * (two * tt * exx + two * t * SMxx) +
+            szzh * dt * (two * tt * ezz + two * t * SMzz);
+      gtau = sxxh * dt * (pr * exx + lr * ezz + pr * SMxx + lr * SMzz) +
+             szzh * dt * (lr * exx + pr * ezz + lr * SMxx + pr * SMzz);
+    } else {
+      // sxx' = sxx + dt (A + l0 z), z = -N/p0; Mzz'_n depends on z through dt a1_n z
+      const R z = ezs;
+      const R dNl = tt * exx + t * SMxx + t * SMzz, dNm = two * t * SMzz,
+              dNt = lr * exx + lr * SMxx + pr * SMzz;
+      const R dPl = tt, dPm = two * tt, dPt = pr;
+      const R dzl = -dNl / p0 + N / (p0 * p0) * dPl;
+      const R dzm = -dNm / p0 + N / (p0 * p0) * dPm;
+      const R dzt = -dNt / p0 + N / (p0 * p0) * dPt;
+      const R dAl = tt * exx + t * SMxx + t * SMzz, dAm = two * tt * exx + two * t * SMxx,
+              dAt = pr * exx + pr * SMxx + lr * SMzz;
+      glam = sxxh * dt * (dAl + tt * z + l0 * dzl) + dt * Mzzh_sum * dzl;
+      gmu = sxxh * dt * (dAm + l0 * dzm) + dt * Mzzh_sum * dzm;
+      gtau = sxxh * dt * (dAt + lr * z + l0 * dzt) + dt * Mzzh_sum * dzt;
+    }
+    gtau = gtau + sxzh * dt * (mxz * gxz + mxz * SMxz);
+    gst(d.g_lam, cb, gld((const R*)d.g_lam, cb) + glam);
+    gst(d.g_mu, cb, gld((const R*)d.g_mu, cb) + gmu);
+    gst(d.g_muxz, cb, gld((const R*)d.g_muxz, cb) + sxzh * dt * (tt * gxz + t * SMxz));
+    if (L > 0) gst(d.g_tau, cb, gld((const R*)d.g_tau, cb) + gtau);
+  }
+  // CPML adjoint and the adjoints of the raw derivatives
+  R exx_r = exh, exz_r = gh, ezx_r = gh, ezz_r = ezz_raw_h;
+  if (xk >= 0) {
+    R p0h = gld((const R*)d.apsi[0], qx), p1h = gld((const R*)d.apsi[1], qx);
+    exx_r = cpml_adj(exh, &p0h, axv, bxv, kxv);
+    exz_r = cpml_adj(gh, &p1h, axv, bxv, kxv);
+    gst(d.apsi[0], qx, p0h);
+    gst(d.apsi[1], qx, p1h);
+  }
+  if (zk >= 0) {
+    R p4h = gld((const R*)d.apsi[4], qz), p5h = gld((const R*)d.apsi[5], qz);
+    ezx_r = cpml_adj(gh, &p4h, azv, bzv, kzv);
+    ezz_r = cpml_adj(ezz_raw_h, &p5h, azv, bzv, kzv);
+    gst(d.apsi[4], qz, p4h);
+    gst(d.apsi[5], qz, p5h);
+  }
+  gst(d.ehat[0], cb, exx_r);
+  gst(d.ehat[1], cb, ezx_r);
+  gst(d.ehat[2], cb, ezz_r);
+  gst(d.ehat[3], cb, exz_r);
+}
+
+// ---- adjoint of the stress half step (transposed operators): v_hat -= D^T e_hat --------
+template <typename R>
+__global__ __launch_bounds__(256) void k_adj_stress_gather(const AdjDesc<R> d) {
+  constexpr unsigned E = sizeof(R);
+  const int i = 4 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 4 + blockIdx.y * 4 + threadIdx.y;
+  if (i >= d.nx - 4 || j >= d.nz - 4) return;
+  const unsigned cb = (unsigned)(j * d.ldx + i) * E;
+  const int rb = d.ldx * (int)E;
+  // exx = D-x vx, ezx = D+z vx, ezz = D-z vz, exz = D+x vz
+  const R gvx = gp((const R*)d.ehat[0], cb, (int)E, d.Dx) + gm((const R*)d.ehat[1], cb, rb, d.Dz);
+  const R gvz = gp((const R*)d.ehat[2], cb, rb, d.Dz) + gm((const R*)d.ehat[3], cb, (int)E, d.Dx);
+  gst(d.avx, cb, gld((const R*)d.avx, cb) - gvx);
+  gst(d.avz, cb, gld((const R*)d.avz, cb) - gvz);
+}
+
+// ---- adjoint of the velocity half step (pointwise part) ------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_adj_vel_point(const AdjDesc<R> d) {
+  constexpr unsigned E = sizeof(R);
+  const int i = 4 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 4 + blockIdx.y * 4 + threadIdx.y;
+  if (i >= d.nx - 4 || j >= d.nz - 4) return;
+  const int ldx = d.ldx;
+  const unsigned cb = (unsigned)(j * ldx + i) * E;
+  const unsigned mb = (unsigned)(j * d.ldm + i) * E;
+  const int rb = ldx * (int)E;
+  const R dt = d.dt, two = R(2);
+  R dxx = gp(d.fsxx, cb, (int)E, d.Dx), dxzz = gm(d.fsxz, cb, rb, d.Dz);
+  R dxzx = gm(d.fsxz, cb, (int)E, d.Dx), dzz = gp(d.fszz, cb, rb, d.Dz);
+  const int xk = d.T ? strip_x(i, d.nx, d.T, d.xlo, d.xhi) : -1;
+  const int zk = d.T ? strip_x(j, d.nz, d.T, d.zlo, d.zhi) : -1;
+  const unsigned qx = (unsigned)(j * 2 * d.T + xk) * E, qz = (unsigned)(zk * ldx + i) * E;
+  R axv = 0, bxv = 0, kxv = 1, azv = 0, bzv = 0, kzv = 1;
+  if (xk >= 0) {
+    axv = gld(d.ax, i * E); bxv = gld(d.bx, i * E); kxv = gld(d.kx, i * E);
+    dxx = cpml_fwd(dxx, gld(d.fpsi[2], qx), axv, bxv, kxv);
+    dxzx = cpml_fwd(dxzx, gld(d.fpsi[3], qx), axv, bxv, kxv);
+  }
+  if (zk >= 0) {
+    azv = d.az[j]; bzv = d.bz[j]; kzv = d.kz[j];
+    dxzz = cpml_fwd(dxzz, gld(d.fpsi[6], qz), azv, bzv, kzv);
+    dzz = cpml_fwd(dzz, gld(d.fpsi[7], qz), azv, bzv, kzv);
+  }
+  R fx = dxx + dxzz, fz = dzz + dxzx;
+  for (int s = 0; s < d.nsrc; ++s) {
+    if ((unsigned)d.src_cell[s] * E != cb) continue;
+    const R val = d.step < d.snt ? d.src_w[(size_t)s * d.snt + d.step] : R(0);
+    if (d.src_kind[s] == 0) fz = fz + val;
+    else if (d.src_kind[s] == 2) fx = fx + val;
+  }
+  const R rho = gld(d.rho, mb);
+  const R bxb = two / (rho + gld(d.rho, mb + E));
+  const R bzb = two / (rho + gld(d.rho, mb + d.ldm * E));
+  const R vxh = gld((const R*)d.avx, cb), vzh = gld((const R*)d.avz, cb);
+  // gradient with respect to the buoyancies (scattered to rho at the end)
+  gst(d.g_bx, cb, gld((const R*)d.g_bx, cb) + vxh * dt * fx);
+  gst(d.g_bz, cb, gld((const R*)d.g_bz, cb) + vzh * dt * fz);
+  const R hx = dt * bxb * vxh, hz = dt * bzb * vzh;
+  R hxx = hx, hxzz = hx, hzz = hz, hxzx = hz;
+  if (xk >= 0) {
+    R p2h = gld((const R*)d.apsi[2], qx), p3h = gld((const R*)d.apsi[3], qx);
+    hxx = cpml_adj(hx, &p2h, axv, bxv, kxv);
+    hxzx = cpml_adj(hz, &p3h, axv, bxv, kxv);
+    gst(d.apsi[2], qx, p2h);
+    gst(d.apsi[3], qx, p3h);
+  }
+  if (zk >= 0) {
+    R p6h = gld((const R*)d.apsi[6], qz), p7h = gld((const R*)d.apsi[7], qz);
+    hxzz = cpml_adj(hx, &p6h, azv, bzv, kzv);
+    hzz = cpml_adj(hz, &p7h, azv, bzv, kzv);
+    gst(d.apsi[6], qz, p6h);
+    gst(d.apsi[7], qz, p7h);
+  }
+  gst(d.dhat[0], cb, hxx);
+  gst(d.dhat[1], cb, hxzz);
+  gst(d.dhat[2], cb, hxzx);
+  gst(d.dhat[3], cb, hzz);
+}
+
+// ---- adjoint of the velocity half step (transposed operators): s_hat -= D^T d_hat ------
+// Rows 0..3 are the free-surface image rows, whose adjoint is collected here and moved to
+// the interior rows they copy by the next k_adj_stress_point.
+template <typename R>
+__global__ __launch_bounds__(256) void k_adj_vel_gather(const AdjDesc<R> d, int row0) {
+  constexpr unsigned E = sizeof(R);
+  const int i = 4 + blockIdx.x * 64 + threadIdx.x;
+  const int j = row0 + blockIdx.y * 4 + threadIdx.y;
+  if (i >= d.nx - 4 || j >= d.nz - 4) return;
+  const unsigned cb = (unsigned)(j * d.ldx + i) * E;
+  const int rb = d.ldx * (int)E;
+  // dxx = D+x sxx, dxzz = D-z sxz, dxzx = D-x sxz, dzz = D+z szz
+  const R gsxx = gm((const R*)d.dhat[0], cb, (int)E, d.Dx);
+  const R gsxz = gp((const R*)d.dhat[1], cb, rb, d.Dz) + gp((const R*)d.dhat[2], cb, (int)E, d.Dx);
+  const R gszz = gm((const R*)d.dhat[3], cb, rb, d.Dz);
+  gst(d.asxx, cb, gld((const R*)d.asxx, cb) - gsxx);
+  gst(d.asxz, cb, gld((const R*)d.asxz, cb) - gsxz);
+  gst(d.aszz, cb, gld((const R*)d.aszz, cb) - gszz);
+}
+
+// ---- residual injection: u_hat += R^T r^n (receivers sample after the stress half) ----
+template <typename R>
+__global__ void k_adj_inject(R* f0, R* f1, R* f2, R* f3, R* f4, const int* cell,
+                             const int* comp, int nrec, const R* resid_row) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;  // one thread: deterministic order
+  R* f[5] = {f0, f1, f2, f3, f4};
+  for (int r = 0; r < nrec; ++r) {
+    R* p = f[comp[r]] + cell[r];
+    *p = *p + resid_row[r];
+  }
+}
+
+// ---- gradient finalisation: mu_xz and buoyancy accumulators to mu and rho --------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_grad_finalize(const AdjDesc<R> d, R* g_rho_out,
+                                                       R* g_mu_out) {
+  constexpr unsigned E = sizeof(R);
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= d.nx || j >= d.nz) return;
+  const unsigned cb = (unsigned)(j * d.ldx + i) * E;
+  const unsigned mb = (unsigned)(j * d.ldm + i) * E;
+  const int rb = d.ldx * (int)E;
+  const int mrb = d.ldm * (int)E;
+  const R quarter = R(0.25), two = R(2);
+  // mu_xz(c') = 1/4 (mu(c') + mu(c'+1) + mu(c'+ld) + mu(c'+ld+1)): mu(c) feeds c, c-1,
+  // c-ld, c-ld-1
+  R gm_ = gld((const R*)d.g_mu, cb);
+  R gx = gld((const R*)d.g_muxz, cb);
+  if (i >= 1) gx = gx + gld((const R*)d.g_muxz, cb - E);
+  if (j >= 1) gx = gx + gld((const R*)d.g_muxz, cb - rb);
+  if (i >= 1 && j >= 1) gx = gx + gld((const R*)d.g_muxz, cb - rb - E);
+  gst(g_mu_out, cb, gm_ + quarter * gx);
+  // bx(c) = 2/(rho(c) + rho(c+1)), bz(c) = 2/(rho(c) + rho(c+ld)); d b / d rho = -b^2/2
+  auto bxat = [&](unsigned m) { return two / (gld(d.rho, m) + gld(d.rho, m + E)); };
+  auto bzat = [&](unsigned m) { return two / (gld(d.rho, m) + gld(d.rho, m + mrb)); };
+  R gr = 0;
+  R b = bxat(mb);
+  gr = gr - gld((const R*)d.g_bx, cb) * b * b / two;
+  b = bzat(mb);
+  gr = gr - gld((const R*)d.g_bz, cb) * b * b / two;
+  if (i >= 1) {
+    b = bxat(mb - E);
+    gr = gr - gld((const R*)d.g_bx, cb - E) * b * b / two;
+  }
+  if (j >= 1) {
+    b = bzat(mb - mrb);
+    gr = gr - gld((const R*)d.g_bz, cb - rb) * b * b / two;
+  }
+  gst(g_rho_out, cb, gr);
+}
+
+}  // namespace vi

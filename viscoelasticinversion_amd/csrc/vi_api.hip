@@ -7,6 +7,7 @@
 // group), optionally replaying hipGraphs of a block of steps.
 #include "vi_mi355x.h"
 #include "vi_kernels.hpp"
+#include "vi_adjoint.hpp"
 
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
@@ -628,6 +629,196 @@ int batch_finish(vi_batch_s* b, int nsteps) {
   return 0;
 }
 
+
+// ---------------------------------------------------------------- gradient (adjoint)
+template <typename R>
+int gradient_impl(vi_sim_s* h, int nt, const R* dobs, int K, double* misfit, R* out_rho,
+                  R* out_lam, R* out_mu, R* out_tau) {
+  const size_t E = sizeof(R);
+  const int L = h->L;
+  const size_t pb = h->plane * E;
+  hipStream_t st = h->stream;
+  VI_TRY(vi_reset(h));
+  VI_TRY(refresh_own_desc(h));
+  if (K <= 0) K = std::max(1, (int)std::ceil(std::sqrt((double)nt)));
+  K = std::min(K, nt);
+  const int nck = (nt + K - 1) / K;
+  // forward state arrays of the handle, in a fixed order
+  std::vector<DevBuf*> live = mutable_arrays(h);
+  live.pop_back();  // traces are not part of the checkpointed state
+  size_t state_bytes = 0;
+  for (auto* b : live) state_bytes += b->bytes;
+  DevBuf ck;
+  VI_TRY(ck.alloc(state_bytes * nck, st));
+  // per-step slot: s (3 planes), v^{n+1/2} (2), M (3L), psi (8 strips)
+  size_t psi_bytes[8];
+  size_t psi_total = 0;
+  for (int q = 0; q < 8; ++q) psi_total += (psi_bytes[q] = h->psi[q].bytes);
+  const size_t slot_bytes = (5 + 3 * (size_t)L) * pb + psi_total;
+  DevBuf seg;
+  VI_TRY(seg.alloc(slot_bytes * K, st));
+  // adjoint state, work arrays and gradient accumulators
+  // ehat/dhat planes get 4 zero guard rows in front: the transposed z-operators of the
+  // free-surface image rows (rows 0..3) read rows -4..-1, where the derivative adjoints are 0
+  DevBuf adj;
+  const size_t guard = 4 * (size_t)h->ldx * E;
+  const size_t nplanes_adj = 5 + 3 * (size_t)L + 4 + 4 + 6 + 2;
+  VI_TRY(adj.alloc(nplanes_adj * pb + 8 * guard + psi_total, st));
+  char* ab = (char*)adj.p;
+  auto take = [&](size_t bytes) { char* p = ab; ab += bytes; return (R*)p; };
+  auto take_guarded = [&]() { ab += guard; return take(pb); };
+  vi::AdjDesc<R> d{};
+  d.avx = take(pb); d.avz = take(pb); d.asxx = take(pb); d.aszz = take(pb); d.asxz = take(pb);
+  d.aM = take(3 * (size_t)L * pb);
+  for (int q = 0; q < 4; ++q) d.ehat[q] = take_guarded();
+  for (int q = 0; q < 4; ++q) d.dhat[q] = take_guarded();
+  d.g_lam = take(pb); d.g_mu = take(pb); d.g_muxz = take(pb); d.g_tau = take(pb);
+  d.g_bx = take(pb); d.g_bz = take(pb);
+  R* g_rho_out = take(pb);
+  R* g_mu_out = take(pb);
+  for (int q = 0; q < 8; ++q) d.apsi[q] = take(psi_bytes[q]);
+  // residual [nt][nrec], flat receiver and source lists
+  DevBuf resid, lists;
+  const int nrec = h->nrec, nsrc = h->nsrc;
+  VI_TRY(resid.alloc((size_t)nt * nrec * E, st));
+  std::vector<int> lst(2 * (size_t)nrec + 2 * (size_t)nsrc + 1);
+  for (int r = 0; r < nrec; ++r) {
+    lst[r] = h->rec_iz[r] * h->ldx + h->rec_ix[r];
+    lst[nrec + r] = h->rec_comp[r];
+  }
+  for (int q = 0; q < nsrc; ++q) {
+    lst[2 * nrec + q] = h->src_iz[q] * h->ldx + h->src_ix[q];
+    lst[2 * nrec + nsrc + q] = h->src_kind[q];
+  }
+  VI_TRY(lists.alloc(lst.size() * sizeof(int), st));
+  VI_HIP(hipMemcpyAsync(lists.p, lst.data(), lst.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  const int* rec_cell = (const int*)lists.p;
+  const int* rec_comp = rec_cell + nrec;
+  // static part of the adjoint descriptor
+  d.nx = h->g.nx; d.nz = h->g.nz; d.ldx = h->ldx; d.ldm = h->model->ldm;
+  {
+    const vi::ShotDesc<R>* sd = (const vi::ShotDesc<R>*)h->desc_host.data();
+    d.dt = sd->dt;
+    for (int l = 0; l < 4; ++l) { d.Dx[l] = sd->Dx[l]; d.Dz[l] = sd->Dz[l]; }
+    for (int k = 0; k < L; ++k) { d.a1[k] = sd->a1[k]; d.a2[k] = sd->a2[k]; }
+    d.T = sd->T; d.xlo = sd->xlo; d.xhi = sd->xhi; d.zlo = sd->zlo; d.zhi = sd->zhi; d.fs = sd->fs;
+    d.rho = sd->rho; d.lam = sd->lam; d.mu = sd->mu; d.tau = sd->tau;
+    d.ax = sd->ax; d.bx = sd->bx; d.kx = sd->kx; d.az = sd->az; d.bz = sd->bz; d.kz = sd->kz;
+    d.src_w = sd->src_w; d.snt = sd->snt;
+  }
+  d.L = L;
+  d.plane = h->plane;
+  d.nsrc = nsrc;
+  d.src_cell = (const int*)lists.p + 2 * nrec;
+  d.src_kind = d.src_cell + nsrc;
+  auto copy_state = [&](char* dst, bool to_ck) -> int {
+    size_t off = 0;
+    for (auto* b : live) {
+      if (!b->p) continue;
+      if (to_ck) VI_HIP(hipMemcpyAsync(dst + off, b->p, b->bytes, hipMemcpyDeviceToDevice, st));
+      else VI_HIP(hipMemcpyAsync(b->p, dst + off, b->bytes, hipMemcpyDeviceToDevice, st));
+      off += b->bytes;
+    }
+    return 0;
+  };
+  // ---- forward pass with checkpoints
+  for (int s = 0; s < nck; ++s) {
+    VI_TRY(copy_state((char*)ck.p + (size_t)s * state_bytes, true));
+    VI_TRY(launch_steps(st, E, h->rz, L, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
+                        h->ntiles, 1, std::min(K, nt - s * K), nullptr));
+  }
+  h->step = nt;
+  // residual and misfit (host, double accumulation)
+  std::vector<R> tr((size_t)nt * nrec);
+  VI_HIP(hipMemcpyAsync(tr.data(), h->traces.p, tr.size() * E, hipMemcpyDeviceToHost, st));
+  VI_HIP(hipStreamSynchronize(st));
+  double J = 0.0;
+  for (int r = 0; r < nrec; ++r)
+    for (int t = 0; t < nt; ++t) {
+      const R res = tr[(size_t)t * nrec + r] - dobs[(size_t)r * nt + t];
+      tr[(size_t)t * nrec + r] = res;
+      J += 0.5 * (double)res * (double)res;
+    }
+  if (misfit) *misfit = J;
+  VI_HIP(hipMemcpyAsync(resid.p, tr.data(), tr.size() * E, hipMemcpyHostToDevice, st));
+  // ---- backward pass, one checkpoint segment at a time
+  const dim3 blk(64, 4);
+  const dim3 gint((d.nx - 8 + 63) / 64, (d.nz - 8 + 3) / 4);
+  const int row0 = d.fs ? 0 : 4;
+  const dim3 gvg((d.nx - 8 + 63) / 64, (d.nz - 4 - row0 + 3) / 4);
+  R* M0 = (R*)h->M.p;
+  for (int s = nck - 1; s >= 0; --s) {
+    const int n0 = s * K, n1 = std::min(n0 + K, nt);
+    VI_TRY(copy_state((char*)ck.p + (size_t)s * state_bytes, false));
+    h->step = n0;
+    touch(h);
+    VI_TRY(refresh_own_desc(h));
+    for (int n = n0; n < n1; ++n) {  // recompute the segment, storing each step's inputs
+      char* slot = (char*)seg.p + (size_t)(n - n0) * slot_bytes;
+      for (int q = 0; q < 3; ++q)
+        VI_HIP(hipMemcpyAsync(slot + q * pb, h->state[2 + q].p, pb, hipMemcpyDeviceToDevice, st));
+      if (L)
+        VI_HIP(hipMemcpyAsync(slot + 5 * pb, M0, 3 * (size_t)L * pb, hipMemcpyDeviceToDevice, st));
+      size_t po = (5 + 3 * (size_t)L) * pb;
+      for (int q = 0; q < 8; ++q) {
+        if (psi_bytes[q])
+          VI_HIP(hipMemcpyAsync(slot + po, h->psi[q].p, psi_bytes[q], hipMemcpyDeviceToDevice, st));
+        po += psi_bytes[q];
+      }
+      VI_TRY(launch_one<R>(st, h->rz, L, 0, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
+                           h->ntiles, n - n0, nullptr, nullptr));
+      for (int q = 0; q < 2; ++q)
+        VI_HIP(hipMemcpyAsync(slot + (3 + q) * pb, h->state[q].p, pb, hipMemcpyDeviceToDevice, st));
+      VI_TRY(launch_one<R>(st, h->rz, L, 1, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
+                           h->ntiles, n - n0, nullptr, nullptr));
+    }
+    for (int n = n1 - 1; n >= n0; --n) {
+      const char* slot = (const char*)seg.p + (size_t)(n - n0) * slot_bytes;
+      d.fsxx = (const R*)(slot);
+      d.fszz = (const R*)(slot + pb);
+      d.fsxz = (const R*)(slot + 2 * pb);
+      d.fvx = (const R*)(slot + 3 * pb);
+      d.fvz = (const R*)(slot + 4 * pb);
+      d.fM = (const R*)(slot + 5 * pb);
+      size_t po = (5 + 3 * (size_t)L) * pb;
+      for (int q = 0; q < 8; ++q) {
+        d.fpsi[q] = (const R*)(slot + po);
+        po += psi_bytes[q];
+      }
+      d.step = n;
+      if (nrec) {
+        hipLaunchKernelGGL(vi::k_adj_inject<R>, dim3(1), dim3(1), 0, st, d.avx, d.avz, d.asxx,
+                           d.aszz, d.asxz, rec_cell, rec_comp, nrec,
+                           (const R*)resid.p + (size_t)n * nrec);
+      }
+      switch (L) {
+        case 0: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 0>), gint, blk, 0, st, d); break;
+        case 1: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 1>), gint, blk, 0, st, d); break;
+        case 2: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 2>), gint, blk, 0, st, d); break;
+        case 3: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 3>), gint, blk, 0, st, d); break;
+        case 4: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 4>), gint, blk, 0, st, d); break;
+        case 5: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 5>), gint, blk, 0, st, d); break;
+        case 6: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 6>), gint, blk, 0, st, d); break;
+      }
+      hipLaunchKernelGGL(vi::k_adj_stress_gather<R>, gint, blk, 0, st, d);
+      hipLaunchKernelGGL(vi::k_adj_vel_point<R>, gint, blk, 0, st, d);
+      hipLaunchKernelGGL(vi::k_adj_vel_gather<R>, gvg, blk, 0, st, d, row0);
+      VI_HIP(hipGetLastError());
+    }
+  }
+  const dim3 gall((d.nx + 63) / 64, (d.nz + 3) / 4);
+  hipLaunchKernelGGL(vi::k_grad_finalize<R>, gall, blk, 0, st, d, g_rho_out, g_mu_out);
+  VI_HIP(hipGetLastError());
+  const int nx = h->g.nx, nz = h->g.nz;
+  if (out_rho) VI_TRY(download_plane(st, out_rho, g_rho_out, h->ldx, nx, nz, E));
+  if (out_lam) VI_TRY(download_plane(st, out_lam, d.g_lam, h->ldx, nx, nz, E));
+  if (out_mu) VI_TRY(download_plane(st, out_mu, g_mu_out, h->ldx, nx, nz, E));
+  if (out_tau && L) VI_TRY(download_plane(st, out_tau, d.g_tau, h->ldx, nx, nz, E));
+  VI_HIP(hipStreamSynchronize(st));
+  // leave the handle at a zero state, step 0
+  return vi_reset(h);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1125,6 +1316,24 @@ int vi_algorithmic_bytes(vi_handle h, int kernel, double* bytes) {
   }
   *bytes = total * (double)h->esz;
   return 0;
+}
+
+
+int vi_gradient(vi_handle h, int nt, const void* dobs, int checkpoint_every, double* misfit,
+                void* g_rho, void* g_lam, void* g_mu, void* g_tau) {
+  VI_TRY(check_handle(h));
+  if (nt < 1 || !dobs || checkpoint_every < 0)
+    return fail(VI_EINVAL, "vi_gradient: need nt >= 1, dobs, checkpoint_every >= 0");
+  VI_TRY(check_ready(h));
+  if (h->nrec < 1 || h->rec_nt < nt)
+    return fail(VI_ESTATE, "vi_gradient: receivers with capacity >= nt are required");
+  VI_TRY(wait_external(h));
+  VI_HIP(hipStreamSynchronize(h->stream));
+  if (h->esz == 4)
+    return gradient_impl<float>(h, nt, (const float*)dobs, checkpoint_every, misfit,
+                                (float*)g_rho, (float*)g_lam, (float*)g_mu, (float*)g_tau);
+  return gradient_impl<double>(h, nt, (const double*)dobs, checkpoint_every, misfit,
+                               (double*)g_rho, (double*)g_lam, (double*)g_mu, (double*)g_tau);
 }
 
 // ---------------------------------------------------------------- batches

@@ -164,6 +164,25 @@ class Simulation:
         L.check(L.lib().vi_get_traces(self._h, self._ptr(out), nt))
         return out
 
+    # -- gradient ----------------------------------------------------------------------
+    def gradient(self, dobs, nt=None, checkpoint_every=0):
+        """L2 trace misfit J and its gradient with respect to the model arrays (rho, lam, mu,
+        tau) by the discrete adjoint of ``nt`` steps from a zero state. ``dobs`` is
+        (nrec, nt). Leaves the simulation reset. Returns (J, {"rho", "lam", "mu", "tau"})."""
+        d = self._arr(dobs)
+        nt = d.shape[1] if nt is None else int(nt)
+        d = np.ascontiguousarray(d.reshape(self.nrec, -1)[:, :nt])
+        shp = (self.nz, self.nx)
+        g = {k: np.zeros(shp, dtype=self.dtype) for k in ("rho", "lam", "mu", "tau")}
+        J = ctypes.c_double()
+        L.check(L.lib().vi_gradient(self._h, nt, self._ptr(d), int(checkpoint_every),
+                                    ctypes.byref(J), self._ptr(g["rho"]), self._ptr(g["lam"]),
+                                    self._ptr(g["mu"]),
+                                    self._ptr(g["tau"]) if self.nrelax else None))
+        if not self.nrelax:
+            g.pop("tau")
+        return J.value, g
+
     # -- tiling ------------------------------------------------------------------------
     def set_tiling(self, rows_per_wave):
         L.check(L.lib().vi_set_tiling(self._h, int(rows_per_wave)))
