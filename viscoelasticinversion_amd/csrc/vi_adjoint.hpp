@@ -316,7 +316,9 @@ __global__ __launch_bounds__(256) void k_adj_vel_point(const AdjDesc<R> d) {
 
 // ---- adjoint of the velocity half step (transposed operators): s_hat -= D^T d_hat ------
 // Rows 0..3 are the free-surface image rows, whose adjoint is collected here and moved to
-// the interior rows they copy by the next k_adj_stress_point.
+// the interior rows they copy by the next k_adj_stress_point. Their z-stencils reach rows
+// -4..-1, so the d_hat planes are read from a base moved back by 4 rows into the zero guard
+// rows in front of each plane: every 32-bit byte offset stays non-negative.
 template <typename R>
 __global__ __launch_bounds__(256) void k_adj_vel_gather(const AdjDesc<R> d, int row0) {
   constexpr unsigned E = sizeof(R);
@@ -324,11 +326,14 @@ __global__ __launch_bounds__(256) void k_adj_vel_gather(const AdjDesc<R> d, int 
   const int j = row0 + blockIdx.y * 4 + threadIdx.y;
   if (i >= d.nx - 4 || j >= d.nz - 4) return;
   const unsigned cb = (unsigned)(j * d.ldx + i) * E;
+  const unsigned gb = (unsigned)((j + 4) * d.ldx + i) * E;  // offset from the guarded base
+  const size_t back = (size_t)4 * d.ldx;
   const int rb = d.ldx * (int)E;
   // dxx = D+x sxx, dxzz = D-z sxz, dxzx = D-x sxz, dzz = D+z szz
-  const R gsxx = gm((const R*)d.dhat[0], cb, (int)E, d.Dx);
-  const R gsxz = gp((const R*)d.dhat[1], cb, rb, d.Dz) + gp((const R*)d.dhat[2], cb, (int)E, d.Dx);
-  const R gszz = gm((const R*)d.dhat[3], cb, rb, d.Dz);
+  const R gsxx = gm((const R*)d.dhat[0] - back, gb, (int)E, d.Dx);
+  const R gsxz = gp((const R*)d.dhat[1] - back, gb, rb, d.Dz) +
+                 gp((const R*)d.dhat[2] - back, gb, (int)E, d.Dx);
+  const R gszz = gm((const R*)d.dhat[3] - back, gb, rb, d.Dz);
   gst(d.asxx, cb, gld((const R*)d.asxx, cb) - gsxx);
   gst(d.asxz, cb, gld((const R*)d.asxz, cb) - gsxz);
   gst(d.aszz, cb, gld((const R*)d.aszz, cb) - gszz);
