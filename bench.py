@@ -144,6 +144,55 @@ def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True):
                                      "latency_p99_s": round(float(np.percentile(done, 99)), 4)}}
 
 
+def bench_gradient(nshots=16, nt=3000, checkpoint_every=0):
+    """Adjoint gradient workload: 1600 x 600, L=3, CPML + free surface, ``nshots`` shots
+    with 301 receivers, ``nt`` steps, FP32. Observed data come from the true model; each
+    shot's gradient (forward with sqrt(nt) checkpoints, segment recomputation, transposed
+    steps) runs on a simulation already resident on the device."""
+    from viscoelasticinversion_amd.api import gradient_to_velocity_params
+    from viscoelasticinversion_amd.configs import (adjoint_shot, build_simulation,
+                                                   config_adjoint, observed_data)
+    acfg = config_adjoint(nshots=nshots, nt=nt)
+    dobs = observed_data(acfg, "float32")
+    sims = [build_simulation(adjoint_shot(acfg, s, "start"), "float32") for s in range(nshots)]
+    sims[0].gradient(dobs[0], nt, checkpoint_every)  # first-call set-up
+    t0 = time.perf_counter()
+    sims[0].reset()
+    sims[0].step(nt)
+    sims[0].sync()
+    t_fwd = time.perf_counter() - t0
+    sims[0].reset()
+    per = []
+    J = 0.0
+    total = None
+    st = acfg["start"]
+    t_chain = 0.0
+    t0 = time.perf_counter()
+    for s, sim in enumerate(sims):
+        ts = time.perf_counter()
+        Js, g = sim.gradient(dobs[s], nt, checkpoint_every)
+        per.append(time.perf_counter() - ts)
+        tc = time.perf_counter()
+        gv = gradient_to_velocity_params(g, st["rho"], st["lam"], st["mu"], st["tau"], st["Q"])
+        total = gv if total is None else {k: total[k] + gv[k] for k in gv}
+        t_chain += time.perf_counter() - tc
+        J += Js
+    wall = time.perf_counter() - t0
+    for sim in sims:
+        sim.close()
+    cells = acfg["nx"] * acfg["nz"]
+    return {"workload": f"{nshots} shots, {acfg['nx']}x{acfg['nz']}, 301 receivers, L=3, "
+                        f"CPML + free surface, {nt} steps, FP32, sqrt(nt) checkpointing",
+            "wall_s": round(wall, 4), "shots_per_s": round(nshots / wall, 3),
+            "gradient_s_per_shot_p50": round(float(np.percentile(per, 50)), 4),
+            "gradient_s_per_shot_p99": round(float(np.percentile(per, 99)), 4),
+            "forward_s_per_shot": round(t_fwd, 4),
+            "gradient_over_forward": round(float(np.median(per)) / t_fwd, 3),
+            "host_chain_rule_s": round(t_chain, 3),
+            "forward_equivalent_GPt_per_s": round(cells * nt * nshots / wall / 1e9, 3),
+            "misfit": J, "gradient_finite": bool(all(np.isfinite(v).all() for v in total.values()))}
+
+
 def pmc_probe(nshots, steps, copy_bytes=1 << 30):
     """Short run for rocprofv3 --pmc passes, three phases in order:
     1. the single-shot 1024 x 512 L=3 workload, 20 steps, rows per wave from VI_ROWS1 (1);
@@ -198,6 +247,10 @@ def main():
     ap.add_argument("--multishot-shots", type=int, default=64)
     ap.add_argument("--multishot-steps", type=int, default=3000)
     ap.add_argument("--no-multishot", action="store_true")
+    ap.add_argument("--no-gradient", action="store_true", help="skip the adjoint-gradient leg")
+    ap.add_argument("--gradient-shots", type=int, default=16)
+    ap.add_argument("--only-gradient", action="store_true",
+                    help="run only the adjoint-gradient leg and print its JSON")
     ap.add_argument("--rows", type=int, default=0,
                     help="fix the kernel variant (rows per wavefront) instead of autotuning")
     ap.add_argument("--no-serial", action="store_true",
@@ -222,6 +275,9 @@ def main():
         print(json.dumps({"multishot": bench_multishot(
             args.multishot_shots, args.multishot_steps, rows=args.rows,
             serial=not args.no_serial)}), flush=True)
+        return
+    if args.only_gradient:
+        print(json.dumps({"gradient": bench_gradient(args.gradient_shots)}), flush=True)
         return
     from viscoelasticinversion_amd.configs import build_simulation, config_layered_visco
     nt_total = args.warmup + 2 * args.steps
@@ -277,6 +333,8 @@ def main():
     if not args.no_multishot:
         extra["multishot"] = bench_multishot(args.multishot_shots, args.multishot_steps,
                                              serial=not args.no_serial)
+    if not args.no_gradient:
+        extra["gradient"] = bench_gradient(args.gradient_shots)
     cpu = None
     if not args.no_cpu:
         cfg_cpu = config_layered_visco(nx=args.nx, nz=args.nz, nt=args.steps)

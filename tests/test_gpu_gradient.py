@@ -25,14 +25,20 @@ def _misfit(cfg, model, dobs, nt):
     return 0.5 * float(np.sum((tr - dobs) ** 2))
 
 
-def _setup(nrelax, fs, cpml, seed):
-    cfg = small_cfg(nx=72, nz=56, nrelax=nrelax, cpml=cpml, fs=fs, seed=seed, nt=70)
+def _setup(nrelax, fs, cpml, seed, nt=120):
+    """Small heterogeneous case whose data depend on the boundaries: receivers on the rows
+    just below the free surface (row 4 is the surface), inside the CPML strips and in the
+    interior, on every component; a shallow extra source."""
+    cfg = small_cfg(nx=72, nz=56, nrelax=nrelax, cpml=cpml, fs=fs, seed=seed, nt=nt)
     rng = np.random.default_rng(seed + 100)
-    # receivers on every component, spread over the interior
-    nrec = 24
-    cfg["rec_ix"] = rng.integers(6, cfg["nx"] - 6, nrec).astype(np.int32)
-    cfg["rec_iz"] = rng.integers(5, cfg["nz"] - 6, nrec).astype(np.int32)
-    cfg["rec_comp"] = (np.arange(nrec) % 5).astype(np.int32)
+    nx, nz = cfg["nx"], cfg["nz"]
+    rix = np.concatenate([rng.integers(6, nx - 6, 8), rng.integers(4, 8, 4),
+                          rng.integers(nx - 8, nx - 4, 4), rng.integers(6, nx - 6, 8)])
+    riz = np.concatenate([rng.integers(4, 7, 8), rng.integers(5, nz - 5, 8),
+                          rng.integers(nz - 8, nz - 4, 4), rng.integers(8, nz - 8, 4)])
+    cfg["rec_ix"] = rix.astype(np.int32)
+    cfg["rec_iz"] = riz.astype(np.int32)
+    cfg["rec_comp"] = (np.arange(len(rix)) % 5).astype(np.int32)
     # observed data: the same experiment in a perturbed model
     true = {k: v * (1 + 0.03 * rng.standard_normal(v.shape)) for k, v in cfg["model"].items()}
     sim = _sim(cfg, true)
@@ -117,3 +123,47 @@ def test_gradient_fp32_close_to_fp64():
         den = np.linalg.norm(g64[k])
         print(f"  {k}: rel L2 {num / den:.2e}", flush=True)
         assert num <= 1e-3 * den, k
+
+
+def test_workload_gradient_vp_vs_rho_q_at_random_points():
+    """The gradient workload (smooth layered background with Gaussian anomalies, Q per layer,
+    L = 3, CPML + free surface, several shots, surface receivers) at reduced size in FP64:
+    dJ/d(vp, vs, rho, Q) from the adjoint and the chain rule against central differences of
+    the multi-shot misfit at 20 random grid points per parameter (drawn among the cells whose
+    sensitivity |g m| is at least 1 % of the largest, so that every check is non-trivial)."""
+    from viscoelasticinversion_amd.configs import (config_adjoint, moduli_from_velocity,
+                                                   multishot_gradient, multishot_misfit,
+                                                   observed_data)
+    print("[case] workload gradient 200x100 3 shots FP64", flush=True)
+    acfg = config_adjoint(nx=200, nz=100, nshots=3, nrec=41, nt=500, seed=3, smooth_cells=8)
+    dobs = observed_data(acfg, "float64")
+    st = acfg["start"]
+    J, g = multishot_gradient(acfg, dobs, "float64", "start", checkpoint_every=0)
+    assert abs(J - multishot_misfit(acfg, st, dobs, "float64")) <= 1e-12 * J
+    rng = np.random.default_rng(7)
+    nz, nx = acfg["nz"], acfg["nx"]
+    keys = ("vp", "vs", "rho", "Q")
+    for key in keys:
+        sens = np.abs(g[key] * st[key])
+        sens[:5, :] = 0
+        sens[-4:, :] = 0
+        sens[:, :4] = 0
+        sens[:, -4:] = 0
+        cand = np.argwhere(sens >= 0.01 * sens.max())
+        pts = cand[rng.choice(len(cand), 20, replace=False)]
+        worst = 0.0
+        for j, i in pts:
+            eps = 1e-4 * st[key][j, i]
+
+            def J_at(sign):
+                p = {k: st[k].copy() for k in keys}
+                p[key][j, i] += sign * eps
+                m = moduli_from_velocity(p["vp"], p["vs"], p["rho"], p["Q"], acfg["kbar"], 3)
+                return multishot_misfit(acfg, m, dobs, "float64")
+
+            fd = (J_at(1) - J_at(-1)) / (2 * eps)
+            ad = float(g[key][j, i])
+            err = abs(ad - fd) / abs(fd)
+            worst = max(worst, err)
+            assert err <= 1e-5, (key, int(j), int(i), ad, fd)
+        print(f"  {key}: 20 points, worst relative difference {worst:.2e}", flush=True)

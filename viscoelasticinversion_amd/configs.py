@@ -185,3 +185,123 @@ def build_multishot(mcfg, precision="float32"):
         sim.set_receivers(sh["rec_ix"], sh["rec_iz"], sh["rec_comp"], mcfg["nt"])
         sims.append(sim)
     return model, sims
+
+
+def moduli_from_velocity(vp, vs, rho, Q, kbar, L=3):
+    """(vp, vs, rho, Q) -> model dict (rho, lam, mu, tau) with lam = rho (vp^2 - 2 vs^2),
+    mu = rho vs^2 and tau = L / (Kbar Q) (Models.jl:67)."""
+    mu = rho * vs ** 2
+    return dict(rho=rho, lam=rho * vp ** 2 - 2.0 * mu, mu=mu, tau=get_tau(Q, kbar, L),
+                vp=vp, vs=vs, Q=Q)
+
+
+def adjoint_models(nx=1600, nz=600, seed=0, nlayers=6, nanom=6, smooth_cells=40.0, L=3,
+                   taun=TAUN_2_40HZ_L3, f1=2.0, f2=40.0):
+    """True and starting models of the gradient workload: a smooth layered background
+    (vp 1800-4000 m/s with tanh interfaces, vs = vp/1.8, Gardner rho) plus ``nanom`` seeded
+    Gaussian anomalies of +-10 % in vp and vs with radii U[20, 60] cells; Q ~ U[30, 100] per
+    layer. The starting model is the true model smoothed by a ``smooth_cells``-cell Gaussian.
+    Returns (true, start, kbar); true and start are dicts of rho, lam, mu, tau, vp, vs, Q."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    z = np.arange(nz, dtype=np.float64)
+    depths = np.sort(rng.uniform(0.1, 0.95, nlayers - 1)) * nz
+    vp_l = np.sort(rng.uniform(1800.0, 4000.0, nlayers))
+    q_l = rng.uniform(30.0, 100.0, nlayers)
+    vp1 = np.full(nz, vp_l[0])
+    q1 = np.full(nz, q_l[0])
+    for k, dz in enumerate(depths):
+        w = 0.5 * (1 + np.tanh((z - dz) / 4.0))
+        vp1 = vp1 + (vp_l[k + 1] - vp_l[k]) * w
+        q1 = q1 + (q_l[k + 1] - q_l[k]) * w
+    vp = np.repeat(vp1[:, None], nx, axis=1)
+    vs = vp / 1.8
+    Q = np.repeat(q1[:, None], nx, axis=1)
+    X, Z = np.meshgrid(np.arange(nx), np.arange(nz))
+    for _ in range(nanom):
+        cx, cz = rng.uniform(0.1, 0.9) * nx, rng.uniform(0.2, 0.9) * nz
+        r = rng.uniform(20.0, 60.0)
+        amp = rng.choice([-0.1, 0.1])
+        g = 1 + amp * np.exp(-((X - cx) ** 2 + (Z - cz) ** 2) / (2 * r * r))
+        vp = vp * g
+        vs = vs * g
+    kbar = float(np.mean(K(np.linspace(2 * np.pi * f1, 2 * np.pi * f2, 100), taun)))
+
+    def pack(vp, vs, Q):
+        return moduli_from_velocity(vp, vs, 310.0 * vp ** 0.25, Q, kbar, L)
+
+    true = pack(vp, vs, Q)
+    sm = smooth_cells
+    start = pack(gaussian_filter(vp, sm, mode="nearest"), gaussian_filter(vs, sm, mode="nearest"),
+                 gaussian_filter(Q, sm, mode="nearest"))
+    return true, start, kbar
+
+
+def config_adjoint(nx=1600, nz=600, nshots=16, nrec=301, nt=3000, seed=0, smooth_cells=40.0):
+    """Gradient workload: nx x nz (default 1600 x 600), dx 5 m, L = 3, CPML 20 + free
+    surface, ``nshots`` 12 Hz Ricker force sources evenly along the surface 10 m deep,
+    ``nrec`` receivers (vz) 10 m deep, ``nt`` steps at dt 0.4 ms; L2 trace misfit of the
+    starting model against data of the true model."""
+    true, start, kbar = adjoint_models(nx, nz, seed, smooth_cells=smooth_cells)
+    dx, dt, T = 5.0, 4e-4, 20
+    vmax = float(true["vp"].max() * np.sqrt(1 + true["tau"].max()))
+    d0 = -(2.0 + 1) * vmax * np.log(1e-5) / (2 * T * dx)
+    depth = JS + int(round(10.0 / dx))
+    src_x = np.linspace(T + 10, nx - T - 11, nshots).round().astype(np.int32)
+    rec_x = np.linspace(T, nx - T - 1, nrec).round().astype(np.int32)
+    wav = ricker_samples(12.0, 1.0 / 12.0, nt, dt)
+    return dict(name=f"adjoint_{nx}x{nz}", nx=nx, nz=nz, dx=dx, dz=dx, dt=dt, nt=nt, nrelax=3,
+                taun=np.array(TAUN_2_40HZ_L3), kbar=kbar, true=true, start=start,
+                free_surface=True,
+                cpml=dict(thickness=T, d0=d0, alpha_max=0.0, kappa_max=1.0, power=2.0,
+                          left=True, right=True, top=False, bottom=True),
+                src_x=src_x, src_z=depth, wavelet=wav, rec_ix=rec_x,
+                rec_iz=np.full(nrec, depth, np.int32), rec_comp=np.full(nrec, 1, np.int32))
+
+
+def adjoint_shot(acfg, s, model):
+    """Single-shot config dict of shot ``s`` of a gradient workload in ``model`` (a dict with
+    rho, lam, mu, tau, or the name of one of the workload's models)."""
+    m = acfg[model] if isinstance(model, str) else model
+    return dict(name=f"{acfg['name']}_shot{s}", nx=acfg["nx"], nz=acfg["nz"], dx=acfg["dx"],
+                dz=acfg["dz"], dt=acfg["dt"], nt=acfg["nt"], nrelax=acfg["nrelax"],
+                taun=acfg["taun"], model={k: m[k] for k in ("rho", "lam", "mu", "tau")},
+                free_surface=acfg["free_surface"], cpml=acfg["cpml"],
+                src_ix=np.array([acfg["src_x"][s]], np.int32),
+                src_iz=np.array([acfg["src_z"]], np.int32), src_kind=np.array([0], np.int32),
+                wavelets=acfg["wavelet"][None, :], rec_ix=acfg["rec_ix"], rec_iz=acfg["rec_iz"],
+                rec_comp=acfg["rec_comp"])
+
+
+def observed_data(acfg, precision="float32", model="true"):
+    """Traces [nrec, nt] of every shot in ``model`` (default: the true model)."""
+    out = []
+    for s in range(len(acfg["src_x"])):
+        sim = build_simulation(adjoint_shot(acfg, s, model), precision)
+        sim.step(acfg["nt"])
+        out.append(sim.traces(acfg["nt"]))
+        sim.close()
+    return out
+
+
+def multishot_misfit(acfg, model, dobs, precision="float32"):
+    """Sum over shots of 1/2 |d(model) - dobs|^2."""
+    syn = observed_data(acfg, precision, model)
+    return float(sum(0.5 * np.sum((a.astype(np.float64) - b) ** 2) for a, b in zip(syn, dobs)))
+
+
+def multishot_gradient(acfg, dobs, precision="float32", model="start", checkpoint_every=0):
+    """Misfit and its gradient with respect to (vp, vs, rho, Q), summed over the shots,
+    from the discrete adjoint of each shot."""
+    from .api import gradient_to_velocity_params
+    m = acfg[model] if isinstance(model, str) else model
+    J = 0.0
+    total = None
+    for s in range(len(acfg["src_x"])):
+        sim = build_simulation(adjoint_shot(acfg, s, m), precision)
+        Js, g = sim.gradient(dobs[s], acfg["nt"], checkpoint_every)
+        sim.close()
+        gv = gradient_to_velocity_params(g, m["rho"], m["lam"], m["mu"], m["tau"], m["Q"])
+        J += Js
+        total = gv if total is None else {k: total[k] + gv[k] for k in gv}
+    return J, total

@@ -30,7 +30,8 @@ struct AdjDesc {
   const R *rho, *lam, *mu, *tau;  // model (pitch ldm)
   const R *ax, *bx, *kx, *az, *bz, *kz;
   // forward state of the current step (pitch ldx): stress s^n (pre-step) with images, the
-  // velocity v^{n+1/2} after the velocity half, memory variables M^n, CPML memory psi^n
+  // velocity v^{n+1/2} after the velocity half, the memory-variable sums sum_k M^n_k of the
+  // xx, zz and xz components (three planes), CPML memory psi^n
   const R *fsxx, *fszz, *fsxz, *fvx, *fvz, *fM;
   const R* fpsi[8];
   // sources of the shot (flat list) and the sample index of this step
@@ -118,12 +119,10 @@ __global__ __launch_bounds__(256) void k_adj_stress_point(const AdjDesc<R> d) {
   const R t = L > 0 ? gld(d.tau, mb) : R(0);
   const R p1 = pr * t, p0 = p1 + pr, l1 = lr * t, l0 = l1 + lr, m1 = mxz * t, m0 = m1 + mxz;
   R SMxx = 0, SMzz = 0, SMxz = 0;
-  R* Mf[3 * LM];
-  for (int k = 0; k < 3 * L; ++k) Mf[k] = (R*)d.fM + (size_t)k * d.plane;
-  for (int k = 0; k < L; ++k) {
-    SMxx = SMxx + gld((const R*)Mf[k], cb);
-    SMzz = SMzz + gld((const R*)Mf[L + k], cb);
-    SMxz = SMxz + gld((const R*)Mf[2 * L + k], cb);
+  if (L > 0) {
+    SMxx = gld(d.fM, cb);
+    SMzz = gld(d.fM + d.plane, cb);
+    SMxz = gld(d.fM + 2 * d.plane, cb);
   }
   const bool fs_row = d.fs && j == 4;
   const R N = l0 * exx + l1 * SMxx + p1 * SMzz;  // free-surface row: ezz* = -N/p0
@@ -340,15 +339,65 @@ __global__ __launch_bounds__(256) void k_adj_vel_gather(const AdjDesc<R> d, int 
 }
 
 // ---- residual injection: u_hat += R^T r^n (receivers sample after the stress half) ----
+// One thread per distinct (component, cell) target; receivers that share a target are
+// summed in ascending receiver order (tptr/trec, a CSR list), so the result does not depend
+// on thread scheduling.
 template <typename R>
-__global__ void k_adj_inject(R* f0, R* f1, R* f2, R* f3, R* f4, const int* cell,
-                             const int* comp, int nrec, const R* resid_row) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;  // one thread: deterministic order
-  R* f[5] = {f0, f1, f2, f3, f4};
-  for (int r = 0; r < nrec; ++r) {
-    R* p = f[comp[r]] + cell[r];
-    *p = *p + resid_row[r];
+__global__ __launch_bounds__(256) void k_adj_inject(R* f0, R* f1, R* f2, R* f3, R* f4,
+                                                    const int* tcell, const int* tcomp,
+                                                    const int* tptr, const int* trec, int ntgt,
+                                                    const R* resid_row) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntgt) return;
+  R s = 0;
+  for (int k = tptr[t]; k < tptr[t + 1]; ++k) s = s + resid_row[trec[k]];
+  R* f;
+  switch (tcomp[t]) {
+    case 0: f = f0; break;
+    case 1: f = f1; break;
+    case 2: f = f2; break;
+    case 3: f = f3; break;
+    default: f = f4; break;
   }
+  R* p = f + tcell[t];
+  *p = *p + s;
+}
+
+// ---- memory-variable sums of the forward state: dst[c] = sum_k M[c L + k], c = xx, zz, xz,
+// summed in ascending k from zero (the order the stress update uses)
+template <typename R>
+__global__ __launch_bounds__(256) void k_msum(const R* M, R* dst, size_t plane, int L) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= plane) return;
+  for (int c = 0; c < 3; ++c) {
+    R s = 0;
+    for (int k = 0; k < L; ++k) s = s + M[(size_t)(c * L + k) * plane + idx];
+    dst[(size_t)c * plane + idx] = s;
+  }
+}
+
+// ---- batched device-to-device copy: up to 16 regions per launch (blockIdx.y = region).
+// Regions start 16-byte aligned and hold a multiple of 4 bytes.
+struct CopyList {
+  const char* src[16];
+  char* dst[16];
+  unsigned long long bytes[16];
+  int n;
+};
+
+__global__ __launch_bounds__(256) void k_copy_list(const CopyList c) {
+  const int e = blockIdx.y;
+  if (e >= c.n) return;
+  const char* s = c.src[e];
+  char* d = c.dst[e];
+  const size_t nb = c.bytes[e];
+  const size_t n16 = nb / 16;
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = tid; k < n16; k += stride) ((uint4*)d)[k] = ((const uint4*)s)[k];
+  const size_t t0 = n16 * 16;
+  if (tid < 4 && t0 + 4 * tid < nb)
+    ((unsigned*)(d + t0))[tid] = ((const unsigned*)(s + t0))[tid];
 }
 
 // ---- gradient finalisation: mu_xz and buoyancy accumulators to mu and rho --------------

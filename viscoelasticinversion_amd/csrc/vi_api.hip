@@ -646,15 +646,40 @@ int gradient_impl(vi_sim_s* h, int nt, const R* dobs, int K, double* misfit, R* 
   // forward state arrays of the handle, in a fixed order
   std::vector<DevBuf*> live = mutable_arrays(h);
   live.pop_back();  // traces are not part of the checkpointed state
+  // regions inside checkpoint and segment buffers start on 256-byte boundaries
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   size_t state_bytes = 0;
-  for (auto* b : live) state_bytes += b->bytes;
+  for (auto* b : live) state_bytes += al(b->bytes);
+  // device-to-device copies, 16 regions per k_copy_list launch
+  struct Cp { const void* s; void* d; size_t b; };
+  auto run_copies = [&](const std::vector<Cp>& v) -> int {
+    for (size_t i = 0; i < v.size(); i += 16) {
+      vi::CopyList c{};
+      size_t mx = 0;
+      c.n = (int)std::min<size_t>(16, v.size() - i);
+      for (int k = 0; k < c.n; ++k) {
+        c.src[k] = (const char*)v[i + k].s;
+        c.dst[k] = (char*)v[i + k].d;
+        c.bytes[k] = v[i + k].b;
+        mx = std::max(mx, v[i + k].b);
+      }
+      const unsigned gx = (unsigned)std::min<size_t>(1024, std::max<size_t>(1, (mx / 16 + 255) / 256));
+      hipLaunchKernelGGL(vi::k_copy_list, dim3(gx, c.n), dim3(256), 0, st, c);
+    }
+    VI_HIP(hipGetLastError());
+    return 0;
+  };
   DevBuf ck;
   VI_TRY(ck.alloc(state_bytes * nck, st));
-  // per-step slot: s (3 planes), v^{n+1/2} (2), M (3L), psi (8 strips)
-  size_t psi_bytes[8];
+  // per-step slot: s (3 planes), v^{n+1/2} (2), memory-variable sums (3 if L > 0), psi
+  size_t psi_bytes[8], psi_off[8];
   size_t psi_total = 0;
-  for (int q = 0; q < 8; ++q) psi_total += (psi_bytes[q] = h->psi[q].bytes);
-  const size_t slot_bytes = (5 + 3 * (size_t)L) * pb + psi_total;
+  size_t slot_bytes = (L ? 8 : 5) * pb;
+  for (int q = 0; q < 8; ++q) {
+    psi_total += (psi_bytes[q] = h->psi[q].bytes);
+    psi_off[q] = slot_bytes;
+    slot_bytes += al(psi_bytes[q]);
+  }
   DevBuf seg;
   VI_TRY(seg.alloc(slot_bytes * K, st));
   // adjoint state, work arrays and gradient accumulators
@@ -681,19 +706,37 @@ int gradient_impl(vi_sim_s* h, int nt, const R* dobs, int K, double* misfit, R* 
   DevBuf resid, lists;
   const int nrec = h->nrec, nsrc = h->nsrc;
   VI_TRY(resid.alloc((size_t)nt * nrec * E, st));
-  std::vector<int> lst(2 * (size_t)nrec + 2 * (size_t)nsrc + 1);
-  for (int r = 0; r < nrec; ++r) {
-    lst[r] = h->rec_iz[r] * h->ldx + h->rec_ix[r];
-    lst[nrec + r] = h->rec_comp[r];
+  // injection targets: distinct (component, cell) pairs with their receivers in ascending
+  // order; layout [tcell | tcomp | tptr (ntgt+1) | trec (nrec) | src_cell | src_kind]
+  std::map<std::pair<int, int>, std::vector<int>> targets;
+  for (int r = 0; r < nrec; ++r)
+    targets[{h->rec_comp[r], h->rec_iz[r] * h->ldx + h->rec_ix[r]}].push_back(r);
+  const int ntgt = (int)targets.size();
+  std::vector<int> lst(3 * (size_t)ntgt + 1 + (size_t)nrec + 2 * (size_t)nsrc);
+  {
+    int t = 0, k = 0;
+    int* tptr = lst.data() + 2 * ntgt;
+    int* trec = tptr + ntgt + 1;
+    for (auto& kv : targets) {
+      lst[t] = kv.first.second;
+      lst[ntgt + t] = kv.first.first;
+      tptr[t] = k;
+      for (int r : kv.second) trec[k++] = r;
+      ++t;
+    }
+    tptr[ntgt] = k;
   }
+  const size_t src0 = 3 * (size_t)ntgt + 1 + nrec;
   for (int q = 0; q < nsrc; ++q) {
-    lst[2 * nrec + q] = h->src_iz[q] * h->ldx + h->src_ix[q];
-    lst[2 * nrec + nsrc + q] = h->src_kind[q];
+    lst[src0 + q] = h->src_iz[q] * h->ldx + h->src_ix[q];
+    lst[src0 + nsrc + q] = h->src_kind[q];
   }
   VI_TRY(lists.alloc(lst.size() * sizeof(int), st));
   VI_HIP(hipMemcpyAsync(lists.p, lst.data(), lst.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  const int* rec_cell = (const int*)lists.p;
-  const int* rec_comp = rec_cell + nrec;
+  const int* t_cell = (const int*)lists.p;
+  const int* t_comp = t_cell + ntgt;
+  const int* t_ptr = t_comp + ntgt;
+  const int* t_rec = t_ptr + ntgt + 1;
   // static part of the adjoint descriptor
   d.nx = h->g.nx; d.nz = h->g.nz; d.ldx = h->ldx; d.ldm = h->model->ldm;
   {
@@ -709,17 +752,18 @@ int gradient_impl(vi_sim_s* h, int nt, const R* dobs, int K, double* misfit, R* 
   d.L = L;
   d.plane = h->plane;
   d.nsrc = nsrc;
-  d.src_cell = (const int*)lists.p + 2 * nrec;
+  d.src_cell = (const int*)lists.p + src0;
   d.src_kind = d.src_cell + nsrc;
   auto copy_state = [&](char* dst, bool to_ck) -> int {
+    std::vector<Cp> v;
     size_t off = 0;
     for (auto* b : live) {
       if (!b->p) continue;
-      if (to_ck) VI_HIP(hipMemcpyAsync(dst + off, b->p, b->bytes, hipMemcpyDeviceToDevice, st));
-      else VI_HIP(hipMemcpyAsync(b->p, dst + off, b->bytes, hipMemcpyDeviceToDevice, st));
-      off += b->bytes;
+      if (to_ck) v.push_back({b->p, dst + off, b->bytes});
+      else v.push_back({dst + off, b->p, b->bytes});
+      off += al(b->bytes);
     }
-    return 0;
+    return run_copies(v);
   };
   // ---- forward pass with checkpoints
   for (int s = 0; s < nck; ++s) {
@@ -755,20 +799,17 @@ int gradient_impl(vi_sim_s* h, int nt, const R* dobs, int K, double* misfit, R* 
     VI_TRY(refresh_own_desc(h));
     for (int n = n0; n < n1; ++n) {  // recompute the segment, storing each step's inputs
       char* slot = (char*)seg.p + (size_t)(n - n0) * slot_bytes;
-      for (int q = 0; q < 3; ++q)
-        VI_HIP(hipMemcpyAsync(slot + q * pb, h->state[2 + q].p, pb, hipMemcpyDeviceToDevice, st));
+      std::vector<Cp> pre;
+      for (int q = 0; q < 3; ++q) pre.push_back({h->state[2 + q].p, slot + q * pb, pb});
+      for (int q = 0; q < 8; ++q)
+        if (psi_bytes[q]) pre.push_back({h->psi[q].p, slot + psi_off[q], psi_bytes[q]});
+      VI_TRY(run_copies(pre));
       if (L)
-        VI_HIP(hipMemcpyAsync(slot + 5 * pb, M0, 3 * (size_t)L * pb, hipMemcpyDeviceToDevice, st));
-      size_t po = (5 + 3 * (size_t)L) * pb;
-      for (int q = 0; q < 8; ++q) {
-        if (psi_bytes[q])
-          VI_HIP(hipMemcpyAsync(slot + po, h->psi[q].p, psi_bytes[q], hipMemcpyDeviceToDevice, st));
-        po += psi_bytes[q];
-      }
+        hipLaunchKernelGGL(vi::k_msum<R>, dim3((unsigned)((h->plane + 255) / 256)), dim3(256), 0,
+                           st, (const R*)M0, (R*)(slot + 5 * pb), h->plane, L);
       VI_TRY(launch_one<R>(st, h->rz, L, 0, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
                            h->ntiles, n - n0, nullptr, nullptr));
-      for (int q = 0; q < 2; ++q)
-        VI_HIP(hipMemcpyAsync(slot + (3 + q) * pb, h->state[q].p, pb, hipMemcpyDeviceToDevice, st));
+      VI_TRY(run_copies({{h->state[0].p, slot + 3 * pb, pb}, {h->state[1].p, slot + 4 * pb, pb}}));
       VI_TRY(launch_one<R>(st, h->rz, L, 1, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
                            h->ntiles, n - n0, nullptr, nullptr));
     }
@@ -780,16 +821,12 @@ int gradient_impl(vi_sim_s* h, int nt, const R* dobs, int K, double* misfit, R* 
       d.fvx = (const R*)(slot + 3 * pb);
       d.fvz = (const R*)(slot + 4 * pb);
       d.fM = (const R*)(slot + 5 * pb);
-      size_t po = (5 + 3 * (size_t)L) * pb;
-      for (int q = 0; q < 8; ++q) {
-        d.fpsi[q] = (const R*)(slot + po);
-        po += psi_bytes[q];
-      }
+      for (int q = 0; q < 8; ++q) d.fpsi[q] = (const R*)(slot + psi_off[q]);
       d.step = n;
-      if (nrec) {
-        hipLaunchKernelGGL(vi::k_adj_inject<R>, dim3(1), dim3(1), 0, st, d.avx, d.avz, d.asxx,
-                           d.aszz, d.asxz, rec_cell, rec_comp, nrec,
-                           (const R*)resid.p + (size_t)n * nrec);
+      if (ntgt) {
+        hipLaunchKernelGGL(vi::k_adj_inject<R>, dim3((ntgt + 255) / 256), dim3(256), 0, st,
+                           d.avx, d.avz, d.asxx, d.aszz, d.asxz, t_cell, t_comp, t_ptr, t_rec,
+                           ntgt, (const R*)resid.p + (size_t)n * nrec);
       }
       switch (L) {
         case 0: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 0>), gint, blk, 0, st, d); break;
