@@ -15,6 +15,7 @@
  *   FreeSurfaceBC / apply_free_surface!   src/sciml_solver/CPML.jl:366-393 -> vi_set_free_surface
  *   IsotropicModel / QModel         src/sciml_solver/Models.jl:3-25, 90-107 -> vi_set_model,
  *                                                                              vi_set_relaxation
+ *   VTIModel                        src/sciml_solver/Models.jl:27-33   -> vi_set_model_vti
  *
  * Conventions
  *   - Every 2D host array is [nz][nx] with x fastest (Julia's column-major (Nx, Nz)).
@@ -103,6 +104,12 @@ int vi_mem_info(size_t* free_bytes, size_t* total_bytes);
  * each reading a column window of it (multi-shot acquisition on one large model). ------ */
 int vi_model_create(int nx, int nz, int precision, const void* rho, const void* lam,
                     const void* mu, const void* tau /* NULL for elastic */, vi_model* out);
+/* VTI model: stiffnesses C11, C13, C33, C55 (VTIModel, Models.jl:27-33; the stress update
+ * is the TTI form of ViscoelasticProblem.jl:21-35 with C15 = C35 = 0, with the same SLS
+ * relaxation tau applied to every modulus) */
+int vi_model_create_vti(int nx, int nz, int precision, const void* rho, const void* c11,
+                        const void* c13, const void* c33, const void* c55,
+                        const void* tau /* NULL for elastic */, vi_model* out);
 /* releases the caller's reference; shots that use the model keep it alive */
 int vi_model_destroy(vi_model m);
 
@@ -112,6 +119,9 @@ int vi_destroy(vi_handle h);
 /* rho, lam, mu (and tau when nrelax > 0) as [nz][nx]; tau may be NULL when nrelax == 0 */
 int vi_set_model(vi_handle h, const void* rho, const void* lam, const void* mu,
                  const void* tau);
+/* VTI medium (VTIModel, Models.jl:27-33): rho, C11, C13, C33, C55 (and tau) as [nz][nx] */
+int vi_set_model_vti(vi_handle h, const void* rho, const void* c11, const void* c13,
+                     const void* c33, const void* c55, const void* tau);
 /* use columns [x0, x0 + nx) of a shared model (same nz and precision) */
 int vi_set_model_window(vi_handle h, vi_model m, int x0);
 /* taun[L] relaxation times of the SLS mechanisms (QModel.τns, Models.jl:56-62) */
@@ -150,6 +160,11 @@ int vi_device_bytes(vi_handle h, size_t* bytes);
  * step 0). */
 int vi_gradient(vi_handle h, int nt, const void* dobs /*[nrec][nt]*/, int checkpoint_every,
                 double* misfit, void* g_rho, void* g_lam, void* g_mu, void* g_tau);
+/* the same for a VTI model (VTIModel, Models.jl:27-33): gradients with respect to rho, C11,
+ * C13, C33, C55 and tau */
+int vi_gradient_vti(vi_handle h, int nt, const void* dobs /*[nrec][nt]*/, int checkpoint_every,
+                    double* misfit, void* g_rho, void* g_c11, void* g_c13, void* g_c33,
+                    void* g_c55, void* g_tau);
 
 /* ---- tiling (no counterpart in the reference: KA picks workgroup shapes itself) ------ */
 /* rows per wavefront of the kernel variant: 1, 2, 4 or 8 (results are identical) */

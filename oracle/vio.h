@@ -63,6 +63,8 @@ void vio_cpml_axis(int n, int thickness, double d0, double alpha_max, double kap
 /* float instance */
 void* vio_create_f32(const vio_desc* d, const float* rho, const float* lam, const float* mu,
                      const float* tau, const float* taun);
+void* vio_create_vti_f32(const vio_desc* d, const float* rho, const float* c11, const float* c13,
+                         const float* c33, const float* c55, const float* tau, const float* taun);
 int vio_set_sources_f32(void* h, int n, const int* ix, const int* iz, const int* kind,
                         int nt, const float* wavelets);
 int vio_set_receivers_f32(void* h, int n, const int* ix, const int* iz, const int* comp);
@@ -75,6 +77,9 @@ void vio_destroy_f32(void* h);
 /* double instance */
 void* vio_create_f64(const vio_desc* d, const double* rho, const double* lam, const double* mu,
                      const double* tau, const double* taun);
+void* vio_create_vti_f64(const vio_desc* d, const double* rho, const double* c11,
+                         const double* c13, const double* c33, const double* c55,
+                         const double* tau, const double* taun);
 int vio_set_sources_f64(void* h, int n, const int* ix, const int* iz, const int* kind,
                         int nt, const double* wavelets);
 int vio_set_receivers_f64(void* h, int n, const int* ix, const int* iz, const int* comp);

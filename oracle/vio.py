@@ -49,6 +49,9 @@ def lib():
             f = getattr(_lib, "vio_create" + suf)
             f.restype = vp
             f.argtypes = [ctypes.POINTER(Desc), rp, rp, rp, rp, rp]
+            f = getattr(_lib, "vio_create_vti" + suf)
+            f.restype = vp
+            f.argtypes = [ctypes.POINTER(Desc), rp, rp, rp, rp, rp, rp, rp]
             f = getattr(_lib, "vio_set_sources" + suf)
             f.restype = ctypes.c_int
             f.argtypes = [vp, ctypes.c_int, ip, ip, ip, ctypes.c_int, rp]
@@ -95,7 +98,8 @@ def cpml_axis(n, thickness, d0, alpha_max, kappa_max, power, dt, lo, hi):
 class Oracle:
     """One oracle simulation. ``dtype`` is np.float32 or np.float64.
 
-    ``model`` holds rho, lam, mu (and tau when nrelax > 0) as [nz][nx] arrays; ``taun`` the
+    ``model`` holds rho, lam, mu (and tau when nrelax > 0) as [nz][nx] arrays, or for a VTI
+    medium rho, c11, c13, c33, c55 (and tau); ``taun`` the
     L relaxation times; ``cpml`` a dict with thickness, d0, alpha_max, kappa_max, power,
     left, right, top, bottom (or None).
     """
@@ -130,9 +134,14 @@ class Oracle:
 
         tau = model.get("tau", np.zeros((nz, nx)))
         tn = np.zeros(max(nrelax, 1)) if taun is None else np.asarray(taun)
-        self.h = getattr(lib(), "vio_create" + self.suf)(
-            ctypes.byref(d), arr(model["rho"]), arr(model["lam"]), arr(model["mu"]), arr(tau),
-            arr(tn))
+        if "c11" in model:
+            self.h = getattr(lib(), "vio_create_vti" + self.suf)(
+                ctypes.byref(d), arr(model["rho"]), arr(model["c11"]), arr(model["c13"]),
+                arr(model["c33"]), arr(model["c55"]), arr(tau), arr(tn))
+        else:
+            self.h = getattr(lib(), "vio_create" + self.suf)(
+                ctypes.byref(d), arr(model["rho"]), arr(model["lam"]), arr(model["mu"]),
+                arr(tau), arr(tn))
         if not self.h:
             raise ValueError("oracle rejected the configuration")
         self._keep.clear()

@@ -35,6 +35,13 @@
  *   159-167 intent), and szz is set to 0 after the update; then
  *     szz[js-m] = -szz[js+m],  sxz[js-m] = -sxz[js+m-1],  m = 1..4, columns i in [4, nx-4).
  *   Receivers sample the stored value of their field at (ix, iz) after the stress half.
+ *   VTI (VTIModel, Models.jl:27-33, with the TTI form ViscoelasticProblem.jl:21-35 at
+ *   C15 = C35 = 0): lam holds C13, mu holds C55 (averaged to the sxz node like mu) and the
+ *   diagonal moduli C11, C33 replace pi in the xx and zz rows:
+ *     dsxx = C11_0 exx + C13_0 ezz + C11_1 SMxx + C13_1 SMzz
+ *     dszz = C13_0 exx + C33_0 ezz + C13_1 SMxx + C33_1 SMzz,   X_1 = X tau, X_0 = X_1 + X
+ *   and the free-surface rate is ezz = -((C13_0 exx + C13_1 SMxx) + C33_1 SMzz)/C33_0.
+ *   With C11 = C33 = lam + 2 mu this is the isotropic update operation for operation.
  *
  * Defects of the draft resolved here: _ddz := ddzp (visco-kernels.jl:67 undefined); the
  * velocity kernel's operator choice is mirrored to match its own stress kernel
@@ -59,6 +66,7 @@ typedef struct {
   REAL *vx, *vz, *sxx, *szz, *sxz;
   REAL *M; /* [3][L][nz][nx] */
   REAL *rho, *lam, *mu, *tau;
+  REAL *c11, *c33; /* VTI only (NULL: isotropic, pi = lam + 2 mu) */
   REAL taun[VIO_MAXL], a1[VIO_MAXL], a2[VIO_MAXL];
   REAL Dx[4], Dz[4];
   REAL *ax, *bx, *kx, *az, *bz, *kz;
@@ -141,7 +149,7 @@ void NAME(vio_destroy)(void* h) {
   SIM* s = (SIM*)h;
   if (!s) return;
   free(s->vx); free(s->vz); free(s->sxx); free(s->szz); free(s->sxz); free(s->M);
-  free(s->rho); free(s->lam); free(s->mu); free(s->tau);
+  free(s->rho); free(s->lam); free(s->mu); free(s->tau); free(s->c11); free(s->c33);
   free(s->ax); free(s->bx); free(s->kx); free(s->az); free(s->bz); free(s->kz);
   for (int q = 0; q < 8; ++q) free(s->psi[q]);
   free(s->six); free(s->siz); free(s->skind); free(s->sw); free(s->srcmark);
@@ -151,6 +159,19 @@ void NAME(vio_destroy)(void* h) {
 }
 
 long NAME(vio_step_index)(void* h) { return ((SIM*)h)->step; }
+
+/* VTI instance: lam <- C13, mu <- C55, plus the C11 and C33 planes */
+void* NAME(vio_create_vti)(const vio_desc* d, const REAL* rho, const REAL* c11, const REAL* c13,
+                           const REAL* c33, const REAL* c55, const REAL* tau, const REAL* taun) {
+  if (!c11 || !c33) return NULL;
+  SIM* s = (SIM*)NAME(vio_create)(d, rho, c13, c55, tau, taun);
+  if (!s) return NULL;
+  s->c11 = NAME(vio_alloc)(s->ncell);
+  s->c33 = NAME(vio_alloc)(s->ncell);
+  memcpy(s->c11, c11, s->ncell * sizeof(REAL));
+  memcpy(s->c33, c33, s->ncell * sizeof(REAL));
+  return s;
+}
 
 static int NAME(in_interior)(const SIM* s, int i, int j) {
   return i >= 4 && i < s->d.nx - 4 && j >= 4 && j < s->d.nz - 4;
@@ -325,11 +346,13 @@ static inline void NAME(stress_cell)(SIM* s, size_t c, long n, REAL exx, REAL ez
   const size_t nc = s->ncell;
   const REAL lr = s->lam[c], mr = s->mu[c];
   const REAL mxz = quarter * (((s->mu[c] + s->mu[c + 1]) + s->mu[c + nx]) + s->mu[c + nx + 1]);
-  const REAL pr = lr + two * mr;
+  const REAL px = s->c11 ? s->c11[c] : lr + two * mr; /* C11 (pi when isotropic) */
+  const REAL pz = s->c11 ? s->c33[c] : px;             /* C33 */
   REAL dsxx, dszz, dsxz;
   if (L > 0) {
     const REAL t = s->tau[c];
-    const REAL p1 = pr * t, p0 = p1 + pr;
+    const REAL px1 = px * t, px0 = px1 + px;
+    const REAL pz1 = pz * t, pz0 = pz1 + pz;
     const REAL l1 = lr * t, l0 = l1 + lr;
     const REAL m1 = mxz * t, m0 = m1 + mxz;
     REAL* Mxx = s->M;
@@ -341,14 +364,14 @@ static inline void NAME(stress_cell)(SIM* s, size_t c, long n, REAL exx, REAL ez
       SMzz = SMzz + Mzz[(size_t)k * nc + c];
       SMxz = SMxz + Mxz[(size_t)k * nc + c];
     }
-    if (fs_row) ezz = -(((l0 * exx) + l1 * SMxx) + p1 * SMzz) / p0;
-    dsxx = p0 * exx + l0 * ezz + p1 * SMxx + l1 * SMzz;
-    dszz = l0 * exx + p0 * ezz + l1 * SMxx + p1 * SMzz;
+    if (fs_row) ezz = -(((l0 * exx) + l1 * SMxx) + pz1 * SMzz) / pz0;
+    dsxx = px0 * exx + l0 * ezz + px1 * SMxx + l1 * SMzz;
+    dszz = l0 * exx + pz0 * ezz + l1 * SMxx + pz1 * SMzz;
     dsxz = m0 * gxz + m1 * SMxz;
   } else {
-    if (fs_row) ezz = -(lr * exx) / pr;
-    dsxx = pr * exx + lr * ezz;
-    dszz = lr * exx + pr * ezz;
+    if (fs_row) ezz = -(lr * exx) / pz;
+    dsxx = px * exx + lr * ezz;
+    dszz = lr * exx + pz * ezz;
     dsxz = mxz * gxz;
   }
   if (s->srcmark[c]) {
@@ -535,25 +558,27 @@ static void NAME(stress_multipass)(SIM* s, long n) {
       e3[c] = gxz;
       const REAL lr = s->lam[c], mr = s->mu[c];
       const REAL mxz = quarter * (((s->mu[c] + s->mu[c + 1]) + s->mu[c + nx]) + s->mu[c + nx + 1]);
-      const REAL pr = lr + two * mr;
+      const REAL px = s->c11 ? s->c11[c] : lr + two * mr;
+      const REAL pz = s->c11 ? s->c33[c] : px;
       const int fs_row = fs && j == 4;
       if (L > 0) {
         const REAL t = s->tau[c];
-        const REAL p1 = pr * t, p0 = p1 + pr, l1 = lr * t, l0 = l1 + lr, m1 = mxz * t, m0 = m1 + mxz;
+        const REAL px1 = px * t, px0 = px1 + px, pz1 = pz * t, pz0 = pz1 + pz;
+        const REAL l1 = lr * t, l0 = l1 + lr, m1 = mxz * t, m0 = m1 + mxz;
         REAL SMxx = Mxx[c], SMzz = Mzz[c], SMxz = Mxz[c];
         for (int k = 1; k < L; ++k) {
           SMxx = SMxx + Mxx[(size_t)k * nc + c];
           SMzz = SMzz + Mzz[(size_t)k * nc + c];
           SMxz = SMxz + Mxz[(size_t)k * nc + c];
         }
-        if (fs_row) ezz = -(((l0 * exx) + l1 * SMxx) + p1 * SMzz) / p0;
-        ds1[c] = p0 * exx + l0 * ezz + p1 * SMxx + l1 * SMzz;
-        ds2[c] = l0 * exx + p0 * ezz + l1 * SMxx + p1 * SMzz;
+        if (fs_row) ezz = -(((l0 * exx) + l1 * SMxx) + pz1 * SMzz) / pz0;
+        ds1[c] = px0 * exx + l0 * ezz + px1 * SMxx + l1 * SMzz;
+        ds2[c] = l0 * exx + pz0 * ezz + l1 * SMxx + pz1 * SMzz;
         ds3[c] = m0 * gxz + m1 * SMxz;
       } else {
-        if (fs_row) ezz = -(lr * exx) / pr;
-        ds1[c] = pr * exx + lr * ezz;
-        ds2[c] = lr * exx + pr * ezz;
+        if (fs_row) ezz = -(lr * exx) / pz;
+        ds1[c] = px * exx + lr * ezz;
+        ds2[c] = lr * exx + pz * ezz;
         ds3[c] = mxz * gxz;
       }
       e2[c] = ezz;

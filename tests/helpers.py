@@ -95,8 +95,9 @@ def compare(gpu, ref, dtype, label=""):
 
 
 def small_cfg(nx=77, nz=45, nrelax=2, cpml=True, fs=True, seed=1, nt=60, sources="all",
-              dt=2e-4):
-    """Heterogeneous small case exercising every feature of the step."""
+              dt=2e-4, vti=False):
+    """Heterogeneous small case exercising every feature of the step (``vti``: a VTI medium
+    with Thomsen eps ~ U[0, 0.25], delta ~ U[-0.1, 0.15] per cell)."""
     rng = np.random.default_rng(seed)
     vp = 2000.0 + 800.0 * rng.random((nz, nx))
     vs = vp / (1.6 + 0.4 * rng.random((nz, nx)))
@@ -104,6 +105,12 @@ def small_cfg(nx=77, nz=45, nrelax=2, cpml=True, fs=True, seed=1, nt=60, sources
     mu = rho * vs ** 2
     lam = rho * vp ** 2 - 2 * mu
     model = dict(rho=rho, lam=lam, mu=mu)
+    if vti:
+        from viscoelasticinversion_amd.api import thomsen_to_stiffness
+        eps = 0.25 * rng.random((nz, nx))
+        delta = -0.1 + 0.25 * rng.random((nz, nx))
+        c11, c13, c33, c55 = thomsen_to_stiffness(vp, vs, rho, eps, delta)
+        model = dict(rho=rho, c11=c11, c13=c13, c33=c33, c55=c55)
     taun = None
     if nrelax:
         model["tau"] = 0.05 + 0.05 * rng.random((nz, nx))

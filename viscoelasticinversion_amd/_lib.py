@@ -56,6 +56,7 @@ SIGNATURES = {
     "vi_device_count": (_I, []),
     "vi_mem_info": (_I, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "vi_model_create": (_I, [_I, _I, _I, _V, _V, _V, _V, ctypes.POINTER(_H)]),
+    "vi_model_create_vti": (_I, [_I, _I, _I, _V, _V, _V, _V, _V, _V, ctypes.POINTER(_H)]),
     "vi_model_destroy": (_I, [_H]),
     "vi_set_model_window": (_I, [_H, _H, _I]),
     "vi_device_bytes": (_I, [_H, ctypes.POINTER(ctypes.c_size_t)]),
@@ -73,6 +74,7 @@ SIGNATURES = {
     "vi_create": (_I, [ctypes.POINTER(GridDesc), ctypes.POINTER(_H)]),
     "vi_destroy": (_I, [_H]),
     "vi_set_model": (_I, [_H, _V, _V, _V, _V]),
+    "vi_set_model_vti": (_I, [_H, _V, _V, _V, _V, _V, _V]),
     "vi_set_relaxation": (_I, [_H, _DP]),
     "vi_set_cpml": (_I, [_H, ctypes.POINTER(CpmlDesc)]),
     "vi_set_free_surface": (_I, [_H, _I]),
@@ -86,6 +88,8 @@ SIGNATURES = {
     "vi_step_index": (_L, [_H]),
     "vi_get_traces": (_I, [_H, _V, _I]),
     "vi_gradient": (_I, [_H, _I, _V, _I, ctypes.POINTER(ctypes.c_double), _V, _V, _V, _V]),
+    "vi_gradient_vti": (_I, [_H, _I, _V, _I, ctypes.POINTER(ctypes.c_double), _V, _V, _V, _V, _V,
+                             _V]),
     "vi_set_tiling": (_I, [_H, _I]),
     "vi_get_tiling": (_I, [_H]),
     "vi_autotune": (_I, [_H, _I, _IP, _DP]),

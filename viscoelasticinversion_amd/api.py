@@ -145,6 +145,43 @@ class IsotropicModel:
         return float(np.sqrt((self.lam + 2 * self.mu) / self.rho).max())
 
 
+def thomsen_to_stiffness(vp0, vs0, rho, eps, delta):
+    """Thomsen parameters of a VTI medium to stiffnesses (Pa): C33 = rho vp0^2,
+    C55 = rho vs0^2, C11 = C33 (1 + 2 eps) and, from
+    delta = ((C13 + C55)^2 - (C33 - C55)^2) / (2 C33 (C33 - C55)),
+    C13 = sqrt((C33 - C55) (C33 (1 + 2 delta) - C55)) - C55."""
+    vp0, vs0, rho, eps, delta = (np.asarray(a, dtype=np.float64)
+                                 for a in (vp0, vs0, rho, eps, delta))
+    c33 = rho * vp0 ** 2
+    c55 = rho * vs0 ** 2
+    c11 = c33 * (1.0 + 2.0 * eps)
+    c13 = np.sqrt((c33 - c55) * (c33 * (1.0 + 2.0 * delta) - c55)) - c55
+    return c11, c13, c33, c55
+
+
+class VTIModel:
+    """Vertically transverse isotropic model from density and the stiffnesses C11, C13,
+    C33, C55, scalars or (Nz, Nx) arrays (Models.jl:27-33; fields b, C11, C13, C33, C55).
+    The stress update is the TTI form of ViscoelasticProblem.jl:21-35 with C15 = C35 = 0."""
+
+    def __init__(self, rho, C11, C13, C33, C55, Nx, Nz):
+        shp = (int(Nz), int(Nx))
+        f = lambda a: np.array(np.broadcast_to(a, shp), dtype=np.float64)
+        self.rho, self.C11, self.C13, self.C33, self.C55 = f(rho), f(C11), f(C13), f(C33), f(C55)
+
+    @property
+    def b(self):
+        return 1.0 / self.rho
+
+    @classmethod
+    def from_thomsen(cls, vp0, vs0, rho, eps, delta, Nx, Nz):
+        c11, c13, c33, c55 = thomsen_to_stiffness(vp0, vs0, rho, eps, delta)
+        return cls(rho, c11, c13, c33, c55, Nx, Nz)
+
+    def vmax(self):
+        return float(np.sqrt(np.maximum(self.C11, self.C33) / self.rho).max())
+
+
 def K(w, taun):
     """K(ω) = Σ ωτₙ/(1+τₙ²ω²)   (Models.jl:65)."""
     w = np.asarray(w, dtype=np.float64)[..., None]

@@ -14,11 +14,15 @@ from . import _lib as L
 
 
 class Model:
-    """Device-resident model arrays (rho, lam, mu[, tau]) of shape (nz, nx)."""
+    """Device-resident model arrays of shape (nz, nx): rho, lam, mu[, tau] (isotropic) or,
+    with ``vti`` = (c11, c13, c33, c55), rho and the VTI stiffnesses [, tau]; lam and mu
+    are then ignored (pass None)."""
 
-    def __init__(self, rho, lam, mu, tau=None, precision="float32"):
+    def __init__(self, rho, lam, mu, tau=None, precision="float32", vti=None):
         self.dtype = np.dtype(np.float32 if precision in ("float32", np.float32, "f32") else np.float64)
-        arrs = [np.ascontiguousarray(np.asarray(a, dtype=self.dtype)) for a in (rho, lam, mu)]
+        self.vti = vti is not None
+        planes = (rho, *vti) if self.vti else (rho, lam, mu)
+        arrs = [np.ascontiguousarray(np.asarray(a, dtype=self.dtype)) for a in planes]
         self.nz, self.nx = arrs[0].shape
         for a in arrs[1:]:
             if a.shape != arrs[0].shape:
@@ -28,11 +32,24 @@ class Model:
             raise ValueError("tau must have the model's shape")
         h = ctypes.c_void_p()
         P = ctypes.c_void_p
-        L.check(L.lib().vi_model_create(
-            self.nx, self.nz, L.VI_F32 if self.dtype == np.float32 else L.VI_F64,
-            *(a.ctypes.data_as(P) for a in arrs), t.ctypes.data_as(P) if t is not None else None,
-            ctypes.byref(h)))
+        prec = L.VI_F32 if self.dtype == np.float32 else L.VI_F64
+        tp = t.ctypes.data_as(P) if t is not None else None
+        if self.vti:
+            L.check(L.lib().vi_model_create_vti(self.nx, self.nz, prec,
+                                                *(a.ctypes.data_as(P) for a in arrs), tp,
+                                                ctypes.byref(h)))
+        else:
+            L.check(L.lib().vi_model_create(self.nx, self.nz, prec,
+                                            *(a.ctypes.data_as(P) for a in arrs), tp,
+                                            ctypes.byref(h)))
         self._h = h
+
+    @classmethod
+    def from_dict(cls, m, precision="float32"):
+        if "c11" in m:
+            return cls(m["rho"], None, None, m.get("tau"), precision,
+                       vti=(m["c11"], m["c13"], m["c33"], m["c55"]))
+        return cls(m["rho"], m["lam"], m["mu"], m.get("tau"), precision)
 
     @property
     def handle(self):

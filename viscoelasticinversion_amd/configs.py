@@ -103,8 +103,7 @@ def build_simulation(cfg, precision="float32"):
     from .stepper import Simulation
     sim = Simulation(cfg["nx"], cfg["nz"], cfg["dx"], cfg["dz"], cfg["dt"], precision,
                      cfg["nrelax"])
-    m = cfg["model"]
-    sim.set_model(m["rho"], m["lam"], m["mu"], m.get("tau"))
+    sim.set_model_dict(cfg["model"])
     if cfg["nrelax"]:
         sim.set_relaxation(cfg["taun"])
     if cfg["cpml"]:

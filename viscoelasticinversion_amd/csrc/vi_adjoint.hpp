@@ -27,7 +27,9 @@ struct AdjDesc {
   R a1[MAXL], a2[MAXL];
   int L, T, xlo, xhi, zlo, zhi, fs;
   size_t plane;
-  const R *rho, *lam, *mu, *tau;  // model (pitch ldm)
+  const R *rho, *lam, *mu, *tau;  // model (pitch ldm); VTI: lam = C13, mu = C55
+  const R *c11, *c33;             // VTI only
+  int vti;
   const R *ax, *bx, *kx, *az, *bz, *kz;
   // forward state of the current step (pitch ldx): stress s^n (pre-step) with images, the
   // velocity v^{n+1/2} after the velocity half, the memory-variable sums sum_k M^n_k of the
@@ -44,8 +46,9 @@ struct AdjDesc {
   R* apsi[8];
   R* ehat[4];  // adjoint of raw D-x vx, D+z vx, D-z vz, D+x vz (stress half)
   R* dhat[4];  // adjoint of raw D+x sxx, D-z sxz, D-x sxz, D+z szz (velocity half)
-  // gradient accumulators (pitch ldx): lam, mu (through pi), mu_xz, tau, bx, bz
-  R *g_lam, *g_mu, *g_muxz, *g_tau, *g_bx, *g_bz;
+  // gradient accumulators (pitch ldx): lam, mu (through C11 and C33), mu_xz, tau, bx, bz;
+  // VTI: g_lam = C13, g_mu = C11, g_c33 = C33, g_muxz = C55 at the sxz node
+  R *g_lam, *g_mu, *g_muxz, *g_tau, *g_bx, *g_bz, *g_c33;
 };
 
 template <typename R>
@@ -112,12 +115,15 @@ __global__ __launch_bounds__(256) void k_adj_stress_point(const AdjDesc<R> d) {
     ezz = cpml_fwd(ezz, gld(d.fpsi[5], qz), azv, bzv, kzv);
   }
   const R gxz = ezx + exz;
+  // moduli: lam | C13, mu | C55 (averaged to the sxz node), C11, C33 (lam + 2 mu if isotropic)
   const R lr = gld(d.lam, mb), mr = gld(d.mu, mb);
   const R mxz = quarter * (((mr + gld(d.mu, mb + E)) + gld(d.mu, mb + d.ldm * E)) +
                            gld(d.mu, mb + d.ldm * E + E));
-  const R pr = lr + two * mr;
+  const R px = d.vti ? gld(d.c11, mb) : lr + two * mr;
+  const R pz = d.vti ? gld(d.c33, mb) : px;
   const R t = L > 0 ? gld(d.tau, mb) : R(0);
-  const R p1 = pr * t, p0 = p1 + pr, l1 = lr * t, l0 = l1 + lr, m1 = mxz * t, m0 = m1 + mxz;
+  const R px1 = px * t, px0 = px1 + px, pz1 = pz * t, pz0 = pz1 + pz;
+  const R l1 = lr * t, l0 = l1 + lr, m1 = mxz * t, m0 = m1 + mxz;
   R SMxx = 0, SMzz = 0, SMxz = 0;
   if (L > 0) {
     SMxx = gld(d.fM, cb);
@@ -125,8 +131,8 @@ __global__ __launch_bounds__(256) void k_adj_stress_point(const AdjDesc<R> d) {
     SMxz = gld(d.fM + 2 * d.plane, cb);
   }
   const bool fs_row = d.fs && j == 4;
-  const R N = l0 * exx + l1 * SMxx + p1 * SMzz;  // free-surface row: ezz* = -N/p0
-  const R ezs = fs_row ? -N / p0 : ezz;
+  const R N = l0 * exx + l1 * SMxx + pz1 * SMzz;  // free-surface row: ezz* = -N/C33_0
+  const R ezs = fs_row ? -N / pz0 : ezz;
   // stress images: their adjoint goes to the interior rows they copy, then they are cleared
   if (d.fs) {
     if (j >= 5 && j <= 8) {
@@ -145,11 +151,11 @@ __global__ __launch_bounds__(256) void k_adj_stress_point(const AdjDesc<R> d) {
   const R sxzh = gld((const R*)d.asxz, cb);
   if (fs_row) gst(d.aszz, cb, R(0));
   // adjoints of the (stretched) rates and of the memory sums
-  R exh = dt * (p0 * sxxh + l0 * szzh);
-  R ezh = dt * (l0 * sxxh + p0 * szzh);
+  R exh = dt * (px0 * sxxh + l0 * szzh);
+  R ezh = dt * (l0 * sxxh + pz0 * szzh);
   R gh = dt * (m0 * sxzh);
-  const R SMxxh = dt * (p1 * sxxh + l1 * szzh);
-  const R SMzzh = dt * (l1 * sxxh + p1 * szzh);
+  const R SMxxh = dt * (px1 * sxxh + l1 * szzh);
+  const R SMzzh = dt * (l1 * sxxh + pz1 * szzh);
   const R SMxzh = dt * (m1 * sxzh);
   R Mzzh_sum = 0;  // sum_n a1_n * M_hat'_zz,n (free-surface gradient term)
   R* Ma[3 * LM];
@@ -169,45 +175,50 @@ __global__ __launch_bounds__(256) void k_adj_stress_point(const AdjDesc<R> d) {
   }
   ezh = ezh + ezh_from_M;
   R ezz_raw_h = ezh;
-  if (fs_row) {  // ezz* = -(l0 exx + l1 SMxx + p1 SMzz)/p0 replaces the raw rate
-    exh = exh - l0 / p0 * ezh;
-    fsMxx = -l1 / p0 * ezh;
-    fsMzz = -p1 / p0 * ezh;
+  if (fs_row) {  // ezz* = -(C13_0 exx + C13_1 SMxx + C33_1 SMzz)/C33_0 replaces the raw rate
+    exh = exh - l0 / pz0 * ezh;
+    fsMxx = -l1 / pz0 * ezh;
+    fsMzz = -pz1 / pz0 * ezh;
     for (int k = 0; k < L; ++k) {
       gst(Ma[k], cb, gld((const R*)Ma[k], cb) + fsMxx);
       gst(Ma[L + k], cb, gld((const R*)Ma[L + k], cb) + fsMzz);
     }
     ezz_raw_h = 0;
   }
-  // gradient with respect to lam, mu (through pi), mu_xz and tau of this cell
+  // gradient with respect to C11, C13, C33 (lam and mu through them when isotropic),
+  // C55 at the sxz node and tau of this cell
   {
     const R tt = R(1) + t;
-    R glam, gmu, gtau;
+    R g11, g13, g33, gtau;
     if (!fs_row) {
-      const R cl = dt * (tt * (exx + ezz) + t * (SMxx + SMzz));
-      glam = sxxh * cl + szzh * cl;
-      gmu = sxxh * dt * (two * tt * exx + two * t * SMxx) +
-            szzh * dt * (two * tt * ezz + two * t * SMzz);
-      gtau = sxxh * dt * (pr * exx + lr * ezz + pr * SMxx + lr * SMzz) +
-             szzh * dt * (lr * exx + pr * ezz + lr * SMxx + pr * SMzz);
+      const R ax_ = dt * (tt * exx + t * SMxx), az_ = dt * (tt * ezz + t * SMzz);
+      g11 = sxxh * ax_;
+      g33 = szzh * az_;
+      g13 = sxxh * az_ + szzh * ax_;
+      gtau = sxxh * dt * (px * exx + lr * ezz + px * SMxx + lr * SMzz) +
+             szzh * dt * (lr * exx + pz * ezz + lr * SMxx + pz * SMzz);
     } else {
-      // sxx' = sxx + dt (A + l0 z), z = -N/p0; Mzz'_n depends on z through dt a1_n z
+      // sxx' = sxx + dt (A + C13_0 z) with z = -N/C33_0; Mzz'_n depends on z via dt a1_n z
       const R z = ezs;
-      const R dNl = tt * exx + t * SMxx + t * SMzz, dNm = two * t * SMzz,
-              dNt = lr * exx + lr * SMxx + pr * SMzz;
-      const R dPl = tt, dPm = two * tt, dPt = pr;
-      const R dzl = -dNl / p0 + N / (p0 * p0) * dPl;
-      const R dzm = -dNm / p0 + N / (p0 * p0) * dPm;
-      const R dzt = -dNt / p0 + N / (p0 * p0) * dPt;
-      const R dAl = tt * exx + t * SMxx + t * SMzz, dAm = two * tt * exx + two * t * SMxx,
-              dAt = pr * exx + pr * SMxx + lr * SMzz;
-      glam = sxxh * dt * (dAl + tt * z + l0 * dzl) + dt * Mzzh_sum * dzl;
-      gmu = sxxh * dt * (dAm + l0 * dzm) + dt * Mzzh_sum * dzm;
-      gtau = sxxh * dt * (dAt + lr * z + l0 * dzt) + dt * Mzzh_sum * dzt;
+      const R q = N / (pz0 * pz0);
+      const R dz13 = -(tt * exx + t * SMxx) / pz0;
+      const R dz33 = -(t * SMzz) / pz0 + q * tt;
+      const R dzt = -(lr * exx + lr * SMxx + pz * SMzz) / pz0 + q * pz;
+      g11 = sxxh * dt * (tt * exx + t * SMxx);
+      g13 = sxxh * dt * (t * SMzz + tt * z + l0 * dz13) + dt * Mzzh_sum * dz13;
+      g33 = sxxh * dt * (l0 * dz33) + dt * Mzzh_sum * dz33;
+      gtau = sxxh * dt * (px * exx + px * SMxx + lr * SMzz + lr * z + l0 * dzt) +
+             dt * Mzzh_sum * dzt;
     }
     gtau = gtau + sxzh * dt * (mxz * gxz + mxz * SMxz);
-    gst(d.g_lam, cb, gld((const R*)d.g_lam, cb) + glam);
-    gst(d.g_mu, cb, gld((const R*)d.g_mu, cb) + gmu);
+    if (d.vti) {
+      gst(d.g_lam, cb, gld((const R*)d.g_lam, cb) + g13);
+      gst(d.g_mu, cb, gld((const R*)d.g_mu, cb) + g11);
+      gst(d.g_c33, cb, gld((const R*)d.g_c33, cb) + g33);
+    } else {  // C11 = C33 = lam + 2 mu, C13 = lam
+      gst(d.g_lam, cb, gld((const R*)d.g_lam, cb) + ((g11 + g33) + g13));
+      gst(d.g_mu, cb, gld((const R*)d.g_mu, cb) + two * (g11 + g33));
+    }
     gst(d.g_muxz, cb, gld((const R*)d.g_muxz, cb) + sxzh * dt * (tt * gxz + t * SMxz));
     if (L > 0) gst(d.g_tau, cb, gld((const R*)d.g_tau, cb) + gtau);
   }
@@ -415,7 +426,7 @@ __global__ __launch_bounds__(256) void k_grad_finalize(const AdjDesc<R> d, R* g_
   const R quarter = R(0.25), two = R(2);
   // mu_xz(c') = 1/4 (mu(c') + mu(c'+1) + mu(c'+ld) + mu(c'+ld+1)): mu(c) feeds c, c-1,
   // c-ld, c-ld-1
-  R gm_ = gld((const R*)d.g_mu, cb);
+  R gm_ = d.vti ? R(0) : gld((const R*)d.g_mu, cb);  // VTI: C55 acts only at sxz nodes
   R gx = gld((const R*)d.g_muxz, cb);
   if (i >= 1) gx = gx + gld((const R*)d.g_muxz, cb - E);
   if (j >= 1) gx = gx + gld((const R*)d.g_muxz, cb - rb);

@@ -155,7 +155,8 @@ struct ModelStore {
   int nx = 0, nz = 0, ldm = 0, nzp = 0;
   size_t esz = 4;
   bool has_tau = false;
-  DevBuf arr[4];
+  bool vti = false;
+  DevBuf arr[6];  // rho, lam | C13, mu | C55, tau, C11, C33 (the last two for VTI only)
 };
 
 struct vi_model_s {
@@ -346,6 +347,9 @@ void build_desc(const vi_sim_s* h, int tile_base, vi::ShotDesc<R>* d) {
   d->lam = (const R*)m.arr[1].p + h->model_x0;
   d->mu = (const R*)m.arr[2].p + h->model_x0;
   d->tau = m.arr[3].p ? (const R*)m.arr[3].p + h->model_x0 : nullptr;
+  d->vti = m.vti ? 1 : 0;
+  d->c11 = m.vti ? (const R*)m.arr[4].p + h->model_x0 : nullptr;
+  d->c33 = m.vti ? (const R*)m.arr[5].p + h->model_x0 : nullptr;
   if (h->cpml_on) {
     d->T = h->cpml.thickness;
     d->xlo = h->cpml.left;
@@ -514,7 +518,8 @@ std::vector<DevBuf*> mutable_arrays(vi_sim_s* h) {
   return v;
 }
 
-int make_model_store(int nx, int nz, size_t esz, bool with_tau, const void* const src[4],
+// src = {rho, lam | C13, mu | C55, tau, C11, C33}; C11 and C33 given only for VTI
+int make_model_store(int nx, int nz, size_t esz, bool with_tau, const void* const src[6],
                      hipStream_t s, std::shared_ptr<ModelStore>* out) {
   auto m = std::make_shared<ModelStore>();
   m->nx = nx;
@@ -523,9 +528,12 @@ int make_model_store(int nx, int nz, size_t esz, bool with_tau, const void* cons
   m->ldm = (int)round_up((size_t)nx + 72, 64);
   m->nzp = nz + vi::NW * vi::RZ_MAX + 16;
   m->has_tau = with_tau;
+  m->vti = src[4] != nullptr;
   const size_t pb = (size_t)m->ldm * m->nzp * esz;
   if (pb >= (size_t(1) << 31)) return fail(VI_EINVAL, "model plane must stay below 2 GiB");
-  for (int q = 0; q < (with_tau ? 4 : 3); ++q) {
+  for (int q = 0; q < 6; ++q) {
+    if (q == 3 && !with_tau) continue;
+    if (q >= 4 && !m->vti) continue;
     VI_TRY(m->arr[q].alloc(pb, s));
     VI_TRY(upload_plane(s, m->arr[q].p, m->ldm, src[q], nx, nz, esz));
   }
@@ -631,9 +639,9 @@ int batch_finish(vi_batch_s* b, int nsteps) {
 
 
 // ---------------------------------------------------------------- gradient (adjoint)
+// out = {rho, lam | C13, mu | C55, tau, C11, C33} (host arrays [nz][nx], each may be NULL)
 template <typename R>
-int gradient_impl(vi_sim_s* h, int nt, const R* dobs, int K, double* misfit, R* out_rho,
-                  R* out_lam, R* out_mu, R* out_tau) {
+int gradient_impl(vi_sim_s* h, int nt, const R* dobs, int K, double* misfit, R* const out[6]) {
   const size_t E = sizeof(R);
   const int L = h->L;
   const size_t pb = h->plane * E;
@@ -687,7 +695,7 @@ int gradient_impl(vi_sim_s* h, int nt, const R* dobs, int K, double* misfit, R* 
   // free-surface image rows (rows 0..3) read rows -4..-1, where the derivative adjoints are 0
   DevBuf adj;
   const size_t guard = 4 * (size_t)h->ldx * E;
-  const size_t nplanes_adj = 5 + 3 * (size_t)L + 4 + 4 + 6 + 2;
+  const size_t nplanes_adj = 5 + 3 * (size_t)L + 4 + 4 + 7 + 2;
   VI_TRY(adj.alloc(nplanes_adj * pb + 8 * guard + psi_total, st));
   char* ab = (char*)adj.p;
   auto take = [&](size_t bytes) { char* p = ab; ab += bytes; return (R*)p; };
@@ -698,7 +706,7 @@ int gradient_impl(vi_sim_s* h, int nt, const R* dobs, int K, double* misfit, R* 
   for (int q = 0; q < 4; ++q) d.ehat[q] = take_guarded();
   for (int q = 0; q < 4; ++q) d.dhat[q] = take_guarded();
   d.g_lam = take(pb); d.g_mu = take(pb); d.g_muxz = take(pb); d.g_tau = take(pb);
-  d.g_bx = take(pb); d.g_bz = take(pb);
+  d.g_bx = take(pb); d.g_bz = take(pb); d.g_c33 = take(pb);
   R* g_rho_out = take(pb);
   R* g_mu_out = take(pb);
   for (int q = 0; q < 8; ++q) d.apsi[q] = take(psi_bytes[q]);
@@ -746,6 +754,7 @@ int gradient_impl(vi_sim_s* h, int nt, const R* dobs, int K, double* misfit, R* 
     for (int k = 0; k < L; ++k) { d.a1[k] = sd->a1[k]; d.a2[k] = sd->a2[k]; }
     d.T = sd->T; d.xlo = sd->xlo; d.xhi = sd->xhi; d.zlo = sd->zlo; d.zhi = sd->zhi; d.fs = sd->fs;
     d.rho = sd->rho; d.lam = sd->lam; d.mu = sd->mu; d.tau = sd->tau;
+    d.c11 = sd->c11; d.c33 = sd->c33; d.vti = sd->vti;
     d.ax = sd->ax; d.bx = sd->bx; d.kx = sd->kx; d.az = sd->az; d.bz = sd->bz; d.kz = sd->kz;
     d.src_w = sd->src_w; d.snt = sd->snt;
   }
@@ -847,10 +856,10 @@ int gradient_impl(vi_sim_s* h, int nt, const R* dobs, int K, double* misfit, R* 
   hipLaunchKernelGGL(vi::k_grad_finalize<R>, gall, blk, 0, st, d, g_rho_out, g_mu_out);
   VI_HIP(hipGetLastError());
   const int nx = h->g.nx, nz = h->g.nz;
-  if (out_rho) VI_TRY(download_plane(st, out_rho, g_rho_out, h->ldx, nx, nz, E));
-  if (out_lam) VI_TRY(download_plane(st, out_lam, d.g_lam, h->ldx, nx, nz, E));
-  if (out_mu) VI_TRY(download_plane(st, out_mu, g_mu_out, h->ldx, nx, nz, E));
-  if (out_tau && L) VI_TRY(download_plane(st, out_tau, d.g_tau, h->ldx, nx, nz, E));
+  const R* planes[6] = {g_rho_out, d.g_lam, g_mu_out, L ? d.g_tau : nullptr,
+                        d.vti ? d.g_mu : nullptr, d.vti ? d.g_c33 : nullptr};
+  for (int q = 0; q < 6; ++q)
+    if (out[q] && planes[q]) VI_TRY(download_plane(st, out[q], planes[q], h->ldx, nx, nz, E));
   VI_HIP(hipStreamSynchronize(st));
   // leave the handle at a zero state, step 0
   return vi_reset(h);
@@ -910,7 +919,28 @@ int vi_model_create(int nx, int nz, int precision, const void* rho, const void* 
   if (vi_device_count() < 1) return fail(VI_EDEVICE, "vi_model_create: no HIP device visible");
   hipStream_t s;
   VI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  const void* src[4] = {rho, lam, mu, tau};
+  const void* src[6] = {rho, lam, mu, tau, nullptr, nullptr};
+  std::shared_ptr<ModelStore> m;
+  int rc = make_model_store(nx, nz, precision == VI_F32 ? 4 : 8, tau != nullptr, src, s, &m);
+  (void)hipStreamDestroy(s);
+  if (rc) return rc;
+  *out = new vi_model_s{m};
+  return 0;
+}
+
+int vi_model_create_vti(int nx, int nz, int precision, const void* rho, const void* c11,
+                        const void* c13, const void* c33, const void* c55, const void* tau,
+                        vi_model* out) {
+  if (!out || !rho || !c11 || !c13 || !c33 || !c55)
+    return fail(VI_EINVAL, "vi_model_create_vti: null argument");
+  *out = nullptr;
+  if (nx < 16 || nz < 16) return fail(VI_EINVAL, "vi_model_create_vti: nx and nz must be >= 16");
+  if (precision != VI_F32 && precision != VI_F64)
+    return fail(VI_EINVAL, "vi_model_create_vti: bad precision");
+  if (vi_device_count() < 1) return fail(VI_EDEVICE, "vi_model_create_vti: no HIP device visible");
+  hipStream_t s;
+  VI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  const void* src[6] = {rho, c13, c55, tau, c11, c33};
   std::shared_ptr<ModelStore> m;
   int rc = make_model_store(nx, nz, precision == VI_F32 ? 4 : 8, tau != nullptr, src, s, &m);
   (void)hipStreamDestroy(s);
@@ -979,7 +1009,24 @@ int vi_set_model(vi_handle h, const void* rho, const void* lam, const void* mu,
   if (!rho || !lam || !mu) return fail(VI_EINVAL, "vi_set_model: rho, lam, mu required");
   if (h->L && !tau) return fail(VI_EINVAL, "vi_set_model: tau required when nrelax > 0");
   VI_TRY(wait_external(h));
-  const void* src[4] = {rho, lam, mu, h->L ? tau : nullptr};
+  const void* src[6] = {rho, lam, mu, h->L ? tau : nullptr, nullptr, nullptr};
+  VI_HIP(hipStreamSynchronize(h->stream));
+  std::shared_ptr<ModelStore> m;
+  VI_TRY(make_model_store(h->g.nx, h->g.nz, h->esz, h->L > 0, src, h->stream, &m));
+  h->model = m;
+  h->model_x0 = 0;
+  touch(h);
+  return 0;
+}
+
+int vi_set_model_vti(vi_handle h, const void* rho, const void* c11, const void* c13,
+                     const void* c33, const void* c55, const void* tau) {
+  VI_TRY(check_handle(h));
+  if (!rho || !c11 || !c13 || !c33 || !c55)
+    return fail(VI_EINVAL, "vi_set_model_vti: rho, c11, c13, c33, c55 required");
+  if (h->L && !tau) return fail(VI_EINVAL, "vi_set_model_vti: tau required when nrelax > 0");
+  VI_TRY(wait_external(h));
+  const void* src[6] = {rho, c13, c55, h->L ? tau : nullptr, c11, c33};
   VI_HIP(hipStreamSynchronize(h->stream));
   std::shared_ptr<ModelStore> m;
   VI_TRY(make_model_store(h->g.nx, h->g.nz, h->esz, h->L > 0, src, h->stream, &m));
@@ -1341,6 +1388,7 @@ int vi_algorithmic_bytes(vi_handle h, int kernel, double* bytes) {
   // velocity: v R+W (4) + s R (3) + rho (1); stress: v R (2) + s R+W (6) + lam, mu (2)
   // [+ tau (1) + M R+W (6L)]
   double words = kernel == 0 ? 8.0 : (h->L ? 11.0 + 6.0 * h->L : 10.0);
+  if (kernel == 1 && h->model && h->model->vti) words += 2.0;  // C11, C33
   double total = words * ni;
   if (h->cpml_on) {  // two psi arrays read and written per strip direction per kernel
     const int T = h->cpml.thickness;
@@ -1364,13 +1412,35 @@ int vi_gradient(vi_handle h, int nt, const void* dobs, int checkpoint_every, dou
   VI_TRY(check_ready(h));
   if (h->nrec < 1 || h->rec_nt < nt)
     return fail(VI_ESTATE, "vi_gradient: receivers with capacity >= nt are required");
+  if (h->model->vti) return fail(VI_ESTATE, "vi_gradient: VTI model, use vi_gradient_vti");
   VI_TRY(wait_external(h));
   VI_HIP(hipStreamSynchronize(h->stream));
+  void* out[6] = {g_rho, g_lam, g_mu, g_tau, nullptr, nullptr};
   if (h->esz == 4)
     return gradient_impl<float>(h, nt, (const float*)dobs, checkpoint_every, misfit,
-                                (float*)g_rho, (float*)g_lam, (float*)g_mu, (float*)g_tau);
+                                (float* const*)out);
   return gradient_impl<double>(h, nt, (const double*)dobs, checkpoint_every, misfit,
-                               (double*)g_rho, (double*)g_lam, (double*)g_mu, (double*)g_tau);
+                               (double* const*)out);
+}
+
+int vi_gradient_vti(vi_handle h, int nt, const void* dobs, int checkpoint_every,
+                    double* misfit, void* g_rho, void* g_c11, void* g_c13, void* g_c33,
+                    void* g_c55, void* g_tau) {
+  VI_TRY(check_handle(h));
+  if (nt < 1 || !dobs || checkpoint_every < 0)
+    return fail(VI_EINVAL, "vi_gradient_vti: need nt >= 1, dobs, checkpoint_every >= 0");
+  VI_TRY(check_ready(h));
+  if (h->nrec < 1 || h->rec_nt < nt)
+    return fail(VI_ESTATE, "vi_gradient_vti: receivers with capacity >= nt are required");
+  if (!h->model->vti) return fail(VI_ESTATE, "vi_gradient_vti: isotropic model, use vi_gradient");
+  VI_TRY(wait_external(h));
+  VI_HIP(hipStreamSynchronize(h->stream));
+  void* out[6] = {g_rho, g_c13, g_c55, g_tau, g_c11, g_c33};
+  if (h->esz == 4)
+    return gradient_impl<float>(h, nt, (const float*)dobs, checkpoint_every, misfit,
+                                (float* const*)out);
+  return gradient_impl<double>(h, nt, (const double*)dobs, checkpoint_every, misfit,
+                               (double* const*)out);
 }
 
 // ---------------------------------------------------------------- batches

@@ -24,8 +24,10 @@
 // Arithmetic follows the oracle's operation order exactly (no contraction: the library is
 // built with -ffp-contract=off), so FP32 and FP64 results match the CPU oracle bit for
 // bit. References: operators src/diff-funcs.jl:3-32; velocity src/visco-kernels.jl:3-20;
-// constitutive + memory src/visco-kernels.jl:22-85; CPML src/sciml_solver/CPML.jl:291-328;
-// imaging src/sciml_solver/CPML.jl:372-380; source src/sciml_solver/Source.jl:35-40.
+// constitutive + memory src/visco-kernels.jl:22-85; VTI stiffnesses
+// src/sciml_solver/Models.jl:27-33 in the form of ViscoelasticProblem.jl:21-35; CPML
+// src/sciml_solver/CPML.jl:291-328; imaging src/sciml_solver/CPML.jl:372-380; source
+// src/sciml_solver/Source.jl:35-40.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
@@ -48,6 +50,10 @@ struct ShotDesc {
   R a1[MAXL], a2[MAXL];
   R *vx, *vz, *sxx, *szz, *sxz, *M;
   const R *rho, *lam, *mu, *tau;  // model window (column offset applied), pitch ldm
+  // VTI (vti = 1): lam holds C13, mu holds C55 and c11, c33 the diagonal stiffnesses;
+  // isotropic (vti = 0): C11 = C33 = lam + 2 mu computed in the kernel
+  const R *c11, *c33;
+  int vti;
   int T, xlo, xhi, zlo, zhi, fs;
   const R *ax, *bx, *kx, *az, *bz, *kz;
   R* psi[8];  // 0..3: [nz][2T] (vx_x, vz_x, sxx_x, sxz_x); 4..7: [2T][ldx] (vx_z, vz_z, sxz_z, szz_z)
@@ -290,10 +296,18 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
 #pragma unroll
   for (int k = 0; k < 3 * LM; ++k) Mp[k] = d.M + (size_t)k * d.plane;
   R wmu[RZ + 1], wmu1[RZ + 1];  // mu at columns i and i+1, rows jw .. jw+RZ
-  R olam[RH], otau[RH], osxx[RH], oszz[RH], osxz[RH], om[3 * LM][RH];
+  const bool vti = d.vti;
+  R olam[RH], otau[RH], osxx[RH], oszz[RH], osxz[RH], om[3 * LM][RH], oc11[RH], oc33[RH];
   auto load_cell = [&](int r, int h) {
     const unsigned cb = cb0 + r * rowb, mb = mb0 + r * mrowb;
     olam[h] = gld(lam, mb);
+    if (vti) {
+      oc11[h] = gld(d.c11, mb);
+      oc33[h] = gld(d.c33, mb);
+    } else {
+      oc11[h] = R(0);
+      oc33[h] = R(0);
+    }
     osxx[h] = gld((const R*)sxx, cb);
     oszz[h] = gld((const R*)szz, cb);
     osxz[h] = gld((const R*)sxz, cb);
@@ -367,12 +381,14 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
     const R gxz = ezx + exz;
     const R lr = olam[h], mr = wmu[r];
     const R mxz = quarter * (((wmu[r] + wmu1[r]) + wmu[r + 1]) + wmu1[r + 1]);
-    const R pr = lr + two * mr;
+    const R px = vti ? oc11[h] : lr + two * mr;  // C11
+    const R pz = vti ? oc33[h] : px;              // C33
     const bool fs_row = fs && j == 4;
     R dsxx, dszz, dsxz;
     if constexpr (L > 0) {
       const R t = otau[h];
-      const R p1 = pr * t, p0 = p1 + pr;
+      const R px1 = px * t, px0 = px1 + px;
+      const R pz1 = pz * t, pz0 = pz1 + pz;
       const R l1 = lr * t, l0 = l1 + lr;
       const R m1 = mxz * t, m0 = m1 + mxz;
       R SMxx = om[0][h], SMzz = om[L][h], SMxz = om[2 * L][h];
@@ -382,14 +398,14 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
         SMzz = SMzz + om[L + k][h];
         SMxz = SMxz + om[2 * L + k][h];
       }
-      if (fs_row) ezz = -(((l0 * exx) + l1 * SMxx) + p1 * SMzz) / p0;
-      dsxx = p0 * exx + l0 * ezz + p1 * SMxx + l1 * SMzz;
-      dszz = l0 * exx + p0 * ezz + l1 * SMxx + p1 * SMzz;
+      if (fs_row) ezz = -(((l0 * exx) + l1 * SMxx) + pz1 * SMzz) / pz0;
+      dsxx = px0 * exx + l0 * ezz + px1 * SMxx + l1 * SMzz;
+      dszz = l0 * exx + pz0 * ezz + l1 * SMxx + pz1 * SMzz;
       dsxz = m0 * gxz + m1 * SMxz;
     } else {
-      if (fs_row) ezz = -(lr * exx) / pr;
-      dsxx = pr * exx + lr * ezz;
-      dszz = lr * exx + pr * ezz;
+      if (fs_row) ezz = -(lr * exx) / pz;
+      dsxx = px * exx + lr * ezz;
+      dszz = lr * exx + pz * ezz;
       dsxz = mxz * gxz;
     }
     for (int s = s0; s < s1; ++s) {

@@ -42,6 +42,7 @@ class Simulation:
         L.check(L.lib().vi_create(ctypes.byref(g), ctypes.byref(h)))
         self._h = h
         self.cpml_thickness = 0
+        self.vti = False
         self.nrec = 0
         self.rec_nt = 0
 
@@ -72,11 +73,34 @@ class Simulation:
             t = self._arr(np.broadcast_to(tau, shp), shp)
         L.check(L.lib().vi_set_model(self._h, *(self._ptr(a) for a in arrs),
                                      self._ptr(t) if t is not None else None))
+        self.vti = False
+
+    def set_model_vti(self, rho, c11, c13, c33, c55, tau=None):
+        """VTI medium: density and stiffnesses C11, C13, C33, C55 (VTIModel, Models.jl:27-33)."""
+        shp = (self.nz, self.nx)
+        arrs = [self._arr(np.broadcast_to(x, shp), shp) for x in (rho, c11, c13, c33, c55)]
+        t = None
+        if self.nrelax:
+            if tau is None:
+                raise ValueError("tau is required when nrelax > 0")
+            t = self._arr(np.broadcast_to(tau, shp), shp)
+        L.check(L.lib().vi_set_model_vti(self._h, *(self._ptr(a) for a in arrs),
+                                         self._ptr(t) if t is not None else None))
+        self.vti = True
+
+    def set_model_dict(self, m):
+        """Model from a dict: rho, lam, mu (isotropic) or rho, c11, c13, c33, c55 (VTI),
+        plus tau when nrelax > 0."""
+        if "c11" in m:
+            self.set_model_vti(m["rho"], m["c11"], m["c13"], m["c33"], m["c55"], m.get("tau"))
+        else:
+            self.set_model(m["rho"], m["lam"], m["mu"], m.get("tau"))
 
     def set_model_window(self, model, x0):
         """Read the model from columns [x0, x0 + nx) of a shared device Model."""
         L.check(L.lib().vi_set_model_window(self._h, model.handle, int(x0)))
         self._model_ref = model
+        self.vti = bool(getattr(model, "vti", False))
 
     def device_bytes(self):
         b = ctypes.c_size_t()
@@ -166,19 +190,20 @@ class Simulation:
 
     # -- gradient ----------------------------------------------------------------------
     def gradient(self, dobs, nt=None, checkpoint_every=0):
-        """L2 trace misfit J and its gradient with respect to the model arrays (rho, lam, mu,
-        tau) by the discrete adjoint of ``nt`` steps from a zero state. ``dobs`` is
-        (nrec, nt). Leaves the simulation reset. Returns (J, {"rho", "lam", "mu", "tau"})."""
+        """L2 trace misfit J and its gradient with respect to the model arrays by the discrete
+        adjoint of ``nt`` steps from a zero state. ``dobs`` is (nrec, nt). Leaves the
+        simulation reset. Returns (J, g) with g = {"rho", "lam", "mu"[, "tau"]} for an
+        isotropic model and {"rho", "c11", "c13", "c33", "c55"[, "tau"]} for a VTI model."""
         d = self._arr(dobs)
         nt = d.shape[1] if nt is None else int(nt)
         d = np.ascontiguousarray(d.reshape(self.nrec, -1)[:, :nt])
         shp = (self.nz, self.nx)
-        g = {k: np.zeros(shp, dtype=self.dtype) for k in ("rho", "lam", "mu", "tau")}
+        keys = ("rho", "c11", "c13", "c33", "c55", "tau") if self.vti else ("rho", "lam", "mu", "tau")
+        g = {k: np.zeros(shp, dtype=self.dtype) for k in keys}
         J = ctypes.c_double()
-        L.check(L.lib().vi_gradient(self._h, nt, self._ptr(d), int(checkpoint_every),
-                                    ctypes.byref(J), self._ptr(g["rho"]), self._ptr(g["lam"]),
-                                    self._ptr(g["mu"]),
-                                    self._ptr(g["tau"]) if self.nrelax else None))
+        ptrs = [self._ptr(g[k]) for k in keys[:-1]] + [self._ptr(g["tau"]) if self.nrelax else None]
+        fn = L.lib().vi_gradient_vti if self.vti else L.lib().vi_gradient
+        L.check(fn(self._h, nt, self._ptr(d), int(checkpoint_every), ctypes.byref(J), *ptrs))
         if not self.nrelax:
             g.pop("tau")
         return J.value, g
