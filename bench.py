@@ -137,11 +137,12 @@ def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True):
                         "k_velocity_hbm_frac": round(bytes_v / avg_v / 1e9 / HBM_PEAK_GBS, 4),
                         "autotune_us_per_step": {str(k): round(v, 1) for k, v in table.items()},
                         "traces_finite": finite},
-            "serial_plain_stepper": {"wall_s": round(wall_s, 4),
-                                     "GPt_per_s": round(cells * nt / wall_s / 1e9, 3),
-                                     "shots_per_s": round(nshots / wall_s, 3),
-                                     "latency_p50_s": round(float(np.percentile(done, 50)), 4),
-                                     "latency_p99_s": round(float(np.percentile(done, 99)), 4)}}
+            "serial_plain_stepper": None if not serial else {
+                "wall_s": round(wall_s, 4),
+                "GPt_per_s": round(cells * nt / wall_s / 1e9, 3),
+                "shots_per_s": round(nshots / wall_s, 3),
+                "latency_p50_s": round(float(np.percentile(done, 50)), 4),
+                "latency_p99_s": round(float(np.percentile(done, 99)), 4)}}
 
 
 def bench_gradient(nshots=16, nt=3000, checkpoint_every=0):

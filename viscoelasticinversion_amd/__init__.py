@@ -3,10 +3,12 @@
 The compute path is the HIP library ``libvi_mi355x.so`` behind the C ABI of
 ``include/vi_mi355x.h``; this package is its host-side mirror of the reference solver API.
 """
+from . import segy
 from .api import (CPMLConfig, DirichletBC, Fields, FreeSurfaceBC, IsotropicModel, K,
-                  Parameters, Problem, QModel, RickerSource, Solution, _ricker, dK_dw,
+                  Parameters, Problem, QModel, RickerSource, Solution, VTIModel, _ricker, dK_dw,
                   find_optimal_taun, get_bc, get_Q, get_tau, gradient_to_velocity_params,
-                  leapfrog_step, make_problem, misfit_and_pullback, solve_problem)
+                  leapfrog_step, make_problem, misfit_and_pullback, solve_problem,
+                  thomsen_to_stiffness)
 from .batch import Batch, Model, mem_info
 from .stepper import COMPONENTS, FIELDS, SOURCE_KINDS, Simulation
 
@@ -15,5 +17,6 @@ __all__ = [
     "Parameters", "Problem", "QModel", "RickerSource", "Solution", "_ricker", "dK_dw",
     "find_optimal_taun", "get_bc", "get_Q", "get_tau", "leapfrog_step", "make_problem",
     "solve_problem", "Simulation", "FIELDS", "COMPONENTS", "SOURCE_KINDS", "Batch", "Model",
-    "mem_info", "gradient_to_velocity_params", "misfit_and_pullback",
+    "mem_info", "gradient_to_velocity_params", "misfit_and_pullback", "VTIModel",
+    "thomsen_to_stiffness", "segy",
 ]
