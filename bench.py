@@ -194,6 +194,44 @@ def bench_gradient(nshots=16, nt=3000, checkpoint_every=0):
             "misfit": J, "gradient_finite": bool(all(np.isfinite(v).all() for v in total.values()))}
 
 
+def bench_serving(njobs=24, rate=20.0, seed=0, block_steps=100, max_gradients=4):
+    """Serving workload: a seeded Poisson stream of ``njobs`` shot jobs at ``rate`` per
+    second (70 % forward, 30 % gradient; grids 256^2, 512x1024, 1024x2048, 2048^2; isotropic
+    or VTI; L=3; FP32; 1000-4000 steps) served with continuous batching under an HBM budget,
+    then the same stream through the plain single-shot stepper one job at a time."""
+    from viscoelasticinversion_amd.configs import serving_workload
+    from viscoelasticinversion_amd.serve import Server, run_serial, summarize
+    def progress(tag, kinds):
+        def ev(what, jid, t):
+            if what == "done":
+                print(f"[serving {tag}] job {jid} ({kinds[jid]}) done at {t:.2f} s",
+                      file=sys.stderr, flush=True)
+        return ev
+
+    warm = serving_workload(2, rate=1000.0, seed=seed + 1, grids=((128, 96),), steps=(50, 60),
+                            gradient_fraction=0.5)
+    Server(keep_outputs=False).run(warm)
+    run_serial(warm, keep_outputs=False)
+    t0 = time.perf_counter()
+    jobs = serving_workload(njobs, rate, seed)
+    kinds = {j.id: j.kind for j in jobs}
+    print(f"[serving] {njobs} jobs generated in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    srv = Server(block_steps=block_steps, max_gradients=max_gradients, keep_outputs=False)
+    served = summarize(srv.run(jobs, on_event=progress("served", kinds)))
+    served.update({"hbm_budget_GB": round(srv.budget / 1e9, 2),
+                   "peak_admitted_GB": round(srv.peak_bytes / 1e9, 2),
+                   "block_steps": block_steps, "max_gradients": max_gradients})
+    serial = summarize(run_serial(jobs, keep_outputs=False, on_event=progress("serial", kinds)))
+    ng = sum(j.kind == "gradient" for j in jobs)
+    return {"workload": f"{njobs} jobs, Poisson {rate}/s, {ng} gradient / {njobs - ng} forward, "
+                        "grids {256^2, 512x1024, 1024x2048, 2048^2}, iso or VTI, L=3, FP32, "
+                        "1000-4000 steps", "rate_per_s": rate, "served": served,
+            "serial_plain_stepper": serial,
+            "shots_per_s_ratio": round(served["shots_per_s"] / serial["shots_per_s"], 3),
+            "p99_latency_ratio": round(served["latency_p99_s"] / serial["latency_p99_s"], 3)}
+
+
 def pmc_probe(nshots, steps, copy_bytes=1 << 30):
     """Short run for rocprofv3 --pmc passes, three phases in order:
     1. the single-shot 1024 x 512 L=3 workload, 20 steps, rows per wave from VI_ROWS1 (1);
@@ -252,6 +290,11 @@ def main():
     ap.add_argument("--gradient-shots", type=int, default=16)
     ap.add_argument("--only-gradient", action="store_true",
                     help="run only the adjoint-gradient leg and print its JSON")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serving-jobs", type=int, default=24)
+    ap.add_argument("--serving-rate", type=float, default=20.0)
+    ap.add_argument("--serving-rates", type=str, default="",
+                    help="comma-separated arrival rates: run only the serving leg at each")
     ap.add_argument("--rows", type=int, default=0,
                     help="fix the kernel variant (rows per wavefront) instead of autotuning")
     ap.add_argument("--no-serial", action="store_true",
@@ -276,6 +319,10 @@ def main():
         print(json.dumps({"multishot": bench_multishot(
             args.multishot_shots, args.multishot_steps, rows=args.rows,
             serial=not args.no_serial)}), flush=True)
+        return
+    if args.serving_rates:
+        for r in (float(x) for x in args.serving_rates.split(",")):
+            print(json.dumps({"serving": bench_serving(args.serving_jobs, r)}), flush=True)
         return
     if args.only_gradient:
         print(json.dumps({"gradient": bench_gradient(args.gradient_shots)}), flush=True)
@@ -336,6 +383,8 @@ def main():
                                              serial=not args.no_serial)
     if not args.no_gradient:
         extra["gradient"] = bench_gradient(args.gradient_shots)
+    if not args.no_serving:
+        extra["serving"] = bench_serving(args.serving_jobs, args.serving_rate)
     cpu = None
     if not args.no_cpu:
         cfg_cpu = config_layered_visco(nx=args.nx, nz=args.nz, nt=args.steps)

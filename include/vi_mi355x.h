@@ -38,6 +38,7 @@ extern "C" {
 typedef struct vi_sim_s* vi_handle;
 typedef struct vi_model_s* vi_model;
 typedef struct vi_batch_s* vi_batch;
+typedef struct vi_grad_s* vi_grad;
 
 enum { VI_F32 = 0, VI_F64 = 1 };
 
@@ -165,6 +166,34 @@ int vi_gradient(vi_handle h, int nt, const void* dobs /*[nrec][nt]*/, int checkp
 int vi_gradient_vti(vi_handle h, int nt, const void* dobs /*[nrec][nt]*/, int checkpoint_every,
                     double* misfit, void* g_rho, void* g_c11, void* g_c13, void* g_c33,
                     void* g_c55, void* g_tau);
+/* The same computation as a job that advances in chunks without blocking the caller, so one
+ * host thread can interleave several gradients with other device work (serving).
+ * begin copies dobs and allocates the job's buffers (vi_gradient_bytes); the handle then
+ * belongs to the job (other calls on it fail with VI_ESTATE) until end or cancel.
+ * advance enqueues at most max_steps forward, recomputed or transposed steps on the handle's
+ * stream and returns without waiting (with nothing enqueued while the forward traces are
+ * still being copied to the host for the residual); *finished = 1 once all work is
+ * enqueued. idle returns 1 when everything enqueued so far has completed, 0 otherwise.
+ * end finishes any remaining work, waits, writes the misfit and the gradients (isotropic:
+ * rho, lam, mu, tau; VTI: rho, C13, C55, tau, C11, C33; any may be NULL), resets the handle
+ * and frees the job; cancel frees it without results. */
+int vi_gradient_begin(vi_handle h, int nt, const void* dobs /*[nrec][nt]*/, int checkpoint_every,
+                      vi_grad* out);
+int vi_gradient_advance(vi_grad g, int max_steps, int* finished);
+int vi_gradient_idle(vi_grad g);
+int vi_gradient_end(vi_grad g, double* misfit, void* g_rho, void* g_lam_or_c13,
+                    void* g_mu_or_c55, void* g_tau, void* g_c11, void* g_c33);
+int vi_gradient_cancel(vi_grad g);
+
+/* ---- device-memory planning (no counterpart in the reference; used for admission by an
+ * HBM budget when many shots share one GPU) ---------------------------------------------- */
+/* temporary device bytes one vi_gradient(h, nt, ..., checkpoint_every, ...) call allocates */
+int vi_gradient_bytes(vi_handle h, int nt, int checkpoint_every, size_t* bytes);
+/* upper bound of the device bytes of a shot before it is created: grid, model (VTI or not),
+ * CPML strips, sources, receiver traces and, when grad_nt > 0, one gradient call; needs no
+ * device */
+int vi_plan_bytes(const vi_grid_desc* g, int vti, int cpml_thickness, int nsrc, int src_nt,
+                  int nrec, int rec_nt, int grad_nt, int checkpoint_every, size_t* bytes);
 
 /* ---- tiling (no counterpart in the reference: KA picks workgroup shapes itself) ------ */
 /* rows per wavefront of the kernel variant: 1, 2, 4 or 8 (results are identical) */

@@ -167,3 +167,30 @@ def test_workload_gradient_vp_vs_rho_q_at_random_points():
             worst = max(worst, err)
             assert err <= 1e-5, (key, int(j), int(i), ad, fd)
         print(f"  {key}: 20 points, worst relative difference {worst:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("vti", [False, True])
+def test_chunked_gradient_job_equals_gradient(vti):
+    """The gradient as a chunked job (begin / advance by a few steps / end) is bit-identical
+    to the one-call gradient, and the handle is locked while the job owns it."""
+    from viscoelasticinversion_amd.configs import build_simulation
+    from viscoelasticinversion_amd.stepper import GradientJob
+    from viscoelasticinversion_amd import _lib
+    cfg = small_cfg(nx=80, nz=60, nrelax=3, seed=51, nt=70, vti=vti)
+    sim = build_simulation(cfg, "float32")
+    sim.step(70)
+    dobs = sim.traces(70) * 0.9
+    sim.reset()
+    print(f"[case] chunked gradient vti={vti}", flush=True)
+    J, g = sim.gradient(dobs, 70, checkpoint_every=8)
+    job = GradientJob(sim, dobs, 70, checkpoint_every=8)
+    with pytest.raises(_lib.ViError):
+        sim.step(1)
+    while not job.finished:
+        job.advance(5)
+    J2, g2 = job.end()
+    assert J2 == J
+    for k in g:
+        assert np.array_equal(g[k], g2[k]), k
+    sim.step(2)  # the handle is free again
+    sim.close()

@@ -88,6 +88,14 @@ SIGNATURES = {
     "vi_step_index": (_L, [_H]),
     "vi_get_traces": (_I, [_H, _V, _I]),
     "vi_gradient": (_I, [_H, _I, _V, _I, ctypes.POINTER(ctypes.c_double), _V, _V, _V, _V]),
+    "vi_gradient_bytes": (_I, [_H, _I, _I, ctypes.POINTER(ctypes.c_size_t)]),
+    "vi_gradient_begin": (_I, [_H, _I, _V, _I, ctypes.POINTER(_H)]),
+    "vi_gradient_advance": (_I, [_H, _I, ctypes.POINTER(_I)]),
+    "vi_gradient_idle": (_I, [_H]),
+    "vi_gradient_end": (_I, [_H, ctypes.POINTER(ctypes.c_double), _V, _V, _V, _V, _V, _V]),
+    "vi_gradient_cancel": (_I, [_H]),
+    "vi_plan_bytes": (_I, [ctypes.POINTER(GridDesc), _I, _I, _I, _I, _I, _I, _I, _I,
+                           ctypes.POINTER(ctypes.c_size_t)]),
     "vi_gradient_vti": (_I, [_H, _I, _V, _I, ctypes.POINTER(ctypes.c_double), _V, _V, _V, _V, _V,
                              _V]),
     "vi_set_tiling": (_I, [_H, _I]),

@@ -263,7 +263,7 @@ class Parameters:
     bc: str = "dirichlet"
     source: Optional[object] = None
     device: str = "mi355x"
-    model: Optional[IsotropicModel] = None
+    model: Optional[object] = None  # IsotropicModel or VTIModel
     cpml: Optional[CPMLConfig] = None
     free_surface: bool = False
     precision: str = "float32"
@@ -292,7 +292,11 @@ class Parameters:
         sim = Simulation(self.Nx, self.Nz, self.dx, self.dz, self.dt, self.precision,
                          self.nrelax)
         m = self.model
-        sim.set_model(m.rho, m.lam, m.mu, self.qmodel.tau if self.qmodel is not None else None)
+        tau = self.qmodel.tau if self.qmodel is not None else None
+        if isinstance(m, VTIModel):
+            sim.set_model_vti(m.rho, m.C11, m.C13, m.C33, m.C55, tau)
+        else:
+            sim.set_model(m.rho, m.lam, m.mu, tau)
         if self.qmodel is not None:
             sim.set_relaxation(self.qmodel.taun)
         if self.cpml is not None:

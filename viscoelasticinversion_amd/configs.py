@@ -304,3 +304,60 @@ def multishot_gradient(acfg, dobs, precision="float32", model="start", checkpoin
         J += Js
         total = gv if total is None else {k: total[k] + gv[k] for k in gv}
     return J, total
+
+
+# grid sizes (nx, nz) of the serving workload: 256^2, 512 x 1024, 1024 x 2048, 2048^2
+SERVING_GRIDS = ((256, 256), (512, 1024), (1024, 2048), (2048, 2048))
+
+
+def serving_job_config(nx, nz, nt, seed, vti=False, eps=0.0, delta=0.0, rec_every=8,
+                       dtype=np.float32):
+    """One shot of the serving workload: seeded 8-layer model (as the single-shot workload),
+    isotropic or VTI with the given Thomsen eps, delta; L = 3, Q = 50, CPML 20 + free surface,
+    12 Hz Ricker force source 10 m deep at a seeded x, receivers every ``rec_every`` columns
+    10 m deep; dx 5 m, dt 0.4 ms. Model arrays are stored in ``dtype``."""
+    from .api import thomsen_to_stiffness
+    rng = np.random.default_rng(seed + 7919)
+    model, vmax = layered_model(nx, nz, seed=seed)
+    if vti:
+        vp = np.sqrt((model["lam"] + 2 * model["mu"]) / model["rho"])
+        vs = np.sqrt(model["mu"] / model["rho"])
+        c11, c13, c33, c55 = thomsen_to_stiffness(vp, vs, model["rho"], eps, delta)
+        model = dict(rho=model["rho"], c11=c11, c13=c13, c33=c33, c55=c55, tau=model["tau"])
+        vmax *= np.sqrt(1 + 2 * max(eps, 0.0))
+    model = {k: np.ascontiguousarray(v, dtype=dtype) for k, v in model.items()}
+    dx, dt, T = 5.0, 4e-4, 20
+    d0 = -(2.0 + 1) * vmax * np.log(1e-5) / (2 * T * dx)
+    depth = JS + int(round(10.0 / dx))
+    rec_ix = np.arange(T, nx - T, rec_every, dtype=np.int32)
+    return dict(name=f"serve_{nx}x{nz}", nx=nx, nz=nz, dx=dx, dz=dx, dt=dt, nt=int(nt), nrelax=3,
+                taun=np.array(TAUN_2_40HZ_L3), model=model, free_surface=True,
+                cpml=dict(thickness=T, d0=d0, alpha_max=0.0, kappa_max=1.0, power=2.0,
+                          left=True, right=True, top=False, bottom=True),
+                src_ix=np.array([int(rng.integers(T + 10, nx - T - 10))], np.int32),
+                src_iz=np.array([depth], np.int32), src_kind=np.array([0], np.int32),
+                wavelets=ricker_samples(12.0, 1.0 / 12.0, int(nt), dt)[None, :],
+                rec_ix=rec_ix, rec_iz=np.full(len(rec_ix), depth, np.int32),
+                rec_comp=np.full(len(rec_ix), 1, np.int32))
+
+
+def serving_workload(njobs, rate, seed=0, grids=SERVING_GRIDS, steps=(1000, 4000),
+                     gradient_fraction=0.3, vti_fraction=0.5):
+    """Seeded stream of shot jobs: Poisson arrivals at ``rate`` per second, 30 % gradient
+    and 70 % forward, grid drawn uniformly from ``grids``, isotropic or VTI
+    (eps ~ U[0, 0.25], delta ~ U[-0.1, 0.15]), L = 3, FP32, steps ~ U[steps]. Gradient jobs
+    carry zero observed data, so their misfit is 1/2 |d|^2 of the synthetic traces."""
+    from .serve import Job
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, njobs))
+    jobs = []
+    for k in range(njobs):
+        nx, nz = grids[int(rng.integers(len(grids)))]
+        nt = int(rng.integers(steps[0], steps[1] + 1))
+        vti = bool(rng.random() < vti_fraction)
+        eps, delta = float(rng.uniform(0.0, 0.25)), float(rng.uniform(-0.1, 0.15))
+        kind = "gradient" if rng.random() < gradient_fraction else "forward"
+        cfg = serving_job_config(nx, nz, nt, seed=seed * 100003 + k, vti=vti, eps=eps, delta=delta)
+        dobs = np.zeros((len(cfg["rec_ix"]), nt), np.float32) if kind == "gradient" else None
+        jobs.append(Job(id=k, kind=kind, cfg=cfg, arrival=float(t[k]), dobs=dobs))
+    return jobs

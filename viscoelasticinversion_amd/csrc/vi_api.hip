@@ -198,6 +198,7 @@ struct vi_sim_s {
   hipEvent_t own_ev = nullptr, ext_ev = nullptr;
   bool own_pending = false, ext_pending = false;
   Prof prof;
+  bool grad_busy = false;  // a gradient job owns the handle
 };
 
 struct BatchGroup {
@@ -221,10 +222,23 @@ struct vi_batch_s {
   Prof prof;
 };
 
+// A gradient computation in flight on one handle (see GradRun).
+struct vi_grad_s {
+  vi_sim_s* h = nullptr;
+  hipEvent_t ev = nullptr;  // recorded after the last enqueued chunk
+  bool done = false;
+  virtual ~vi_grad_s() {
+    if (ev) (void)hipEventDestroy(ev);
+  }
+  virtual int advance(int max_steps) = 0;
+  virtual int finish(double* misfit, void* const out[6]) = 0;
+};
+
 namespace {
 
 int check_handle(vi_handle h) {
   if (!h) return fail(VI_EINVAL, "null handle");
+  if (h->grad_busy) return fail(VI_ESTATE, "handle in use by a gradient job");
   return 0;
 }
 
@@ -639,28 +653,79 @@ int batch_finish(vi_batch_s* b, int nsteps) {
 
 
 // ---------------------------------------------------------------- gradient (adjoint)
-// out = {rho, lam | C13, mu | C55, tau, C11, C33} (host arrays [nz][nx], each may be NULL)
-template <typename R>
-int gradient_impl(vi_sim_s* h, int nt, const R* dobs, int K, double* misfit, R* const out[6]) {
-  const size_t E = sizeof(R);
-  const int L = h->L;
-  const size_t pb = h->plane * E;
-  hipStream_t st = h->stream;
-  VI_TRY(vi_reset(h));
-  VI_TRY(refresh_own_desc(h));
+// Device buffers of one gradient call: checkpoints (forward state every K steps), the
+// segment slots (per step: s, v^{n+1/2}, memory-variable sums, psi), the adjoint state with
+// work planes and accumulators, the residual and the injection/source lists. Regions inside
+// the checkpoint and slot buffers start on 256-byte boundaries.
+struct GradLayout {
+  int K = 1, nck = 1;
+  size_t state_bytes = 0, slot_bytes = 0, psi_total = 0, guard = 0, adj_bytes = 0;
+  size_t resid_bytes = 0, lists_bytes = 0;
+  size_t psi_off[8] = {};
+  size_t total() const {
+    return state_bytes * nck + slot_bytes * K + adj_bytes + resid_bytes + lists_bytes;
+  }
+};
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// live[] = byte sizes of the checkpointed arrays (5 state planes, M, 8 psi strips), in the
+// order the gradient copies them; lists_ints bounds the injection + source lists
+GradLayout grad_layout(int nt, int K, int L, size_t pb, size_t ldx, size_t esz,
+                       const size_t live[14], int nrec, size_t lists_ints) {
+  GradLayout g;
   if (K <= 0) K = std::max(1, (int)std::ceil(std::sqrt((double)nt)));
-  K = std::min(K, nt);
-  const int nck = (nt + K - 1) / K;
-  // forward state arrays of the handle, in a fixed order
-  std::vector<DevBuf*> live = mutable_arrays(h);
-  live.pop_back();  // traces are not part of the checkpointed state
-  // regions inside checkpoint and segment buffers start on 256-byte boundaries
-  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  size_t state_bytes = 0;
-  for (auto* b : live) state_bytes += al(b->bytes);
-  // device-to-device copies, 16 regions per k_copy_list launch
+  g.K = std::min(K, nt);
+  g.nck = (nt + g.K - 1) / g.K;
+  for (int q = 0; q < 14; ++q) g.state_bytes += align256(live[q]);
+  g.slot_bytes = (L ? 8 : 5) * pb;
+  for (int q = 0; q < 8; ++q) {
+    g.psi_total += live[6 + q];
+    g.psi_off[q] = g.slot_bytes;
+    g.slot_bytes += align256(live[6 + q]);
+  }
+  g.guard = 4 * ldx * esz;
+  const size_t nplanes_adj = 5 + 3 * (size_t)L + 4 + 4 + 7 + 2;
+  g.adj_bytes = nplanes_adj * pb + 8 * g.guard + g.psi_total;
+  g.resid_bytes = (size_t)nt * nrec * esz;
+  g.lists_bytes = lists_ints * sizeof(int);
+  return g;
+}
+
+// A gradient computation as a sequence of asynchronous chunks on the handle's stream:
+//   forward pass storing a checkpoint every K steps -> traces to pinned host memory ->
+//   residual and misfit on the host -> for each segment, last first: restore its
+//   checkpoint, recompute it storing every step's forward state in the slots, then the
+//   transposed steps backwards -> finalisation of the accumulators.
+// advance() enqueues at most max_steps forward, recomputed or transposed steps and never
+// waits for the device (it returns without work while the traces are still in flight), so
+// one host thread can interleave many gradients with other work on the GPU.
+template <typename R>
+struct GradRun : vi_grad_s {
+  int nt = 0, K = 1, nck = 1, L = 0, nrec = 0, nsrc = 0, ntgt = 0;
+  size_t E = sizeof(R), pb = 0;
+  GradLayout lay;
+  size_t psi_bytes[8] = {};
+  std::vector<DevBuf*> live;
+  DevBuf ck, seg, adj, resid, lists;
+  R* host_tr = nullptr;     // pinned: traces, then the residual [nt][nrec]
+  std::vector<R> dobs;      // [nrec][nt]
+  double J = 0.0;
+  vi::AdjDesc<R> d{};
+  R* g_rho_out = nullptr;
+  R* g_mu_out = nullptr;
+  const int *t_cell = nullptr, *t_comp = nullptr, *t_ptr = nullptr, *t_rec = nullptr;
+  hipEvent_t ev_tr = nullptr;
+  enum Phase { FWD, RESID, RESTORE, RECOMP, BACK, DONE } ph = FWD;
+  int s = 0, n = 0;  // segment, step (FWD: steps done in segment s)
+
+  ~GradRun() override {
+    if (host_tr) (void)hipHostFree(host_tr);
+    if (ev_tr) (void)hipEventDestroy(ev_tr);
+  }
+
   struct Cp { const void* s; void* d; size_t b; };
-  auto run_copies = [&](const std::vector<Cp>& v) -> int {
+  int run_copies(const std::vector<Cp>& v) {
     for (size_t i = 0; i < v.size(); i += 16) {
       vi::CopyList c{};
       size_t mx = 0;
@@ -672,197 +737,281 @@ int gradient_impl(vi_sim_s* h, int nt, const R* dobs, int K, double* misfit, R* 
         mx = std::max(mx, v[i + k].b);
       }
       const unsigned gx = (unsigned)std::min<size_t>(1024, std::max<size_t>(1, (mx / 16 + 255) / 256));
-      hipLaunchKernelGGL(vi::k_copy_list, dim3(gx, c.n), dim3(256), 0, st, c);
+      hipLaunchKernelGGL(vi::k_copy_list, dim3(gx, c.n), dim3(256), 0, h->stream, c);
     }
     VI_HIP(hipGetLastError());
     return 0;
-  };
-  DevBuf ck;
-  VI_TRY(ck.alloc(state_bytes * nck, st));
-  // per-step slot: s (3 planes), v^{n+1/2} (2), memory-variable sums (3 if L > 0), psi
-  size_t psi_bytes[8], psi_off[8];
-  size_t psi_total = 0;
-  size_t slot_bytes = (L ? 8 : 5) * pb;
-  for (int q = 0; q < 8; ++q) {
-    psi_total += (psi_bytes[q] = h->psi[q].bytes);
-    psi_off[q] = slot_bytes;
-    slot_bytes += al(psi_bytes[q]);
   }
-  DevBuf seg;
-  VI_TRY(seg.alloc(slot_bytes * K, st));
-  // adjoint state, work arrays and gradient accumulators
-  // ehat/dhat planes get 4 zero guard rows in front: the transposed z-operators of the
-  // free-surface image rows (rows 0..3) read rows -4..-1, where the derivative adjoints are 0
-  DevBuf adj;
-  const size_t guard = 4 * (size_t)h->ldx * E;
-  const size_t nplanes_adj = 5 + 3 * (size_t)L + 4 + 4 + 7 + 2;
-  VI_TRY(adj.alloc(nplanes_adj * pb + 8 * guard + psi_total, st));
-  char* ab = (char*)adj.p;
-  auto take = [&](size_t bytes) { char* p = ab; ab += bytes; return (R*)p; };
-  auto take_guarded = [&]() { ab += guard; return take(pb); };
-  vi::AdjDesc<R> d{};
-  d.avx = take(pb); d.avz = take(pb); d.asxx = take(pb); d.aszz = take(pb); d.asxz = take(pb);
-  d.aM = take(3 * (size_t)L * pb);
-  for (int q = 0; q < 4; ++q) d.ehat[q] = take_guarded();
-  for (int q = 0; q < 4; ++q) d.dhat[q] = take_guarded();
-  d.g_lam = take(pb); d.g_mu = take(pb); d.g_muxz = take(pb); d.g_tau = take(pb);
-  d.g_bx = take(pb); d.g_bz = take(pb); d.g_c33 = take(pb);
-  R* g_rho_out = take(pb);
-  R* g_mu_out = take(pb);
-  for (int q = 0; q < 8; ++q) d.apsi[q] = take(psi_bytes[q]);
-  // residual [nt][nrec], flat receiver and source lists
-  DevBuf resid, lists;
-  const int nrec = h->nrec, nsrc = h->nsrc;
-  VI_TRY(resid.alloc((size_t)nt * nrec * E, st));
-  // injection targets: distinct (component, cell) pairs with their receivers in ascending
-  // order; layout [tcell | tcomp | tptr (ntgt+1) | trec (nrec) | src_cell | src_kind]
-  std::map<std::pair<int, int>, std::vector<int>> targets;
-  for (int r = 0; r < nrec; ++r)
-    targets[{h->rec_comp[r], h->rec_iz[r] * h->ldx + h->rec_ix[r]}].push_back(r);
-  const int ntgt = (int)targets.size();
-  std::vector<int> lst(3 * (size_t)ntgt + 1 + (size_t)nrec + 2 * (size_t)nsrc);
-  {
-    int t = 0, k = 0;
-    int* tptr = lst.data() + 2 * ntgt;
-    int* trec = tptr + ntgt + 1;
-    for (auto& kv : targets) {
-      lst[t] = kv.first.second;
-      lst[ntgt + t] = kv.first.first;
-      tptr[t] = k;
-      for (int r : kv.second) trec[k++] = r;
-      ++t;
-    }
-    tptr[ntgt] = k;
-  }
-  const size_t src0 = 3 * (size_t)ntgt + 1 + nrec;
-  for (int q = 0; q < nsrc; ++q) {
-    lst[src0 + q] = h->src_iz[q] * h->ldx + h->src_ix[q];
-    lst[src0 + nsrc + q] = h->src_kind[q];
-  }
-  VI_TRY(lists.alloc(lst.size() * sizeof(int), st));
-  VI_HIP(hipMemcpyAsync(lists.p, lst.data(), lst.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  const int* t_cell = (const int*)lists.p;
-  const int* t_comp = t_cell + ntgt;
-  const int* t_ptr = t_comp + ntgt;
-  const int* t_rec = t_ptr + ntgt + 1;
-  // static part of the adjoint descriptor
-  d.nx = h->g.nx; d.nz = h->g.nz; d.ldx = h->ldx; d.ldm = h->model->ldm;
-  {
-    const vi::ShotDesc<R>* sd = (const vi::ShotDesc<R>*)h->desc_host.data();
-    d.dt = sd->dt;
-    for (int l = 0; l < 4; ++l) { d.Dx[l] = sd->Dx[l]; d.Dz[l] = sd->Dz[l]; }
-    for (int k = 0; k < L; ++k) { d.a1[k] = sd->a1[k]; d.a2[k] = sd->a2[k]; }
-    d.T = sd->T; d.xlo = sd->xlo; d.xhi = sd->xhi; d.zlo = sd->zlo; d.zhi = sd->zhi; d.fs = sd->fs;
-    d.rho = sd->rho; d.lam = sd->lam; d.mu = sd->mu; d.tau = sd->tau;
-    d.c11 = sd->c11; d.c33 = sd->c33; d.vti = sd->vti;
-    d.ax = sd->ax; d.bx = sd->bx; d.kx = sd->kx; d.az = sd->az; d.bz = sd->bz; d.kz = sd->kz;
-    d.src_w = sd->src_w; d.snt = sd->snt;
-  }
-  d.L = L;
-  d.plane = h->plane;
-  d.nsrc = nsrc;
-  d.src_cell = (const int*)lists.p + src0;
-  d.src_kind = d.src_cell + nsrc;
-  auto copy_state = [&](char* dst, bool to_ck) -> int {
+  int copy_state(int seg_idx, bool to_ck) {
+    char* dst = (char*)ck.p + (size_t)seg_idx * lay.state_bytes;
     std::vector<Cp> v;
     size_t off = 0;
     for (auto* b : live) {
       if (!b->p) continue;
       if (to_ck) v.push_back({b->p, dst + off, b->bytes});
       else v.push_back({dst + off, b->p, b->bytes});
-      off += al(b->bytes);
+      off += align256(b->bytes);
     }
     return run_copies(v);
-  };
-  // ---- forward pass with checkpoints
-  for (int s = 0; s < nck; ++s) {
-    VI_TRY(copy_state((char*)ck.p + (size_t)s * state_bytes, true));
-    VI_TRY(launch_steps(st, E, h->rz, L, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
-                        h->ntiles, 1, std::min(K, nt - s * K), nullptr));
   }
-  h->step = nt;
-  // residual and misfit (host, double accumulation)
-  std::vector<R> tr((size_t)nt * nrec);
-  VI_HIP(hipMemcpyAsync(tr.data(), h->traces.p, tr.size() * E, hipMemcpyDeviceToHost, st));
-  VI_HIP(hipStreamSynchronize(st));
-  double J = 0.0;
-  for (int r = 0; r < nrec; ++r)
-    for (int t = 0; t < nt; ++t) {
-      const R res = tr[(size_t)t * nrec + r] - dobs[(size_t)r * nt + t];
-      tr[(size_t)t * nrec + r] = res;
-      J += 0.5 * (double)res * (double)res;
-    }
-  if (misfit) *misfit = J;
-  VI_HIP(hipMemcpyAsync(resid.p, tr.data(), tr.size() * E, hipMemcpyHostToDevice, st));
-  // ---- backward pass, one checkpoint segment at a time
-  const dim3 blk(64, 4);
-  const dim3 gint((d.nx - 8 + 63) / 64, (d.nz - 8 + 3) / 4);
-  const int row0 = d.fs ? 0 : 4;
-  const dim3 gvg((d.nx - 8 + 63) / 64, (d.nz - 4 - row0 + 3) / 4);
-  R* M0 = (R*)h->M.p;
-  for (int s = nck - 1; s >= 0; --s) {
-    const int n0 = s * K, n1 = std::min(n0 + K, nt);
-    VI_TRY(copy_state((char*)ck.p + (size_t)s * state_bytes, false));
-    h->step = n0;
-    touch(h);
+
+  int init(vi_sim_s* hh, int nt_, const R* dobs_, int K_) {
+    h = hh;
+    nt = nt_;
+    L = h->L;
+    pb = h->plane * E;
+    nrec = h->nrec;
+    nsrc = h->nsrc;
+    hipStream_t st = h->stream;
+    VI_TRY(vi_reset(h));
     VI_TRY(refresh_own_desc(h));
-    for (int n = n0; n < n1; ++n) {  // recompute the segment, storing each step's inputs
-      char* slot = (char*)seg.p + (size_t)(n - n0) * slot_bytes;
-      std::vector<Cp> pre;
-      for (int q = 0; q < 3; ++q) pre.push_back({h->state[2 + q].p, slot + q * pb, pb});
-      for (int q = 0; q < 8; ++q)
-        if (psi_bytes[q]) pre.push_back({h->psi[q].p, slot + psi_off[q], psi_bytes[q]});
-      VI_TRY(run_copies(pre));
-      if (L)
-        hipLaunchKernelGGL(vi::k_msum<R>, dim3((unsigned)((h->plane + 255) / 256)), dim3(256), 0,
-                           st, (const R*)M0, (R*)(slot + 5 * pb), h->plane, L);
-      VI_TRY(launch_one<R>(st, h->rz, L, 0, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
-                           h->ntiles, n - n0, nullptr, nullptr));
-      VI_TRY(run_copies({{h->state[0].p, slot + 3 * pb, pb}, {h->state[1].p, slot + 4 * pb, pb}}));
-      VI_TRY(launch_one<R>(st, h->rz, L, 1, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
-                           h->ntiles, n - n0, nullptr, nullptr));
-    }
-    for (int n = n1 - 1; n >= n0; --n) {
-      const char* slot = (const char*)seg.p + (size_t)(n - n0) * slot_bytes;
-      d.fsxx = (const R*)(slot);
-      d.fszz = (const R*)(slot + pb);
-      d.fsxz = (const R*)(slot + 2 * pb);
-      d.fvx = (const R*)(slot + 3 * pb);
-      d.fvz = (const R*)(slot + 4 * pb);
-      d.fM = (const R*)(slot + 5 * pb);
-      for (int q = 0; q < 8; ++q) d.fpsi[q] = (const R*)(slot + psi_off[q]);
-      d.step = n;
-      if (ntgt) {
-        hipLaunchKernelGGL(vi::k_adj_inject<R>, dim3((ntgt + 255) / 256), dim3(256), 0, st,
-                           d.avx, d.avz, d.asxx, d.aszz, d.asxz, t_cell, t_comp, t_ptr, t_rec,
-                           ntgt, (const R*)resid.p + (size_t)n * nrec);
+    live = mutable_arrays(h);
+    live.pop_back();  // traces are not part of the checkpointed state
+    size_t live_bytes[14];
+    for (int q = 0; q < 14; ++q) live_bytes[q] = live[q]->bytes;
+    for (int q = 0; q < 8; ++q) psi_bytes[q] = live_bytes[6 + q];
+    // injection targets: distinct (component, cell) pairs with their receivers in ascending
+    // order; layout [tcell | tcomp | tptr (ntgt+1) | trec (nrec) | src_cell | src_kind]
+    std::map<std::pair<int, int>, std::vector<int>> targets;
+    for (int r = 0; r < nrec; ++r)
+      targets[{h->rec_comp[r], h->rec_iz[r] * h->ldx + h->rec_ix[r]}].push_back(r);
+    ntgt = (int)targets.size();
+    std::vector<int> lst(3 * (size_t)ntgt + 1 + (size_t)nrec + 2 * (size_t)nsrc);
+    lay = grad_layout(nt, K_, L, pb, h->ldx, E, live_bytes, nrec, lst.size());
+    K = lay.K;
+    nck = lay.nck;
+    VI_TRY(ck.alloc(lay.state_bytes * nck, st));
+    VI_TRY(seg.alloc(lay.slot_bytes * K, st));
+    // adjoint state, work arrays and accumulators; ehat/dhat planes get 4 zero guard rows in
+    // front: the transposed z-operators of the free-surface image rows (rows 0..3) read rows
+    // -4..-1, where the derivative adjoints are 0
+    VI_TRY(adj.alloc(lay.adj_bytes, st));
+    char* ab = (char*)adj.p;
+    auto take = [&](size_t bytes) { char* p = ab; ab += bytes; return (R*)p; };
+    auto take_guarded = [&]() { ab += lay.guard; return take(pb); };
+    d.avx = take(pb); d.avz = take(pb); d.asxx = take(pb); d.aszz = take(pb); d.asxz = take(pb);
+    d.aM = take(3 * (size_t)L * pb);
+    for (int q = 0; q < 4; ++q) d.ehat[q] = take_guarded();
+    for (int q = 0; q < 4; ++q) d.dhat[q] = take_guarded();
+    d.g_lam = take(pb); d.g_mu = take(pb); d.g_muxz = take(pb); d.g_tau = take(pb);
+    d.g_bx = take(pb); d.g_bz = take(pb); d.g_c33 = take(pb);
+    g_rho_out = take(pb);
+    g_mu_out = take(pb);
+    for (int q = 0; q < 8; ++q) d.apsi[q] = take(psi_bytes[q]);
+    VI_TRY(resid.alloc(lay.resid_bytes, st));
+    {
+      int t = 0, k = 0;
+      int* tptr = lst.data() + 2 * ntgt;
+      int* trec = tptr + ntgt + 1;
+      for (auto& kv : targets) {
+        lst[t] = kv.first.second;
+        lst[ntgt + t] = kv.first.first;
+        tptr[t] = k;
+        for (int r : kv.second) trec[k++] = r;
+        ++t;
       }
-      switch (L) {
-        case 0: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 0>), gint, blk, 0, st, d); break;
-        case 1: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 1>), gint, blk, 0, st, d); break;
-        case 2: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 2>), gint, blk, 0, st, d); break;
-        case 3: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 3>), gint, blk, 0, st, d); break;
-        case 4: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 4>), gint, blk, 0, st, d); break;
-        case 5: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 5>), gint, blk, 0, st, d); break;
-        case 6: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 6>), gint, blk, 0, st, d); break;
-      }
-      hipLaunchKernelGGL(vi::k_adj_stress_gather<R>, gint, blk, 0, st, d);
-      hipLaunchKernelGGL(vi::k_adj_vel_point<R>, gint, blk, 0, st, d);
-      hipLaunchKernelGGL(vi::k_adj_vel_gather<R>, gvg, blk, 0, st, d, row0);
-      VI_HIP(hipGetLastError());
+      tptr[ntgt] = k;
     }
+    const size_t src0 = 3 * (size_t)ntgt + 1 + nrec;
+    for (int q = 0; q < nsrc; ++q) {
+      lst[src0 + q] = h->src_iz[q] * h->ldx + h->src_ix[q];
+      lst[src0 + nsrc + q] = h->src_kind[q];
+    }
+    VI_TRY(lists.alloc(lst.size() * sizeof(int), st));
+    VI_HIP(hipMemcpyAsync(lists.p, lst.data(), lst.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    t_cell = (const int*)lists.p;
+    t_comp = t_cell + ntgt;
+    t_ptr = t_comp + ntgt;
+    t_rec = t_ptr + ntgt + 1;
+    // static part of the adjoint descriptor
+    d.nx = h->g.nx; d.nz = h->g.nz; d.ldx = h->ldx; d.ldm = h->model->ldm;
+    {
+      const vi::ShotDesc<R>* sd = (const vi::ShotDesc<R>*)h->desc_host.data();
+      d.dt = sd->dt;
+      for (int l = 0; l < 4; ++l) { d.Dx[l] = sd->Dx[l]; d.Dz[l] = sd->Dz[l]; }
+      for (int k = 0; k < L; ++k) { d.a1[k] = sd->a1[k]; d.a2[k] = sd->a2[k]; }
+      d.T = sd->T; d.xlo = sd->xlo; d.xhi = sd->xhi; d.zlo = sd->zlo; d.zhi = sd->zhi; d.fs = sd->fs;
+      d.rho = sd->rho; d.lam = sd->lam; d.mu = sd->mu; d.tau = sd->tau;
+      d.c11 = sd->c11; d.c33 = sd->c33; d.vti = sd->vti;
+      d.ax = sd->ax; d.bx = sd->bx; d.kx = sd->kx; d.az = sd->az; d.bz = sd->bz; d.kz = sd->kz;
+      d.src_w = sd->src_w; d.snt = sd->snt;
+    }
+    d.L = L;
+    d.plane = h->plane;
+    d.nsrc = nsrc;
+    d.src_cell = (const int*)lists.p + src0;
+    d.src_kind = d.src_cell + nsrc;
+    dobs.assign(dobs_, dobs_ + (size_t)nrec * nt);
+    VI_HIP(hipHostMalloc((void**)&host_tr, (size_t)nt * nrec * E, 0));
+    VI_HIP(hipEventCreateWithFlags(&ev_tr, hipEventDisableTiming));
+    VI_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    VI_HIP(hipStreamSynchronize(st));
+    h->own_pending = true;
+    h->grad_busy = true;
+    return 0;
   }
-  const dim3 gall((d.nx + 63) / 64, (d.nz + 3) / 4);
-  hipLaunchKernelGGL(vi::k_grad_finalize<R>, gall, blk, 0, st, d, g_rho_out, g_mu_out);
-  VI_HIP(hipGetLastError());
-  const int nx = h->g.nx, nz = h->g.nz;
-  const R* planes[6] = {g_rho_out, d.g_lam, g_mu_out, L ? d.g_tau : nullptr,
-                        d.vti ? d.g_mu : nullptr, d.vti ? d.g_c33 : nullptr};
-  for (int q = 0; q < 6; ++q)
-    if (out[q] && planes[q]) VI_TRY(download_plane(st, out[q], planes[q], h->ldx, nx, nz, E));
-  VI_HIP(hipStreamSynchronize(st));
-  // leave the handle at a zero state, step 0
-  return vi_reset(h);
+
+  int backward_step(int step, const char* slot) {
+    hipStream_t st = h->stream;
+    const dim3 blk(64, 4);
+    const dim3 gint((d.nx - 8 + 63) / 64, (d.nz - 8 + 3) / 4);
+    const int row0 = d.fs ? 0 : 4;
+    const dim3 gvg((d.nx - 8 + 63) / 64, (d.nz - 4 - row0 + 3) / 4);
+    d.fsxx = (const R*)(slot);
+    d.fszz = (const R*)(slot + pb);
+    d.fsxz = (const R*)(slot + 2 * pb);
+    d.fvx = (const R*)(slot + 3 * pb);
+    d.fvz = (const R*)(slot + 4 * pb);
+    d.fM = (const R*)(slot + 5 * pb);
+    for (int q = 0; q < 8; ++q) d.fpsi[q] = (const R*)(slot + lay.psi_off[q]);
+    d.step = step;
+    if (ntgt)
+      hipLaunchKernelGGL(vi::k_adj_inject<R>, dim3((ntgt + 255) / 256), dim3(256), 0, st, d.avx,
+                         d.avz, d.asxx, d.aszz, d.asxz, t_cell, t_comp, t_ptr, t_rec, ntgt,
+                         (const R*)resid.p + (size_t)step * nrec);
+    switch (L) {
+      case 0: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 0>), gint, blk, 0, st, d); break;
+      case 1: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 1>), gint, blk, 0, st, d); break;
+      case 2: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 2>), gint, blk, 0, st, d); break;
+      case 3: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 3>), gint, blk, 0, st, d); break;
+      case 4: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 4>), gint, blk, 0, st, d); break;
+      case 5: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 5>), gint, blk, 0, st, d); break;
+      case 6: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 6>), gint, blk, 0, st, d); break;
+    }
+    hipLaunchKernelGGL(vi::k_adj_stress_gather<R>, gint, blk, 0, st, d);
+    hipLaunchKernelGGL(vi::k_adj_vel_point<R>, gint, blk, 0, st, d);
+    hipLaunchKernelGGL(vi::k_adj_vel_gather<R>, gvg, blk, 0, st, d, row0);
+    VI_HIP(hipGetLastError());
+    return 0;
+  }
+
+  int advance(int max_steps) override {
+    hipStream_t st = h->stream;
+    int budget = max_steps;
+    while (budget > 0 && ph != DONE) {
+      if (ph == FWD) {
+        const int len = std::min(K, nt - s * K);
+        if (n == 0) VI_TRY(copy_state(s, true));
+        const int m = std::min(len - n, budget);
+        VI_TRY(launch_steps(st, E, h->rz, L, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
+                            h->ntiles, 1, m, nullptr));
+        n += m;
+        budget -= m;
+        if (n == len) {
+          ++s;
+          n = 0;
+        }
+        if (s == nck) {
+          VI_HIP(hipMemcpyAsync(host_tr, h->traces.p, (size_t)nt * nrec * E,
+                                hipMemcpyDeviceToHost, st));
+          VI_HIP(hipEventRecord(ev_tr, st));
+          ph = RESID;
+        }
+      } else if (ph == RESID) {
+        const hipError_t q = hipEventQuery(ev_tr);
+        if (q == hipErrorNotReady) break;  // traces still on their way: nothing to enqueue
+        VI_HIP(q);
+        J = 0.0;  // residual and misfit, double accumulation
+        for (int r = 0; r < nrec; ++r)
+          for (int t = 0; t < nt; ++t) {
+            const R res = host_tr[(size_t)t * nrec + r] - dobs[(size_t)r * nt + t];
+            host_tr[(size_t)t * nrec + r] = res;
+            J += 0.5 * (double)res * (double)res;
+          }
+        VI_HIP(hipMemcpyAsync(resid.p, host_tr, (size_t)nt * nrec * E, hipMemcpyHostToDevice, st));
+        s = nck - 1;
+        ph = RESTORE;
+      } else if (ph == RESTORE) {
+        VI_TRY(copy_state(s, false));
+        hipLaunchKernelGGL(vi::k_set_step<R>, dim3(1), dim3(64), 0, st,
+                           (vi::ShotDesc<R>*)h->desc_dev.p, 0, (long)s * K);
+        VI_HIP(hipGetLastError());
+        n = s * K;
+        ph = RECOMP;
+      } else if (ph == RECOMP) {  // recompute the segment, storing each step's inputs
+        const int n0 = s * K, n1 = std::min(n0 + K, nt);
+        R* M0 = (R*)h->M.p;
+        for (; n < n1 && budget > 0; ++n, --budget) {
+          char* slot = (char*)seg.p + (size_t)(n - n0) * lay.slot_bytes;
+          std::vector<Cp> pre;
+          for (int q = 0; q < 3; ++q) pre.push_back({h->state[2 + q].p, slot + q * pb, pb});
+          for (int q = 0; q < 8; ++q)
+            if (psi_bytes[q]) pre.push_back({h->psi[q].p, slot + lay.psi_off[q], psi_bytes[q]});
+          VI_TRY(run_copies(pre));
+          if (L)
+            hipLaunchKernelGGL(vi::k_msum<R>, dim3((unsigned)((h->plane + 255) / 256)),
+                               dim3(256), 0, st, (const R*)M0, (R*)(slot + 5 * pb), h->plane, L);
+          VI_TRY(launch_one<R>(st, h->rz, L, 0, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
+                               h->ntiles, n - n0, nullptr, nullptr));
+          VI_TRY(run_copies({{h->state[0].p, slot + 3 * pb, pb}, {h->state[1].p, slot + 4 * pb, pb}}));
+          VI_TRY(launch_one<R>(st, h->rz, L, 1, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
+                               h->ntiles, n - n0, nullptr, nullptr));
+        }
+        if (n == n1) {
+          n = n1 - 1;
+          ph = BACK;
+        }
+      } else if (ph == BACK) {
+        const int n0 = s * K;
+        for (; n >= n0 && budget > 0; --n, --budget)
+          VI_TRY(backward_step(n, (const char*)seg.p + (size_t)(n - n0) * lay.slot_bytes));
+        if (n < n0) {
+          if (s == 0) {
+            const dim3 gall((d.nx + 63) / 64, (d.nz + 3) / 4);
+            hipLaunchKernelGGL(vi::k_grad_finalize<R>, gall, dim3(64, 4), 0, st, d, g_rho_out,
+                               g_mu_out);
+            VI_HIP(hipGetLastError());
+            ph = DONE;
+          } else {
+            --s;
+            ph = RESTORE;
+          }
+        }
+      }
+    }
+    done = ph == DONE;
+    VI_HIP(hipEventRecord(ev, st));
+    return 0;
+  }
+
+  int finish(double* misfit, void* const out[6]) override {
+    while (ph != DONE) {  // drive the rest synchronously
+      VI_TRY(advance(1 << 30));
+      if (ph == RESID) VI_HIP(hipEventSynchronize(ev_tr));
+    }
+    hipStream_t st = h->stream;
+    const int nx = h->g.nx, nz = h->g.nz;
+    const R* planes[6] = {g_rho_out, d.g_lam, g_mu_out, L ? d.g_tau : nullptr,
+                          d.vti ? d.g_mu : nullptr, d.vti ? d.g_c33 : nullptr};
+    for (int q = 0; q < 6; ++q)
+      if (out && out[q] && planes[q])
+        VI_TRY(download_plane(st, out[q], planes[q], h->ldx, nx, nz, E));
+    VI_HIP(hipStreamSynchronize(st));
+    if (misfit) *misfit = J;
+    // leave the handle at a zero state, step 0
+    h->grad_busy = false;
+    return vi_reset(h);
+  }
+};
+
+int gradient_begin(vi_sim_s* h, int nt, const void* dobs, int K, vi_grad* out) {
+  std::unique_ptr<vi_grad_s> g;
+  int rc;
+  if (h->esz == 4) {
+    auto* r = new GradRun<float>();
+    g.reset(r);
+    rc = r->init(h, nt, (const float*)dobs, K);
+  } else {
+    auto* r = new GradRun<double>();
+    g.reset(r);
+    rc = r->init(h, nt, (const double*)dobs, K);
+  }
+  if (rc) {
+    (void)hipStreamSynchronize(h->stream);
+    return rc;
+  }
+  *out = g.release();
+  return 0;
 }
 
 }  // namespace
@@ -992,6 +1141,7 @@ int vi_create(const vi_grid_desc* g, vi_handle* out) {
 
 int vi_destroy(vi_handle h) {
   if (!h) return 0;
+  if (h->grad_busy) return fail(VI_ESTATE, "vi_destroy: end or cancel the gradient job first");
   (void)wait_external(h);
   (void)hipStreamSynchronize(h->stream);
   hipStream_t s = h->stream;
@@ -1285,8 +1435,55 @@ int vi_device_bytes(vi_handle h, size_t* bytes) {
   size_t t = 0;
   for (auto* b : mutable_arrays(h)) t += b->bytes;
   t += h->coef.bytes + h->src_tab.bytes + h->src_w.bytes + h->rec_tab.bytes;
+  t += h->desc_dev.bytes + h->tile_shot_dev.bytes;
   if (h->model && h->model.use_count() == 1)
     for (auto& b : h->model->arr) t += b.bytes;
+  *bytes = t;
+  return 0;
+}
+
+int vi_gradient_bytes(vi_handle h, int nt, int checkpoint_every, size_t* bytes) {
+  VI_TRY(check_handle(h));
+  if (!bytes || nt < 1 || checkpoint_every < 0)
+    return fail(VI_EINVAL, "vi_gradient_bytes: need nt >= 1, checkpoint_every >= 0, bytes");
+  std::vector<DevBuf*> live = mutable_arrays(h);
+  size_t lb[14];
+  for (int q = 0; q < 14; ++q) lb[q] = live[q]->bytes;
+  const size_t ints = 4 * (size_t)h->nrec + 1 + 2 * (size_t)h->nsrc;  // bound: targets <= nrec
+  *bytes = grad_layout(nt, checkpoint_every, h->L, h->plane * h->esz, h->ldx, h->esz, lb,
+                       h->nrec, ints).total();
+  return 0;
+}
+
+int vi_plan_bytes(const vi_grid_desc* g, int vti, int cpml_thickness, int nsrc, int src_nt,
+                  int nrec, int rec_nt, int grad_nt, int checkpoint_every, size_t* bytes) {
+  if (!g || !bytes) return fail(VI_EINVAL, "vi_plan_bytes: null argument");
+  if (g->nx < 16 || g->nz < 16 || (g->precision != VI_F32 && g->precision != VI_F64) ||
+      g->nrelax < 0 || g->nrelax > vi::MAXL)
+    return fail(VI_EINVAL, "vi_plan_bytes: bad grid description");
+  const int T = cpml_thickness;
+  if (T < 0 || 2 * T > g->nx || 2 * T > g->nz || nsrc < 0 || src_nt < 0 || nrec < 0 ||
+      rec_nt < 0 || grad_nt < 0 || checkpoint_every < 0)
+    return fail(VI_EINVAL, "vi_plan_bytes: negative count or CPML wider than the grid");
+  const size_t esz = g->precision == VI_F32 ? 4 : 8;
+  const size_t nx = g->nx, nz = g->nz, L = g->nrelax;
+  const size_t ldx = round_up(nx + 72, 64), nzp = nz + vi::NW * vi::RZ_MAX + 16;
+  const size_t pb = ldx * nzp * esz;
+  size_t live[14] = {pb, pb, pb, pb, pb, 3 * L * pb};
+  for (int q = 0; q < 8; ++q)
+    live[6 + q] = T ? (q < 4 ? nz * 2 * T * esz : 2 * (size_t)T * ldx * esz) : 0;
+  size_t t = 0;
+  for (size_t b : live) t += b;
+  t += (3 + (L ? 1 : 0) + (vti ? 2 : 0)) * pb;  // model planes (same padding as the state)
+  if (T) t += (3 * nx + 3 * nz) * esz;           // CPML a, b, kappa
+  // tile tables at the finest tiling, descriptors, source wavelets and traces
+  const size_t ntiles = ((nx - 8 + vi::TX - 1) / vi::TX) * ((nz - 8 + vi::NW - 1) / vi::NW);
+  t += (2 * (ntiles + 1) + 3 * ((size_t)nsrc + nrec)) * sizeof(int) + ntiles * sizeof(int);
+  t += std::max(desc_size(4), desc_size(8));
+  t += (size_t)nsrc * src_nt * esz + (size_t)nrec * rec_nt * esz;
+  if (grad_nt > 0)
+    t += grad_layout(grad_nt, checkpoint_every, (int)L, pb, ldx, esz, live, nrec,
+                     4 * (size_t)nrec + 1 + 2 * (size_t)nsrc).total();
   *bytes = t;
   return 0;
 }
@@ -1404,43 +1601,86 @@ int vi_algorithmic_bytes(vi_handle h, int kernel, double* bytes) {
 }
 
 
-int vi_gradient(vi_handle h, int nt, const void* dobs, int checkpoint_every, double* misfit,
-                void* g_rho, void* g_lam, void* g_mu, void* g_tau) {
+static int gradient_checks(vi_handle h, int nt, const void* dobs, int checkpoint_every,
+                           const char* who) {
   VI_TRY(check_handle(h));
   if (nt < 1 || !dobs || checkpoint_every < 0)
-    return fail(VI_EINVAL, "vi_gradient: need nt >= 1, dobs, checkpoint_every >= 0");
+    return fail(VI_EINVAL, std::string(who) + ": need nt >= 1, dobs, checkpoint_every >= 0");
   VI_TRY(check_ready(h));
   if (h->nrec < 1 || h->rec_nt < nt)
-    return fail(VI_ESTATE, "vi_gradient: receivers with capacity >= nt are required");
-  if (h->model->vti) return fail(VI_ESTATE, "vi_gradient: VTI model, use vi_gradient_vti");
+    return fail(VI_ESTATE, std::string(who) + ": receivers with capacity >= nt are required");
   VI_TRY(wait_external(h));
   VI_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+static int gradient_run(vi_handle h, int nt, const void* dobs, int checkpoint_every,
+                        double* misfit, void* const out[6]) {
+  vi_grad g = nullptr;
+  VI_TRY(gradient_begin(h, nt, dobs, checkpoint_every, &g));
+  std::unique_ptr<vi_grad_s> own(g);
+  const int rc = g->finish(misfit, out);
+  if (rc) h->grad_busy = false;
+  return rc;
+}
+
+int vi_gradient(vi_handle h, int nt, const void* dobs, int checkpoint_every, double* misfit,
+                void* g_rho, void* g_lam, void* g_mu, void* g_tau) {
+  VI_TRY(gradient_checks(h, nt, dobs, checkpoint_every, "vi_gradient"));
+  if (h->model->vti) return fail(VI_ESTATE, "vi_gradient: VTI model, use vi_gradient_vti");
   void* out[6] = {g_rho, g_lam, g_mu, g_tau, nullptr, nullptr};
-  if (h->esz == 4)
-    return gradient_impl<float>(h, nt, (const float*)dobs, checkpoint_every, misfit,
-                                (float* const*)out);
-  return gradient_impl<double>(h, nt, (const double*)dobs, checkpoint_every, misfit,
-                               (double* const*)out);
+  return gradient_run(h, nt, dobs, checkpoint_every, misfit, out);
 }
 
 int vi_gradient_vti(vi_handle h, int nt, const void* dobs, int checkpoint_every,
                     double* misfit, void* g_rho, void* g_c11, void* g_c13, void* g_c33,
                     void* g_c55, void* g_tau) {
-  VI_TRY(check_handle(h));
-  if (nt < 1 || !dobs || checkpoint_every < 0)
-    return fail(VI_EINVAL, "vi_gradient_vti: need nt >= 1, dobs, checkpoint_every >= 0");
-  VI_TRY(check_ready(h));
-  if (h->nrec < 1 || h->rec_nt < nt)
-    return fail(VI_ESTATE, "vi_gradient_vti: receivers with capacity >= nt are required");
+  VI_TRY(gradient_checks(h, nt, dobs, checkpoint_every, "vi_gradient_vti"));
   if (!h->model->vti) return fail(VI_ESTATE, "vi_gradient_vti: isotropic model, use vi_gradient");
-  VI_TRY(wait_external(h));
-  VI_HIP(hipStreamSynchronize(h->stream));
   void* out[6] = {g_rho, g_c13, g_c55, g_tau, g_c11, g_c33};
-  if (h->esz == 4)
-    return gradient_impl<float>(h, nt, (const float*)dobs, checkpoint_every, misfit,
-                                (float* const*)out);
-  return gradient_impl<double>(h, nt, (const double*)dobs, checkpoint_every, misfit,
-                               (double* const*)out);
+  return gradient_run(h, nt, dobs, checkpoint_every, misfit, out);
+}
+
+int vi_gradient_begin(vi_handle h, int nt, const void* dobs, int checkpoint_every,
+                      vi_grad* out) {
+  if (!out) return fail(VI_EINVAL, "vi_gradient_begin: null output");
+  *out = nullptr;
+  VI_TRY(gradient_checks(h, nt, dobs, checkpoint_every, "vi_gradient_begin"));
+  return gradient_begin(h, nt, dobs, checkpoint_every, out);
+}
+
+int vi_gradient_advance(vi_grad g, int max_steps, int* finished) {
+  if (!g || max_steps < 1) return fail(VI_EINVAL, "vi_gradient_advance: bad arguments");
+  VI_TRY(g->advance(max_steps));
+  if (finished) *finished = g->done ? 1 : 0;
+  return 0;
+}
+
+int vi_gradient_idle(vi_grad g) {
+  if (!g) return fail(VI_EINVAL, "vi_gradient_idle: null job");
+  const hipError_t e = hipEventQuery(g->ev);
+  if (e == hipErrorNotReady) return 0;
+  VI_HIP(e);
+  return 1;
+}
+
+int vi_gradient_end(vi_grad g, double* misfit, void* g_rho, void* g_lam_or_c13,
+                    void* g_mu_or_c55, void* g_tau, void* g_c11, void* g_c33) {
+  if (!g) return fail(VI_EINVAL, "vi_gradient_end: null job");
+  std::unique_ptr<vi_grad_s> own(g);
+  void* out[6] = {g_rho, g_lam_or_c13, g_mu_or_c55, g_tau, g_c11, g_c33};
+  const int rc = g->finish(misfit, out);
+  if (rc) g->h->grad_busy = false;
+  return rc;
+}
+
+int vi_gradient_cancel(vi_grad g) {
+  if (!g) return 0;
+  std::unique_ptr<vi_grad_s> own(g);
+  vi_sim_s* h = g->h;
+  (void)hipStreamSynchronize(h->stream);
+  h->grad_busy = false;
+  return vi_reset(h);
 }
 
 // ---------------------------------------------------------------- batches

@@ -446,6 +446,12 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   }
 }
 
+// Set the step index of shot s (stream-ordered, so later launches see it).
+template <typename R>
+__global__ void k_set_step(ShotDesc<R>* shots, int s, long step) {
+  if (blockIdx.x * blockDim.x + threadIdx.x == 0) shots[s].step = step;
+}
+
 // Advance the step index of every shot of a batch after a block of k steps.
 template <typename R>
 __global__ void k_advance(ShotDesc<R>* shots, int nshots, int k) {

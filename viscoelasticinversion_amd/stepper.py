@@ -107,6 +107,12 @@ class Simulation:
         L.check(L.lib().vi_device_bytes(self._h, ctypes.byref(b)))
         return b.value
 
+    def gradient_bytes(self, nt, checkpoint_every=0):
+        """Temporary device bytes of one gradient(…, nt, checkpoint_every) call."""
+        b = ctypes.c_size_t()
+        L.check(L.lib().vi_gradient_bytes(self._h, int(nt), int(checkpoint_every), ctypes.byref(b)))
+        return b.value
+
     def set_relaxation(self, taun):
         t = np.ascontiguousarray(np.asarray(taun, dtype=np.float64).ravel())
         if t.size != self.nrelax:
@@ -256,3 +262,66 @@ class Simulation:
             self.close()
         except Exception:
             pass
+
+
+class GradientJob:
+    """The gradient of one Simulation (as Simulation.gradient) computed as a job that
+    advances in chunks without blocking: ``advance(k)`` enqueues at most k steps of work,
+    ``idle()`` tells whether the device has finished what was enqueued, ``finished`` whether
+    everything is enqueued, ``end()`` returns (J, g). The simulation belongs to the job until
+    end() or cancel()."""
+
+    def __init__(self, sim, dobs, nt=None, checkpoint_every=0):
+        d = sim._arr(dobs)
+        nt = d.shape[1] if nt is None else int(nt)
+        d = np.ascontiguousarray(d.reshape(sim.nrec, -1)[:, :nt])
+        self.sim, self.nt = sim, nt
+        h = ctypes.c_void_p()
+        L.check(L.lib().vi_gradient_begin(sim.handle, nt, sim._ptr(d), int(checkpoint_every),
+                                          ctypes.byref(h)))
+        self._h = h
+        self.finished = False
+
+    def advance(self, max_steps):
+        fin = ctypes.c_int()
+        L.check(L.lib().vi_gradient_advance(self._h, int(max_steps), ctypes.byref(fin)))
+        self.finished = bool(fin.value)
+        return self.finished
+
+    def idle(self):
+        r = L.lib().vi_gradient_idle(self._h)
+        if r < 0:
+            L.check(r)
+        return r == 1
+
+    def end(self):
+        sim = self.sim
+        shp = (sim.nz, sim.nx)
+        keys = (("rho", "c13", "c55", "tau", "c11", "c33") if sim.vti
+                else ("rho", "lam", "mu", "tau"))
+        g = {k: np.zeros(shp, dtype=sim.dtype) for k in keys if k != "tau" or sim.nrelax}
+        ptr = lambda k: sim._ptr(g[k]) if k in g else None
+        J = ctypes.c_double()
+        order = keys + (None,) * (6 - len(keys))
+        h, self._h = self._h, None
+        L.check(L.lib().vi_gradient_end(h, ctypes.byref(J), *(ptr(k) for k in order)))
+        return J.value, g
+
+    def cancel(self):
+        if self._h:
+            h, self._h = self._h, None
+            L.check(L.lib().vi_gradient_cancel(h))
+
+
+def plan_bytes(nx, nz, precision="float32", nrelax=0, vti=False, cpml_thickness=0, nsrc=0,
+               src_nt=0, nrec=0, rec_nt=0, grad_nt=0, checkpoint_every=0):
+    """Upper bound of the device bytes a shot holds (plus one gradient call when
+    grad_nt > 0), computed by the library without a device."""
+    dt = np.dtype(np.float32 if precision in ("float32", np.float32, "f32") else np.float64)
+    g = L.GridDesc(int(nx), int(nz), 1.0, 1.0, 1.0, L.VI_F32 if dt == np.float32 else L.VI_F64,
+                   int(nrelax), 8, 8)
+    b = ctypes.c_size_t()
+    L.check(L.lib().vi_plan_bytes(ctypes.byref(g), int(bool(vti)), int(cpml_thickness), int(nsrc),
+                                  int(src_nt), int(nrec), int(rec_nt), int(grad_nt),
+                                  int(checkpoint_every), ctypes.byref(b)))
+    return b.value
