@@ -84,22 +84,18 @@ __device__ __forceinline__ R cpml_adj(R h, R* ph, R a, R b, R k) {
   return h / k + a * tot;
 }
 
-// ---- adjoint of the stress half step (pointwise part) --------------------------------
+// ---- adjoint of the stress half step (pointwise part of one interior cell) ------------
+// exx = D-x vx, ezx = D+z vx, ezz = D-z vz, exz = D+x vz: the raw forward rates of the cell
 template <typename R, int L>
-__global__ __launch_bounds__(256) void k_adj_stress_point(const AdjDesc<R> d) {
+__device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, int i, int j, R exx, R ezx,
+                                                R ezz, R exz) {
   constexpr unsigned E = sizeof(R);
   constexpr int LM = L > 0 ? L : 1;
-  const int i = 4 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 4 + blockIdx.y * 4 + threadIdx.y;
-  if (i >= d.nx - 4 || j >= d.nz - 4) return;
   const int ldx = d.ldx;
   const unsigned cb = (unsigned)(j * ldx + i) * E;
   const unsigned mb = (unsigned)(j * d.ldm + i) * E;
-  const int rb = ldx * (int)E;
   const R dt = d.dt, two = R(2), quarter = R(0.25);
   // forward strain rates of this step (CPML-stretched)
-  R exx = gm(d.fvx, cb, (int)E, d.Dx), ezx = gp(d.fvx, cb, rb, d.Dz);
-  R ezz = gm(d.fvz, cb, rb, d.Dz), exz = gp(d.fvz, cb, (int)E, d.Dx);
   const int xk = d.T ? strip_x(i, d.nx, d.T, d.xlo, d.xhi) : -1;
   const int zk = d.T ? strip_x(j, d.nz, d.T, d.zlo, d.zhi) : -1;
   const unsigned qx = (unsigned)(j * 2 * d.T + xk) * E, qz = (unsigned)(zk * ldx + i) * E;
@@ -244,36 +240,17 @@ __global__ __launch_bounds__(256) void k_adj_stress_point(const AdjDesc<R> d) {
   gst(d.ehat[3], cb, exz_r);
 }
 
-// ---- adjoint of the stress half step (transposed operators): v_hat -= D^T e_hat --------
+// ---- adjoint of the velocity half step (pointwise part of one interior cell) ----------
+// dxx = D+x sxx, dxzz = D-z sxz, dxzx = D-x sxz, dzz = D+z szz: the raw forward derivatives;
+// vxh, vzh: the velocity adjoint of the cell
 template <typename R>
-__global__ __launch_bounds__(256) void k_adj_stress_gather(const AdjDesc<R> d) {
+__device__ __forceinline__ void adj_vel_cell(const AdjDesc<R>& d, int i, int j, R dxx, R dxzz,
+                                             R dxzx, R dzz, R vxh, R vzh) {
   constexpr unsigned E = sizeof(R);
-  const int i = 4 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 4 + blockIdx.y * 4 + threadIdx.y;
-  if (i >= d.nx - 4 || j >= d.nz - 4) return;
-  const unsigned cb = (unsigned)(j * d.ldx + i) * E;
-  const int rb = d.ldx * (int)E;
-  // exx = D-x vx, ezx = D+z vx, ezz = D-z vz, exz = D+x vz
-  const R gvx = gp((const R*)d.ehat[0], cb, (int)E, d.Dx) + gm((const R*)d.ehat[1], cb, rb, d.Dz);
-  const R gvz = gp((const R*)d.ehat[2], cb, rb, d.Dz) + gm((const R*)d.ehat[3], cb, (int)E, d.Dx);
-  gst(d.avx, cb, gld((const R*)d.avx, cb) - gvx);
-  gst(d.avz, cb, gld((const R*)d.avz, cb) - gvz);
-}
-
-// ---- adjoint of the velocity half step (pointwise part) ------------------------------
-template <typename R>
-__global__ __launch_bounds__(256) void k_adj_vel_point(const AdjDesc<R> d) {
-  constexpr unsigned E = sizeof(R);
-  const int i = 4 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 4 + blockIdx.y * 4 + threadIdx.y;
-  if (i >= d.nx - 4 || j >= d.nz - 4) return;
   const int ldx = d.ldx;
   const unsigned cb = (unsigned)(j * ldx + i) * E;
   const unsigned mb = (unsigned)(j * d.ldm + i) * E;
-  const int rb = ldx * (int)E;
   const R dt = d.dt, two = R(2);
-  R dxx = gp(d.fsxx, cb, (int)E, d.Dx), dxzz = gm(d.fsxz, cb, rb, d.Dz);
-  R dxzx = gm(d.fsxz, cb, (int)E, d.Dx), dzz = gp(d.fszz, cb, rb, d.Dz);
   const int xk = d.T ? strip_x(i, d.nx, d.T, d.xlo, d.xhi) : -1;
   const int zk = d.T ? strip_x(j, d.nz, d.T, d.zlo, d.zhi) : -1;
   const unsigned qx = (unsigned)(j * 2 * d.T + xk) * E, qz = (unsigned)(zk * ldx + i) * E;
@@ -298,7 +275,6 @@ __global__ __launch_bounds__(256) void k_adj_vel_point(const AdjDesc<R> d) {
   const R rho = gld(d.rho, mb);
   const R bxb = two / (rho + gld(d.rho, mb + E));
   const R bzb = two / (rho + gld(d.rho, mb + d.ldm * E));
-  const R vxh = gld((const R*)d.avx, cb), vzh = gld((const R*)d.avz, cb);
   // gradient with respect to the buoyancies (scattered to rho at the end)
   gst(d.g_bx, cb, gld((const R*)d.g_bx, cb) + vxh * dt * fx);
   gst(d.g_bz, cb, gld((const R*)d.g_bz, cb) + vzh * dt * fz);
@@ -324,29 +300,145 @@ __global__ __launch_bounds__(256) void k_adj_vel_point(const AdjDesc<R> d) {
   gst(d.dhat[3], cb, hzz);
 }
 
-// ---- adjoint of the velocity half step (transposed operators): s_hat -= D^T d_hat ------
-// Rows 0..3 are the free-surface image rows, whose adjoint is collected here and moved to
-// the interior rows they copy by the next k_adj_stress_point. Their z-stencils reach rows
-// -4..-1, so the d_hat planes are read from a base moved back by 4 rows into the zero guard
-// rows in front of each plane: every 32-bit byte offset stays non-negative.
-template <typename R>
-__global__ __launch_bounds__(256) void k_adj_vel_gather(const AdjDesc<R> d, int row0) {
+// ---- tiled transposed steps ------------------------------------------------------------
+// The adjoint kernels use the forward kernels' tiles: 64 columns x (4 RZ) rows, 4 wavefronts,
+// XCD-aware tile order, every stenciled field staged once in LDS with its halo (TileLoad).
+// The sums are the ones of the one-cell formulas, in the same order.
+
+// Transposed stress half, pointwise: stencils of the stored v^{n+1/2} from LDS.
+template <typename R, int L, int RZ>
+__global__ __launch_bounds__(TX* NW) void k_adj_stress(const AdjDesc<R> d) {
+  constexpr int TZ = NW * RZ, WX = TX + 8;
+  __shared__ R s_vx[(TZ + 8) * WX];
+  __shared__ R s_vz[(TZ + 8) * WX];
+  const int lane = threadIdx.x, w = threadIdx.y;
+  const int ntx = (d.nx - 8 + TX - 1) / TX;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int i0 = 4 + (tile % ntx) * TX, j0 = 4 + (tile / ntx) * TZ;
+  TileLoad<R, TZ + 8, 4> t_vx, t_vz;
+  t_vx.fetch(d.fvx, j0 - 4, i0, d.ldx, lane, w);
+  t_vz.fetch(d.fvz, j0 - 4, i0, d.ldx, lane, w);
+  t_vx.store(s_vx, lane, w);
+  t_vz.store(s_vz, lane, w);
+  __syncthreads();
+  const int i = i0 + lane;
+  if (i >= d.nx - 4) return;
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const int j = j0 + w * RZ + r;
+    if (j >= d.nz - 4) return;
+    const R* pvx = &s_vx[(w * RZ + r + 4) * WX + lane + 4];
+    const R* pvz = &s_vz[(w * RZ + r + 4) * WX + lane + 4];
+    adj_stress_cell<R, L>(d, i, j, op_m(pvx, 1, d.Dx), op_p(pvx, WX, d.Dz), op_m(pvz, WX, d.Dz),
+                          op_p(pvz, 1, d.Dx));
+  }
+}
+
+// Transposed stress half, gather (v_hat -= D^T e_hat), fused with the transposed velocity
+// half, pointwise (which needs only the cell's new v_hat): e_hat and the stored s^n from LDS.
+template <typename R, int RZ>
+__global__ __launch_bounds__(TX* NW) void k_adj_vel(const AdjDesc<R> d) {
+  constexpr int TZ = NW * RZ, WX = TX + 8;
   constexpr unsigned E = sizeof(R);
-  const int i = 4 + blockIdx.x * 64 + threadIdx.x;
-  const int j = row0 + blockIdx.y * 4 + threadIdx.y;
-  if (i >= d.nx - 4 || j >= d.nz - 4) return;
-  const unsigned cb = (unsigned)(j * d.ldx + i) * E;
-  const unsigned gb = (unsigned)((j + 4) * d.ldx + i) * E;  // offset from the guarded base
-  const size_t back = (size_t)4 * d.ldx;
-  const int rb = d.ldx * (int)E;
-  // dxx = D+x sxx, dxzz = D-z sxz, dxzx = D-x sxz, dzz = D+z szz
-  const R gsxx = gm((const R*)d.dhat[0] - back, gb, (int)E, d.Dx);
-  const R gsxz = gp((const R*)d.dhat[1] - back, gb, rb, d.Dz) +
-                 gp((const R*)d.dhat[2] - back, gb, (int)E, d.Dx);
-  const R gszz = gm((const R*)d.dhat[3] - back, gb, rb, d.Dz);
-  gst(d.asxx, cb, gld((const R*)d.asxx, cb) - gsxx);
-  gst(d.asxz, cb, gld((const R*)d.asxz, cb) - gsxz);
-  gst(d.aszz, cb, gld((const R*)d.aszz, cb) - gszz);
+  __shared__ R s_e0[TZ * WX];        // e_hat exx: x halo
+  __shared__ R s_e3[TZ * WX];        // e_hat exz: x halo
+  __shared__ R s_e1[(TZ + 8) * TX];  // e_hat ezx: z halo
+  __shared__ R s_e2[(TZ + 8) * TX];  // e_hat ezz: z halo
+  __shared__ R s_xx[TZ * WX];        // sxx: x halo
+  __shared__ R s_zz[(TZ + 8) * TX];  // szz: z halo
+  __shared__ R s_xz[(TZ + 8) * WX];  // sxz: x and z halo
+  const int lane = threadIdx.x, w = threadIdx.y;
+  const int ntx = (d.nx - 8 + TX - 1) / TX;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int i0 = 4 + (tile % ntx) * TX, j0 = 4 + (tile / ntx) * TZ;
+  const int ldx = d.ldx;
+  TileLoad<R, TZ, 4> t_e0, t_e3, t_xx;
+  TileLoad<R, TZ + 8, 0> t_e1, t_e2, t_zz;
+  TileLoad<R, TZ + 8, 4> t_xz;
+  t_e0.fetch(d.ehat[0], j0, i0, ldx, lane, w);
+  t_e3.fetch(d.ehat[3], j0, i0, ldx, lane, w);
+  t_e1.fetch(d.ehat[1], j0 - 4, i0, ldx, lane, w);
+  t_e2.fetch(d.ehat[2], j0 - 4, i0, ldx, lane, w);
+  t_xx.fetch(d.fsxx, j0, i0, ldx, lane, w);
+  t_zz.fetch(d.fszz, j0 - 4, i0, ldx, lane, w);
+  t_xz.fetch(d.fsxz, j0 - 4, i0, ldx, lane, w);
+  t_e0.store(s_e0, lane, w);
+  t_e3.store(s_e3, lane, w);
+  t_e1.store(s_e1, lane, w);
+  t_e2.store(s_e2, lane, w);
+  t_xx.store(s_xx, lane, w);
+  t_zz.store(s_zz, lane, w);
+  t_xz.store(s_xz, lane, w);
+  __syncthreads();
+  const int i = i0 + lane;
+  if (i >= d.nx - 4) return;
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const int j = j0 + w * RZ + r;
+    if (j >= d.nz - 4) return;
+    const int tr = w * RZ + r;
+    const unsigned cb = (unsigned)(j * ldx + i) * E;
+    // exx = D-x vx, ezx = D+z vx, ezz = D-z vz, exz = D+x vz
+    const R gvx = op_p(&s_e0[tr * WX + lane + 4], 1, d.Dx) +
+                  op_m(&s_e1[(tr + 4) * TX + lane], TX, d.Dz);
+    const R gvz = op_p(&s_e2[(tr + 4) * TX + lane], TX, d.Dz) +
+                  op_m(&s_e3[tr * WX + lane + 4], 1, d.Dx);
+    const R vxh = gld((const R*)d.avx, cb) - gvx;
+    const R vzh = gld((const R*)d.avz, cb) - gvz;
+    gst(d.avx, cb, vxh);
+    gst(d.avz, cb, vzh);
+    const R* pxz = &s_xz[(tr + 4) * WX + lane + 4];
+    adj_vel_cell<R>(d, i, j, op_p(&s_xx[tr * WX + lane + 4], 1, d.Dx), op_m(pxz, WX, d.Dz),
+                    op_m(pxz, 1, d.Dx), op_p(&s_zz[(tr + 4) * TX + lane], TX, d.Dz), vxh, vzh);
+  }
+}
+
+// Transposed velocity half, gather: s_hat -= D^T d_hat over rows [row0, nz-4). Rows 0..3 are
+// the free-surface image rows (row0 = 0), whose adjoint the next k_adj_stress moves to the
+// interior rows they copy. Their z-stencils reach rows -4..-1, so the d_hat planes are read
+// from a base moved back by 4 rows into the zero guard rows in front of each plane: every
+// 32-bit byte offset stays non-negative.
+template <typename R, int RZ>
+__global__ __launch_bounds__(TX* NW) void k_adj_sgather(const AdjDesc<R> d, int row0) {
+  constexpr int TZ = NW * RZ, WX = TX + 8;
+  constexpr unsigned E = sizeof(R);
+  __shared__ R s_d0[TZ * WX];        // d_hat dxx = D+x sxx: x halo
+  __shared__ R s_d2[TZ * WX];        // d_hat dxzx = D-x sxz: x halo
+  __shared__ R s_d1[(TZ + 8) * TX];  // d_hat dxzz = D-z sxz: z halo
+  __shared__ R s_d3[(TZ + 8) * TX];  // d_hat dzz = D+z szz: z halo
+  const int lane = threadIdx.x, w = threadIdx.y;
+  const int ntx = (d.nx - 8 + TX - 1) / TX;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int i0 = 4 + (tile % ntx) * TX, j0 = row0 + (tile / ntx) * TZ;
+  const int ldx = d.ldx;
+  const size_t back = (size_t)4 * ldx;  // guarded bases: row r of the plane is row r + 4
+  TileLoad<R, TZ, 4> t_d0, t_d2;
+  TileLoad<R, TZ + 8, 0> t_d1, t_d3;
+  t_d0.fetch(d.dhat[0] - back, j0 + 4, i0, ldx, lane, w);
+  t_d2.fetch(d.dhat[2] - back, j0 + 4, i0, ldx, lane, w);
+  t_d1.fetch(d.dhat[1] - back, j0, i0, ldx, lane, w);
+  t_d3.fetch(d.dhat[3] - back, j0, i0, ldx, lane, w);
+  t_d0.store(s_d0, lane, w);
+  t_d2.store(s_d2, lane, w);
+  t_d1.store(s_d1, lane, w);
+  t_d3.store(s_d3, lane, w);
+  __syncthreads();
+  const int i = i0 + lane;
+  if (i >= d.nx - 4) return;
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const int j = j0 + w * RZ + r;
+    if (j >= d.nz - 4) return;
+    const int tr = w * RZ + r;
+    const unsigned cb = (unsigned)(j * ldx + i) * E;
+    const R gsxx = op_m(&s_d0[tr * WX + lane + 4], 1, d.Dx);
+    const R gsxz = op_p(&s_d1[(tr + 4) * TX + lane], TX, d.Dz) +
+                   op_p(&s_d2[tr * WX + lane + 4], 1, d.Dx);
+    const R gszz = op_m(&s_d3[(tr + 4) * TX + lane], TX, d.Dz);
+    gst(d.asxx, cb, gld((const R*)d.asxx, cb) - gsxx);
+    gst(d.asxz, cb, gld((const R*)d.asxz, cb) - gsxz);
+    gst(d.aszz, cb, gld((const R*)d.aszz, cb) - gszz);
+  }
 }
 
 // ---- residual injection: u_hat += R^T r^n (receivers sample after the stress half) ----
@@ -372,19 +464,6 @@ __global__ __launch_bounds__(256) void k_adj_inject(R* f0, R* f1, R* f2, R* f3, 
   }
   R* p = f + tcell[t];
   *p = *p + s;
-}
-
-// ---- memory-variable sums of the forward state: dst[c] = sum_k M[c L + k], c = xx, zz, xz,
-// summed in ascending k from zero (the order the stress update uses)
-template <typename R>
-__global__ __launch_bounds__(256) void k_msum(const R* M, R* dst, size_t plane, int L) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= plane) return;
-  for (int c = 0; c < 3; ++c) {
-    R s = 0;
-    for (int k = 0; k < L; ++k) s = s + M[(size_t)(c * L + k) * plane + idx];
-    dst[(size_t)c * plane + idx] = s;
-  }
 }
 
 // ---- batched device-to-device copy: up to 16 regions per launch (blockIdx.y = region).

@@ -445,22 +445,58 @@ int launch_kernel(hipStream_t s, int L, int k, const void* descs, const int* til
   const dim3 block(vi::TX, vi::NW);
   const dim3 grid(ntiles);
   const vi::ShotDesc<R>* d = (const vi::ShotDesc<R>*)descs;
+  const vi::RecSlot rs{};
   if (k == 0) {
-    launch(vi::k_velocity<R, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep);
+    launch(vi::k_velocity<R, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs);
   } else {
     switch (L) {
-      case 0: launch(vi::k_stress<R, 0, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep); break;
-      case 1: launch(vi::k_stress<R, 1, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep); break;
-      case 2: launch(vi::k_stress<R, 2, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep); break;
-      case 3: launch(vi::k_stress<R, 3, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep); break;
-      case 4: launch(vi::k_stress<R, 4, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep); break;
-      case 5: launch(vi::k_stress<R, 5, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep); break;
-      case 6: launch(vi::k_stress<R, 6, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep); break;
+      case 0: launch(vi::k_stress<R, 0, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
+      case 1: launch(vi::k_stress<R, 1, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
+      case 2: launch(vi::k_stress<R, 2, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
+      case 3: launch(vi::k_stress<R, 3, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
+      case 4: launch(vi::k_stress<R, 4, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
+      case 5: launch(vi::k_stress<R, 5, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
+      case 6: launch(vi::k_stress<R, 6, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
       default: return fail(VI_EINVAL, "nrelax out of range");
     }
   }
   VI_HIP(hipGetLastError());
   return 0;
+}
+
+// One half step of one shot (null tile table) that also records the forward state of the
+// step into a slot (recording variants exist for 1 and 2 rows per wavefront).
+template <typename R, int RZ>
+int launch_rec_kernel(hipStream_t s, int L, int k, const void* descs, int ntiles, int kstep,
+                      const vi::RecSlot& rs) {
+  const dim3 block(vi::TX, vi::NW);
+  const dim3 grid(ntiles);
+  const vi::ShotDesc<R>* d = (const vi::ShotDesc<R>*)descs;
+  const int* ts = nullptr;
+  if (k == 0) {
+    hipLaunchKernelGGL((vi::k_velocity<R, RZ, true>), grid, block, 0, s, d, ts, kstep, rs);
+  } else {
+    switch (L) {
+      case 0: hipLaunchKernelGGL((vi::k_stress<R, 0, RZ, true>), grid, block, 0, s, d, ts, kstep, rs); break;
+      case 1: hipLaunchKernelGGL((vi::k_stress<R, 1, RZ, true>), grid, block, 0, s, d, ts, kstep, rs); break;
+      case 2: hipLaunchKernelGGL((vi::k_stress<R, 2, RZ, true>), grid, block, 0, s, d, ts, kstep, rs); break;
+      case 3: hipLaunchKernelGGL((vi::k_stress<R, 3, RZ, true>), grid, block, 0, s, d, ts, kstep, rs); break;
+      case 4: hipLaunchKernelGGL((vi::k_stress<R, 4, RZ, true>), grid, block, 0, s, d, ts, kstep, rs); break;
+      case 5: hipLaunchKernelGGL((vi::k_stress<R, 5, RZ, true>), grid, block, 0, s, d, ts, kstep, rs); break;
+      case 6: hipLaunchKernelGGL((vi::k_stress<R, 6, RZ, true>), grid, block, 0, s, d, ts, kstep, rs); break;
+      default: return fail(VI_EINVAL, "nrelax out of range");
+    }
+  }
+  VI_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int launch_rec(hipStream_t s, int rz, int L, int k, const void* descs, int ntiles, int kstep,
+               const vi::RecSlot& rs) {
+  if (rz == 1) return launch_rec_kernel<R, 1>(s, L, k, descs, ntiles, kstep, rs);
+  if (rz == 2) return launch_rec_kernel<R, 2>(s, L, k, descs, ntiles, kstep, rs);
+  return fail(VI_EINVAL, "recording launches need 1 or 2 rows per wavefront");
 }
 
 template <typename R>
@@ -479,6 +515,7 @@ int launch_one(hipStream_t s, int rz, int L, int k, const void* descs, const int
 // indices on the device.
 int launch_steps(hipStream_t s, size_t esz, int rz, int L, void* descs, const int* tile_shot,
                  int ntiles, int nshots, int nsteps, Prof* prof) {
+  if (nshots == 1) tile_shot = nullptr;  // every tile belongs to shot 0
   for (int q = 0; q < nsteps; ++q) {
     for (int k = 0; k < 2; ++k) {
       hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -717,6 +754,7 @@ struct GradRun : vi_grad_s {
   const int *t_cell = nullptr, *t_comp = nullptr, *t_ptr = nullptr, *t_rec = nullptr;
   hipEvent_t ev_tr = nullptr;
   enum Phase { FWD, RESID, RESTORE, RECOMP, BACK, DONE } ph = FWD;
+  int rz0 = 0;  // the handle's tiling before the job (restored at the end)
   int s = 0, n = 0;  // segment, step (FWD: steps done in segment s)
 
   ~GradRun() override {
@@ -764,6 +802,8 @@ struct GradRun : vi_grad_s {
     nsrc = h->nsrc;
     hipStream_t st = h->stream;
     VI_TRY(vi_reset(h));
+    rz0 = h->rz;
+    if (h->rz > 2) VI_TRY(apply_tiling(h, 2));  // the recording kernels use short tiles
     VI_TRY(refresh_own_desc(h));
     live = mutable_arrays(h);
     live.pop_back();  // traces are not part of the checkpointed state
@@ -845,18 +885,44 @@ struct GradRun : vi_grad_s {
     VI_HIP(hipHostMalloc((void**)&host_tr, (size_t)nt * nrec * E, 0));
     VI_HIP(hipEventCreateWithFlags(&ev_tr, hipEventDisableTiming));
     VI_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    {  // short tiles when the grid is small (enough waves in flight)
+      const long ntx = (d.nx - 8 + vi::TX - 1) / vi::TX;
+      arz = ntx * ((d.nz - 8 + 1) / 2) >= 256L * 16 ? 2 : 1;
+    }
     VI_HIP(hipStreamSynchronize(st));
     h->own_pending = true;
     h->grad_busy = true;
     return 0;
   }
 
+  int arz = 2;  // rows per wavefront of the transposed-step tiles
+
+  template <int RZ>
+  int backward_kernels() {
+    hipStream_t st = h->stream;
+    constexpr int TZ = vi::NW * RZ;
+    const dim3 blk(vi::TX, vi::NW);
+    const int ntx = (d.nx - 8 + vi::TX - 1) / vi::TX;
+    const dim3 gint(ntx * ((d.nz - 8 + TZ - 1) / TZ));
+    const int row0 = d.fs ? 0 : 4;
+    const dim3 gsg(ntx * ((d.nz - 4 - row0 + TZ - 1) / TZ));
+    switch (L) {
+      case 0: hipLaunchKernelGGL((vi::k_adj_stress<R, 0, RZ>), gint, blk, 0, st, d); break;
+      case 1: hipLaunchKernelGGL((vi::k_adj_stress<R, 1, RZ>), gint, blk, 0, st, d); break;
+      case 2: hipLaunchKernelGGL((vi::k_adj_stress<R, 2, RZ>), gint, blk, 0, st, d); break;
+      case 3: hipLaunchKernelGGL((vi::k_adj_stress<R, 3, RZ>), gint, blk, 0, st, d); break;
+      case 4: hipLaunchKernelGGL((vi::k_adj_stress<R, 4, RZ>), gint, blk, 0, st, d); break;
+      case 5: hipLaunchKernelGGL((vi::k_adj_stress<R, 5, RZ>), gint, blk, 0, st, d); break;
+      case 6: hipLaunchKernelGGL((vi::k_adj_stress<R, 6, RZ>), gint, blk, 0, st, d); break;
+    }
+    hipLaunchKernelGGL((vi::k_adj_vel<R, RZ>), gint, blk, 0, st, d);
+    hipLaunchKernelGGL((vi::k_adj_sgather<R, RZ>), gsg, blk, 0, st, d, row0);
+    VI_HIP(hipGetLastError());
+    return 0;
+  }
+
   int backward_step(int step, const char* slot) {
     hipStream_t st = h->stream;
-    const dim3 blk(64, 4);
-    const dim3 gint((d.nx - 8 + 63) / 64, (d.nz - 8 + 3) / 4);
-    const int row0 = d.fs ? 0 : 4;
-    const dim3 gvg((d.nx - 8 + 63) / 64, (d.nz - 4 - row0 + 3) / 4);
     d.fsxx = (const R*)(slot);
     d.fszz = (const R*)(slot + pb);
     d.fsxz = (const R*)(slot + 2 * pb);
@@ -869,20 +935,7 @@ struct GradRun : vi_grad_s {
       hipLaunchKernelGGL(vi::k_adj_inject<R>, dim3((ntgt + 255) / 256), dim3(256), 0, st, d.avx,
                          d.avz, d.asxx, d.aszz, d.asxz, t_cell, t_comp, t_ptr, t_rec, ntgt,
                          (const R*)resid.p + (size_t)step * nrec);
-    switch (L) {
-      case 0: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 0>), gint, blk, 0, st, d); break;
-      case 1: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 1>), gint, blk, 0, st, d); break;
-      case 2: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 2>), gint, blk, 0, st, d); break;
-      case 3: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 3>), gint, blk, 0, st, d); break;
-      case 4: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 4>), gint, blk, 0, st, d); break;
-      case 5: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 5>), gint, blk, 0, st, d); break;
-      case 6: hipLaunchKernelGGL((vi::k_adj_stress_point<R, 6>), gint, blk, 0, st, d); break;
-    }
-    hipLaunchKernelGGL(vi::k_adj_stress_gather<R>, gint, blk, 0, st, d);
-    hipLaunchKernelGGL(vi::k_adj_vel_point<R>, gint, blk, 0, st, d);
-    hipLaunchKernelGGL(vi::k_adj_vel_gather<R>, gvg, blk, 0, st, d, row0);
-    VI_HIP(hipGetLastError());
-    return 0;
+    return arz == 1 ? backward_kernels<1>() : backward_kernels<2>();
   }
 
   int advance(int max_steps) override {
@@ -930,22 +983,14 @@ struct GradRun : vi_grad_s {
         ph = RECOMP;
       } else if (ph == RECOMP) {  // recompute the segment, storing each step's inputs
         const int n0 = s * K, n1 = std::min(n0 + K, nt);
-        R* M0 = (R*)h->M.p;
         for (; n < n1 && budget > 0; ++n, --budget) {
-          char* slot = (char*)seg.p + (size_t)(n - n0) * lay.slot_bytes;
-          std::vector<Cp> pre;
-          for (int q = 0; q < 3; ++q) pre.push_back({h->state[2 + q].p, slot + q * pb, pb});
-          for (int q = 0; q < 8; ++q)
-            if (psi_bytes[q]) pre.push_back({h->psi[q].p, slot + lay.psi_off[q], psi_bytes[q]});
-          VI_TRY(run_copies(pre));
-          if (L)
-            hipLaunchKernelGGL(vi::k_msum<R>, dim3((unsigned)((h->plane + 255) / 256)),
-                               dim3(256), 0, st, (const R*)M0, (R*)(slot + 5 * pb), h->plane, L);
-          VI_TRY(launch_one<R>(st, h->rz, L, 0, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
-                               h->ntiles, n - n0, nullptr, nullptr));
-          VI_TRY(run_copies({{h->state[0].p, slot + 3 * pb, pb}, {h->state[1].p, slot + 4 * pb, pb}}));
-          VI_TRY(launch_one<R>(st, h->rz, L, 1, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
-                               h->ntiles, n - n0, nullptr, nullptr));
+          // both half steps record the step's forward state into its slot as they go
+          vi::RecSlot rs{};
+          rs.base = (char*)seg.p + (size_t)(n - n0) * lay.slot_bytes;
+          rs.pb = pb;
+          for (int q = 0; q < 8; ++q) rs.psi_off[q] = lay.psi_off[q];
+          VI_TRY(launch_rec<R>(st, h->rz, L, 0, h->desc_dev.p, h->ntiles, n - n0, rs));
+          VI_TRY(launch_rec<R>(st, h->rz, L, 1, h->desc_dev.p, h->ntiles, n - n0, rs));
         }
         if (n == n1) {
           n = n1 - 1;
@@ -988,8 +1033,9 @@ struct GradRun : vi_grad_s {
         VI_TRY(download_plane(st, out[q], planes[q], h->ldx, nx, nz, E));
     VI_HIP(hipStreamSynchronize(st));
     if (misfit) *misfit = J;
-    // leave the handle at a zero state, step 0
+    // leave the handle at a zero state, step 0, with its own tiling
     h->grad_busy = false;
+    if (rz0 && h->rz != rz0) VI_TRY(apply_tiling(h, rz0));
     return vi_reset(h);
   }
 };

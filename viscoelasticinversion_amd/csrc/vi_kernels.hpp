@@ -5,7 +5,7 @@
 // shots: every shot is described by a ShotDesc in device memory, its interior
 // [4, nx-4) x [4, nz-4) is cut into tiles of TX x (NW*RZ) cells, and the tiles of all shots
 // form one grid (tile_shot[] maps a global tile to its shot). A single simulation is a
-// batch of one.
+// batch of one, launched with a null tile_shot (one dependent load less per workgroup).
 //   - a workgroup is NW wavefronts of 64 lanes; lane = x offset, wave = band of RZ rows;
 //   - the differentiated fields of the tile are staged once in LDS with a 4-cell halo in x
 //     and z (the halo is shared by the NW wavefronts, so each global row segment is a
@@ -65,6 +65,22 @@ struct ShotDesc {
   R* traces;  // [rec_nt][nrec]
   int nrec, rec_nt;
 };
+
+// Where a recording launch (REC) stores the forward state of its step for the adjoint:
+// planes (pitch ldx, pb bytes apart) 0..2 = sxx, szz, sxz before the step (with the
+// free-surface images), 3..4 = vx, vz after the velocity half, 5..7 = the memory-variable
+// sums of xx, zz, xz before the step; the 8 CPML memory arrays before the step at
+// base + psi_off[q]. Only written by the variants with REC = true.
+struct RecSlot {
+  char* base;
+  unsigned long pb;
+  unsigned long psi_off[8];
+};
+
+template <typename R>
+__device__ __forceinline__ R* rec_plane(const RecSlot& rs, int q) {
+  return (R*)(rs.base + (unsigned long)q * rs.pb);
+}
 
 // Workgroup id -> global tile id such that each of the 8 XCDs (which receive workgroups
 // round robin) owns one contiguous range of tiles.
@@ -157,10 +173,57 @@ struct TileLoad {
   }
 };
 
+// CPML memory of one lane's RZ cells for two x-strip and two z-strip arrays, requested
+// before the barrier together with the tile loads (for short tiles), so the strip tiles,
+// which finish last, do not wait for a separate round trip after the stencils.
 template <typename R, int RZ>
+struct CpmlRows {
+  R px0[RZ], px1[RZ], pz0[RZ], pz1[RZ];
+  R ax = 0, bx = 0, kx = 1;
+  int xk = -1;
+  __device__ __forceinline__ void fetch(const ShotDesc<R>& d, int i, int jw, bool col_ok,
+                                        int ix0, int ix1, int iz0, int iz1) {
+    constexpr unsigned E = sizeof(R);
+    const int T = d.T;
+    xk = (T && col_ok) ? strip_x(i, d.nx, T, d.xlo, d.xhi) : -1;
+    if (xk >= 0) {
+      ax = gld(d.ax, i * E);
+      bx = gld(d.bx, i * E);
+      kx = gld(d.kx, i * E);
+    }
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      const int j = jw + r;
+      const bool ok = col_ok && j < d.nz - 4;
+      px0[r] = px1[r] = pz0[r] = pz1[r] = R(0);
+      if (ok && xk >= 0) {
+        const unsigned q = (unsigned)(j * 2 * T + xk) * E;
+        px0[r] = gld((const R*)d.psi[ix0], q);
+        px1[r] = gld((const R*)d.psi[ix1], q);
+      }
+      const int zk = (T && ok) ? strip_x(j, d.nz, T, d.zlo, d.zhi) : -1;
+      if (zk >= 0) {
+        const unsigned q = (unsigned)(zk * d.ldx + i) * E;
+        pz0[r] = gld((const R*)d.psi[iz0], q);
+        pz1[r] = gld((const R*)d.psi[iz1], q);
+      }
+    }
+  }
+};
+
+// psi' = b psi + a d (stored), return d/kappa + psi' -- with psi already in a register
+template <typename R>
+__device__ __forceinline__ R cpml_apply_pre(R* psi, unsigned pb, R p, R a, R b, R k, R d) {
+  const R q = b * p + a * d;
+  gst(psi, pb, q);
+  return d / k + q;
+}
+
+template <typename R, int RZ, bool REC = false>
 __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restrict__ shots,
                                                      const int* __restrict__ tile_shot,
-                                                     int kstep) {
+                                                     int kstep, const RecSlot rs) {
+  static_assert(!REC || RZ <= 2, "recording needs the prefetched CPML memory");
   constexpr int TZ = NW * RZ;
   constexpr int WX = TX + 8;
   constexpr unsigned E = sizeof(R);
@@ -169,7 +232,7 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
   __shared__ R s_xx[TZ * WX];        // sxx rows j0 .. j0+TZ-1, columns i0-4 .. i0+67
   const int lane = threadIdx.x, w = threadIdx.y;
   const int gtile = xcd_tile(blockIdx.x, gridDim.x);
-  const ShotDesc<R>& d = shots[tile_shot[gtile]];
+  const ShotDesc<R>& d = shots[tile_shot ? tile_shot[gtile] : 0];  // null: one shot
   const int tile = gtile - d.tile_base;
   const int ntx = d.ntx;
   const int tx = tile % ntx, tz = tile / ntx;
@@ -190,6 +253,10 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
   const R* __restrict__ rho = d.rho;
   R* __restrict__ vx = d.vx;
   R* __restrict__ vz = d.vz;
+  constexpr bool PRE = RZ <= 2;  // CPML memory requested before the barrier
+  CpmlRows<R, PRE ? RZ : 1> cp;
+  if constexpr (PRE) cp.fetch(d, i, jw, i < d.nx - 4, 2, 3, 6, 7);
+  const int s0 = d.src_off[tile], s1 = d.src_off[tile + 1];
   R wrho[RZ + 1], wrho1[RZ], ovx[RZ], ovz[RZ];
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
@@ -210,7 +277,6 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
   const R Dx[4] = {d.Dx[0], d.Dx[1], d.Dx[2], d.Dx[3]};
   const R Dz[4] = {d.Dz[0], d.Dz[1], d.Dz[2], d.Dz[3]};
   const int xk = T ? strip_x(i, nx, T, d.xlo, d.xhi) : -1;
-  const int s0 = d.src_off[tile], s1 = d.src_off[tile + 1];
   const long step = d.step + kstep;
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
@@ -225,16 +291,34 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
     if (T && ok) {
       if (xk >= 0) {
         const unsigned q = (unsigned)(j * 2 * T + xk) * E;
-        const R a = gld(d.ax, i * E), b = gld(d.bx, i * E), k = gld(d.kx, i * E);
-        dsxx_x = cpml_apply(d.psi[2], q, a, b, k, dsxx_x);
-        dsxz_x = cpml_apply(d.psi[3], q, a, b, k, dsxz_x);
+        if constexpr (PRE) {
+          if constexpr (REC) {
+            gst((R*)(rs.base + rs.psi_off[2]), q, cp.px0[r]);
+            gst((R*)(rs.base + rs.psi_off[3]), q, cp.px1[r]);
+          }
+          dsxx_x = cpml_apply_pre(d.psi[2], q, cp.px0[r], cp.ax, cp.bx, cp.kx, dsxx_x);
+          dsxz_x = cpml_apply_pre(d.psi[3], q, cp.px1[r], cp.ax, cp.bx, cp.kx, dsxz_x);
+        } else {
+          const R a = gld(d.ax, i * E), b = gld(d.bx, i * E), k = gld(d.kx, i * E);
+          dsxx_x = cpml_apply(d.psi[2], q, a, b, k, dsxx_x);
+          dsxz_x = cpml_apply(d.psi[3], q, a, b, k, dsxz_x);
+        }
       }
       const int zk = strip_x(j, nz, T, d.zlo, d.zhi);
       if (zk >= 0) {
         const unsigned q = (unsigned)(zk * ldx + i) * E;
         const R a = d.az[j], b = d.bz[j], k = d.kz[j];
-        dsxz_z = cpml_apply(d.psi[6], q, a, b, k, dsxz_z);
-        dszz_z = cpml_apply(d.psi[7], q, a, b, k, dszz_z);
+        if constexpr (PRE) {
+          if constexpr (REC) {
+            gst((R*)(rs.base + rs.psi_off[6]), q, cp.pz0[r]);
+            gst((R*)(rs.base + rs.psi_off[7]), q, cp.pz1[r]);
+          }
+          dsxz_z = cpml_apply_pre(d.psi[6], q, cp.pz0[r], a, b, k, dsxz_z);
+          dszz_z = cpml_apply_pre(d.psi[7], q, cp.pz1[r], a, b, k, dszz_z);
+        } else {
+          dsxz_z = cpml_apply(d.psi[6], q, a, b, k, dsxz_z);
+          dszz_z = cpml_apply(d.psi[7], q, a, b, k, dszz_z);
+        }
       }
     }
     R fx = dsxx_x + dsxz_z;
@@ -249,16 +333,22 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
     const R bxv = two / (wrho[r] + wrho1[r]);
     const R bzv = two / (wrho[r] + wrho[r + 1]);
     if (ok) {
-      gst(vx, cb, ovx[r] + dt * (bxv * fx));
-      gst(vz, cb, ovz[r] + dt * (bzv * fz));
+      const R nvx = ovx[r] + dt * (bxv * fx), nvz = ovz[r] + dt * (bzv * fz);
+      gst(vx, cb, nvx);
+      gst(vz, cb, nvz);
+      if constexpr (REC) {
+        gst(rec_plane<R>(rs, 3), cb, nvx);
+        gst(rec_plane<R>(rs, 4), cb, nvz);
+      }
     }
   }
 }
 
-template <typename R, int L, int RZ>
+template <typename R, int L, int RZ, bool REC = false>
 __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict__ shots,
                                                    const int* __restrict__ tile_shot,
-                                                   int kstep) {
+                                                   int kstep, const RecSlot rs) {
+  static_assert(!REC || RZ <= 2, "recording needs the prefetched CPML memory");
   constexpr int TZ = NW * RZ;
   constexpr int WX = TX + 8;
   constexpr unsigned E = sizeof(R);
@@ -271,7 +361,7 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   __shared__ R s_vz[(TZ + 8) * WX];  // vz, same window
   const int lane = threadIdx.x, w = threadIdx.y;
   const int gtile = xcd_tile(blockIdx.x, gridDim.x);
-  const ShotDesc<R>& d = shots[tile_shot[gtile]];
+  const ShotDesc<R>& d = shots[tile_shot ? tile_shot[gtile] : 0];  // null: one shot
   const int tile = gtile - d.tile_base;
   const int ntx = d.ntx;
   const int tx = tile % ntx, tz = tile / ntx;
@@ -296,6 +386,10 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
 #pragma unroll
   for (int k = 0; k < 3 * LM; ++k) Mp[k] = d.M + (size_t)k * d.plane;
   R wmu[RZ + 1], wmu1[RZ + 1];  // mu at columns i and i+1, rows jw .. jw+RZ
+  CpmlRows<R, HOIST ? RZ : 1> cp;
+  if constexpr (HOIST) cp.fetch(d, i, jw, i < d.nx - 4, 0, 1, 4, 5);
+  const int s0 = d.src_off[tile], s1 = d.src_off[tile + 1];
+  const int r0 = d.rec_off[tile], r1 = d.rec_off[tile + 1];
   const bool vti = d.vti;
   R olam[RH], otau[RH], osxx[RH], oszz[RH], osxz[RH], om[3 * LM][RH], oc11[RH], oc33[RH];
   auto load_cell = [&](int r, int h) {
@@ -338,8 +432,6 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   const R Dx[4] = {d.Dx[0], d.Dx[1], d.Dx[2], d.Dx[3]};
   const R Dz[4] = {d.Dz[0], d.Dz[1], d.Dz[2], d.Dz[3]};
   const int xk = T ? strip_x(i, nx, T, d.xlo, d.xhi) : -1;
-  const int s0 = d.src_off[tile], s1 = d.src_off[tile + 1];
-  const int r0 = d.rec_off[tile], r1 = d.rec_off[tile + 1];
   const long step = d.step + kstep;
   const int fs = d.fs;
   R a1[LM], a2[LM];
@@ -366,16 +458,34 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
     if (T) {
       if (xk >= 0) {
         const unsigned q = (unsigned)(j * 2 * T + xk) * E;
-        const R a = gld(d.ax, i * E), b = gld(d.bx, i * E), k = gld(d.kx, i * E);
-        exx = cpml_apply(d.psi[0], q, a, b, k, exx);
-        exz = cpml_apply(d.psi[1], q, a, b, k, exz);
+        if constexpr (HOIST) {
+          if constexpr (REC) {
+            gst((R*)(rs.base + rs.psi_off[0]), q, cp.px0[r]);
+            gst((R*)(rs.base + rs.psi_off[1]), q, cp.px1[r]);
+          }
+          exx = cpml_apply_pre(d.psi[0], q, cp.px0[r], cp.ax, cp.bx, cp.kx, exx);
+          exz = cpml_apply_pre(d.psi[1], q, cp.px1[r], cp.ax, cp.bx, cp.kx, exz);
+        } else {
+          const R a = gld(d.ax, i * E), b = gld(d.bx, i * E), k = gld(d.kx, i * E);
+          exx = cpml_apply(d.psi[0], q, a, b, k, exx);
+          exz = cpml_apply(d.psi[1], q, a, b, k, exz);
+        }
       }
       const int zk = strip_x(j, nz, T, d.zlo, d.zhi);
       if (zk >= 0) {
         const unsigned q = (unsigned)(zk * ldx + i) * E;
         const R a = d.az[j], b = d.bz[j], k = d.kz[j];
-        ezx = cpml_apply(d.psi[4], q, a, b, k, ezx);
-        ezz = cpml_apply(d.psi[5], q, a, b, k, ezz);
+        if constexpr (HOIST) {
+          if constexpr (REC) {
+            gst((R*)(rs.base + rs.psi_off[4]), q, cp.pz0[r]);
+            gst((R*)(rs.base + rs.psi_off[5]), q, cp.pz1[r]);
+          }
+          ezx = cpml_apply_pre(d.psi[4], q, cp.pz0[r], a, b, k, ezx);
+          ezz = cpml_apply_pre(d.psi[5], q, cp.pz1[r], a, b, k, ezz);
+        } else {
+          ezx = cpml_apply(d.psi[4], q, a, b, k, ezx);
+          ezz = cpml_apply(d.psi[5], q, a, b, k, ezz);
+        }
       }
     }
     const R gxz = ezx + exz;
@@ -398,6 +508,11 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
         SMzz = SMzz + om[L + k][h];
         SMxz = SMxz + om[2 * L + k][h];
       }
+      if constexpr (REC) {
+        gst(rec_plane<R>(rs, 5), cb, SMxx);
+        gst(rec_plane<R>(rs, 6), cb, SMzz);
+        gst(rec_plane<R>(rs, 7), cb, SMxz);
+      }
       if (fs_row) ezz = -(((l0 * exx) + l1 * SMxx) + pz1 * SMzz) / pz0;
       dsxx = px0 * exx + l0 * ezz + px1 * SMxx + l1 * SMzz;
       dszz = l0 * exx + pz0 * ezz + l1 * SMxx + pz1 * SMzz;
@@ -413,6 +528,15 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
       const R val = step < d.snt ? d.src_w[(size_t)d.src_row[s] * d.snt + step] : R(0);
       dsxx = dsxx + val;
       dszz = dszz + val;
+    }
+    if constexpr (REC) {  // the stress before the step, and its images (always -interior)
+      gst(rec_plane<R>(rs, 0), cb, osxx[h]);
+      gst(rec_plane<R>(rs, 1), cb, oszz[h]);
+      gst(rec_plane<R>(rs, 2), cb, osxz[h]);
+      if (fs) {
+        if (j >= 5 && j <= 8) gst(rec_plane<R>(rs, 1), (unsigned)((8 - j) * ldx + i) * E, -oszz[h]);
+        if (j >= 4 && j <= 7) gst(rec_plane<R>(rs, 2), (unsigned)((7 - j) * ldx + i) * E, -osxz[h]);
+      }
     }
     const R nsxx = osxx[h] + dt * dsxx;
     R nszz = oszz[h] + dt * dszz;
