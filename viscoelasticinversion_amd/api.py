@@ -417,11 +417,37 @@ def gradient_to_velocity_params(grads, rho, lam, mu, tau=None, Q=None):
     return out
 
 
+def gradient_to_thomsen_params(grads, rho, vp0, vs0, eps, delta, Q=None, tau=None):
+    """Chain rule from the gradient with respect to (rho, C11, C13, C33, C55[, tau]) of a VTI
+    model to (vp0, vs0, rho, eps, delta[, Q]) with C33 = rho vp0^2, C55 = rho vs0^2,
+    C11 = C33 (1 + 2 eps), C13 = sqrt((C33 - C55)(C33 (1 + 2 delta) - C55)) - C55
+    (thomsen_to_stiffness) and tau = L/(Kbar Q)."""
+    rho, vp0, vs0, eps, delta = (np.asarray(a, np.float64) for a in (rho, vp0, vs0, eps, delta))
+    g11, g13, g33, g55, gr = (np.asarray(grads[k], np.float64)
+                              for k in ("c11", "c13", "c33", "c55", "rho"))
+    c11, c13, c33, c55 = thomsen_to_stiffness(vp0, vs0, rho, eps, delta)
+    A = c33 - c55
+    B = c33 * (1.0 + 2.0 * delta) - c55
+    S = np.sqrt(A * B)
+    d13_d33 = (B + A * (1.0 + 2.0 * delta)) / (2.0 * S)
+    d13_d55 = -(A + B) / (2.0 * S) - 1.0
+    out = {"vp0": (g11 * (1.0 + 2.0 * eps) + g33 + g13 * d13_d33) * 2.0 * rho * vp0,
+           "vs0": (g55 + g13 * d13_d55) * 2.0 * rho * vs0,
+           "rho": gr + (g11 * c11 + g13 * c13 + g33 * c33 + g55 * c55) / rho,
+           "eps": g11 * 2.0 * c33,
+           "delta": g13 * A * c33 / S}
+    if Q is not None and tau is not None and "tau" in grads:
+        out["Q"] = np.asarray(grads["tau"], np.float64) * (-np.asarray(tau) / np.asarray(Q))
+    return out
+
+
 def misfit_and_pullback(parameters, dobs, tspan=(0.0, 1.0), checkpoint_every=0):
     """Forward solve and L2 trace misfit with a pullback for the model, the analogue of a
     ChainRulesCore rrule for J(model): returns (J, pullback) where pullback(dJ) gives
-    dJ * dJ/d(vp, vs, rho[, Q]) as arrays (Nz, Nx). The gradient comes from the discrete
-    adjoint on the GPU with sqrt(nt) checkpointing."""
+    dJ * dJ/d(vp, vs, rho[, Q]) as arrays (Nz, Nx) for an IsotropicModel, and
+    dJ * dJ/d(rho, C11, C13, C33, C55[, Q]) for a VTIModel (gradient_to_thomsen_params
+    chains those to Thomsen parameters). The gradient comes from the discrete adjoint on the
+    GPU with sqrt(nt) checkpointing."""
     p = parameters
     nt = int(round((tspan[1] - tspan[0]) / p.dt))
     if p.receivers is None:
@@ -442,7 +468,13 @@ def misfit_and_pullback(parameters, dobs, tspan=(0.0, 1.0), checkpoint_every=0):
     Qarr = None
     if q is not None:
         Qarr = q.N / (q.K * q.tau)
-    gv = gradient_to_velocity_params(g, m.rho, m.lam, m.mu, q.tau if q is not None else None, Qarr)
+    if isinstance(m, VTIModel):
+        gv = {k: np.asarray(g[k], np.float64) for k in ("rho", "c11", "c13", "c33", "c55")}
+        if q is not None:
+            gv["Q"] = np.asarray(g["tau"], np.float64) * (-q.tau / Qarr)
+    else:
+        gv = gradient_to_velocity_params(g, m.rho, m.lam, m.mu, q.tau if q is not None else None,
+                                         Qarr)
 
     def pullback(dJ=1.0):
         return {k: dJ * v for k, v in gv.items()}
