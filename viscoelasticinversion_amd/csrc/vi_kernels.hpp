@@ -386,8 +386,11 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
 #pragma unroll
   for (int k = 0; k < 3 * LM; ++k) Mp[k] = d.M + (size_t)k * d.plane;
   R wmu[RZ + 1], wmu1[RZ + 1];  // mu at columns i and i+1, rows jw .. jw+RZ
-  CpmlRows<R, HOIST ? RZ : 1> cp;
-  if constexpr (HOIST) cp.fetch(d, i, jw, i < d.nx - 4, 0, 1, 4, 5);
+  // CPML memory before the barrier: pays for 2-row tiles (measured); the 1-row variant
+  // keeps its registers for occupancy unless it records
+  constexpr bool PRE = REC || RZ == 2;
+  CpmlRows<R, PRE ? RZ : 1> cp;
+  if constexpr (PRE) cp.fetch(d, i, jw, i < d.nx - 4, 0, 1, 4, 5);
   const int s0 = d.src_off[tile], s1 = d.src_off[tile + 1];
   const int r0 = d.rec_off[tile], r1 = d.rec_off[tile + 1];
   const bool vti = d.vti;
@@ -458,7 +461,7 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
     if (T) {
       if (xk >= 0) {
         const unsigned q = (unsigned)(j * 2 * T + xk) * E;
-        if constexpr (HOIST) {
+        if constexpr (PRE) {
           if constexpr (REC) {
             gst((R*)(rs.base + rs.psi_off[0]), q, cp.px0[r]);
             gst((R*)(rs.base + rs.psi_off[1]), q, cp.px1[r]);
@@ -475,7 +478,7 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
       if (zk >= 0) {
         const unsigned q = (unsigned)(zk * ldx + i) * E;
         const R a = d.az[j], b = d.bz[j], k = d.kz[j];
-        if constexpr (HOIST) {
+        if constexpr (PRE) {
           if constexpr (REC) {
             gst((R*)(rs.base + rs.psi_off[4]), q, cp.pz0[r]);
             gst((R*)(rs.base + rs.psi_off[5]), q, cp.pz1[r]);
