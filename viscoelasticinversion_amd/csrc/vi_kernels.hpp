@@ -219,6 +219,36 @@ __device__ __forceinline__ R cpml_apply_pre(R* psi, unsigned pb, R p, R a, R b, 
   return d / k + q;
 }
 
+// The first 64 entries of a tile's source and receiver lists (CSR order), copied to LDS by
+// the first wavefront together with the tile loads, before the tile's barrier. The per-cell
+// loops then read them at uniform LDS addresses instead of making one dependent global
+// round trip per entry (which made the tiles with many receivers the slowest of a launch).
+// Entries past 64 are read from global memory.
+constexpr int LIST = 64;
+template <typename R>
+struct TileLists {
+  int src_pk[LIST];   // source cell * 4 + kind
+  R src_w[LIST];      // its sample of this step
+  int rec_cell[LIST];
+  int rec_code[LIST];  // receiver id * 8 + component
+};
+
+template <typename R>
+__device__ __forceinline__ void fill_lists(TileLists<R>& tl, const ShotDesc<R>& d, int s0,
+                                           int s1, int r0, int r1, long step, int lane, int w,
+                                           bool with_receivers) {
+  if (w != 0) return;
+  if (lane < s1 - s0) {
+    const int s = s0 + lane;
+    tl.src_pk[lane] = d.src_cell[s] * 4 + d.src_kind[s];
+    tl.src_w[lane] = step < d.snt ? d.src_w[(size_t)d.src_row[s] * d.snt + step] : R(0);
+  }
+  if (with_receivers && lane < r1 - r0) {
+    tl.rec_cell[lane] = d.rec_cell[r0 + lane];
+    tl.rec_code[lane] = d.rec_id[r0 + lane] * 8 + d.rec_comp[r0 + lane];
+  }
+}
+
 template <typename R, int RZ, bool REC = false>
 __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restrict__ shots,
                                                      const int* __restrict__ tile_shot,
@@ -257,6 +287,9 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
   CpmlRows<R, PRE ? RZ : 1> cp;
   if constexpr (PRE) cp.fetch(d, i, jw, i < d.nx - 4, 2, 3, 6, 7);
   const int s0 = d.src_off[tile], s1 = d.src_off[tile + 1];
+  const long step = d.step + kstep;
+  __shared__ TileLists<R> tl;
+  fill_lists(tl, d, s0, s1, 0, 0, step, lane, w, false);
   R wrho[RZ + 1], wrho1[RZ], ovx[RZ], ovz[RZ];
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
@@ -277,7 +310,6 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
   const R Dx[4] = {d.Dx[0], d.Dx[1], d.Dx[2], d.Dx[3]};
   const R Dz[4] = {d.Dz[0], d.Dz[1], d.Dz[2], d.Dz[3]};
   const int xk = T ? strip_x(i, nx, T, d.xlo, d.xhi) : -1;
-  const long step = d.step + kstep;
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     const int j = jw + r;
@@ -323,7 +355,15 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
     }
     R fx = dsxx_x + dsxz_z;
     R fz = dszz_z + dsxz_x;
-    for (int s = s0; s < s1; ++s) {
+    const unsigned cell = cb / E;
+    for (int q = 0; q < min(s1 - s0, LIST); ++q) {  // sources in CSR order
+      const int pk = tl.src_pk[q];
+      if ((unsigned)(pk >> 2) != cell) continue;
+      const R val = tl.src_w[q];
+      if ((pk & 3) == 0) fz = fz + val;
+      else if ((pk & 3) == 2) fx = fx + val;
+    }
+    for (int s = s0 + LIST; s < s1; ++s) {
       if ((unsigned)d.src_cell[s] * E != cb) continue;
       const int kind = d.src_kind[s];
       const R val = step < d.snt ? d.src_w[(size_t)d.src_row[s] * d.snt + step] : R(0);
@@ -393,6 +433,9 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   if constexpr (PRE) cp.fetch(d, i, jw, i < d.nx - 4, 0, 1, 4, 5);
   const int s0 = d.src_off[tile], s1 = d.src_off[tile + 1];
   const int r0 = d.rec_off[tile], r1 = d.rec_off[tile + 1];
+  const long step = d.step + kstep;
+  __shared__ TileLists<R> tl;
+  fill_lists(tl, d, s0, s1, r0, r1, step, lane, w, true);
   const bool vti = d.vti;
   R olam[RH], otau[RH], osxx[RH], oszz[RH], osxz[RH], om[3 * LM][RH], oc11[RH], oc33[RH];
   auto load_cell = [&](int r, int h) {
@@ -435,7 +478,6 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   const R Dx[4] = {d.Dx[0], d.Dx[1], d.Dx[2], d.Dx[3]};
   const R Dz[4] = {d.Dz[0], d.Dz[1], d.Dz[2], d.Dz[3]};
   const int xk = T ? strip_x(i, nx, T, d.xlo, d.xhi) : -1;
-  const long step = d.step + kstep;
   const int fs = d.fs;
   R a1[LM], a2[LM];
 #pragma unroll
@@ -526,7 +568,15 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
       dszz = lr * exx + pz * ezz;
       dsxz = mxz * gxz;
     }
-    for (int s = s0; s < s1; ++s) {
+    const unsigned cell = cb / E;
+    for (int q = 0; q < min(s1 - s0, LIST); ++q) {  // explosive sources in CSR order
+      const int pk = tl.src_pk[q];
+      if ((unsigned)(pk >> 2) != cell || (pk & 3) != 1) continue;
+      const R val = tl.src_w[q];
+      dsxx = dsxx + val;
+      dszz = dszz + val;
+    }
+    for (int s = s0 + LIST; s < s1; ++s) {
       if ((unsigned)d.src_cell[s] * E != cb || d.src_kind[s] != 1) continue;
       const R val = step < d.snt ? d.src_w[(size_t)d.src_row[s] * d.snt + step] : R(0);
       dsxx = dsxx + val;
@@ -560,7 +610,20 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
       if (j >= 5 && j <= 8) gst(szz, (unsigned)((8 - j) * ldx + i) * E, -nszz);
       if (j >= 4 && j <= 7) gst(sxz, (unsigned)((7 - j) * ldx + i) * E, -nsxz);
     }
-    for (int q = r0; q < r1; ++q) {
+    if (step < d.rec_nt) {
+      for (int q = 0; q < min(r1 - r0, LIST); ++q) {  // receivers of the tile, from LDS
+        if ((unsigned)tl.rec_cell[q] != cell) continue;
+        const int code = tl.rec_code[q];
+        const int comp = code & 7;
+        const R val = comp == 0   ? pvx[0]
+                      : comp == 1 ? pvz[0]
+                      : comp == 2 ? nsxx
+                      : comp == 3 ? nszz
+                                  : nsxz;
+        d.traces[(size_t)step * d.nrec + (code >> 3)] = val;
+      }
+    }
+    for (int q = r0 + LIST; q < r1; ++q) {
       if ((unsigned)d.rec_cell[q] * E != cb || step >= d.rec_nt) continue;
       const int comp = d.rec_comp[q];
       const R val = comp == 0   ? pvx[0]
