@@ -318,14 +318,12 @@ int apply_tiling(vi_sim_s* h, int rz) {
   return 0;
 }
 
-// Default rows per wave: the largest variant that still gives about 16 waves per CU.
+// Default rows per wave (measured on MI355X over 256^2 ... 2048^2 grids): one-row tiles while
+// the grid has fewer than 4096 of them (the launch is then one round of workgroups and
+// latency decides), two-row tiles above that (halo re-reads amortised).
 int default_rz(int nx, int nz) {
-  const long cols = (nx - 8 + vi::TX - 1) / vi::TX;
-  for (int rz : {8, 4, 2}) {
-    const long waves = cols * ((nz - 8 + rz - 1) / rz);
-    if (waves >= 256L * 16) return rz;
-  }
-  return 1;
+  const long tiles = (long)((nx - 8 + vi::TX - 1) / vi::TX) * ((nz - 8 + vi::NW - 1) / vi::NW);
+  return tiles >= 4096 ? 2 : 1;
 }
 
 template <typename R>
