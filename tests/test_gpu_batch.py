@@ -114,6 +114,34 @@ def test_batch_then_single_steps_and_membership_change():
     b.close()
 
 
+@pytest.mark.parametrize("graph", [0, 4])
+def test_member_stepped_alone_between_batch_steps(graph):
+    """A member advanced on its own between two batch steps of the same member set continues
+    from its own step index (the batch rebuilds its descriptors), with and without graphs."""
+    from viscoelasticinversion_amd import Batch
+    from viscoelasticinversion_amd.configs import build_multishot, shot_config
+    mcfg = _small_multishot(nshots=3, nt=30, precision_seed=6)
+    print(f"[case] member stepped alone between batch steps, graph={graph}", flush=True)
+    model, sims = build_multishot(mcfg, "float32")
+    b = Batch(sims)
+    b.set_graph(graph)
+    b.step(9)
+    sims[1].step(4)
+    b.step(9)
+    sims[0].step(3)
+    b.step(8)
+    b.sync()
+    for s, nt in enumerate((29, 30, 26)):
+        assert sims[s].step_index == nt
+        got = _collect(sims[s], nt, 3)
+        ref = run_oracle(shot_config(mcfg, s), np.float32, nt=nt)
+        compare(got, {k: ref[k] for k in got}, np.float32, f"alone-between shot {s}")
+    b.close()
+    for sim in sims:
+        sim.close()
+    model.close()
+
+
 def test_batch_autotune_keeps_state():
     from viscoelasticinversion_amd import Batch
     from viscoelasticinversion_amd.configs import build_multishot, shot_config

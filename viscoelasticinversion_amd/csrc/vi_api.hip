@@ -212,6 +212,7 @@ struct BatchGroup {
 struct vi_batch_s {
   std::vector<vi_sim_s*> members;
   std::vector<unsigned long> versions;
+  std::vector<long> steps;  // each member's step index as the batch descriptors hold it
   std::vector<std::unique_ptr<BatchGroup>> groups;
   bool built = false;
   int rz = 8;
@@ -636,7 +637,11 @@ int batch_build(vi_batch_s* b) {
     VI_HIP(hipStreamSynchronize(b->stream));
   }
   b->versions.resize(b->members.size());
-  for (size_t q = 0; q < b->members.size(); ++q) b->versions[q] = b->members[q]->version;
+  b->steps.resize(b->members.size());
+  for (size_t q = 0; q < b->members.size(); ++q) {
+    b->versions[q] = b->members[q]->version;
+    b->steps[q] = b->members[q]->step;
+  }
   b->built = true;
   return 0;
 }
@@ -653,8 +658,10 @@ int batch_launch(vi_batch_s* b, int nsteps) {
 
 int batch_prepare(vi_batch_s* b) {
   bool stale = !b->built;
+  // a member stepped on its own since the last batch step has another step index than the
+  // batch descriptors (sources and receivers would use the wrong time sample)
   for (size_t q = 0; !stale && q < b->members.size(); ++q)
-    stale = b->versions[q] != b->members[q]->version;
+    stale = b->versions[q] != b->members[q]->version || b->steps[q] != b->members[q]->step;
   if (stale) {
     for (auto* h : b->members) {
       VI_TRY(wait_external(h));
@@ -679,6 +686,7 @@ int batch_finish(vi_batch_s* b, int nsteps) {
   for (size_t q = 0; q < b->members.size(); ++q) {
     vi_sim_s* h = b->members[q];
     h->step += nsteps;
+    b->steps[q] = h->step;
     h->desc_dirty = true;  // its own descriptor holds the old step
     VI_HIP(hipEventRecord(h->ext_ev, b->stream));
     h->ext_pending = true;
