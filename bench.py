@@ -149,13 +149,15 @@ def bench_gradient(nshots=16, nt=3000, checkpoint_every=0):
     """Adjoint gradient workload: 1600 x 600, L=3, CPML + free surface, ``nshots`` shots
     with 301 receivers, ``nt`` steps, FP32. Observed data come from the true model; each
     shot's gradient (forward with sqrt(nt) checkpoints, segment recomputation, transposed
-    steps) runs on a simulation already resident on the device."""
+    steps; with checkpoint_every >= nt every step is recorded in the forward pass and nothing
+    is recomputed) runs on a simulation already resident on the device."""
     from viscoelasticinversion_amd.api import gradient_to_velocity_params
     from viscoelasticinversion_amd.configs import (adjoint_shot, build_simulation,
                                                    config_adjoint, observed_data)
     acfg = config_adjoint(nshots=nshots, nt=nt)
     dobs = observed_data(acfg, "float32")
     sims = [build_simulation(adjoint_shot(acfg, s, "start"), "float32") for s in range(nshots)]
+    slot_bytes = sims[0].gradient_bytes(nt, checkpoint_every)
     sims[0].gradient(dobs[0], nt, checkpoint_every)  # first-call set-up
     t0 = time.perf_counter()
     sims[0].reset()
@@ -182,8 +184,11 @@ def bench_gradient(nshots=16, nt=3000, checkpoint_every=0):
     for sim in sims:
         sim.close()
     cells = acfg["nx"] * acfg["nz"]
+    scheme = ("every step stored in the forward pass" if checkpoint_every >= nt
+              else f"checkpoints every {checkpoint_every or 'ceil(sqrt(nt))'} steps")
     return {"workload": f"{nshots} shots, {acfg['nx']}x{acfg['nz']}, 301 receivers, L=3, "
-                        f"CPML + free surface, {nt} steps, FP32, sqrt(nt) checkpointing",
+                        f"CPML + free surface, {nt} steps, FP32, {scheme}",
+            "device_bytes_per_gradient": slot_bytes,
             "wall_s": round(wall, 4), "shots_per_s": round(nshots / wall, 3),
             "gradient_s_per_shot_p50": round(float(np.percentile(per, 50)), 4),
             "gradient_s_per_shot_p99": round(float(np.percentile(per, 99)), 4),
@@ -288,6 +293,9 @@ def main():
     ap.add_argument("--no-multishot", action="store_true")
     ap.add_argument("--no-gradient", action="store_true", help="skip the adjoint-gradient leg")
     ap.add_argument("--gradient-shots", type=int, default=16)
+    ap.add_argument("--checkpoint-every", type=int, default=0,
+                    help="gradient checkpoint interval (0: ceil(sqrt(nt)); >= nt: every "
+                         "step stored, nothing recomputed)")
     ap.add_argument("--only-gradient", action="store_true",
                     help="run only the adjoint-gradient leg and print its JSON")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
@@ -325,7 +333,9 @@ def main():
             print(json.dumps({"serving": bench_serving(args.serving_jobs, r)}), flush=True)
         return
     if args.only_gradient:
-        print(json.dumps({"gradient": bench_gradient(args.gradient_shots)}), flush=True)
+        print(json.dumps({"gradient": bench_gradient(args.gradient_shots,
+                                                     checkpoint_every=args.checkpoint_every)}),
+              flush=True)
         return
     from viscoelasticinversion_amd.configs import build_simulation, config_layered_visco
     nt_total = args.warmup + 2 * args.steps

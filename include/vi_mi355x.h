@@ -157,8 +157,11 @@ int vi_device_bytes(vi_handle h, size_t* bytes);
  * zero initial state with the handle's sources and receivers (capacity >= nt); gradients of J
  * with respect to the handle's rho, lam, mu, tau arrays ([nz][nx], handle precision; any may
  * be NULL). Checkpoints every checkpoint_every steps (0 = ceil(sqrt(nt))) and recomputes each
- * segment, so device memory is about 2 sqrt(nt) states. The handle is left reset (zero state,
- * step 0). */
+ * segment but the last (recorded during the forward pass), so device memory is about
+ * 2 sqrt(nt) states. checkpoint_every >= nt stores every step's forward state during the
+ * forward pass and recomputes nothing (memory for nt steps; vi_gradient_bytes tells the
+ * size). The results do not depend on checkpoint_every. The handle is left reset (zero
+ * state, step 0). */
 int vi_gradient(vi_handle h, int nt, const void* dobs /*[nrec][nt]*/, int checkpoint_every,
                 double* misfit, void* g_rho, void* g_lam, void* g_mu, void* g_tau);
 /* the same for a VTI model (VTIModel, Models.jl:27-33): gradients with respect to rho, C11,

@@ -92,12 +92,13 @@ def test_gradient_matches_finite_differences(nrelax, fs, cpml):
 
 
 def test_gradient_checkpoint_interval_invariant():
-    """The gradient does not depend on the checkpoint interval (1, 7, sqrt(nt), nt)."""
+    """The gradient does not depend on the checkpoint interval (1, 7, sqrt(nt), and nt or more:
+    one segment, recorded during the forward pass without recomputation)."""
     cfg, dobs = _setup(2, True, True, seed=3)
     nt = cfg["nt"]
     print("[case] gradient checkpoint intervals", flush=True)
     res = []
-    for K in (1, 7, 0, nt):
+    for K in (1, 7, 0, nt, nt + 5):
         sim = _sim(cfg, cfg["model"])
         res.append(sim.gradient(dobs, nt, checkpoint_every=K))
         sim.close()
@@ -169,10 +170,11 @@ def test_workload_gradient_vp_vs_rho_q_at_random_points():
         print(f"  {key}: 20 points, worst relative difference {worst:.2e}", flush=True)
 
 
-@pytest.mark.parametrize("vti", [False, True])
-def test_chunked_gradient_job_equals_gradient(vti):
+@pytest.mark.parametrize("vti,K", [(False, 8), (True, 8), (False, 70)])
+def test_chunked_gradient_job_equals_gradient(vti, K):
     """The gradient as a chunked job (begin / advance by a few steps / end) is bit-identical
-    to the one-call gradient, and the handle is locked while the job owns it."""
+    to the one-call gradient (K = 70: every step recorded in the forward pass), and the handle
+    is locked while the job owns it."""
     from viscoelasticinversion_amd.configs import build_simulation
     from viscoelasticinversion_amd.stepper import GradientJob
     from viscoelasticinversion_amd import _lib
@@ -181,9 +183,9 @@ def test_chunked_gradient_job_equals_gradient(vti):
     sim.step(70)
     dobs = sim.traces(70) * 0.9
     sim.reset()
-    print(f"[case] chunked gradient vti={vti}", flush=True)
+    print(f"[case] chunked gradient vti={vti} K={K}", flush=True)
     J, g = sim.gradient(dobs, 70, checkpoint_every=8)
-    job = GradientJob(sim, dobs, 70, checkpoint_every=8)
+    job = GradientJob(sim, dobs, 70, checkpoint_every=K)
     with pytest.raises(_lib.ViError):
         sim.step(1)
     while not job.finished:

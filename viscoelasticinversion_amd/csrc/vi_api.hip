@@ -740,6 +740,8 @@ GradLayout grad_layout(int nt, int K, int L, size_t pb, size_t ldx, size_t esz,
 //   residual and misfit on the host -> for each segment, last first: restore its
 //   checkpoint, recompute it storing every step's forward state in the slots, then the
 //   transposed steps backwards -> finalisation of the accumulators.
+// The last segment is recorded into the slots while the forward pass computes it, so it is
+// never recomputed; with one segment (K >= nt) nothing is recomputed at all.
 // advance() enqueues at most max_steps forward, recomputed or transposed steps and never
 // waits for the device (it returns without work while the traces are still in flight), so
 // one host thread can interleave many gradients with other work on the GPU.
@@ -944,18 +946,36 @@ struct GradRun : vi_grad_s {
     return arz == 1 ? backward_kernels<1>() : backward_kernels<2>();
   }
 
+  vi::RecSlot slot_of(int k) const {
+    vi::RecSlot rs{};
+    rs.base = (char*)seg.p + (size_t)k * lay.slot_bytes;
+    rs.pb = pb;
+    for (int q = 0; q < 8; ++q) rs.psi_off[q] = lay.psi_off[q];
+    return rs;
+  }
+
   int advance(int max_steps) override {
     hipStream_t st = h->stream;
     int budget = max_steps;
     while (budget > 0 && ph != DONE) {
       if (ph == FWD) {
         const int len = std::min(K, nt - s * K);
-        if (n == 0) VI_TRY(copy_state(s, true));
-        const int m = std::min(len - n, budget);
-        VI_TRY(launch_steps(st, E, h->rz, L, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
-                            h->ntiles, 1, m, nullptr));
-        n += m;
-        budget -= m;
+        if (s == nck - 1) {
+          // last segment: record each step into its slot as it is computed (the descriptor
+          // step is s K here, advanced by the plain launches of the earlier segments)
+          for (; n < len && budget > 0; ++n, --budget) {
+            const vi::RecSlot rs = slot_of(n);
+            VI_TRY(launch_rec<R>(st, h->rz, L, 0, h->desc_dev.p, h->ntiles, n, rs));
+            VI_TRY(launch_rec<R>(st, h->rz, L, 1, h->desc_dev.p, h->ntiles, n, rs));
+          }
+        } else {
+          if (n == 0) VI_TRY(copy_state(s, true));
+          const int m = std::min(len - n, budget);
+          VI_TRY(launch_steps(st, E, h->rz, L, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
+                              h->ntiles, 1, m, nullptr));
+          n += m;
+          budget -= m;
+        }
         if (n == len) {
           ++s;
           n = 0;
@@ -978,8 +998,9 @@ struct GradRun : vi_grad_s {
             J += 0.5 * (double)res * (double)res;
           }
         VI_HIP(hipMemcpyAsync(resid.p, host_tr, (size_t)nt * nrec * E, hipMemcpyHostToDevice, st));
-        s = nck - 1;
-        ph = RESTORE;
+        s = nck - 1;  // its slots were filled by the forward pass
+        n = nt - 1;
+        ph = BACK;
       } else if (ph == RESTORE) {
         VI_TRY(copy_state(s, false));
         hipLaunchKernelGGL(vi::k_set_step<R>, dim3(1), dim3(64), 0, st,
@@ -991,10 +1012,7 @@ struct GradRun : vi_grad_s {
         const int n0 = s * K, n1 = std::min(n0 + K, nt);
         for (; n < n1 && budget > 0; ++n, --budget) {
           // both half steps record the step's forward state into its slot as they go
-          vi::RecSlot rs{};
-          rs.base = (char*)seg.p + (size_t)(n - n0) * lay.slot_bytes;
-          rs.pb = pb;
-          for (int q = 0; q < 8; ++q) rs.psi_off[q] = lay.psi_off[q];
+          const vi::RecSlot rs = slot_of(n - n0);
           VI_TRY(launch_rec<R>(st, h->rz, L, 0, h->desc_dev.p, h->ntiles, n - n0, rs));
           VI_TRY(launch_rec<R>(st, h->rz, L, 1, h->desc_dev.p, h->ntiles, n - n0, rs));
         }
