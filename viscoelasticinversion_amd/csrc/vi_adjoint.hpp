@@ -86,65 +86,118 @@ __device__ __forceinline__ R cpml_adj(R h, R* ph, R a, R b, R k) {
 
 // ---- adjoint of the stress half step (pointwise part of one interior cell) ------------
 // exx = D-x vx, ezx = D+z vx, ezz = D-z vz, exz = D+x vz: the raw forward rates of the cell
+// Everything one cell of the transposed stress half reads from memory: forward CPML state and
+// model, stored memory sums, and every value the cell reads back (adjoint state, memory-
+// variable adjoints, gradient accumulators, CPML adjoints, image rows). The kernels request it
+// for all rows of a wavefront before the tile barrier, so that the whole tile makes one memory
+// round trip: the planes are distinct, but the compiler cannot prove it and would otherwise
+// wait for each read-modify-write in turn. Only for interior cells.
 template <typename R, int L>
-__device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, int i, int j, R exx, R ezx,
-                                                R ezz, R exz) {
+struct AdjStressIn {
+  static constexpr int LM = L > 0 ? L : 1;
+  R axv = 0, bxv = 0, kxv = 1, azv = 0, bzv = 0, kzv = 1;
+  R fp0 = 0, fp1 = 0, fp4 = 0, fp5 = 0, p0h = 0, p1h = 0, p4h = 0, p5h = 0;
+  R lr, mr, mu1, mu2, mu3, c11v, c33v, t, SMxx = 0, SMzz = 0, SMxz = 0;
+  R sxxh, szz_c, sxz_c, szz_im, sxz_im;
+  R mh[3 * LM];
+  R acc_lam, acc_mu, acc_c33, acc_muxz, acc_tau;
+  __device__ __forceinline__ void load(const AdjDesc<R>& d, int i, int j) {
+    constexpr unsigned E = sizeof(R);
+    const int ldx = d.ldx;
+    const unsigned cb = (unsigned)(j * ldx + i) * E;
+    const unsigned mb = (unsigned)(j * d.ldm + i) * E;
+    const int xk = d.T ? strip_x(i, d.nx, d.T, d.xlo, d.xhi) : -1;
+    const int zk = d.T ? strip_x(j, d.nz, d.T, d.zlo, d.zhi) : -1;
+    const unsigned qx = (unsigned)(j * 2 * d.T + xk) * E, qz = (unsigned)(zk * ldx + i) * E;
+    if (xk >= 0) {
+      axv = gld(d.ax, i * E); bxv = gld(d.bx, i * E); kxv = gld(d.kx, i * E);
+      fp0 = gld(d.fpsi[0], qx); fp1 = gld(d.fpsi[1], qx);
+      p0h = gld((const R*)d.apsi[0], qx); p1h = gld((const R*)d.apsi[1], qx);
+    }
+    if (zk >= 0) {
+      azv = d.az[j]; bzv = d.bz[j]; kzv = d.kz[j];
+      fp4 = gld(d.fpsi[4], qz); fp5 = gld(d.fpsi[5], qz);
+      p4h = gld((const R*)d.apsi[4], qz); p5h = gld((const R*)d.apsi[5], qz);
+    }
+    lr = gld(d.lam, mb);
+    mr = gld(d.mu, mb);
+    mu1 = gld(d.mu, mb + E);
+    mu2 = gld(d.mu, mb + d.ldm * E);
+    mu3 = gld(d.mu, mb + d.ldm * E + E);
+    c11v = d.vti ? gld(d.c11, mb) : R(0);
+    c33v = d.vti ? gld(d.c33, mb) : R(0);
+    t = L > 0 ? gld(d.tau, mb) : R(0);
+    if (L > 0) {
+      SMxx = gld(d.fM, cb);
+      SMzz = gld(d.fM + d.plane, cb);
+      SMxz = gld(d.fM + 2 * d.plane, cb);
+    }
+    sxxh = gld((const R*)d.asxx, cb);
+    szz_c = gld((const R*)d.aszz, cb);
+    sxz_c = gld((const R*)d.asxz, cb);
+    const bool imz = d.fs && j >= 5 && j <= 8, imx = d.fs && j >= 4 && j <= 7;
+    szz_im = imz ? gld((const R*)d.aszz, (unsigned)((8 - j) * ldx + i) * E) : R(0);
+    sxz_im = imx ? gld((const R*)d.asxz, (unsigned)((7 - j) * ldx + i) * E) : R(0);
+    for (int k = 0; k < 3 * L; ++k) mh[k] = gld((const R*)(d.aM + (size_t)k * d.plane), cb);
+    acc_lam = gld((const R*)d.g_lam, cb);
+    acc_mu = gld((const R*)d.g_mu, cb);
+    acc_c33 = d.vti ? gld((const R*)d.g_c33, cb) : R(0);
+    acc_muxz = gld((const R*)d.g_muxz, cb);
+    acc_tau = L > 0 ? gld((const R*)d.g_tau, cb) : R(0);
+  }
+};
+
+template <typename R, int L>
+__device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, const AdjStressIn<R, L>& in,
+                                                int i, int j, R exx, R ezx, R ezz, R exz) {
   constexpr unsigned E = sizeof(R);
   constexpr int LM = L > 0 ? L : 1;
   const int ldx = d.ldx;
   const unsigned cb = (unsigned)(j * ldx + i) * E;
-  const unsigned mb = (unsigned)(j * d.ldm + i) * E;
   const R dt = d.dt, two = R(2), quarter = R(0.25);
-  // forward strain rates of this step (CPML-stretched)
   const int xk = d.T ? strip_x(i, d.nx, d.T, d.xlo, d.xhi) : -1;
   const int zk = d.T ? strip_x(j, d.nz, d.T, d.zlo, d.zhi) : -1;
   const unsigned qx = (unsigned)(j * 2 * d.T + xk) * E, qz = (unsigned)(zk * ldx + i) * E;
-  R axv = 0, bxv = 0, kxv = 1, azv = 0, bzv = 0, kzv = 1;
+  const bool fs_row = d.fs && j == 4;
+  const R axv = in.axv, bxv = in.bxv, kxv = in.kxv, azv = in.azv, bzv = in.bzv, kzv = in.kzv;
+  R p0h = in.p0h, p1h = in.p1h, p4h = in.p4h, p5h = in.p5h;
+  const R lr = in.lr, mr = in.mr, t = in.t, SMxx = in.SMxx, SMzz = in.SMzz, SMxz = in.SMxz;
+  const R sxxh = in.sxxh;
+  R szz_c = in.szz_c, sxz_c = in.sxz_c;
+  const bool imz = d.fs && j >= 5 && j <= 8, imx = d.fs && j >= 4 && j <= 7;
+  const unsigned ibz = (unsigned)((8 - j) * ldx + i) * E, ibx = (unsigned)((7 - j) * ldx + i) * E;
+  R* Ma[3 * LM];
+  for (int k = 0; k < 3 * L; ++k) Ma[k] = d.aM + (size_t)k * d.plane;
+  // ---- forward strain rates of this step (CPML-stretched)
   if (xk >= 0) {
-    axv = gld(d.ax, i * E); bxv = gld(d.bx, i * E); kxv = gld(d.kx, i * E);
-    exx = cpml_fwd(exx, gld(d.fpsi[0], qx), axv, bxv, kxv);
-    exz = cpml_fwd(exz, gld(d.fpsi[1], qx), axv, bxv, kxv);
+    exx = cpml_fwd(exx, in.fp0, axv, bxv, kxv);
+    exz = cpml_fwd(exz, in.fp1, axv, bxv, kxv);
   }
   if (zk >= 0) {
-    azv = d.az[j]; bzv = d.bz[j]; kzv = d.kz[j];
-    ezx = cpml_fwd(ezx, gld(d.fpsi[4], qz), azv, bzv, kzv);
-    ezz = cpml_fwd(ezz, gld(d.fpsi[5], qz), azv, bzv, kzv);
+    ezx = cpml_fwd(ezx, in.fp4, azv, bzv, kzv);
+    ezz = cpml_fwd(ezz, in.fp5, azv, bzv, kzv);
   }
   const R gxz = ezx + exz;
-  // moduli: lam | C13, mu | C55 (averaged to the sxz node), C11, C33 (lam + 2 mu if isotropic)
-  const R lr = gld(d.lam, mb), mr = gld(d.mu, mb);
-  const R mxz = quarter * (((mr + gld(d.mu, mb + E)) + gld(d.mu, mb + d.ldm * E)) +
-                           gld(d.mu, mb + d.ldm * E + E));
-  const R px = d.vti ? gld(d.c11, mb) : lr + two * mr;
-  const R pz = d.vti ? gld(d.c33, mb) : px;
-  const R t = L > 0 ? gld(d.tau, mb) : R(0);
+  const R mxz = quarter * (((mr + in.mu1) + in.mu2) + in.mu3);
+  const R px = d.vti ? in.c11v : lr + two * mr;
+  const R pz = d.vti ? in.c33v : px;
   const R px1 = px * t, px0 = px1 + px, pz1 = pz * t, pz0 = pz1 + pz;
   const R l1 = lr * t, l0 = l1 + lr, m1 = mxz * t, m0 = m1 + mxz;
-  R SMxx = 0, SMzz = 0, SMxz = 0;
-  if (L > 0) {
-    SMxx = gld(d.fM, cb);
-    SMzz = gld(d.fM + d.plane, cb);
-    SMxz = gld(d.fM + 2 * d.plane, cb);
-  }
-  const bool fs_row = d.fs && j == 4;
   const R N = l0 * exx + l1 * SMxx + pz1 * SMzz;  // free-surface row: ezz* = -N/C33_0
   const R ezs = fs_row ? -N / pz0 : ezz;
   // stress images: their adjoint goes to the interior rows they copy, then they are cleared
-  if (d.fs) {
-    if (j >= 5 && j <= 8) {
-      const unsigned ib = (unsigned)((8 - j) * ldx + i) * E;
-      gst(d.aszz, cb, gld((const R*)d.aszz, cb) - gld((const R*)d.aszz, ib));
-      gst(d.aszz, ib, R(0));
-    }
-    if (j >= 4 && j <= 7) {
-      const unsigned ib = (unsigned)((7 - j) * ldx + i) * E;
-      gst(d.asxz, cb, gld((const R*)d.asxz, cb) - gld((const R*)d.asxz, ib));
-      gst(d.asxz, ib, R(0));
-    }
+  if (imz) {
+    szz_c = szz_c - in.szz_im;
+    gst(d.aszz, cb, szz_c);
+    gst(d.aszz, ibz, R(0));
   }
-  const R sxxh = gld((const R*)d.asxx, cb);
-  const R szzh = fs_row ? R(0) : gld((const R*)d.aszz, cb);
-  const R sxzh = gld((const R*)d.asxz, cb);
+  if (imx) {
+    sxz_c = sxz_c - in.sxz_im;
+    gst(d.asxz, cb, sxz_c);
+    gst(d.asxz, ibx, R(0));
+  }
+  const R szzh = fs_row ? R(0) : szz_c;
+  const R sxzh = sxz_c;
   if (fs_row) gst(d.aszz, cb, R(0));
   // adjoints of the (stretched) rates and of the memory sums
   R exh = dt * (px0 * sxxh + l0 * szzh);
@@ -154,20 +207,18 @@ __device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, int i, int 
   const R SMzzh = dt * (l1 * sxxh + pz1 * szzh);
   const R SMxzh = dt * (m1 * sxzh);
   R Mzzh_sum = 0;  // sum_n a1_n * M_hat'_zz,n (free-surface gradient term)
-  R* Ma[3 * LM];
-  for (int k = 0; k < 3 * L; ++k) Ma[k] = d.aM + (size_t)k * d.plane;
   R fsMxx = 0, fsMzz = 0;  // extra adjoint of each M_n through ezz* on the free-surface row
   R ezh_from_M = 0;
+  R nm[3 * LM];  // the new memory-variable adjoints
   for (int k = 0; k < L; ++k) {
-    const R hxx = gld((const R*)Ma[k], cb), hzz = gld((const R*)Ma[L + k], cb),
-            hxz = gld((const R*)Ma[2 * L + k], cb);
+    const R hxx = in.mh[k], hzz = in.mh[L + k], hxz = in.mh[2 * L + k];
     exh = exh + dt * d.a1[k] * hxx;
     ezh_from_M = ezh_from_M + dt * d.a1[k] * hzz;
     gh = gh + dt * d.a1[k] * hxz;
     Mzzh_sum = Mzzh_sum + d.a1[k] * hzz;
-    gst(Ma[k], cb, (R(1) + dt * d.a2[k]) * hxx + SMxxh);
-    gst(Ma[L + k], cb, (R(1) + dt * d.a2[k]) * hzz + SMzzh);
-    gst(Ma[2 * L + k], cb, (R(1) + dt * d.a2[k]) * hxz + SMxzh);
+    nm[k] = (R(1) + dt * d.a2[k]) * hxx + SMxxh;
+    nm[L + k] = (R(1) + dt * d.a2[k]) * hzz + SMzzh;
+    nm[2 * L + k] = (R(1) + dt * d.a2[k]) * hxz + SMxzh;
   }
   ezh = ezh + ezh_from_M;
   R ezz_raw_h = ezh;
@@ -176,11 +227,12 @@ __device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, int i, int 
     fsMxx = -l1 / pz0 * ezh;
     fsMzz = -pz1 / pz0 * ezh;
     for (int k = 0; k < L; ++k) {
-      gst(Ma[k], cb, gld((const R*)Ma[k], cb) + fsMxx);
-      gst(Ma[L + k], cb, gld((const R*)Ma[L + k], cb) + fsMzz);
+      nm[k] = nm[k] + fsMxx;
+      nm[L + k] = nm[L + k] + fsMzz;
     }
     ezz_raw_h = 0;
   }
+  for (int k = 0; k < 3 * L; ++k) gst(Ma[k], cb, nm[k]);
   // gradient with respect to C11, C13, C33 (lam and mu through them when isotropic),
   // C55 at the sxz node and tau of this cell
   {
@@ -208,27 +260,25 @@ __device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, int i, int 
     }
     gtau = gtau + sxzh * dt * (mxz * gxz + mxz * SMxz);
     if (d.vti) {
-      gst(d.g_lam, cb, gld((const R*)d.g_lam, cb) + g13);
-      gst(d.g_mu, cb, gld((const R*)d.g_mu, cb) + g11);
-      gst(d.g_c33, cb, gld((const R*)d.g_c33, cb) + g33);
+      gst(d.g_lam, cb, in.acc_lam + g13);
+      gst(d.g_mu, cb, in.acc_mu + g11);
+      gst(d.g_c33, cb, in.acc_c33 + g33);
     } else {  // C11 = C33 = lam + 2 mu, C13 = lam
-      gst(d.g_lam, cb, gld((const R*)d.g_lam, cb) + ((g11 + g33) + g13));
-      gst(d.g_mu, cb, gld((const R*)d.g_mu, cb) + two * (g11 + g33));
+      gst(d.g_lam, cb, in.acc_lam + ((g11 + g33) + g13));
+      gst(d.g_mu, cb, in.acc_mu + two * (g11 + g33));
     }
-    gst(d.g_muxz, cb, gld((const R*)d.g_muxz, cb) + sxzh * dt * (tt * gxz + t * SMxz));
-    if (L > 0) gst(d.g_tau, cb, gld((const R*)d.g_tau, cb) + gtau);
+    gst(d.g_muxz, cb, in.acc_muxz + sxzh * dt * (tt * gxz + t * SMxz));
+    if (L > 0) gst(d.g_tau, cb, in.acc_tau + gtau);
   }
   // CPML adjoint and the adjoints of the raw derivatives
   R exx_r = exh, exz_r = gh, ezx_r = gh, ezz_r = ezz_raw_h;
   if (xk >= 0) {
-    R p0h = gld((const R*)d.apsi[0], qx), p1h = gld((const R*)d.apsi[1], qx);
     exx_r = cpml_adj(exh, &p0h, axv, bxv, kxv);
     exz_r = cpml_adj(gh, &p1h, axv, bxv, kxv);
     gst(d.apsi[0], qx, p0h);
     gst(d.apsi[1], qx, p1h);
   }
   if (zk >= 0) {
-    R p4h = gld((const R*)d.apsi[4], qz), p5h = gld((const R*)d.apsi[5], qz);
     ezx_r = cpml_adj(gh, &p4h, azv, bzv, kzv);
     ezz_r = cpml_adj(ezz_raw_h, &p5h, azv, bzv, kzv);
     gst(d.apsi[4], qz, p4h);
@@ -243,52 +293,87 @@ __device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, int i, int 
 // ---- adjoint of the velocity half step (pointwise part of one interior cell) ----------
 // dxx = D+x sxx, dxzz = D-z sxz, dxzx = D-x sxz, dzz = D+z szz: the raw forward derivatives;
 // vxh, vzh: the velocity adjoint of the cell
+// Everything one cell of the transposed velocity half reads from memory (see AdjStressIn),
+// including the velocity adjoint that the gather updates.
 template <typename R>
-__device__ __forceinline__ void adj_vel_cell(const AdjDesc<R>& d, int i, int j, R dxx, R dxzz,
-                                             R dxzx, R dzz, R vxh, R vzh) {
+struct AdjVelIn {
+  R axv = 0, bxv = 0, kxv = 1, azv = 0, bzv = 0, kzv = 1;
+  R fp2 = 0, fp3 = 0, fp6 = 0, fp7 = 0, p2h = 0, p3h = 0, p6h = 0, p7h = 0;
+  R rho, rho1, rho2, acc_bx, acc_bz, avx, avz;
+  __device__ __forceinline__ void load(const AdjDesc<R>& d, int i, int j) {
+    constexpr unsigned E = sizeof(R);
+    const int ldx = d.ldx;
+    const unsigned cb = (unsigned)(j * ldx + i) * E;
+    const unsigned mb = (unsigned)(j * d.ldm + i) * E;
+    const int xk = d.T ? strip_x(i, d.nx, d.T, d.xlo, d.xhi) : -1;
+    const int zk = d.T ? strip_x(j, d.nz, d.T, d.zlo, d.zhi) : -1;
+    const unsigned qx = (unsigned)(j * 2 * d.T + xk) * E, qz = (unsigned)(zk * ldx + i) * E;
+    if (xk >= 0) {
+      axv = gld(d.ax, i * E); bxv = gld(d.bx, i * E); kxv = gld(d.kx, i * E);
+      fp2 = gld(d.fpsi[2], qx); fp3 = gld(d.fpsi[3], qx);
+      p2h = gld((const R*)d.apsi[2], qx); p3h = gld((const R*)d.apsi[3], qx);
+    }
+    if (zk >= 0) {
+      azv = d.az[j]; bzv = d.bz[j]; kzv = d.kz[j];
+      fp6 = gld(d.fpsi[6], qz); fp7 = gld(d.fpsi[7], qz);
+      p6h = gld((const R*)d.apsi[6], qz); p7h = gld((const R*)d.apsi[7], qz);
+    }
+    rho = gld(d.rho, mb);
+    rho1 = gld(d.rho, mb + E);
+    rho2 = gld(d.rho, mb + d.ldm * E);
+    acc_bx = gld((const R*)d.g_bx, cb);
+    acc_bz = gld((const R*)d.g_bz, cb);
+    avx = gld((const R*)d.avx, cb);
+    avz = gld((const R*)d.avz, cb);
+  }
+};
+
+template <typename R>
+__device__ __forceinline__ void adj_vel_cell(const AdjDesc<R>& d, const AdjVelIn<R>& in, int i,
+                                             int j, R dxx, R dxzz, R dxzx, R dzz, R vxh,
+                                             R vzh) {
   constexpr unsigned E = sizeof(R);
   const int ldx = d.ldx;
   const unsigned cb = (unsigned)(j * ldx + i) * E;
-  const unsigned mb = (unsigned)(j * d.ldm + i) * E;
   const R dt = d.dt, two = R(2);
   const int xk = d.T ? strip_x(i, d.nx, d.T, d.xlo, d.xhi) : -1;
   const int zk = d.T ? strip_x(j, d.nz, d.T, d.zlo, d.zhi) : -1;
   const unsigned qx = (unsigned)(j * 2 * d.T + xk) * E, qz = (unsigned)(zk * ldx + i) * E;
-  R axv = 0, bxv = 0, kxv = 1, azv = 0, bzv = 0, kzv = 1;
+  const R axv = in.axv, bxv = in.bxv, kxv = in.kxv, azv = in.azv, bzv = in.bzv, kzv = in.kzv;
+  const R fp2 = in.fp2, fp3 = in.fp3, fp6 = in.fp6, fp7 = in.fp7;
+  R p2h = in.p2h, p3h = in.p3h, p6h = in.p6h, p7h = in.p7h;
+  const R rho = in.rho, rho1 = in.rho1, rho2 = in.rho2;
+  const R acc_bx = in.acc_bx, acc_bz = in.acc_bz;
   if (xk >= 0) {
-    axv = gld(d.ax, i * E); bxv = gld(d.bx, i * E); kxv = gld(d.kx, i * E);
-    dxx = cpml_fwd(dxx, gld(d.fpsi[2], qx), axv, bxv, kxv);
-    dxzx = cpml_fwd(dxzx, gld(d.fpsi[3], qx), axv, bxv, kxv);
+    dxx = cpml_fwd(dxx, fp2, axv, bxv, kxv);
+    dxzx = cpml_fwd(dxzx, fp3, axv, bxv, kxv);
   }
   if (zk >= 0) {
-    azv = d.az[j]; bzv = d.bz[j]; kzv = d.kz[j];
-    dxzz = cpml_fwd(dxzz, gld(d.fpsi[6], qz), azv, bzv, kzv);
-    dzz = cpml_fwd(dzz, gld(d.fpsi[7], qz), azv, bzv, kzv);
+    dxzz = cpml_fwd(dxzz, fp6, azv, bzv, kzv);
+    dzz = cpml_fwd(dzz, fp7, azv, bzv, kzv);
   }
   R fx = dxx + dxzz, fz = dzz + dxzx;
-  for (int s = 0; s < d.nsrc; ++s) {
-    if ((unsigned)d.src_cell[s] * E != cb) continue;
+  for (int s = 0; s < d.nsrc; ++s) {  // source list: wave-uniform scalar loads
+    if ((unsigned)sld(d.src_cell, s) * E != cb) continue;
     const R val = d.step < d.snt ? d.src_w[(size_t)s * d.snt + d.step] : R(0);
-    if (d.src_kind[s] == 0) fz = fz + val;
-    else if (d.src_kind[s] == 2) fx = fx + val;
+    const int kind = sld(d.src_kind, s);
+    if (kind == 0) fz = fz + val;
+    else if (kind == 2) fx = fx + val;
   }
-  const R rho = gld(d.rho, mb);
-  const R bxb = two / (rho + gld(d.rho, mb + E));
-  const R bzb = two / (rho + gld(d.rho, mb + d.ldm * E));
+  const R bxb = two / (rho + rho1);
+  const R bzb = two / (rho + rho2);
   // gradient with respect to the buoyancies (scattered to rho at the end)
-  gst(d.g_bx, cb, gld((const R*)d.g_bx, cb) + vxh * dt * fx);
-  gst(d.g_bz, cb, gld((const R*)d.g_bz, cb) + vzh * dt * fz);
+  gst(d.g_bx, cb, acc_bx + vxh * dt * fx);
+  gst(d.g_bz, cb, acc_bz + vzh * dt * fz);
   const R hx = dt * bxb * vxh, hz = dt * bzb * vzh;
   R hxx = hx, hxzz = hx, hzz = hz, hxzx = hz;
   if (xk >= 0) {
-    R p2h = gld((const R*)d.apsi[2], qx), p3h = gld((const R*)d.apsi[3], qx);
     hxx = cpml_adj(hx, &p2h, axv, bxv, kxv);
     hxzx = cpml_adj(hz, &p3h, axv, bxv, kxv);
     gst(d.apsi[2], qx, p2h);
     gst(d.apsi[3], qx, p3h);
   }
   if (zk >= 0) {
-    R p6h = gld((const R*)d.apsi[6], qz), p7h = gld((const R*)d.apsi[7], qz);
     hxzz = cpml_adj(hx, &p6h, azv, bzv, kzv);
     hzz = cpml_adj(hz, &p7h, azv, bzv, kzv);
     gst(d.apsi[6], qz, p6h);
@@ -318,10 +403,16 @@ __global__ __launch_bounds__(TX* NW) void k_adj_stress(const AdjDesc<R> d) {
   TileLoad<R, TZ + 8, 4> t_vx, t_vz;
   t_vx.fetch(d.fvx, j0 - 4, i0, d.ldx, lane, w);
   t_vz.fetch(d.fvz, j0 - 4, i0, d.ldx, lane, w);
+  const int i = i0 + lane;
+  AdjStressIn<R, L> in[RZ];
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const int j = j0 + w * RZ + r;
+    if (i < d.nx - 4 && j < d.nz - 4) in[r].load(d, i, j);
+  }
   t_vx.store(s_vx, lane, w);
   t_vz.store(s_vz, lane, w);
   __syncthreads();
-  const int i = i0 + lane;
   if (i >= d.nx - 4) return;
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
@@ -329,8 +420,8 @@ __global__ __launch_bounds__(TX* NW) void k_adj_stress(const AdjDesc<R> d) {
     if (j >= d.nz - 4) return;
     const R* pvx = &s_vx[(w * RZ + r + 4) * WX + lane + 4];
     const R* pvz = &s_vz[(w * RZ + r + 4) * WX + lane + 4];
-    adj_stress_cell<R, L>(d, i, j, op_m(pvx, 1, d.Dx), op_p(pvx, WX, d.Dz), op_m(pvz, WX, d.Dz),
-                          op_p(pvz, 1, d.Dx));
+    adj_stress_cell<R, L>(d, in[r], i, j, op_m(pvx, 1, d.Dx), op_p(pvx, WX, d.Dz),
+                          op_m(pvz, WX, d.Dz), op_p(pvz, 1, d.Dx));
   }
 }
 
@@ -362,6 +453,13 @@ __global__ __launch_bounds__(TX* NW) void k_adj_vel(const AdjDesc<R> d) {
   t_xx.fetch(d.fsxx, j0, i0, ldx, lane, w);
   t_zz.fetch(d.fszz, j0 - 4, i0, ldx, lane, w);
   t_xz.fetch(d.fsxz, j0 - 4, i0, ldx, lane, w);
+  const int i = i0 + lane;
+  AdjVelIn<R> in[RZ];
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const int j = j0 + w * RZ + r;
+    if (i < d.nx - 4 && j < d.nz - 4) in[r].load(d, i, j);
+  }
   t_e0.store(s_e0, lane, w);
   t_e3.store(s_e3, lane, w);
   t_e1.store(s_e1, lane, w);
@@ -370,7 +468,6 @@ __global__ __launch_bounds__(TX* NW) void k_adj_vel(const AdjDesc<R> d) {
   t_zz.store(s_zz, lane, w);
   t_xz.store(s_xz, lane, w);
   __syncthreads();
-  const int i = i0 + lane;
   if (i >= d.nx - 4) return;
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
@@ -383,13 +480,14 @@ __global__ __launch_bounds__(TX* NW) void k_adj_vel(const AdjDesc<R> d) {
                   op_m(&s_e1[(tr + 4) * TX + lane], TX, d.Dz);
     const R gvz = op_p(&s_e2[(tr + 4) * TX + lane], TX, d.Dz) +
                   op_m(&s_e3[tr * WX + lane + 4], 1, d.Dx);
-    const R vxh = gld((const R*)d.avx, cb) - gvx;
-    const R vzh = gld((const R*)d.avz, cb) - gvz;
+    const R vxh = in[r].avx - gvx;
+    const R vzh = in[r].avz - gvz;
     gst(d.avx, cb, vxh);
     gst(d.avz, cb, vzh);
     const R* pxz = &s_xz[(tr + 4) * WX + lane + 4];
-    adj_vel_cell<R>(d, i, j, op_p(&s_xx[tr * WX + lane + 4], 1, d.Dx), op_m(pxz, WX, d.Dz),
-                    op_m(pxz, 1, d.Dx), op_p(&s_zz[(tr + 4) * TX + lane], TX, d.Dz), vxh, vzh);
+    adj_vel_cell<R>(d, in[r], i, j, op_p(&s_xx[tr * WX + lane + 4], 1, d.Dx),
+                    op_m(pxz, WX, d.Dz), op_m(pxz, 1, d.Dx),
+                    op_p(&s_zz[(tr + 4) * TX + lane], TX, d.Dz), vxh, vzh);
   }
 }
 
@@ -418,12 +516,23 @@ __global__ __launch_bounds__(TX* NW) void k_adj_sgather(const AdjDesc<R> d, int 
   t_d2.fetch(d.dhat[2] - back, j0 + 4, i0, ldx, lane, w);
   t_d1.fetch(d.dhat[1] - back, j0, i0, ldx, lane, w);
   t_d3.fetch(d.dhat[3] - back, j0, i0, ldx, lane, w);
+  const int i = i0 + lane;
+  R oxx[RZ], oxz[RZ], ozz[RZ];  // the stress adjoint of the wave's cells, before the barrier
+#pragma unroll
+  for (int r = 0; r < RZ; ++r) {
+    const int j = j0 + w * RZ + r;
+    if (i < d.nx - 4 && j < d.nz - 4) {
+      const unsigned cb = (unsigned)(j * ldx + i) * E;
+      oxx[r] = gld((const R*)d.asxx, cb);
+      oxz[r] = gld((const R*)d.asxz, cb);
+      ozz[r] = gld((const R*)d.aszz, cb);
+    }
+  }
   t_d0.store(s_d0, lane, w);
   t_d2.store(s_d2, lane, w);
   t_d1.store(s_d1, lane, w);
   t_d3.store(s_d3, lane, w);
   __syncthreads();
-  const int i = i0 + lane;
   if (i >= d.nx - 4) return;
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
@@ -435,9 +544,9 @@ __global__ __launch_bounds__(TX* NW) void k_adj_sgather(const AdjDesc<R> d, int 
     const R gsxz = op_p(&s_d1[(tr + 4) * TX + lane], TX, d.Dz) +
                    op_p(&s_d2[tr * WX + lane + 4], 1, d.Dx);
     const R gszz = op_m(&s_d3[(tr + 4) * TX + lane], TX, d.Dz);
-    gst(d.asxx, cb, gld((const R*)d.asxx, cb) - gsxx);
-    gst(d.asxz, cb, gld((const R*)d.asxz, cb) - gsxz);
-    gst(d.aszz, cb, gld((const R*)d.aszz, cb) - gszz);
+    gst(d.asxx, cb, oxx[r] - gsxx);
+    gst(d.asxz, cb, oxz[r] - gsxz);
+    gst(d.aszz, cb, ozz[r] - gszz);
   }
 }
 

@@ -180,6 +180,8 @@ struct vi_sim_s {
   DevBuf psi[8];
   DevBuf coef;  // ax bx kx (nx each), az bz kz (nz each)
   DevBuf src_tab, src_w, rec_tab, traces;
+  DevBuf src_wc;                   // the wavelets [snt][nsrc] in the CSR order of src_tab
+  std::vector<char> src_w_host;    // the wavelets as given [nsrc][snt]
   double taun[vi::MAXL] = {};
   bool relax_set = false;
   bool cpml_on = false;
@@ -274,8 +276,10 @@ int tile_of(const vi_sim_s* h, int ix, int iz) {
   return ((iz - 4) / (vi::NW * h->rz)) * h->ntx + (ix - 4) / vi::TX;
 }
 
-// Per-tile CSR table [off(ntiles+1) | cell(n) | a(n) | original index(n)], uploaded.
-int upload_table(vi_sim_s* h, DevBuf& buf, int n, const int* ix, const int* iz, const int* a) {
+// Per-tile CSR table [off(ntiles+1) | cell(n) | a(n) | original index(n)], uploaded; *order_out
+// (optional) receives the original index of each CSR entry.
+int upload_table(vi_sim_s* h, DevBuf& buf, int n, const int* ix, const int* iz, const int* a,
+                 std::vector<int>* order_out = nullptr) {
   std::vector<int> order(n);
   for (int q = 0; q < n; ++q) order[q] = q;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -298,6 +302,22 @@ int upload_table(vi_sim_s* h, DevBuf& buf, int n, const int* ix, const int* iz, 
   VI_HIP(hipMemcpyAsync(buf.p, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice,
                         h->stream));
   VI_HIP(hipStreamSynchronize(h->stream));
+  if (order_out) *order_out = order;
+  return 0;
+}
+
+// The wavelet samples transposed to [snt][nsrc] with the sources in CSR order, so a tile's
+// list loads of one step are contiguous and need no source index first.
+int upload_src_wc(vi_sim_s* h, const std::vector<int>& order) {
+  const size_t E = h->esz, n = (size_t)h->nsrc, nt = (size_t)h->snt;
+  std::vector<char> t(n * nt * E);
+  for (size_t k = 0; k < n; ++k)
+    for (size_t st = 0; st < nt; ++st)
+      std::memcpy(t.data() + (st * n + k) * E, h->src_w_host.data() + ((size_t)order[k] * nt + st) * E, E);
+  VI_TRY(h->src_wc.alloc(t.size(), h->stream));
+  if (!t.empty())
+    VI_HIP(hipMemcpyAsync(h->src_wc.p, t.data(), t.size(), hipMemcpyHostToDevice, h->stream));
+  VI_HIP(hipStreamSynchronize(h->stream));
   return 0;
 }
 
@@ -311,8 +331,10 @@ int apply_tiling(vi_sim_s* h, int rz) {
   h->ntx = (h->g.nx - 8 + vi::TX - 1) / vi::TX;
   h->ntz = (h->g.nz - 8 + vi::NW * rz - 1) / (vi::NW * rz);
   h->ntiles = h->ntx * h->ntz;
+  std::vector<int> order;
   VI_TRY(upload_table(h, h->src_tab, h->nsrc, h->src_ix.data(), h->src_iz.data(),
-                      h->src_kind.data()));
+                      h->src_kind.data(), &order));
+  VI_TRY(upload_src_wc(h, order));
   VI_TRY(upload_table(h, h->rec_tab, h->nrec, h->rec_ix.data(), h->rec_iz.data(),
                       h->rec_comp.data()));
   touch(h);
@@ -386,6 +408,8 @@ void build_desc(const vi_sim_s* h, int tile_base, vi::ShotDesc<R>* d) {
   d->src_kind = st + (h->ntiles + 1) + h->nsrc;
   d->src_row = st + (h->ntiles + 1) + 2 * h->nsrc;
   d->src_w = (const R*)h->src_w.p;
+  d->src_wc = (const R*)h->src_wc.p;
+  d->nsrc = h->nsrc;
   d->snt = h->snt;
   const int* rt = (const int*)h->rec_tab.p;
   d->rec_off = rt;
@@ -1352,19 +1376,24 @@ int vi_set_sources(vi_handle h, int n, const int* ix, const int* iz, const int* 
                                  " is outside the interior [4, nx-4) x [4, nz-4)");
     if (kind[q] < 0 || kind[q] > 2) return fail(VI_EINVAL, "vi_set_sources: bad kind");
   }
+  const size_t wbytes = (size_t)n * nt * h->esz;
+  if (wbytes >= (size_t(1) << 31))
+    return fail(VI_EINVAL, "vi_set_sources: the wavelet table must stay below 2 GiB");
   VI_TRY(wait_external(h));
   VI_HIP(hipStreamSynchronize(h->stream));
   h->src_ix.assign(ix, ix + n);
   h->src_iz.assign(iz, iz + n);
   h->src_kind.assign(kind, kind + n);
   h->nsrc = n;
-  VI_TRY(upload_table(h, h->src_tab, n, ix, iz, kind));
-  VI_TRY(h->src_w.alloc((size_t)n * nt * h->esz, h->stream));
-  if (n && nt)
-    VI_HIP(hipMemcpyAsync(h->src_w.p, wavelets, (size_t)n * nt * h->esz,
-                          hipMemcpyHostToDevice, h->stream));
-  VI_HIP(hipStreamSynchronize(h->stream));
   h->snt = nt;
+  h->src_w_host.assign((const char*)wavelets, (const char*)wavelets + (n && nt ? wbytes : 0));
+  std::vector<int> order;
+  VI_TRY(upload_table(h, h->src_tab, n, ix, iz, kind, &order));
+  VI_TRY(upload_src_wc(h, order));
+  VI_TRY(h->src_w.alloc(wbytes, h->stream));
+  if (n && nt)
+    VI_HIP(hipMemcpyAsync(h->src_w.p, wavelets, wbytes, hipMemcpyHostToDevice, h->stream));
+  VI_HIP(hipStreamSynchronize(h->stream));
   touch(h);
   return 0;
 }
@@ -1504,7 +1533,7 @@ int vi_device_bytes(vi_handle h, size_t* bytes) {
   if (!bytes) return fail(VI_EINVAL, "vi_device_bytes: null argument");
   size_t t = 0;
   for (auto* b : mutable_arrays(h)) t += b->bytes;
-  t += h->coef.bytes + h->src_tab.bytes + h->src_w.bytes + h->rec_tab.bytes;
+  t += h->coef.bytes + h->src_tab.bytes + h->src_w.bytes + h->src_wc.bytes + h->rec_tab.bytes;
   t += h->desc_dev.bytes + h->tile_shot_dev.bytes;
   if (h->model && h->model.use_count() == 1)
     for (auto& b : h->model->arr) t += b.bytes;
@@ -1550,7 +1579,7 @@ int vi_plan_bytes(const vi_grid_desc* g, int vti, int cpml_thickness, int nsrc, 
   const size_t ntiles = ((nx - 8 + vi::TX - 1) / vi::TX) * ((nz - 8 + vi::NW - 1) / vi::NW);
   t += (2 * (ntiles + 1) + 3 * ((size_t)nsrc + nrec)) * sizeof(int) + ntiles * sizeof(int);
   t += std::max(desc_size(4), desc_size(8));
-  t += (size_t)nsrc * src_nt * esz + (size_t)nrec * rec_nt * esz;
+  t += 2 * (size_t)nsrc * src_nt * esz + (size_t)nrec * rec_nt * esz;  // wavelets twice
   if (grad_nt > 0)
     t += grad_layout(grad_nt, checkpoint_every, (int)L, pb, ldx, esz, live, nrec,
                      4 * (size_t)nrec + 1 + 2 * (size_t)nsrc).total();

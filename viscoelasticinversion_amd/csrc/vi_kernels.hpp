@@ -59,8 +59,9 @@ struct ShotDesc {
   R* psi[8];  // 0..3: [nz][2T] (vx_x, vz_x, sxx_x, sxz_x); 4..7: [2T][ldx] (vx_z, vz_z, sxz_z, szz_z)
   // sources / receivers grouped by tile (CSR over this shot's tiles)
   const int *src_off, *src_cell, *src_kind, *src_row;
-  const R* src_w;  // [nsrc][snt]
-  int snt;
+  const R* src_w;   // [nsrc][snt]
+  const R* src_wc;  // [snt][nsrc]: the same samples, sources in the CSR order of src_off
+  int snt, nsrc;
   const int *rec_off, *rec_cell, *rec_comp, *rec_id;
   R* traces;  // [rec_nt][nrec]
   int nrec, rec_nt;
@@ -121,6 +122,12 @@ __device__ __forceinline__ T gld(const T* base, unsigned boff) {
 template <typename T>
 __device__ __forceinline__ void gst(T* base, unsigned boff, T v) {
   *(__attribute__((address_space(1))) T*)((__attribute__((address_space(1))) char*)base + boff) = v;
+}
+
+// Load of a wave-uniform index through the scalar data cache (read-only data): waiting for it
+// counts on lgkmcnt only, so it does not drain the vector loads already in flight.
+__device__ __forceinline__ int sld(const int* p, int idx) {
+  return ((const __attribute__((address_space(4))) int*)p)[__builtin_amdgcn_readfirstlane(idx)];
 }
 
 // CPML on one derivative d of a strip cell: psi = b psi + a d; return d/kappa + psi.
@@ -233,21 +240,44 @@ struct TileLists {
   int rec_code[LIST];  // receiver id * 8 + component
 };
 
+// The list entries of one lane of the first wavefront, loaded into registers together with
+// the tile loads (no load depends on another one: the samples are stored in CSR order), and
+// written to LDS with the tiles.
 template <typename R>
-__device__ __forceinline__ void fill_lists(TileLists<R>& tl, const ShotDesc<R>& d, int s0,
-                                           int s1, int r0, int r1, long step, int lane, int w,
-                                           bool with_receivers) {
-  if (w != 0) return;
-  if (lane < s1 - s0) {
-    const int s = s0 + lane;
-    tl.src_pk[lane] = d.src_cell[s] * 4 + d.src_kind[s];
-    tl.src_w[lane] = step < d.snt ? d.src_w[(size_t)d.src_row[s] * d.snt + step] : R(0);
+struct ListRegs {
+  // raw loads only (combined in store()), so that no wait separates the two lists' loads
+  int scell = 0, skind = 0, rcell = 0, rid = 0, rcomp = 0;
+  R sw = R(0);
+  __device__ __forceinline__ void fetch(const ShotDesc<R>& d, int s0, int s1, int r0, int r1,
+                                        long step, int lane, int w, bool with_receivers) {
+    constexpr unsigned E = sizeof(R);
+    if (w != 0) return;
+    if (lane < s1 - s0) {
+      const unsigned s = s0 + lane;
+      scell = gld(d.src_cell, s * 4u);
+      skind = gld(d.src_kind, s * 4u);
+      if (step < d.snt) sw = gld(d.src_wc, ((unsigned)step * d.nsrc + s) * E);
+    }
+    if (with_receivers && lane < r1 - r0) {
+      const unsigned r = r0 + lane;
+      rcell = gld(d.rec_cell, r * 4u);
+      rid = gld(d.rec_id, r * 4u);
+      rcomp = gld(d.rec_comp, r * 4u);
+    }
   }
-  if (with_receivers && lane < r1 - r0) {
-    tl.rec_cell[lane] = d.rec_cell[r0 + lane];
-    tl.rec_code[lane] = d.rec_id[r0 + lane] * 8 + d.rec_comp[r0 + lane];
+  __device__ __forceinline__ void store(TileLists<R>& tl, int s0, int s1, int r0, int r1,
+                                        int lane, int w, bool with_receivers) const {
+    if (w != 0) return;
+    if (lane < s1 - s0) {
+      tl.src_pk[lane] = scell * 4 + skind;
+      tl.src_w[lane] = sw;
+    }
+    if (with_receivers && lane < r1 - r0) {
+      tl.rec_cell[lane] = rcell;
+      tl.rec_code[lane] = rid * 8 + rcomp;
+    }
   }
-}
+};
 
 template <typename R, int RZ, bool REC = false>
 __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restrict__ shots,
@@ -286,10 +316,6 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
   constexpr bool PRE = RZ <= 2;  // CPML memory requested before the barrier
   CpmlRows<R, PRE ? RZ : 1> cp;
   if constexpr (PRE) cp.fetch(d, i, jw, i < d.nx - 4, 2, 3, 6, 7);
-  const int s0 = d.src_off[tile], s1 = d.src_off[tile + 1];
-  const long step = d.step + kstep;
-  __shared__ TileLists<R> tl;
-  fill_lists(tl, d, s0, s1, 0, 0, step, lane, w, false);
   R wrho[RZ + 1], wrho1[RZ], ovx[RZ], ovz[RZ];
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
@@ -299,9 +325,15 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
   }
 #pragma unroll
   for (int r = 0; r <= RZ; ++r) wrho[r] = gld(rho, mb0 + r * mrowb);
+  const int s0 = sld(d.src_off, tile), s1 = sld(d.src_off, tile + 1);
+  const long step = d.step + kstep;
+  __shared__ TileLists<R> tl;
+  ListRegs<R> lr;
+  lr.fetch(d, s0, s1, 0, 0, step, lane, w, false);
   t_xz.store(s_xz, lane, w);
   t_zz.store(s_zz, lane, w);
   t_xx.store(s_xx, lane, w);
+  lr.store(tl, s0, s1, 0, 0, lane, w, false);
   __syncthreads();
 
   const int nx = d.nx, nz = d.nz, T = d.T;
@@ -366,7 +398,7 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
     for (int s = s0 + LIST; s < s1; ++s) {
       if ((unsigned)d.src_cell[s] * E != cb) continue;
       const int kind = d.src_kind[s];
-      const R val = step < d.snt ? d.src_w[(size_t)d.src_row[s] * d.snt + step] : R(0);
+      const R val = step < d.snt ? d.src_wc[(size_t)step * d.nsrc + s] : R(0);
       if (kind == 0) fz = fz + val;
       else if (kind == 2) fx = fx + val;
     }
@@ -431,11 +463,8 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   constexpr bool PRE = REC || RZ == 2;
   CpmlRows<R, PRE ? RZ : 1> cp;
   if constexpr (PRE) cp.fetch(d, i, jw, i < d.nx - 4, 0, 1, 4, 5);
-  const int s0 = d.src_off[tile], s1 = d.src_off[tile + 1];
-  const int r0 = d.rec_off[tile], r1 = d.rec_off[tile + 1];
   const long step = d.step + kstep;
   __shared__ TileLists<R> tl;
-  fill_lists(tl, d, s0, s1, r0, r1, step, lane, w, true);
   const bool vti = d.vti;
   R olam[RH], otau[RH], osxx[RH], oszz[RH], osxz[RH], om[3 * LM][RH], oc11[RH], oc33[RH];
   auto load_cell = [&](int r, int h) {
@@ -468,8 +497,13 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
 #pragma unroll
     for (int r = 0; r < RZ; ++r) load_cell(r, r);
   }
+  const int s0 = sld(d.src_off, tile), s1 = sld(d.src_off, tile + 1);
+  const int r0 = sld(d.rec_off, tile), r1 = sld(d.rec_off, tile + 1);
+  ListRegs<R> lr;
+  lr.fetch(d, s0, s1, r0, r1, step, lane, w, true);
   t_vx.store(s_vx, lane, w);
   t_vz.store(s_vz, lane, w);
+  lr.store(tl, s0, s1, r0, r1, lane, w, true);
   __syncthreads();
 
   const int nx = d.nx, nz = d.nz, T = d.T;
@@ -578,7 +612,7 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
     }
     for (int s = s0 + LIST; s < s1; ++s) {
       if ((unsigned)d.src_cell[s] * E != cb || d.src_kind[s] != 1) continue;
-      const R val = step < d.snt ? d.src_w[(size_t)d.src_row[s] * d.snt + step] : R(0);
+      const R val = step < d.snt ? d.src_wc[(size_t)step * d.nsrc + s] : R(0);
       dsxx = dsxx + val;
       dszz = dszz + val;
     }
