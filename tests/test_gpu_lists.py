@@ -1,0 +1,53 @@
+"""Per-tile source and receiver lists: the first 64 entries of a tile come from LDS, the rest
+from global memory (the samples in CSR order). Sources next to tile corners, two sources on
+one cell, more than 64 sources and more than 64 receivers in one tile, wavelets shorter than
+the run; every tiling, against the CPU oracle (relative L2 <= 1e-5 FP32, <= 1e-12 FP64)."""
+import numpy as np
+import pytest
+
+from helpers import compare, run_gpu, run_oracle, small_cfg
+
+pytestmark = pytest.mark.gpu
+
+
+def _dense_cfg(nt, seed=4, short_wavelets=False):
+    cfg = small_cfg(nx=150, nz=45, nrelax=2, nt=nt, seed=seed)
+    nx, nz = cfg["nx"], cfg["nz"]
+    # tile corners of 1-, 2-, 4- and 8-row tilings (tiles start at columns 4, 68, 132 and
+    # rows 4 + 4k), a repeated cell, the surface row, and a block of 8 x 24 sources
+    pts = [(67, 7, 0), (68, 8, 2), (68, 8, 0), (64, 12, 1), (72, 11, 2), (131, 9, 0),
+           (132, 4, 1), (71, 15, 0), (4, 4, 2), (nx - 5, nz - 5, 1), (100, 20, 0)]
+    pts += [(ix, iz, (ix + iz) % 3) for iz in range(6, 14) for ix in range(56, 80)]
+    rng = np.random.default_rng(7)
+    snt = nt // 2 if short_wavelets else nt + 5
+    t = np.arange(snt) * cfg["dt"]
+    w = np.stack([(1 + rng.random()) * 1e3 * np.sin(2 * np.pi * (15 + 20 * rng.random()) * t)
+                  * np.exp(-((t - 0.003) / 0.002) ** 2) for _ in pts])
+    cfg["src_ix"] = np.array([p[0] for p in pts], np.int32)
+    cfg["src_iz"] = np.array([p[1] for p in pts], np.int32)
+    cfg["src_kind"] = np.array([p[2] for p in pts], np.int32)
+    cfg["wavelets"] = w
+    rix = np.concatenate([np.arange(4, nx - 4), np.arange(4, nx - 4), cfg["rec_ix"]])
+    riz = np.concatenate([np.full(nx - 8, 10), np.full(nx - 8, 11), cfg["rec_iz"]])
+    cfg["rec_ix"] = rix.astype(np.int32)
+    cfg["rec_iz"] = riz.astype(np.int32)
+    cfg["rec_comp"] = (np.arange(len(rix)) % 5).astype(np.int32)
+    return cfg
+
+
+@pytest.mark.parametrize("rows", [1, 2, 4, 8])
+def test_dense_lists_every_tiling(rows):
+    cfg = _dense_cfg(nt=30)
+    print(f"[case] dense source/receiver lists, tiling {rows}", flush=True)
+    ref = run_oracle(cfg, np.float32)
+    gpu = run_gpu(cfg, "float32", tiling=rows, chunks=[7, 23])
+    compare(gpu, ref, np.float32, f"dense lists rows={rows}")
+    assert np.abs(ref["traces"]).max() > 0
+
+
+def test_dense_lists_fp64_short_wavelets():
+    cfg = _dense_cfg(nt=26, seed=5, short_wavelets=True)
+    print("[case] dense lists FP64, wavelets shorter than the run", flush=True)
+    ref = run_oracle(cfg, np.float64)
+    gpu = run_gpu(cfg, "float64", tiling=1)
+    compare(gpu, ref, np.float64, "dense lists fp64")
