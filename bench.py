@@ -311,7 +311,7 @@ def main():
                     help="run only the batched multi-shot leg and print its JSON")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short batched run + 1 GiB copy for rocprofv3 --pmc passes")
-    ap.add_argument("--profile-every", type=int, default=20,
+    ap.add_argument("--profile-every", type=int, default=50,
                     help="time the launches of every N-th step with dispatch events")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
