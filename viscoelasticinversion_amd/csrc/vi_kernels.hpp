@@ -180,6 +180,58 @@ struct TileLoad {
   }
 };
 
+// FP32 tile rows go straight from global memory to LDS (global_load_lds_dword: lane l of a
+// wavefront writes LDS at the wave-uniform base + 4 l), which needs no VGPRs and no LDS stores
+// for the 64 main columns; the 4 + 4 halo columns of a row go through registers (lanes 0..7),
+// since an LDS-direct load has one base address per instruction and the two halo groups need
+// two (a divergent pair of such loads can be merged by the compiler into one with the first
+// lane's base, which is wrong). FP64 tiles go through registers (TileLoad): there is no 8-byte
+// LDS-direct load. Same rows, columns and LDS layout as TileLoad; the barrier after the loads
+// waits for them. Rows past NROWS are not loaded.
+__device__ __forceinline__ void dma_f32(const float* g, float* lds) {
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) float*)g,
+                                   (__attribute__((address_space(3))) float*)lds, 4, 0, 0);
+}
+template <typename R, int NROWS, int HALO>
+struct TileIn {
+  static constexpr int W = TX + 2 * HALO;
+  static constexpr int K = (NROWS + NW - 1) / NW;
+  TileLoad<R, NROWS, HALO> t;  // FP64 path
+  R hv[K];                     // FP32 path: halo values of lanes 0..7
+  __device__ __forceinline__ void fetch(const R* __restrict__ g, int jr0, int i0, int ldx,
+                                        int lane, int w, R* sh) {
+    if constexpr (sizeof(R) == 4) {
+      constexpr unsigned E = sizeof(R);
+      const int hg = lane < 4 ? lane - 4 : TX + lane - 4;  // halo column relative to i0
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int r = w + k * NW;  // wave-uniform
+        if (r < NROWS) dma_f32(g + (size_t)(jr0 + r) * ldx + i0 + lane, &sh[r * W + HALO]);
+        if (HALO) {
+          const unsigned rb = (unsigned)((jr0 + min(r, NROWS - 1)) * ldx + i0) * E;
+          hv[k] = lane < 8 ? gld(g, rb + hg * (int)E) : R(0);
+        }
+      }
+    } else {
+      t.fetch(g, jr0, i0, ldx, lane, w);
+    }
+  }
+  __device__ __forceinline__ void store(R* sh, int lane, int w) const {
+    if constexpr (sizeof(R) == 4) {
+      if (HALO) {
+        const int hc = lane < 4 ? lane : lane + TX;  // LDS column of a halo value
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const int r = w + k * NW;
+          if (r < NROWS && lane < 8) sh[r * W + hc] = hv[k];
+        }
+      }
+    } else {
+      t.store(sh, lane, w);
+    }
+  }
+};
+
 // CPML memory of one lane's RZ cells for two x-strip and two z-strip arrays, requested
 // before the barrier together with the tile loads (for short tiles), so the strip tiles,
 // which finish last, do not wait for a separate round trip after the stencils.
@@ -234,47 +286,66 @@ __device__ __forceinline__ R cpml_apply_pre(R* psi, unsigned pb, R p, R a, R b, 
 constexpr int LIST = 64;
 template <typename R>
 struct TileLists {
-  int src_pk[LIST];   // source cell * 4 + kind
-  R src_w[LIST];      // its sample of this step
-  int rec_cell[LIST];
-  int rec_code[LIST];  // receiver id * 8 + component
+  int src_cell[LIST], src_kind[LIST];
+  R src_w[LIST];  // the source's sample of this step
+  int rec_cell[LIST], rec_id[LIST], rec_comp[LIST];
 };
 
-// The list entries of one lane of the first wavefront, loaded into registers together with
-// the tile loads (no load depends on another one: the samples are stored in CSR order), and
-// written to LDS with the tiles.
+// The list entries, requested by the first wavefront together with the tile loads; no load
+// depends on another one (the samples are stored per step in CSR order). FP32: straight into
+// LDS like the tiles (no VGPRs); FP64: through registers, written to LDS with the tiles.
+__device__ __forceinline__ void dma4(const void* g, void* lds) {
+  dma_f32((const float*)g, (float*)lds);
+}
 template <typename R>
-struct ListRegs {
-  // raw loads only (combined in store()), so that no wait separates the two lists' loads
-  int scell = 0, skind = 0, rcell = 0, rid = 0, rcomp = 0;
+struct ListIn {
+  int scell = 0, skind = 0, rcell = 0, rid = 0, rcomp = 0;  // FP64 path only
   R sw = R(0);
-  __device__ __forceinline__ void fetch(const ShotDesc<R>& d, int s0, int s1, int r0, int r1,
-                                        long step, int lane, int w, bool with_receivers) {
+  __device__ __forceinline__ void fetch(TileLists<R>& tl, const ShotDesc<R>& d, int s0, int s1,
+                                        int r0, int r1, long step, int lane, int w,
+                                        bool with_receivers) {
     constexpr unsigned E = sizeof(R);
     if (w != 0) return;
     if (lane < s1 - s0) {
       const unsigned s = s0 + lane;
-      scell = gld(d.src_cell, s * 4u);
-      skind = gld(d.src_kind, s * 4u);
-      if (step < d.snt) sw = gld(d.src_wc, ((unsigned)step * d.nsrc + s) * E);
+      if constexpr (sizeof(R) == 4) {
+        dma4(d.src_cell + s, tl.src_cell);
+        dma4(d.src_kind + s, tl.src_kind);
+        if (step < d.snt) dma4(d.src_wc + (size_t)step * d.nsrc + s, tl.src_w);
+        else tl.src_w[lane] = R(0);
+      } else {
+        scell = gld(d.src_cell, s * 4u);
+        skind = gld(d.src_kind, s * 4u);
+        if (step < d.snt) sw = gld(d.src_wc, ((unsigned)step * d.nsrc + s) * E);
+      }
     }
     if (with_receivers && lane < r1 - r0) {
       const unsigned r = r0 + lane;
-      rcell = gld(d.rec_cell, r * 4u);
-      rid = gld(d.rec_id, r * 4u);
-      rcomp = gld(d.rec_comp, r * 4u);
+      if constexpr (sizeof(R) == 4) {
+        dma4(d.rec_cell + r, tl.rec_cell);
+        dma4(d.rec_id + r, tl.rec_id);
+        dma4(d.rec_comp + r, tl.rec_comp);
+      } else {
+        rcell = gld(d.rec_cell, r * 4u);
+        rid = gld(d.rec_id, r * 4u);
+        rcomp = gld(d.rec_comp, r * 4u);
+      }
     }
   }
   __device__ __forceinline__ void store(TileLists<R>& tl, int s0, int s1, int r0, int r1,
                                         int lane, int w, bool with_receivers) const {
-    if (w != 0) return;
-    if (lane < s1 - s0) {
-      tl.src_pk[lane] = scell * 4 + skind;
-      tl.src_w[lane] = sw;
-    }
-    if (with_receivers && lane < r1 - r0) {
-      tl.rec_cell[lane] = rcell;
-      tl.rec_code[lane] = rid * 8 + rcomp;
+    if constexpr (sizeof(R) != 4) {
+      if (w != 0) return;
+      if (lane < s1 - s0) {
+        tl.src_cell[lane] = scell;
+        tl.src_kind[lane] = skind;
+        tl.src_w[lane] = sw;
+      }
+      if (with_receivers && lane < r1 - r0) {
+        tl.rec_cell[lane] = rcell;
+        tl.rec_id[lane] = rid;
+        tl.rec_comp[lane] = rcomp;
+      }
     }
   }
 };
@@ -299,12 +370,12 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
   const int i0 = 4 + tx * TX, j0 = 4 + tz * TZ;
   const int i = i0 + lane;
   const int ldx = d.ldx, ldm = d.ldm;
-  TileLoad<R, TZ + 8, 4> t_xz;
-  TileLoad<R, TZ + 8, 0> t_zz;
-  TileLoad<R, TZ, 4> t_xx;
-  t_xz.fetch(d.sxz, j0 - 4, i0, ldx, lane, w);
-  t_zz.fetch(d.szz, j0 - 4, i0, ldx, lane, w);
-  t_xx.fetch(d.sxx, j0, i0, ldx, lane, w);
+  TileIn<R, TZ + 8, 4> t_xz;
+  TileIn<R, TZ + 8, 0> t_zz;
+  TileIn<R, TZ, 4> t_xx;
+  t_xz.fetch(d.sxz, j0 - 4, i0, ldx, lane, w, s_xz);
+  t_zz.fetch(d.szz, j0 - 4, i0, ldx, lane, w, s_zz);
+  t_xx.fetch(d.sxx, j0, i0, ldx, lane, w, s_xx);
 
   const int jw = j0 + w * RZ;  // first row of this wave
   const unsigned rowb = (unsigned)ldx * E, mrowb = (unsigned)ldm * E;
@@ -328,8 +399,8 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
   const int s0 = sld(d.src_off, tile), s1 = sld(d.src_off, tile + 1);
   const long step = d.step + kstep;
   __shared__ TileLists<R> tl;
-  ListRegs<R> lr;
-  lr.fetch(d, s0, s1, 0, 0, step, lane, w, false);
+  ListIn<R> lr;
+  lr.fetch(tl, d, s0, s1, 0, 0, step, lane, w, false);
   t_xz.store(s_xz, lane, w);
   t_zz.store(s_zz, lane, w);
   t_xx.store(s_xx, lane, w);
@@ -389,11 +460,11 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
     R fz = dszz_z + dsxz_x;
     const unsigned cell = cb / E;
     for (int q = 0; q < min(s1 - s0, LIST); ++q) {  // sources in CSR order
-      const int pk = tl.src_pk[q];
-      if ((unsigned)(pk >> 2) != cell) continue;
+      if ((unsigned)tl.src_cell[q] != cell) continue;
+      const int kind = tl.src_kind[q];
       const R val = tl.src_w[q];
-      if ((pk & 3) == 0) fz = fz + val;
-      else if ((pk & 3) == 2) fx = fx + val;
+      if (kind == 0) fz = fz + val;
+      else if (kind == 2) fx = fx + val;
     }
     for (int s = s0 + LIST; s < s1; ++s) {
       if ((unsigned)d.src_cell[s] * E != cb) continue;
@@ -416,8 +487,13 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
   }
 }
 
+template <typename R, int L, int RZ, bool REC>
+constexpr int stress_waves() {
+  return (sizeof(R) == 4 && RZ == 1 && L <= 3 && !REC) ? 8 : 1;
+}
+
 template <typename R, int L, int RZ, bool REC = false>
-__global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict__ shots,
+__global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC>())) void k_stress(const ShotDesc<R>* __restrict__ shots,
                                                    const int* __restrict__ tile_shot,
                                                    int kstep, const RecSlot rs) {
   static_assert(!REC || RZ <= 2, "recording needs the prefetched CPML memory");
@@ -440,9 +516,9 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   const int i0 = 4 + tx * TX, j0 = 4 + tz * TZ;
   const int i = i0 + lane;
   const int ldx = d.ldx, ldm = d.ldm;
-  TileLoad<R, TZ + 8, 4> t_vx, t_vz;
-  t_vx.fetch(d.vx, j0 - 4, i0, ldx, lane, w);
-  t_vz.fetch(d.vz, j0 - 4, i0, ldx, lane, w);
+  TileIn<R, TZ + 8, 4> t_vx, t_vz;
+  t_vx.fetch(d.vx, j0 - 4, i0, ldx, lane, w, s_vx);
+  t_vz.fetch(d.vz, j0 - 4, i0, ldx, lane, w, s_vz);
 
   const int jw = j0 + w * RZ;
   const unsigned rowb = (unsigned)ldx * E, mrowb = (unsigned)ldm * E;
@@ -466,16 +542,18 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   const long step = d.step + kstep;
   __shared__ TileLists<R> tl;
   const bool vti = d.vti;
+  // per cell: lam | C13, C11, C33 (lam + 2 mu when isotropic), mu averaged to the sxz node,
+  // tau, the stress and the memory variables; the moduli are formed as soon as the loads land,
+  // so the four mu values are not held through the CPML code (fewer VGPRs)
   R olam[RH], otau[RH], osxx[RH], oszz[RH], osxz[RH], om[3 * LM][RH], oc11[RH], oc33[RH];
+  R omxz[RH];
+  const R two = R(2), quarter = R(0.25);
   auto load_cell = [&](int r, int h) {
     const unsigned cb = cb0 + r * rowb, mb = mb0 + r * mrowb;
     olam[h] = gld(lam, mb);
     if (vti) {
       oc11[h] = gld(d.c11, mb);
       oc33[h] = gld(d.c33, mb);
-    } else {
-      oc11[h] = R(0);
-      oc33[h] = R(0);
     }
     osxx[h] = gld((const R*)sxx, cb);
     oszz[h] = gld((const R*)szz, cb);
@@ -487,6 +565,14 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
     } else {
       otau[h] = R(0);
     }
+    if (!vti) {
+      oc11[h] = olam[h] + two * wmu[r];  // C11 = C33 = lam + 2 mu
+      oc33[h] = oc11[h];
+    }
+    omxz[h] = quarter * (((wmu[r] + wmu1[r]) + wmu[r + 1]) + wmu1[r + 1]);
+    // keep the compiler from sinking these sums to their uses, which would hold the four mu
+    // values through the CPML code (an empty statement that only pins the value in a VGPR)
+    asm volatile("" : "+v"(omxz[h]), "+v"(oc11[h]), "+v"(oc33[h]));
   };
 #pragma unroll
   for (int r = 0; r <= RZ; ++r) {
@@ -499,8 +585,8 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
   }
   const int s0 = sld(d.src_off, tile), s1 = sld(d.src_off, tile + 1);
   const int r0 = sld(d.rec_off, tile), r1 = sld(d.rec_off, tile + 1);
-  ListRegs<R> lr;
-  lr.fetch(d, s0, s1, r0, r1, step, lane, w, true);
+  ListIn<R> lr;
+  lr.fetch(tl, d, s0, s1, r0, r1, step, lane, w, true);
   t_vx.store(s_vx, lane, w);
   t_vz.store(s_vz, lane, w);
   lr.store(tl, s0, s1, r0, r1, lane, w, true);
@@ -508,7 +594,7 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
 
   const int nx = d.nx, nz = d.nz, T = d.T;
   const bool col_ok = i < nx - 4;
-  const R dt = d.dt, two = R(2), quarter = R(0.25);
+  const R dt = d.dt;
   const R Dx[4] = {d.Dx[0], d.Dx[1], d.Dx[2], d.Dx[3]};
   const R Dz[4] = {d.Dz[0], d.Dz[1], d.Dz[2], d.Dz[3]};
   const int xk = T ? strip_x(i, nx, T, d.xlo, d.xhi) : -1;
@@ -568,10 +654,10 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
       }
     }
     const R gxz = ezx + exz;
-    const R lr = olam[h], mr = wmu[r];
-    const R mxz = quarter * (((wmu[r] + wmu1[r]) + wmu[r + 1]) + wmu1[r + 1]);
-    const R px = vti ? oc11[h] : lr + two * mr;  // C11
-    const R pz = vti ? oc33[h] : px;              // C33
+    const R lr = olam[h];
+    const R mxz = omxz[h];
+    const R px = oc11[h];  // C11
+    const R pz = oc33[h];  // C33
     const bool fs_row = fs && j == 4;
     R dsxx, dszz, dsxz;
     if constexpr (L > 0) {
@@ -602,10 +688,18 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
       dszz = lr * exx + pz * ezz;
       dsxz = mxz * gxz;
     }
+    // memory variables first: their old values are dead after this, before the source loop
+    if constexpr (L > 0) {
+#pragma unroll
+      for (int k = 0; k < L; ++k) {
+        gst(Mp[k], cb, om[k][h] + dt * (a1[k] * exx + a2[k] * om[k][h]));
+        gst(Mp[L + k], cb, om[L + k][h] + dt * (a1[k] * ezz + a2[k] * om[L + k][h]));
+        gst(Mp[2 * L + k], cb, om[2 * L + k][h] + dt * (a1[k] * gxz + a2[k] * om[2 * L + k][h]));
+      }
+    }
     const unsigned cell = cb / E;
     for (int q = 0; q < min(s1 - s0, LIST); ++q) {  // explosive sources in CSR order
-      const int pk = tl.src_pk[q];
-      if ((unsigned)(pk >> 2) != cell || (pk & 3) != 1) continue;
+      if ((unsigned)tl.src_cell[q] != cell || tl.src_kind[q] != 1) continue;
       const R val = tl.src_w[q];
       dsxx = dsxx + val;
       dszz = dszz + val;
@@ -632,14 +726,6 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
     gst(sxx, cb, nsxx);
     gst(szz, cb, nszz);
     gst(sxz, cb, nsxz);
-    if constexpr (L > 0) {
-#pragma unroll
-      for (int k = 0; k < L; ++k) {
-        gst(Mp[k], cb, om[k][h] + dt * (a1[k] * exx + a2[k] * om[k][h]));
-        gst(Mp[L + k], cb, om[L + k][h] + dt * (a1[k] * ezz + a2[k] * om[L + k][h]));
-        gst(Mp[2 * L + k], cb, om[2 * L + k][h] + dt * (a1[k] * gxz + a2[k] * om[2 * L + k][h]));
-      }
-    }
     if (fs) {  // stress images above the surface row 4
       if (j >= 5 && j <= 8) gst(szz, (unsigned)((8 - j) * ldx + i) * E, -nszz);
       if (j >= 4 && j <= 7) gst(sxz, (unsigned)((7 - j) * ldx + i) * E, -nsxz);
@@ -647,14 +733,13 @@ __global__ __launch_bounds__(TX* NW) void k_stress(const ShotDesc<R>* __restrict
     if (step < d.rec_nt) {
       for (int q = 0; q < min(r1 - r0, LIST); ++q) {  // receivers of the tile, from LDS
         if ((unsigned)tl.rec_cell[q] != cell) continue;
-        const int code = tl.rec_code[q];
-        const int comp = code & 7;
+        const int comp = tl.rec_comp[q];
         const R val = comp == 0   ? pvx[0]
                       : comp == 1 ? pvz[0]
                       : comp == 2 ? nsxx
                       : comp == 3 ? nszz
                                   : nsxz;
-        d.traces[(size_t)step * d.nrec + (code >> 3)] = val;
+        d.traces[(size_t)step * d.nrec + tl.rec_id[q]] = val;
       }
     }
     for (int q = r0 + LIST; q < r1; ++q) {
