@@ -13,6 +13,9 @@
  *   RickerSource / inject_source_kernel!  src/sciml_solver/Source.jl:7-58 -> vi_set_sources
  *   CPMLConfig / CPMLCoefficients   src/sciml_solver/CPML.jl:28-221     -> vi_set_cpml
  *   FreeSurfaceBC / apply_free_surface!   src/sciml_solver/CPML.jl:366-393 -> vi_set_free_surface
+ *   H-AFDA surface: FreeSurfaceStencil, Stencil.jl:52-105, and the row kernels of
+ *                   free-surface-kernels.jl:16-214      -> vi_set_free_surface(h, 2),
+ *                                                          vi_set_afda_points, vi_afda_weights
  *   IsotropicModel / QModel         src/sciml_solver/Models.jl:3-25, 90-107 -> vi_set_model,
  *                                                                              vi_set_relaxation
  *   VTIModel                        src/sciml_solver/Models.jl:27-33   -> vi_set_model_vti
@@ -89,6 +92,15 @@ const char* vi_last_error(void);
 const char* vi_version(void);
 /* staggered 8th-order weights / h, as _stencil (Stencil.jl:9-14) gives them */
 void vi_fd_coeffs(double h, double out[4]);
+/* H-AFDA one-sided d/dz weights / h (the FreeSurfaceStencil coefficients, Stencil.jl:52-105,
+ * formulas #1, #2 and #4 of Kristek et al. 2002, carried to every row an 8th-order centred
+ * stencil would take above the surface). Table rows: A0..A3 = d sxz/dz at z = k h (from
+ * sxz(0) = 0 and sxz at (m+1/2) h), B0..B2 = d szz/dz at (k+1/2) h (szz at m h), C1..C3 =
+ * d vz/dz at k h (vz at (m+1/2) h), D0..D2 = d vx/dz at (k+1/2) h (vx at m h); column m
+ * multiplies grid row 4 + m. counts[row] = points used (A: 1..8 besides the boundary point,
+ * others 2..8); NULL = the default {3,4,6,8, 3,4,6, 2,4,6, 3,4,6}. Unused columns are 0. */
+int vi_afda_weights(double h, const int* counts, double out[13][8]);
+int vi_afda_default_points(int out[13]);
 /* d0 = -(power+1) vmax ln(R) / (2 thickness dx)   (CPML.jl:60) */
 double vi_cpml_auto_d0(int thickness, double vmax, double dx, double reflection_coef,
                        double power);
@@ -129,8 +141,16 @@ int vi_set_model_window(vi_handle h, vi_model m, int x0);
 int vi_set_relaxation(vi_handle h, const double* taun);
 /* NULL disables the CPML; resets the psi memory to zero */
 int vi_set_cpml(vi_handle h, const vi_cpml_desc* c);
-/* 1: traction-free surface on row 4 with stress images in rows 0..3 */
-int vi_set_free_surface(vi_handle h, int enabled);
+/* traction-free surface on row 4 (sxx, szz = 0, vx on the surface; vz, sxz half a cell
+ * below). mode 0: none (Dirichlet ring); 1: stress imaging, szz and sxz mirrored with opposite
+ * sign into rows 0..3 (CPML.jl:372-380); 2: H-AFDA, the vertical derivatives of rows 4..7 come
+ * from one-sided stencils on rows 4..11 (vi_afda_weights) and nothing above the surface is
+ * read or written. Needs nz >= 17 and no top CPML strip. The gradient entry points accept
+ * modes 0 and 1 only. */
+int vi_set_free_surface(vi_handle h, int mode);
+/* points per H-AFDA stencil (see vi_afda_weights; NULL = default). All-8 counts, the full
+ * order of the reference's FreeSurfaceStencil(8), diverge in heterogeneous media. */
+int vi_set_afda_points(vi_handle h, const int* counts);
 /* n sources at interior cells (ix, iz); wavelets [n][nt] in the handle's precision,
  * already scaled by the source coefficient; sample k is used by step k */
 int vi_set_sources(vi_handle h, int n, const int* ix, const int* iz, const int* kind, int nt,

@@ -55,6 +55,70 @@ void vio_cpml_axis(int n, int T, double d0, double alpha_max, double kappa_max, 
   }
 }
 
+/* H-AFDA free surface (free_surface = 2): adjusted one-sided FD approximations after Kristek,
+ * Moczo & Archuleta (2002, Stud. Geophys. Geod. 46), the scheme of the reference's draft
+ * src/free-surface-kernels.jl:1-214 with src/common/Stencil.jl:52-105 (formulas #1, #2, #4),
+ * carried to the rows an 8th-order centred stencil needs. The surface is row 4 (z = 0); row
+ * 4+k holds sxx, szz, vx at z = k h and vz, sxz at z = (k+1/2) h; no value above z = 0 is
+ * used. Each vertical derivative whose centred stencil would reach above the surface comes
+ * from the first n rows below it (rows 4 .. 4+n-1 of the field):
+ *   table rows  0.. 3  A_k  d sxz/dz at z = k h (k = 0..3): points 0 (sxz(0) = 0, its weight
+ *                           dropped, formula #1/#4) and (m+1/2) h, m < n
+ *   table rows  4.. 6  B_k  d szz/dz at z = (k+1/2) h: points m h, m < n (formula #2;
+ *                           szz(0) = 0 is the stored surface value)
+ *   table rows  7.. 9  C_k  d vz/dz at z = k h (k = 1..3): points (m+1/2) h, m < n
+ *   table rows 10..12  D_k  d vx/dz at z = (k+1/2) h: points m h, m < n (formula #2)
+ * The weights are those of the derivative of the Lagrange interpolant at x0,
+ *   w_m = sum_{q != m} prod_{l != m, q} (x0 - x_l) / prod_{l != m} (x_m - x_l),
+ * evaluated in integers of h/2 (every factor is at most 16, so numerator and denominator
+ * are exact in int64 and in double) and rounded once by the final division: the correctly
+ * rounded exact rational, then divided by h. Unused columns are 0. */
+static const int vio_afda_default[13] = {3, 4, 6, 8, 3, 4, 6, 2, 4, 6, 3, 4, 6};
+
+void vio_afda_default_points(int out[13]) {
+  for (int r = 0; r < 13; ++r) out[r] = vio_afda_default[r];
+}
+
+int vio_afda_weights(double h, const int* counts, double out[13][8]) {
+  const int* nc = counts ? counts : vio_afda_default;
+  for (int r = 0; r < 13; ++r) {
+    long long X[9], X0; /* positions in units of h/2 */
+    int n = nc[r], skip = 0, np;
+    if (n < (r < 4 ? 1 : 2) || n > 8) return -1;
+    if (r < 4) {
+      X[0] = 0;
+      for (int m = 0; m < n; ++m) X[m + 1] = 2 * m + 1;
+      np = n + 1;
+      skip = 1;
+      X0 = 2 * r;
+    } else if (r < 10 && r >= 7) {
+      for (int m = 0; m < n; ++m) X[m] = 2 * m + 1;
+      np = n;
+      X0 = 2 * (r - 7 + 1);
+    } else {
+      for (int m = 0; m < n; ++m) X[m] = 2 * m;
+      np = n;
+      X0 = 2 * (r < 7 ? r - 4 : r - 10) + 1;
+    }
+    for (int m = 0; m < 8; ++m) out[r][m] = 0.0;
+    for (int m = skip; m < np; ++m) {
+      long long num = 0, den = 1;
+      for (int l = 0; l < np; ++l)
+        if (l != m) den *= X[m] - X[l];
+      for (int q = 0; q < np; ++q) {
+        if (q == m) continue;
+        long long t = 1;
+        for (int l = 0; l < np; ++l)
+          if (l != m && l != q) t *= X0 - X[l];
+        num += t;
+      }
+      /* den carries one factor of h/2 more than num, hence 2 num / den per unit h */
+      out[r][m - skip] = ((double)(2 * num) / (double)den) / h;
+    }
+  }
+  return 0;
+}
+
 #define REAL float
 #define SUF _f32
 #include "vio_body.h"

@@ -14,6 +14,8 @@
  *   - derivative stretching du/kappa + psi                    src/sciml_solver/ViscoelasticProblem.jl:208-216
  *   - Ricker force source added to dvz before buoyancy        src/sciml_solver/ViscoelasticProblem.jl:162-171
  *   - stress imaging free surface                             src/sciml_solver/CPML.jl:372-380
+ *   - H-AFDA free surface (one-sided stencils, Kristek 2002)  src/free-surface-kernels.jl:1-214,
+ *                                                             src/common/Stencil.jl:52-105
  *   - padded (Dirichlet) interior ranges                      src/common/Stencil.jl:432-447, visco-kernels.jl:95-96
  * with the draft defects resolved as listed in the oracle's header comment of vio_body.h.
  *
@@ -46,14 +48,22 @@ typedef struct {
   int nx, nz;          /* cells */
   int nrelax;          /* L: number of SLS mechanisms, 0 = elastic */
   double dx, dz, dt;
-  int free_surface;    /* 1: traction-free surface at row 4 with stress images in rows 0..3 */
+  int free_surface;    /* traction-free surface at row 4: 0 none, 1 stress images in rows
+                          0..3, 2 H-AFDA one-sided stencils (nothing above the surface) */
   int cpml_thickness;  /* 0: no CPML */
   double cpml_d0, cpml_alpha_max, cpml_kappa_max, cpml_power;
   int cpml_left, cpml_right, cpml_top, cpml_bottom;
+  int afda_points[13]; /* H-AFDA point counts per table row (see vio.c); all 0 = default */
 } vio_desc;
 
 /* Exact staggered 8th-order weights (1225/1024, -245/3072, 49/5120, -5/7168) / h. */
 void vio_fd_coeffs(double h, double out[4]);
+
+/* H-AFDA one-sided d/dz weights / h (free_surface = 2): table rows A0..A3, B0..B2, C1..C3,
+ * D0..D2 over field rows 4 .. 11 (see vio.c). counts NULL = the default point counts.
+ * Returns -1 for a count out of range. */
+int vio_afda_weights(double h, const int* counts, double out[13][8]);
+void vio_afda_default_points(int out[13]);
 
 /* CPML a, b, kappa along one axis of n cells (CPML.jl:109-189); lo/hi = strip enabled. */
 void vio_cpml_axis(int n, int thickness, double d0, double alpha_max, double kappa_max,

@@ -18,6 +18,33 @@ _lib = None
 VX, VZ, SXX, SZZ, SXZ, MXX, MZZ, MXZ = range(8)
 PSI_VX_X, PSI_VZ_X, PSI_SXX_X, PSI_SXZ_X, PSI_VX_Z, PSI_VZ_Z, PSI_SXZ_Z, PSI_SZZ_Z = range(8, 16)
 FUSED, MULTIPASS = 0, 1
+FS_NONE, FS_IMAGE, FS_AFDA = 0, 1, 2
+
+
+def fs_mode(free_surface) -> int:
+    """False/None/"none" -> 0, True/"image" -> 1 (stress imaging), "afda" -> 2 (H-AFDA)."""
+    if isinstance(free_surface, str):
+        return {"none": FS_NONE, "image": FS_IMAGE, "afda": FS_AFDA}[free_surface]
+    return int(free_surface or 0)
+
+
+def afda_weights(h, counts=None):
+    """The oracle's H-AFDA one-sided weights / h, shape (13, 8): table rows A0..A3, B0..B2,
+    C1..C3, D0..D2 over the field rows 4..11 (vio.c). ``counts`` = points per table row
+    (None: the default counts)."""
+    out = np.zeros((13, 8))
+    cp = None
+    if counts is not None:
+        cp = (ctypes.c_int * 13)(*[int(c) for c in counts])
+    if lib().vio_afda_weights(float(h), cp, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
+        raise ValueError(f"afda point counts out of range: {counts}")
+    return out
+
+
+def afda_default_points():
+    out = (ctypes.c_int * 13)()
+    lib().vio_afda_default_points(out)
+    return list(out)
 
 
 class Desc(ctypes.Structure):
@@ -29,6 +56,7 @@ class Desc(ctypes.Structure):
         ("cpml_kappa_max", ctypes.c_double), ("cpml_power", ctypes.c_double),
         ("cpml_left", ctypes.c_int), ("cpml_right", ctypes.c_int),
         ("cpml_top", ctypes.c_int), ("cpml_bottom", ctypes.c_int),
+        ("afda_points", ctypes.c_int * 13),
     ]
 
 
@@ -44,6 +72,10 @@ def lib():
             build()
         _lib = ctypes.CDLL(_LIB_PATH)
         vp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)
+        _lib.vio_afda_weights.restype = ctypes.c_int
+        _lib.vio_afda_weights.argtypes = [ctypes.c_double, ip, ctypes.POINTER(ctypes.c_double)]
+        _lib.vio_afda_default_points.restype = None
+        _lib.vio_afda_default_points.argtypes = [ip]
         for suf, ct in (("_f32", ctypes.c_float), ("_f64", ctypes.c_double)):
             rp = ctypes.POINTER(ct)
             f = getattr(_lib, "vio_create" + suf)
@@ -105,7 +137,7 @@ class Oracle:
     """
 
     def __init__(self, nx, nz, dx, dz, dt, model, dtype=np.float64, nrelax=0, taun=None,
-                 free_surface=False, cpml=None):
+                 free_surface=False, cpml=None, afda_points=None):
         self.dtype = np.dtype(dtype)
         self.suf = "_f32" if self.dtype == np.float32 else "_f64"
         self.ct = ctypes.c_float if self.suf == "_f32" else ctypes.c_double
@@ -113,7 +145,10 @@ class Oracle:
         d = Desc()
         d.nx, d.nz, d.nrelax = nx, nz, nrelax
         d.dx, d.dz, d.dt = dx, dz, dt
-        d.free_surface = int(bool(free_surface))
+        d.free_surface = fs_mode(free_surface)
+        if afda_points is not None:
+            for k, c in enumerate(afda_points):
+                d.afda_points[k] = int(c)
         if cpml:
             d.cpml_thickness = int(cpml["thickness"])
             d.cpml_d0 = float(cpml["d0"])

@@ -34,6 +34,12 @@
  *   dszz = 0, i.e. ezz = -((lam0 exx + lam1 SMxx) + pi1 SMzz)/pi0 (free-surface-kernels.jl:
  *   159-167 intent), and szz is set to 0 after the update; then
  *     szz[js-m] = -szz[js+m],  sxz[js-m] = -sxz[js+m-1],  m = 1..4, columns i in [4, nx-4).
+ *   H-AFDA free surface (free_surface = 2; the one-sided weights are described in vio.c):
+ *   the same surface row (ezz from dszz = 0, szz = 0), no image rows; instead, before the
+ *   CPML correction, the vertical derivatives of the rows next to the surface come from the
+ *   one-sided stencils on rows 4..11, each accumulated w0 f4 + w1 f5 + ... + w7 f11 in REAL:
+ *     vx rows 4..7: DZM(sxz) -> A;  vz rows 4..6: DZP(szz) -> B;
+ *     stress rows 5..7: DZM(vz) -> C;  sxz rows 4..6 (z = 1/2, 3/2, 5/2): DZP(vx) -> D.
  *   Receivers sample the stored value of their field at (ix, iz) after the stress half.
  *   VTI (VTIModel, Models.jl:27-33, with the TTI form ViscoelasticProblem.jl:21-35 at
  *   C15 = C35 = 0): lam holds C13, mu holds C55 (averaged to the sxz node like mu) and the
@@ -69,6 +75,7 @@ typedef struct {
   REAL *c11, *c33; /* VTI only (NULL: isotropic, pi = lam + 2 mu) */
   REAL taun[VIO_MAXL], a1[VIO_MAXL], a2[VIO_MAXL];
   REAL Dx[4], Dz[4];
+  REAL afda[13][8]; /* H-AFDA one-sided weights (free_surface = 2) */
   REAL *ax, *bx, *kx, *az, *bz, *kz;
   int xlo, xhi, zlo, zhi;
   REAL* psi[8];
@@ -90,6 +97,10 @@ static REAL* NAME(vio_alloc)(size_t n) {
 void* NAME(vio_create)(const vio_desc* d, const REAL* rho, const REAL* lam, const REAL* mu,
                        const REAL* tau, const REAL* taun) {
   if (!d || d->nx < 16 || d->nz < 16 || d->nrelax < 0 || d->nrelax > VIO_MAXL) return NULL;
+  if (d->free_surface == 2) {
+    double w[13][8];
+    if (vio_afda_weights(d->dz, d->afda_points[0] ? d->afda_points : NULL, w)) return NULL;
+  }
   SIM* s = (SIM*)calloc(1, sizeof(SIM));
   s->d = *d;
   s->L = d->nrelax;
@@ -120,6 +131,12 @@ void* NAME(vio_create)(const vio_desc* d, const REAL* rho, const REAL* lam, cons
   for (int l = 0; l < 4; ++l) s->Dx[l] = (REAL)c[l];
   vio_fd_coeffs(d->dz, c);
   for (int l = 0; l < 4; ++l) s->Dz[l] = (REAL)c[l];
+  if (d->free_surface == 2) {
+    double w[13][8];
+    vio_afda_weights(d->dz, d->afda_points[0] ? d->afda_points : NULL, w); /* validated above */
+    for (int r = 0; r < 13; ++r)
+      for (int m = 0; m < 8; ++m) s->afda[r][m] = (REAL)w[r][m];
+  }
 
   s->ax = NAME(vio_alloc)(nx); s->bx = NAME(vio_alloc)(nx); s->kx = NAME(vio_alloc)(nx);
   s->az = NAME(vio_alloc)(nz); s->bz = NAME(vio_alloc)(nz); s->kz = NAME(vio_alloc)(nz);
@@ -296,9 +313,17 @@ static inline REAL NAME(src_sample)(const SIM* s, int q, long n) {
   return n < s->snt ? s->sw[(size_t)q * s->snt + n] : (REAL)0;
 }
 
+/* one-sided d/dz at column i from rows 4..11 of u with the weights w */
+static inline REAL NAME(afda_dz)(const REAL* u, int i, int nx, const REAL* w) {
+  REAL v = w[0] * u[(size_t)4 * nx + i];
+  for (int m = 1; m < 8; ++m) v = v + w[m] * u[(size_t)(4 + m) * nx + i];
+  return v;
+}
+
 /* ---- fused path ------------------------------------------------------------------- */
 static void NAME(vel_fused)(SIM* s, long n) {
   const int nx = s->d.nx, nz = s->d.nz, T = s->T;
+  const int afda = s->d.free_surface == 2;
   const REAL dt = (REAL)s->d.dt, two = 2;
   const REAL *Dx = s->Dx, *Dz = s->Dz;
 #pragma omp parallel for schedule(static)
@@ -307,9 +332,11 @@ static void NAME(vel_fused)(SIM* s, long n) {
     for (int i = 4; i < nx - 4; ++i) {
       const size_t c = (size_t)j * nx + i;
       REAL dsxx_x = OP_P(s->sxx, c, 1, Dx);
-      REAL dsxz_z = OP_M(s->sxz, c, (size_t)nx, Dz);
+      REAL dsxz_z = afda && j < 8 ? NAME(afda_dz)(s->sxz, i, nx, s->afda[j - 4])
+                                  : OP_M(s->sxz, c, (size_t)nx, Dz);
       REAL dsxz_x = OP_M(s->sxz, c, 1, Dx);
-      REAL dszz_z = OP_P(s->szz, c, (size_t)nx, Dz);
+      REAL dszz_z = afda && j < 7 ? NAME(afda_dz)(s->szz, i, nx, s->afda[4 + j - 4])
+                                  : OP_P(s->szz, c, (size_t)nx, Dz);
       const int xk = NAME(xk)(s, i);
       if (xk >= 0) {
         size_t p = (size_t)j * 2 * T + xk;
@@ -418,14 +445,17 @@ static void NAME(stress_fused)(SIM* s, long n) {
   const int nx = s->d.nx, nz = s->d.nz, T = s->T;
   const REAL *Dx = s->Dx, *Dz = s->Dz;
   const int fs = s->d.free_surface;
+  const int afda = fs == 2;
 #pragma omp parallel for schedule(static)
   for (int j = 4; j < nz - 4; ++j) {
     const int zk = NAME(zk)(s, j);
     for (int i = 4; i < nx - 4; ++i) {
       const size_t c = (size_t)j * nx + i;
       REAL exx = OP_M(s->vx, c, 1, Dx);
-      REAL ezz = OP_M(s->vz, c, (size_t)nx, Dz);
-      REAL ezx = OP_P(s->vx, c, (size_t)nx, Dz);
+      REAL ezz = afda && j >= 5 && j < 8 ? NAME(afda_dz)(s->vz, i, nx, s->afda[7 + j - 5])
+                                         : OP_M(s->vz, c, (size_t)nx, Dz);
+      REAL ezx = afda && j < 7 ? NAME(afda_dz)(s->vx, i, nx, s->afda[10 + j - 4])
+                               : OP_P(s->vx, c, (size_t)nx, Dz);
       REAL exz = OP_P(s->vz, c, 1, Dx);
       const int xk = NAME(xk)(s, i);
       if (xk >= 0) {
@@ -444,7 +474,7 @@ static void NAME(stress_fused)(SIM* s, long n) {
       if (s->L) NAME(memory_cell)(s, c, exx, ezz_used, gxz);
     }
   }
-  if (fs) NAME(images)(s);
+  if (fs == 1) NAME(images)(s);
 }
 
 /* ---- reference-structured multi-pass path (ViscoelasticProblem.jl:62-174) ----------- */
@@ -464,6 +494,16 @@ static void NAME(deriv_pass)(SIM* s, REAL* out, const REAL* u, int plus, int alo
       const size_t c = (size_t)j * nx + i;
       out[c] = plus ? OP_P(u, c, st, D) : OP_M(u, c, st, D);
     }
+}
+
+/* H-AFDA rows: overwrite rows [j0, j1) of a d/dz scratch array with the one-sided stencils
+ * whose table rows start at r0 */
+static void NAME(afda_pass)(SIM* s, REAL* out, const REAL* u, int r0, int j0, int j1) {
+  if (s->d.free_surface != 2) return;
+  const int nx = s->d.nx;
+  for (int j = j0; j < j1; ++j)
+    for (int i = 4; i < nx - 4; ++i)
+      out[(size_t)j * nx + i] = NAME(afda_dz)(u, i, nx, s->afda[r0 + j - j0]);
 }
 
 /* _cpml_scale_{x,z}! analogue with the psi recursion folded in */
@@ -496,6 +536,8 @@ static void NAME(vel_multipass)(SIM* s, long n) {
   NAME(deriv_pass)(s, d2, s->sxz, 0, 1); /* dz_sxz */
   NAME(deriv_pass)(s, d3, s->sxz, 0, 0); /* dx_szx */
   NAME(deriv_pass)(s, d4, s->szz, 1, 1); /* dz_szz */
+  NAME(afda_pass)(s, d2, s->sxz, 0, 4, 8);
+  NAME(afda_pass)(s, d4, s->szz, 4, 4, 7);
   NAME(cpml_pass)(s, d1, s->psi[2], 0);
   NAME(cpml_pass)(s, d3, s->psi[3], 0);
   NAME(cpml_pass)(s, d2, s->psi[6], 1);
@@ -540,6 +582,8 @@ static void NAME(stress_multipass)(SIM* s, long n) {
   NAME(deriv_pass)(s, e2, s->vz, 0, 1); /* dzvz */
   NAME(deriv_pass)(s, e3, s->vx, 1, 1); /* dzvx */
   NAME(deriv_pass)(s, e4, s->vz, 1, 0); /* dxvz */
+  NAME(afda_pass)(s, e2, s->vz, 7, 5, 8);
+  NAME(afda_pass)(s, e3, s->vx, 10, 4, 7);
   NAME(cpml_pass)(s, e1, s->psi[0], 0);
   NAME(cpml_pass)(s, e4, s->psi[1], 0);
   NAME(cpml_pass)(s, e3, s->psi[4], 1);
@@ -607,7 +651,7 @@ static void NAME(stress_multipass)(SIM* s, long n) {
         NAME(memory_cell)(s, c, e1[c], e2[c], e3[c]);
       }
   }
-  if (fs) NAME(images)(s);
+  if (fs == 1) NAME(images)(s);
 }
 
 static void NAME(sample)(SIM* s, REAL* out) {

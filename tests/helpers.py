@@ -25,7 +25,8 @@ def make_oracle(cfg, dtype):
     from oracle import vio
     o = vio.Oracle(cfg["nx"], cfg["nz"], cfg["dx"], cfg["dz"], cfg["dt"], cfg["model"],
                    dtype=dtype, nrelax=cfg["nrelax"], taun=cfg.get("taun"),
-                   free_surface=cfg["free_surface"], cpml=cfg["cpml"])
+                   free_surface=cfg["free_surface"], cpml=cfg["cpml"],
+                   afda_points=cfg.get("afda_points"))
     if len(cfg["src_ix"]):
         o.set_sources(cfg["src_ix"], cfg["src_iz"], cfg["src_kind"], cfg["wavelets"])
     if len(cfg["rec_ix"]):

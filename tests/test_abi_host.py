@@ -55,6 +55,26 @@ def test_cpml_profile_matches_oracle_bitwise(oracle_mod, axis):
     assert np.array_equal(k, ref["kappa"])
 
 
+@pytest.mark.parametrize("counts", [None, [8] * 13, [1, 2, 3, 4, 2, 3, 4, 2, 3, 4, 2, 3, 4]])
+def test_afda_weights_match_oracle_bitwise(oracle_mod, counts):
+    """The library's H-AFDA table (host code, no device) equals the oracle's, which is pinned
+    against the exact Taylor solutions; the default point counts agree too."""
+    from viscoelasticinversion_amd import stepper
+    assert stepper.afda_default_points() == oracle_mod.afda_default_points()
+    for h in (1.0, 2.5, 5.0, 7.3):
+        assert np.array_equal(stepper.afda_weights(h, counts), oracle_mod.afda_weights(h, counts))
+
+
+def test_afda_weights_reject_bad_counts():
+    from viscoelasticinversion_amd import _lib, stepper
+    for bad in ([0] + [4] * 12, [4] * 12 + [9], [4] * 4 + [1] + [4] * 8):
+        with pytest.raises(_lib.ViError):
+            stepper.afda_weights(1.0, bad)
+    with pytest.raises(ValueError):
+        stepper.free_surface_mode("lamb")
+    assert [stepper.free_surface_mode(m) for m in (False, True, "none", "image", "afda")] == [0, 1, 0, 1, 2]
+
+
 def test_cpml_auto_d0_matches_config():
     from viscoelasticinversion_amd import CPMLConfig, _lib
     lib = _lib.lib()

@@ -265,7 +265,7 @@ class Parameters:
     device: str = "mi355x"
     model: Optional[object] = None  # IsotropicModel or VTIModel
     cpml: Optional[CPMLConfig] = None
-    free_surface: bool = False
+    free_surface: object = False  # False, True / "image" (stress imaging) or "afda" (H-AFDA)
     precision: str = "float32"
     qmodel: Optional[QModel] = None
     receivers: Optional[tuple] = None

@@ -109,6 +109,8 @@ def build_simulation(cfg, precision="float32"):
     if cfg["cpml"]:
         sim.set_cpml(**cfg["cpml"])
     sim.set_free_surface(cfg["free_surface"])
+    if cfg.get("afda_points") is not None:
+        sim.set_afda_points(cfg["afda_points"])
     sim.set_sources(cfg["src_ix"], cfg["src_iz"], cfg["src_kind"], cfg["wavelets"])
     sim.set_receivers(cfg["rec_ix"], cfg["rec_iz"], cfg["rec_comp"], cfg["nt"])
     return sim

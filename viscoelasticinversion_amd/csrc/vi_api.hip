@@ -186,7 +186,9 @@ struct vi_sim_s {
   bool relax_set = false;
   bool cpml_on = false;
   vi_cpml_desc cpml{};
-  int fs = 0;
+  int fs = 0;                 // 0 none, 1 stress imaging, 2 H-AFDA
+  int afda_pts[13] = {3, 4, 6, 8, 3, 4, 6, 2, 4, 6, 3, 4, 6};  // points per one-sided stencil
+  DevBuf afda;                // H-AFDA weights / dz, [13][8] in the handle's precision
   int nsrc = 0, snt = 0;
   int nrec = 0, rec_nt = 0;
   std::vector<int> src_ix, src_iz, src_kind, rec_ix, rec_iz, rec_comp;
@@ -205,6 +207,7 @@ struct vi_sim_s {
 
 struct BatchGroup {
   int esz = 4, L = 0;
+  bool afda = false;  // some member has the H-AFDA surface (launch that variant)
   std::vector<vi_sim_s*> members;
   int ntiles = 0;
   std::vector<char> desc_host;
@@ -402,6 +405,7 @@ void build_desc(const vi_sim_s* h, int tile_base, vi::ShotDesc<R>* d) {
     for (int q = 0; q < 8; ++q) d->psi[q] = (R*)h->psi[q].p;
   }
   d->fs = h->fs;
+  d->afda = h->fs == 2 ? (const R*)h->afda.p : nullptr;
   const int* st = (const int*)h->src_tab.p;
   d->src_off = st;
   d->src_cell = st + (h->ntiles + 1);
@@ -462,7 +466,7 @@ void launch(K kern, dim3 grid, dim3 block, hipStream_t s, hipEvent_t e0, hipEven
   else hipLaunchKernelGGL(kern, grid, block, 0, s, args...);
 }
 
-template <typename R, int RZ>
+template <typename R, int RZ, bool AFDA>
 int launch_kernel(hipStream_t s, int L, int k, const void* descs, const int* tile_shot,
                   int ntiles, int kstep, hipEvent_t e0, hipEvent_t e1) {
   const dim3 block(vi::TX, vi::NW);
@@ -470,16 +474,13 @@ int launch_kernel(hipStream_t s, int L, int k, const void* descs, const int* til
   const vi::ShotDesc<R>* d = (const vi::ShotDesc<R>*)descs;
   const vi::RecSlot rs{};
   if (k == 0) {
-    launch(vi::k_velocity<R, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs);
+    launch(vi::k_velocity<R, RZ, false, AFDA>, grid, block, s, e0, e1, d, tile_shot, kstep, rs);
   } else {
     switch (L) {
-      case 0: launch(vi::k_stress<R, 0, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
-      case 1: launch(vi::k_stress<R, 1, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
-      case 2: launch(vi::k_stress<R, 2, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
-      case 3: launch(vi::k_stress<R, 3, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
-      case 4: launch(vi::k_stress<R, 4, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
-      case 5: launch(vi::k_stress<R, 5, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
-      case 6: launch(vi::k_stress<R, 6, RZ>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
+#define VI_STRESS(l) \
+  case l: launch(vi::k_stress<R, l, RZ, false, AFDA>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
+      VI_STRESS(0) VI_STRESS(1) VI_STRESS(2) VI_STRESS(3) VI_STRESS(4) VI_STRESS(5) VI_STRESS(6)
+#undef VI_STRESS
       default: return fail(VI_EINVAL, "nrelax out of range");
     }
   }
@@ -522,29 +523,30 @@ int launch_rec(hipStream_t s, int rz, int L, int k, const void* descs, int ntile
   return fail(VI_EINVAL, "recording launches need 1 or 2 rows per wavefront");
 }
 
+// afda: some shot of the launch has the H-AFDA surface (the variant with those rows)
 template <typename R>
 int launch_one(hipStream_t s, int rz, int L, int k, const void* descs, const int* tile_shot,
-               int ntiles, int kstep, hipEvent_t e0, hipEvent_t e1) {
-  switch (rz) {
-    case 1: return launch_kernel<R, 1>(s, L, k, descs, tile_shot, ntiles, kstep, e0, e1);
-    case 2: return launch_kernel<R, 2>(s, L, k, descs, tile_shot, ntiles, kstep, e0, e1);
-    case 4: return launch_kernel<R, 4>(s, L, k, descs, tile_shot, ntiles, kstep, e0, e1);
-    case 8: return launch_kernel<R, 8>(s, L, k, descs, tile_shot, ntiles, kstep, e0, e1);
-  }
+               int ntiles, int kstep, hipEvent_t e0, hipEvent_t e1, bool afda) {
+#define VI_RZ(r)                                                                              \
+  case r:                                                                                     \
+    return afda ? launch_kernel<R, r, true>(s, L, k, descs, tile_shot, ntiles, kstep, e0, e1) \
+                : launch_kernel<R, r, false>(s, L, k, descs, tile_shot, ntiles, kstep, e0, e1);
+  switch (rz) { VI_RZ(1) VI_RZ(2) VI_RZ(4) VI_RZ(8) }
+#undef VI_RZ
   return fail(VI_EINVAL, "bad tiling");
 }
 
 // nsteps full steps of one group of shots (same precision and L), then advance their step
 // indices on the device.
 int launch_steps(hipStream_t s, size_t esz, int rz, int L, void* descs, const int* tile_shot,
-                 int ntiles, int nshots, int nsteps, Prof* prof) {
+                 int ntiles, int nshots, int nsteps, Prof* prof, bool afda) {
   if (nshots == 1) tile_shot = nullptr;  // every tile belongs to shot 0
   for (int q = 0; q < nsteps; ++q) {
     for (int k = 0; k < 2; ++k) {
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (prof && prof->on && q % prof->every == 0) VI_TRY(prof->begin(k, &e0, &e1));
-      VI_TRY(esz == 4 ? launch_one<float>(s, rz, L, k, descs, tile_shot, ntiles, q, e0, e1)
-                      : launch_one<double>(s, rz, L, k, descs, tile_shot, ntiles, q, e0, e1));
+      VI_TRY(esz == 4 ? launch_one<float>(s, rz, L, k, descs, tile_shot, ntiles, q, e0, e1, afda)
+                      : launch_one<double>(s, rz, L, k, descs, tile_shot, ntiles, q, e0, e1, afda));
     }
   }
   if (nsteps > 0) {
@@ -565,7 +567,8 @@ int step_single(vi_sim_s* h, int nsteps) {
   VI_TRY(wait_external(h));
   VI_TRY(refresh_own_desc(h));
   VI_TRY(launch_steps(h->stream, h->esz, h->rz, h->L, h->desc_dev.p,
-                      (const int*)h->tile_shot_dev.p, h->ntiles, 1, nsteps, &h->prof));
+                      (const int*)h->tile_shot_dev.p, h->ntiles, 1, nsteps, &h->prof,
+                      h->fs == 2));
   h->step += nsteps;
   h->own_pending = true;
   return 0;
@@ -652,6 +655,8 @@ int batch_build(vi_batch_s* b) {
       base += g.members[q]->ntiles;
     }
     g.ntiles = base;
+    g.afda = false;
+    for (auto* m : g.members) g.afda = g.afda || m->fs == 2;
     VI_TRY(g.desc_dev.reserve(g.desc_host.size()));
     VI_TRY(g.tile_shot_dev.reserve(ts.size() * sizeof(int)));
     VI_HIP(hipMemcpyAsync(g.desc_dev.p, g.desc_host.data(), g.desc_host.size(),
@@ -675,7 +680,7 @@ int batch_launch(vi_batch_s* b, int nsteps) {
     BatchGroup& g = *gp;
     VI_TRY(launch_steps(b->stream, g.esz, b->rz, g.L, g.desc_dev.p,
                         (const int*)g.tile_shot_dev.p, g.ntiles, (int)g.members.size(), nsteps,
-                        &b->prof));
+                        &b->prof, g.afda));
   }
   return 0;
 }
@@ -996,7 +1001,7 @@ struct GradRun : vi_grad_s {
           if (n == 0) VI_TRY(copy_state(s, true));
           const int m = std::min(len - n, budget);
           VI_TRY(launch_steps(st, E, h->rz, L, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
-                              h->ntiles, 1, m, nullptr));
+                              h->ntiles, 1, m, nullptr, false));
           n += m;
           budget -= m;
         }
@@ -1122,6 +1127,70 @@ void vi_fd_coeffs(double h, double out[4]) {
   out[1] = (-245.0 / 3072.0) / h;
   out[2] = (49.0 / 5120.0) / h;
   out[3] = (-5.0 / 7168.0) / h;
+}
+
+// H-AFDA one-sided weights (Kristek, Moczo & Archuleta 2002; the reference's draft
+// Stencil.jl:52-105 formulas #1, #2, #4 and free-surface-kernels.jl:16-214), carried to every
+// row an 8th-order centred stencil would take above the surface. Table rows: A0..A3 (d sxz/dz
+// at z = k h from sxz(0) = 0 and sxz at (m+1/2) h), B0..B2 (d szz/dz at (k+1/2) h from szz at
+// m h), C1..C3 (d vz/dz at k h from vz at (m+1/2) h), D0..D2 (d vx/dz at (k+1/2) h from vx at
+// m h); column m multiplies grid row 4 + m, m < counts[row]. Each weight is the derivative of
+// a Lagrange basis polynomial at x0,
+//   w_m = sum_{q != m} prod_{l != m, q} (x0 - x_l) / prod_{l != m} (x_m - x_l),
+// computed in integers of h/2 (every factor is at most 16, so both sums are exact in int64
+// and in double): one rounding, in the final division.
+static const int kAfdaDefault[13] = {3, 4, 6, 8, 3, 4, 6, 2, 4, 6, 3, 4, 6};
+
+int vi_afda_default_points(int out[13]) {
+  if (!out) return fail(VI_EINVAL, "vi_afda_default_points: null output");
+  for (int r = 0; r < 13; ++r) out[r] = kAfdaDefault[r];
+  return 0;
+}
+
+int vi_afda_weights(double h, const int* counts, double out[13][8]) {
+  if (!out || !(h > 0)) return fail(VI_EINVAL, "vi_afda_weights: need h > 0 and an output");
+  const int* nc = counts ? counts : kAfdaDefault;
+  for (int r = 0; r < 13; ++r)
+    if (nc[r] < (r < 4 ? 1 : 2) || nc[r] > 8)
+      return fail(VI_EINVAL, "vi_afda_weights: point count " + std::to_string(r) +
+                                 " outside [" + (r < 4 ? "1" : "2") + ", 8]");
+  for (int r = 0; r < 13; ++r) {
+    long long X[9];  // positions of the points in units of h/2
+    long long X0;    // position of the derivative
+    int np = 0, first = 0;
+    const int n = nc[r];
+    if (r < 4) {  // A_k: the boundary point first (value 0), its weight not stored
+      X[np++] = 0;
+      for (int m = 0; m < n; ++m) X[np++] = 2 * m + 1;
+      first = 1;
+      X0 = 2 * r;
+    } else if (r < 7) {
+      for (int m = 0; m < n; ++m) X[np++] = 2 * m;
+      X0 = 2 * (r - 4) + 1;
+    } else if (r < 10) {
+      for (int m = 0; m < n; ++m) X[np++] = 2 * m + 1;
+      X0 = 2 * (r - 6);
+    } else {
+      for (int m = 0; m < n; ++m) X[np++] = 2 * m;
+      X0 = 2 * (r - 10) + 1;
+    }
+    for (int m = 0; m < 8; ++m) out[r][m] = 0.0;
+    for (int m = first; m < np; ++m) {
+      long long den = 1, num = 0;
+      for (int l = 0; l < np; ++l)
+        if (l != m) den *= X[m] - X[l];
+      for (int q = 0; q < np; ++q) {
+        if (q == m) continue;
+        long long t = 1;
+        for (int l = 0; l < np; ++l)
+          if (l != m && l != q) t *= X0 - X[l];
+        num += t;
+      }
+      // den holds one factor h/2 more than num: 2 num / den per unit h
+      out[r][m - first] = ((double)(2 * num) / (double)den) / h;
+    }
+  }
+  return 0;
 }
 
 double vi_cpml_auto_d0(int thickness, double vmax, double dx, double R, double power) {
@@ -1355,12 +1424,44 @@ int vi_set_cpml(vi_handle h, const vi_cpml_desc* c) {
   return 0;
 }
 
-int vi_set_free_surface(vi_handle h, int enabled) {
+static int upload_afda(vi_sim_s* h) {
+  double w[13][8];
+  VI_TRY(vi_afda_weights(h->g.dz, h->afda_pts, w));
+  std::vector<char> t(13 * 8 * h->esz);
+  for (int r = 0; r < 13; ++r)
+    for (int m = 0; m < 8; ++m) {
+      if (h->esz == 4) ((float*)t.data())[r * 8 + m] = (float)w[r][m];
+      else ((double*)t.data())[r * 8 + m] = w[r][m];
+    }
+  VI_TRY(wait_external(h));
+  VI_HIP(hipStreamSynchronize(h->stream));
+  VI_TRY(h->afda.alloc(t.size(), h->stream));
+  VI_HIP(hipMemcpyAsync(h->afda.p, t.data(), t.size(), hipMemcpyHostToDevice, h->stream));
+  VI_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int vi_set_free_surface(vi_handle h, int mode) {
   VI_TRY(check_handle(h));
-  if (enabled && h->cpml_on && h->cpml.top)
+  if (mode < 0 || mode > 2)
+    return fail(VI_EINVAL, "vi_set_free_surface: mode must be 0 (none), 1 (imaging) or 2 (H-AFDA)");
+  if (mode && h->cpml_on && h->cpml.top)
     return fail(VI_EINVAL, "vi_set_free_surface: conflicts with a top CPML strip");
-  if (enabled && h->g.nz < 17) return fail(VI_EINVAL, "vi_set_free_surface: nz must be >= 17");
-  h->fs = enabled ? 1 : 0;
+  if (mode && h->g.nz < 17) return fail(VI_EINVAL, "vi_set_free_surface: nz must be >= 17");
+  if (mode == 2) VI_TRY(upload_afda(h));
+  h->fs = mode;
+  touch(h);
+  return 0;
+}
+
+int vi_set_afda_points(vi_handle h, const int* counts) {
+  VI_TRY(check_handle(h));
+  int pts[13];
+  for (int r = 0; r < 13; ++r) pts[r] = counts ? counts[r] : kAfdaDefault[r];
+  double w[13][8];
+  VI_TRY(vi_afda_weights(1.0, pts, w));  // range check
+  std::memcpy(h->afda_pts, pts, sizeof pts);
+  if (h->fs == 2) VI_TRY(upload_afda(h));
   touch(h);
   return 0;
 }
@@ -1708,6 +1809,9 @@ static int gradient_checks(vi_handle h, int nt, const void* dobs, int checkpoint
   VI_TRY(check_ready(h));
   if (h->nrec < 1 || h->rec_nt < nt)
     return fail(VI_ESTATE, std::string(who) + ": receivers with capacity >= nt are required");
+  if (h->fs == 2)
+    return fail(VI_ESTATE, std::string(who) + ": the adjoint step has no H-AFDA surface; "
+                                              "use the imaging free surface (mode 1)");
   VI_TRY(wait_external(h));
   VI_HIP(hipStreamSynchronize(h->stream));
   return 0;

@@ -54,7 +54,9 @@ struct ShotDesc {
   // isotropic (vti = 0): C11 = C33 = lam + 2 mu computed in the kernel
   const R *c11, *c33;
   int vti;
-  int T, xlo, xhi, zlo, zhi, fs;
+  int T, xlo, xhi, zlo, zhi;
+  int fs;          // free surface on row 4: 0 none, 1 stress imaging, 2 H-AFDA
+  const R* afda;   // fs = 2: one-sided d/dz weights [13][8] over rows 4..11 (A0..3 B0..2 C1..3 D0..2)
   const R *ax, *bx, *kx, *az, *bz, *kz;
   R* psi[8];  // 0..3: [nz][2T] (vx_x, vz_x, sxx_x, sxz_x); 4..7: [2T][ldx] (vx_z, vz_z, sxz_z, szz_z)
   // sources / receivers grouped by tile (CSR over this shot's tiles)
@@ -107,6 +109,19 @@ __device__ __forceinline__ R op_m(const R* q, int st, const R* D) {
   v = v + D[1] * (q[st] - q[-2 * st]);
   v = v + D[2] * (q[2 * st] - q[-3 * st]);
   v = v + D[3] * (q[3 * st] - q[-4 * st]);
+  return v;
+}
+
+// H-AFDA one-sided d/dz from grid rows 4..11 of a staged tile column (q = row 4, st = LDS
+// row pitch) with the weights of table row tr (wave-uniform), accumulated in the same order
+// as the oracle: w0 q0, then + w_m q_m. The weights are read through the scalar cache.
+template <typename R>
+__device__ __forceinline__ R afda_dz(const R* q, int st, const R* tab, int tr) {
+  const __attribute__((address_space(4))) R* w =
+      (const __attribute__((address_space(4))) R*)tab + __builtin_amdgcn_readfirstlane(tr * 8);
+  R v = w[0] * q[0];
+#pragma unroll
+  for (int m = 1; m < 8; ++m) v = v + w[m] * q[m * st];
   return v;
 }
 
@@ -350,11 +365,14 @@ struct ListIn {
   }
 };
 
-template <typename R, int RZ, bool REC = false>
+// AFDA: the variant with the H-AFDA rows (any shot of the launch may use them; the others
+// run the same code as without)
+template <typename R, int RZ, bool REC = false, bool AFDA = false>
 __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restrict__ shots,
                                                      const int* __restrict__ tile_shot,
                                                      int kstep, const RecSlot rs) {
   static_assert(!REC || RZ <= 2, "recording needs the prefetched CPML memory");
+  static_assert(!(REC && AFDA), "the adjoint has no H-AFDA surface");
   constexpr int TZ = NW * RZ;
   constexpr int WX = TX + 8;
   constexpr unsigned E = sizeof(R);
@@ -409,6 +427,7 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
 
   const int nx = d.nx, nz = d.nz, T = d.T;
   const bool col_ok = i < nx - 4;
+  const bool afda = AFDA && d.fs == 2;
   const R dt = d.dt, two = R(2);
   const R Dx[4] = {d.Dx[0], d.Dx[1], d.Dx[2], d.Dx[3]};
   const R Dz[4] = {d.Dz[0], d.Dz[1], d.Dz[2], d.Dz[3]};
@@ -422,6 +441,13 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
     R dsxz_x = op_m(&s_xz[(tr + 4) * WX + lane + 4], 1, Dx);
     R dsxz_z = op_m(&s_xz[(tr + 4) * WX + lane + 4], WX, Dz);
     R dszz_z = op_p(&s_zz[(tr + 4) * TX + lane], TX, Dz);
+    // H-AFDA rows (only in the first tile row, where LDS row = grid row); ju is j, known to
+    // the compiler as wave-uniform, so this is a scalar branch
+    const int ju = __builtin_amdgcn_readfirstlane(jw) + r;
+    if (AFDA && afda && ju < 8) {
+      dsxz_z = afda_dz(&s_xz[4 * WX + lane + 4], WX, d.afda, ju - 4);      // A_{j-4}
+      if (ju < 7) dszz_z = afda_dz(&s_zz[4 * TX + lane], TX, d.afda, ju);  // B_{j-4}
+    }
     const bool ok = col_ok && j < nz - 4;
     if (T && ok) {
       if (xk >= 0) {
@@ -487,16 +513,17 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
   }
 }
 
-template <typename R, int L, int RZ, bool REC>
+template <typename R, int L, int RZ, bool REC, bool AFDA>
 constexpr int stress_waves() {
-  return (sizeof(R) == 4 && RZ == 1 && L <= 3 && !REC) ? 8 : 1;
+  return (sizeof(R) == 4 && RZ == 1 && L <= 3 && !REC && !AFDA) ? 8 : 1;
 }
 
-template <typename R, int L, int RZ, bool REC = false>
-__global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC>())) void k_stress(const ShotDesc<R>* __restrict__ shots,
+template <typename R, int L, int RZ, bool REC = false, bool AFDA = false>
+__global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void k_stress(const ShotDesc<R>* __restrict__ shots,
                                                    const int* __restrict__ tile_shot,
                                                    int kstep, const RecSlot rs) {
   static_assert(!REC || RZ <= 2, "recording needs the prefetched CPML memory");
+  static_assert(!(REC && AFDA), "the adjoint has no H-AFDA surface");
   constexpr int TZ = NW * RZ;
   constexpr int WX = TX + 8;
   constexpr unsigned E = sizeof(R);
@@ -617,6 +644,13 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC>())) void k_str
     R exz = op_p(pvz, 1, Dx);
     R ezz = op_m(pvz, WX, Dz);
     R ezx = op_p(pvx, WX, Dz);
+    // H-AFDA rows (only in the first tile row, where LDS row = grid row); ju is j, known to
+    // the compiler as wave-uniform, so this is a scalar branch
+    const int ju = __builtin_amdgcn_readfirstlane(jw) + r;
+    if (AFDA && fs == 2 && ju < 8) {
+      if (ju >= 5) ezz = afda_dz(&s_vz[4 * WX + lane + 4], WX, d.afda, ju + 2);  // C_{j-4}
+      if (ju < 7) ezx = afda_dz(&s_vx[4 * WX + lane + 4], WX, d.afda, ju + 6);   // D_{j-4}
+    }
     const bool ok = col_ok && j < nz - 4;
     if (!ok) continue;
     if constexpr (!HOIST) load_cell(r, 0);
@@ -714,7 +748,7 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC>())) void k_str
       gst(rec_plane<R>(rs, 0), cb, osxx[h]);
       gst(rec_plane<R>(rs, 1), cb, oszz[h]);
       gst(rec_plane<R>(rs, 2), cb, osxz[h]);
-      if (fs) {
+      if (fs == 1) {
         if (j >= 5 && j <= 8) gst(rec_plane<R>(rs, 1), (unsigned)((8 - j) * ldx + i) * E, -oszz[h]);
         if (j >= 4 && j <= 7) gst(rec_plane<R>(rs, 2), (unsigned)((7 - j) * ldx + i) * E, -osxz[h]);
       }
@@ -726,7 +760,7 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC>())) void k_str
     gst(sxx, cb, nsxx);
     gst(szz, cb, nszz);
     gst(sxz, cb, nsxz);
-    if (fs) {  // stress images above the surface row 4
+    if (fs == 1) {  // stress images above the surface row 4
       if (j >= 5 && j <= 8) gst(szz, (unsigned)((8 - j) * ldx + i) * E, -nszz);
       if (j >= 4 && j <= 7) gst(sxz, (unsigned)((7 - j) * ldx + i) * E, -nsxz);
     }

@@ -22,8 +22,39 @@ SOURCE_KINDS = {"force_z": L.VI_SRC_FORCE_Z, "explosive": L.VI_SRC_EXPLOSIVE,
 COMPONENTS = {"vx": 0, "vz": 1, "sxx": 2, "szz": 3, "sxz": 4}
 
 
+FREE_SURFACE_MODES = {"none": 0, "image": 1, "imaging": 1, "afda": 2}
+
+
 def _as_int32(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.int32).ravel())
+
+
+def free_surface_mode(mode) -> int:
+    """False/None/"none" -> 0, True/"image" -> 1 (stress imaging), "afda" -> 2 (H-AFDA)."""
+    if isinstance(mode, str):
+        if mode not in FREE_SURFACE_MODES:
+            raise ValueError(f"unknown free surface {mode!r}: none, image or afda")
+        return FREE_SURFACE_MODES[mode]
+    return int(bool(mode))
+
+
+def afda_weights(h, counts=None):
+    """H-AFDA one-sided d/dz weights / h as the library computes them, shape (13, 8): rows
+    A0..A3 (d sxz/dz at z = k h), B0..B2 (d szz/dz at (k+1/2) h), C1..C3 (d vz/dz at k h),
+    D0..D2 (d vx/dz at (k+1/2) h); column m multiplies grid row 4 + m (FreeSurfaceStencil,
+    Stencil.jl:52-105). Host-only; needs no GPU."""
+    out = np.zeros((13, 8))
+    cp = None
+    if counts is not None:
+        cp = (ctypes.c_int * 13)(*[int(c) for c in counts])
+    L.check(L.lib().vi_afda_weights(float(h), cp, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def afda_default_points():
+    out = (ctypes.c_int * 13)()
+    L.check(L.lib().vi_afda_default_points(out))
+    return list(out)
 
 
 class Simulation:
@@ -130,8 +161,17 @@ class Simulation:
         L.check(L.lib().vi_set_cpml(self._h, ctypes.byref(c)))
         self.cpml_thickness = int(thickness)
 
-    def set_free_surface(self, enabled=True):
-        L.check(L.lib().vi_set_free_surface(self._h, int(bool(enabled))))
+    def set_free_surface(self, mode=True):
+        """False/"none": Dirichlet top; True/"image": stress imaging (FreeSurfaceBC,
+        CPML.jl:366-393); "afda": the H-AFDA one-sided surface (free-surface-kernels.jl)."""
+        L.check(L.lib().vi_set_free_surface(self._h, free_surface_mode(mode)))
+
+    def set_afda_points(self, counts=None):
+        """Points of the 13 H-AFDA stencils (None: the default); see ``afda_weights``."""
+        cp = None
+        if counts is not None:
+            cp = (ctypes.c_int * 13)(*[int(c) for c in counts])
+        L.check(L.lib().vi_set_afda_points(self._h, cp))
 
     def set_sources(self, ix, iz, kind, wavelets):
         ix, iz = _as_int32(ix), _as_int32(iz)
