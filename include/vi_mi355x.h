@@ -145,8 +145,8 @@ int vi_set_cpml(vi_handle h, const vi_cpml_desc* c);
  * below). mode 0: none (Dirichlet ring); 1: stress imaging, szz and sxz mirrored with opposite
  * sign into rows 0..3 (CPML.jl:372-380); 2: H-AFDA, the vertical derivatives of rows 4..7 come
  * from one-sided stencils on rows 4..11 (vi_afda_weights) and nothing above the surface is
- * read or written. Needs nz >= 17 and no top CPML strip. The gradient entry points accept
- * modes 0 and 1 only. */
+ * read or written. Needs nz >= 17 and no top CPML strip. The gradient entry points take every
+ * mode (the adjoint transposes the one-sided rows). */
 int vi_set_free_surface(vi_handle h, int mode);
 /* points per H-AFDA stencil (see vi_afda_weights; NULL = default). All-8 counts, the full
  * order of the reference's FreeSurfaceStencil(8), diverge in heterogeneous media. */

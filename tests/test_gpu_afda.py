@@ -1,7 +1,8 @@
 """H-AFDA free surface on the GPU: the HIP stepper against the CPU oracle on identical seeded
 inputs (bar: relative L2 <= 1e-5 FP32, <= 1e-12 FP64; the paths share the operation order, so
 the results are identical in practice), every tiling variant, VTI, custom point counts, and
-shots with different free surfaces mixed in one batched launch."""
+shots with different free surfaces mixed in one batched launch, and the H-AFDA gradient's
+independence of the checkpoint interval."""
 import numpy as np
 import pytest
 
@@ -79,16 +80,24 @@ def test_mixed_free_surfaces_in_one_batch():
         s.close()
 
 
-def test_afda_gradient_is_rejected():
-    from viscoelasticinversion_amd import _lib
+def test_afda_gradient_checkpoint_invariant():
+    """The H-AFDA gradient (finite differences in test_gpu_gradient.py) does not depend on
+    the checkpoint interval: recording recompute (1, 7), sqrt(nt) and store-all agree."""
     from viscoelasticinversion_amd.configs import build_simulation
-    cfg = small_cfg(nx=64, nz=40, nrelax=2, nt=20, fs="afda")
-    sim = build_simulation(cfg, "float32")
-    dobs = np.zeros((len(cfg["rec_ix"]), 20), np.float32)
-    with pytest.raises(_lib.ViError) as e:
-        sim.gradient(dobs, nt=20)
-    assert "H-AFDA" in str(e.value)
-    # the handle stays usable
-    sim.step(5)
-    sim.sync()
+    cfg = small_cfg(nx=80, nz=50, nrelax=2, nt=45, fs="afda")
+    sim = build_simulation(cfg, "float64")
+    sim.step(45)
+    dobs = sim.traces(45) * 1.01 + 1e-3
     sim.close()
+    print("[case] afda gradient checkpoint intervals", flush=True)
+    res = []
+    for K in (1, 7, 0, 45):
+        sim = build_simulation(cfg, "float64")
+        res.append(sim.gradient(dobs, 45, checkpoint_every=K))
+        sim.close()
+    J0, g0 = res[0]
+    assert J0 > 0
+    for J, g in res[1:]:
+        assert J == J0
+        for k in g0:
+            assert np.array_equal(g[k], g0[k]), k

@@ -1,6 +1,7 @@
 """Gradient of the L2 trace misfit by the hand-written discrete adjoint, checked against
 central finite differences of the forward map (FP64) at random grid points and along random
-directions, for elastic and viscoelastic (L = 1, 3) models with CPML and free surface.
+directions, for elastic and viscoelastic (L = 1, 3) models with CPML and free surface (stress
+imaging and H-AFDA).
 The forward map itself is pinned to the CPU oracle by test_gpu_parity.py."""
 import numpy as np
 import pytest
@@ -49,7 +50,8 @@ def _setup(nrelax, fs, cpml, seed, nt=120):
 
 
 @pytest.mark.parametrize("nrelax,fs,cpml", [(0, False, False), (0, True, True), (1, True, True),
-                                            (3, True, True)])
+                                            (3, True, True), (0, "afda", False),
+                                            (3, "afda", True)])
 def test_gradient_matches_finite_differences(nrelax, fs, cpml):
     cfg, dobs = _setup(nrelax, fs, cpml, seed=11 + nrelax)
     nt = cfg["nt"]

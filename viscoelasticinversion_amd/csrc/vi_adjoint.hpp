@@ -13,7 +13,11 @@
 // The forward quantities each adjoint kernel needs (strain rates, stress divergence, CPML
 // state, memory variables) are recomputed from the stored forward state of that step.
 // The forward scheme is the one of src/visco-kernels.jl:3-85 with the CPML of
-// src/sciml_solver/CPML.jl:291-328 and the stress imaging of CPML.jl:372-380.
+// src/sciml_solver/CPML.jl:291-328 and the stress imaging of CPML.jl:372-380 (fs = 1) or the
+// H-AFDA rows of src/free-surface-kernels.jl (fs = 2). With H-AFDA the raw z-derivative G of
+// the rows next to the surface is a one-sided stencil G_a instead of the centred G_c, so
+// G^T e = G_c^T e + (G_a - G_c)^T e_A over those rows (e_A): the gathers add the second term,
+// from a table of G_a - G_c for the target rows 4..11, to the centred transposed stencils.
 #pragma once
 #include "vi_kernels.hpp"
 
@@ -26,6 +30,9 @@ struct AdjDesc {
   R dt, Dx[4], Dz[4];
   R a1[MAXL], a2[MAXL];
   int L, T, xlo, xhi, zlo, zhi, fs;
+  // fs = 2: H-AFDA weights [13][8] (rows A0..3 B0..2 C1..3 D0..2 over grid rows 4..11),
+  // followed by [13][8] of G_a - G_c (the same source rows, target rows 4..11)
+  const R* afda;
   size_t plane;
   const R *rho, *lam, *mu, *tau;  // model (pitch ldm); VTI: lam = C13, mu = C55
   const R *c11, *c33;             // VTI only
@@ -135,7 +142,7 @@ struct AdjStressIn {
     sxxh = gld((const R*)d.asxx, cb);
     szz_c = gld((const R*)d.aszz, cb);
     sxz_c = gld((const R*)d.asxz, cb);
-    const bool imz = d.fs && j >= 5 && j <= 8, imx = d.fs && j >= 4 && j <= 7;
+    const bool imz = d.fs == 1 && j >= 5 && j <= 8, imx = d.fs == 1 && j >= 4 && j <= 7;
     szz_im = imz ? gld((const R*)d.aszz, (unsigned)((8 - j) * ldx + i) * E) : R(0);
     sxz_im = imx ? gld((const R*)d.asxz, (unsigned)((7 - j) * ldx + i) * E) : R(0);
     for (int k = 0; k < 3 * L; ++k) mh[k] = gld((const R*)(d.aM + (size_t)k * d.plane), cb);
@@ -164,7 +171,7 @@ __device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, const AdjSt
   const R lr = in.lr, mr = in.mr, t = in.t, SMxx = in.SMxx, SMzz = in.SMzz, SMxz = in.SMxz;
   const R sxxh = in.sxxh;
   R szz_c = in.szz_c, sxz_c = in.sxz_c;
-  const bool imz = d.fs && j >= 5 && j <= 8, imx = d.fs && j >= 4 && j <= 7;
+  const bool imz = d.fs == 1 && j >= 5 && j <= 8, imx = d.fs == 1 && j >= 4 && j <= 7;
   const unsigned ibz = (unsigned)((8 - j) * ldx + i) * E, ibx = (unsigned)((7 - j) * ldx + i) * E;
   R* Ma[3 * LM];
   for (int k = 0; k < 3 * L; ++k) Ma[k] = d.aM + (size_t)k * d.plane;
@@ -385,6 +392,19 @@ __device__ __forceinline__ void adj_vel_cell(const AdjDesc<R>& d, const AdjVelIn
   gst(d.dhat[3], cb, hzz);
 }
 
+// (G_a - G_c)^T e_A at target row tg (4..11, wave-uniform): sum over the nrows source rows of
+// table rows t0 .. t0+nrows-1; q = LDS address of the first source row, st = LDS row pitch.
+template <typename R>
+__device__ __forceinline__ R afda_corr(const R* q, int st, const R* tab, int t0, int nrows,
+                                       int tg) {
+  const __attribute__((address_space(4))) R* c =
+      (const __attribute__((address_space(4))) R*)tab + 104 + t0 * 8 +
+      __builtin_amdgcn_readfirstlane(tg - 4);
+  R v = c[0] * q[0];
+  for (int k = 1; k < nrows; ++k) v = v + c[k * 8] * q[k * st];
+  return v;
+}
+
 // ---- tiled transposed steps ------------------------------------------------------------
 // The adjoint kernels use the forward kernels' tiles: 64 columns x (4 RZ) rows, 4 wavefronts,
 // XCD-aware tile order, every stenciled field staged once in LDS with its halo (TileLoad).
@@ -420,8 +440,13 @@ __global__ __launch_bounds__(TX* NW) void k_adj_stress(const AdjDesc<R> d) {
     if (j >= d.nz - 4) return;
     const R* pvx = &s_vx[(w * RZ + r + 4) * WX + lane + 4];
     const R* pvz = &s_vz[(w * RZ + r + 4) * WX + lane + 4];
-    adj_stress_cell<R, L>(d, in[r], i, j, op_m(pvx, 1, d.Dx), op_p(pvx, WX, d.Dz),
-                          op_m(pvz, WX, d.Dz), op_p(pvz, 1, d.Dx));
+    R ezx = op_p(pvx, WX, d.Dz), ezz = op_m(pvz, WX, d.Dz);
+    const int ju = __builtin_amdgcn_readfirstlane(j0 + w * RZ) + r;  // j, wave-uniform
+    if (d.fs == 2 && ju < 8) {  // H-AFDA rows (first tile row: LDS row = grid row)
+      if (ju < 7) ezx = afda_dz(&s_vx[4 * WX + lane + 4], WX, d.afda, ju + 6);   // D_{j-4}
+      if (ju >= 5) ezz = afda_dz(&s_vz[4 * WX + lane + 4], WX, d.afda, ju + 2);  // C_{j-4}
+    }
+    adj_stress_cell<R, L>(d, in[r], i, j, op_m(pvx, 1, d.Dx), ezx, ezz, op_p(pvz, 1, d.Dx));
   }
 }
 
@@ -476,24 +501,34 @@ __global__ __launch_bounds__(TX* NW) void k_adj_vel(const AdjDesc<R> d) {
     const int tr = w * RZ + r;
     const unsigned cb = (unsigned)(j * ldx + i) * E;
     // exx = D-x vx, ezx = D+z vx, ezz = D-z vz, exz = D+x vz
-    const R gvx = op_p(&s_e0[tr * WX + lane + 4], 1, d.Dx) +
-                  op_m(&s_e1[(tr + 4) * TX + lane], TX, d.Dz);
-    const R gvz = op_p(&s_e2[(tr + 4) * TX + lane], TX, d.Dz) +
-                  op_m(&s_e3[tr * WX + lane + 4], 1, d.Dx);
+    R gvx = op_p(&s_e0[tr * WX + lane + 4], 1, d.Dx) +
+            op_m(&s_e1[(tr + 4) * TX + lane], TX, d.Dz);
+    R gvz = op_p(&s_e2[(tr + 4) * TX + lane], TX, d.Dz) +
+            op_m(&s_e3[tr * WX + lane + 4], 1, d.Dx);
+    const R* pxz = &s_xz[(tr + 4) * WX + lane + 4];
+    R dsxz_z = op_m(pxz, WX, d.Dz), dszz_z = op_p(&s_zz[(tr + 4) * TX + lane], TX, d.Dz);
+    const int ju = __builtin_amdgcn_readfirstlane(j0 + w * RZ) + r;  // j, wave-uniform
+    if (d.fs == 2 && ju < 12) {  // H-AFDA: rows 4..7 hold the source rows (LDS row 8 - j0 + k)
+      const int s0 = 8 - j0;
+      gvx = gvx - afda_corr(&s_e1[s0 * TX + lane], TX, d.afda, 10, 3, ju);      // D0..2
+      gvz = gvz - afda_corr(&s_e2[(s0 + 1) * TX + lane], TX, d.afda, 7, 3, ju); // C1..3
+      if (ju < 8) {  // recomputed forward divergence (first tile row: LDS row = grid row)
+        dsxz_z = afda_dz(&s_xz[4 * WX + lane + 4], WX, d.afda, ju - 4);       // A_{j-4}
+        if (ju < 7) dszz_z = afda_dz(&s_zz[4 * TX + lane], TX, d.afda, ju);   // B_{j-4}
+      }
+    }
     const R vxh = in[r].avx - gvx;
     const R vzh = in[r].avz - gvz;
     gst(d.avx, cb, vxh);
     gst(d.avz, cb, vzh);
-    const R* pxz = &s_xz[(tr + 4) * WX + lane + 4];
-    adj_vel_cell<R>(d, in[r], i, j, op_p(&s_xx[tr * WX + lane + 4], 1, d.Dx),
-                    op_m(pxz, WX, d.Dz), op_m(pxz, 1, d.Dx),
-                    op_p(&s_zz[(tr + 4) * TX + lane], TX, d.Dz), vxh, vzh);
+    adj_vel_cell<R>(d, in[r], i, j, op_p(&s_xx[tr * WX + lane + 4], 1, d.Dx), dsxz_z,
+                    op_m(pxz, 1, d.Dx), dszz_z, vxh, vzh);
   }
 }
 
 // Transposed velocity half, gather: s_hat -= D^T d_hat over rows [row0, nz-4). Rows 0..3 are
-// the free-surface image rows (row0 = 0), whose adjoint the next k_adj_stress moves to the
-// interior rows they copy. Their z-stencils reach rows -4..-1, so the d_hat planes are read
+// the free-surface image rows (row0 = 0 with imaging), whose adjoint the next k_adj_stress
+// moves to the interior rows they copy. Their z-stencils reach rows -4..-1, so the d_hat planes are read
 // from a base moved back by 4 rows into the zero guard rows in front of each plane: every
 // 32-bit byte offset stays non-negative.
 template <typename R, int RZ>
@@ -541,9 +576,15 @@ __global__ __launch_bounds__(TX* NW) void k_adj_sgather(const AdjDesc<R> d, int 
     const int tr = w * RZ + r;
     const unsigned cb = (unsigned)(j * ldx + i) * E;
     const R gsxx = op_m(&s_d0[tr * WX + lane + 4], 1, d.Dx);
-    const R gsxz = op_p(&s_d1[(tr + 4) * TX + lane], TX, d.Dz) +
-                   op_p(&s_d2[tr * WX + lane + 4], 1, d.Dx);
-    const R gszz = op_m(&s_d3[(tr + 4) * TX + lane], TX, d.Dz);
+    R gsxz = op_p(&s_d1[(tr + 4) * TX + lane], TX, d.Dz) +
+             op_p(&s_d2[tr * WX + lane + 4], 1, d.Dx);
+    R gszz = op_m(&s_d3[(tr + 4) * TX + lane], TX, d.Dz);
+    const int ju = __builtin_amdgcn_readfirstlane(j0 + w * RZ) + r;  // j, wave-uniform
+    if (d.fs == 2 && ju < 12) {  // H-AFDA (row0 = 4): source rows 4..7 at LDS row 8 - j0 + k
+      const int s0 = 8 - j0;
+      gsxz = gsxz - afda_corr(&s_d1[s0 * TX + lane], TX, d.afda, 0, 4, ju);  // A0..3
+      gszz = gszz - afda_corr(&s_d3[s0 * TX + lane], TX, d.afda, 4, 3, ju);  // B0..2
+    }
     gst(d.asxx, cb, oxx[r] - gsxx);
     gst(d.asxz, cb, oxz[r] - gsxz);
     gst(d.aszz, cb, ozz[r] - gszz);

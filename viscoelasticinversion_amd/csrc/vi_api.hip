@@ -490,7 +490,7 @@ int launch_kernel(hipStream_t s, int L, int k, const void* descs, const int* til
 
 // One half step of one shot (null tile table) that also records the forward state of the
 // step into a slot (recording variants exist for 1 and 2 rows per wavefront).
-template <typename R, int RZ>
+template <typename R, int RZ, bool AFDA>
 int launch_rec_kernel(hipStream_t s, int L, int k, const void* descs, int ntiles, int kstep,
                       const vi::RecSlot& rs) {
   const dim3 block(vi::TX, vi::NW);
@@ -498,16 +498,15 @@ int launch_rec_kernel(hipStream_t s, int L, int k, const void* descs, int ntiles
   const vi::ShotDesc<R>* d = (const vi::ShotDesc<R>*)descs;
   const int* ts = nullptr;
   if (k == 0) {
-    hipLaunchKernelGGL((vi::k_velocity<R, RZ, true>), grid, block, 0, s, d, ts, kstep, rs);
+    hipLaunchKernelGGL((vi::k_velocity<R, RZ, true, AFDA>), grid, block, 0, s, d, ts, kstep, rs);
   } else {
     switch (L) {
-      case 0: hipLaunchKernelGGL((vi::k_stress<R, 0, RZ, true>), grid, block, 0, s, d, ts, kstep, rs); break;
-      case 1: hipLaunchKernelGGL((vi::k_stress<R, 1, RZ, true>), grid, block, 0, s, d, ts, kstep, rs); break;
-      case 2: hipLaunchKernelGGL((vi::k_stress<R, 2, RZ, true>), grid, block, 0, s, d, ts, kstep, rs); break;
-      case 3: hipLaunchKernelGGL((vi::k_stress<R, 3, RZ, true>), grid, block, 0, s, d, ts, kstep, rs); break;
-      case 4: hipLaunchKernelGGL((vi::k_stress<R, 4, RZ, true>), grid, block, 0, s, d, ts, kstep, rs); break;
-      case 5: hipLaunchKernelGGL((vi::k_stress<R, 5, RZ, true>), grid, block, 0, s, d, ts, kstep, rs); break;
-      case 6: hipLaunchKernelGGL((vi::k_stress<R, 6, RZ, true>), grid, block, 0, s, d, ts, kstep, rs); break;
+#define VI_STRESS(l)                                                                        \
+  case l:                                                                                   \
+    hipLaunchKernelGGL((vi::k_stress<R, l, RZ, true, AFDA>), grid, block, 0, s, d, ts, kstep, rs); \
+    break;
+      VI_STRESS(0) VI_STRESS(1) VI_STRESS(2) VI_STRESS(3) VI_STRESS(4) VI_STRESS(5) VI_STRESS(6)
+#undef VI_STRESS
       default: return fail(VI_EINVAL, "nrelax out of range");
     }
   }
@@ -517,9 +516,13 @@ int launch_rec_kernel(hipStream_t s, int L, int k, const void* descs, int ntiles
 
 template <typename R>
 int launch_rec(hipStream_t s, int rz, int L, int k, const void* descs, int ntiles, int kstep,
-               const vi::RecSlot& rs) {
-  if (rz == 1) return launch_rec_kernel<R, 1>(s, L, k, descs, ntiles, kstep, rs);
-  if (rz == 2) return launch_rec_kernel<R, 2>(s, L, k, descs, ntiles, kstep, rs);
+               const vi::RecSlot& rs, bool afda) {
+  if (rz == 1)
+    return afda ? launch_rec_kernel<R, 1, true>(s, L, k, descs, ntiles, kstep, rs)
+                : launch_rec_kernel<R, 1, false>(s, L, k, descs, ntiles, kstep, rs);
+  if (rz == 2)
+    return afda ? launch_rec_kernel<R, 2, true>(s, L, k, descs, ntiles, kstep, rs)
+                : launch_rec_kernel<R, 2, false>(s, L, k, descs, ntiles, kstep, rs);
   return fail(VI_EINVAL, "recording launches need 1 or 2 rows per wavefront");
 }
 
@@ -908,6 +911,7 @@ struct GradRun : vi_grad_s {
       for (int l = 0; l < 4; ++l) { d.Dx[l] = sd->Dx[l]; d.Dz[l] = sd->Dz[l]; }
       for (int k = 0; k < L; ++k) { d.a1[k] = sd->a1[k]; d.a2[k] = sd->a2[k]; }
       d.T = sd->T; d.xlo = sd->xlo; d.xhi = sd->xhi; d.zlo = sd->zlo; d.zhi = sd->zhi; d.fs = sd->fs;
+      d.afda = sd->afda;
       d.rho = sd->rho; d.lam = sd->lam; d.mu = sd->mu; d.tau = sd->tau;
       d.c11 = sd->c11; d.c33 = sd->c33; d.vti = sd->vti;
       d.ax = sd->ax; d.bx = sd->bx; d.kx = sd->kx; d.az = sd->az; d.bz = sd->bz; d.kz = sd->kz;
@@ -941,7 +945,7 @@ struct GradRun : vi_grad_s {
     const dim3 blk(vi::TX, vi::NW);
     const int ntx = (d.nx - 8 + vi::TX - 1) / vi::TX;
     const dim3 gint(ntx * ((d.nz - 8 + TZ - 1) / TZ));
-    const int row0 = d.fs ? 0 : 4;
+    const int row0 = d.fs == 1 ? 0 : 4;  // image rows only with imaging
     const dim3 gsg(ntx * ((d.nz - 4 - row0 + TZ - 1) / TZ));
     switch (L) {
       case 0: hipLaunchKernelGGL((vi::k_adj_stress<R, 0, RZ>), gint, blk, 0, st, d); break;
@@ -994,14 +998,14 @@ struct GradRun : vi_grad_s {
           // step is s K here, advanced by the plain launches of the earlier segments)
           for (; n < len && budget > 0; ++n, --budget) {
             const vi::RecSlot rs = slot_of(n);
-            VI_TRY(launch_rec<R>(st, h->rz, L, 0, h->desc_dev.p, h->ntiles, n, rs));
-            VI_TRY(launch_rec<R>(st, h->rz, L, 1, h->desc_dev.p, h->ntiles, n, rs));
+            VI_TRY(launch_rec<R>(st, h->rz, L, 0, h->desc_dev.p, h->ntiles, n, rs, h->fs == 2));
+            VI_TRY(launch_rec<R>(st, h->rz, L, 1, h->desc_dev.p, h->ntiles, n, rs, h->fs == 2));
           }
         } else {
           if (n == 0) VI_TRY(copy_state(s, true));
           const int m = std::min(len - n, budget);
           VI_TRY(launch_steps(st, E, h->rz, L, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
-                              h->ntiles, 1, m, nullptr, false));
+                              h->ntiles, 1, m, nullptr, h->fs == 2));
           n += m;
           budget -= m;
         }
@@ -1042,8 +1046,8 @@ struct GradRun : vi_grad_s {
         for (; n < n1 && budget > 0; ++n, --budget) {
           // both half steps record the step's forward state into its slot as they go
           const vi::RecSlot rs = slot_of(n - n0);
-          VI_TRY(launch_rec<R>(st, h->rz, L, 0, h->desc_dev.p, h->ntiles, n - n0, rs));
-          VI_TRY(launch_rec<R>(st, h->rz, L, 1, h->desc_dev.p, h->ntiles, n - n0, rs));
+          VI_TRY(launch_rec<R>(st, h->rz, L, 0, h->desc_dev.p, h->ntiles, n - n0, rs, h->fs == 2));
+          VI_TRY(launch_rec<R>(st, h->rz, L, 1, h->desc_dev.p, h->ntiles, n - n0, rs, h->fs == 2));
         }
         if (n == n1) {
           n = n1 - 1;
@@ -1424,14 +1428,38 @@ int vi_set_cpml(vi_handle h, const vi_cpml_desc* c) {
   return 0;
 }
 
+// The H-AFDA table of a handle: the weights [13][8], then for the adjoint gathers the
+// difference G_a - G_c between each one-sided row j and the centred stencil it replaces,
+// over the target rows g = 4..11 (G_c of D- at j: D[g-j] for 0 <= g-j <= 3, -D[j-g-1] for
+// 0 <= j-g-1 <= 3; of D+: D[g-j-1], -D[j-g]).
 static int upload_afda(vi_sim_s* h) {
-  double w[13][8];
+  double w[13][8], D[4];
   VI_TRY(vi_afda_weights(h->g.dz, h->afda_pts, w));
-  std::vector<char> t(13 * 8 * h->esz);
+  vi_fd_coeffs(h->g.dz, D);
+  auto gc = [&](bool minus, int j, int g) -> double {
+    const int a = minus ? g - j : g - j - 1, b = minus ? j - g - 1 : j - g;
+    if (a >= 0 && a <= 3) return D[a];
+    if (b >= 0 && b <= 3) return -D[b];
+    return 0.0;
+  };
+  double c[13][8];
+  for (int r = 0; r < 13; ++r) {
+    // source row and operator: A_k D- at 4+k, B_k D+ at 4+k, C_k D- at 4+k (k = 1..3),
+    // D_k D+ at 4+k
+    const int j = r < 4 ? 4 + r : r < 7 ? r : r < 10 ? r - 2 : r - 6;
+    const bool minus = r < 4 || (r >= 7 && r < 10);
+    for (int m = 0; m < 8; ++m) c[r][m] = w[r][m] - gc(minus, j, 4 + m);
+  }
+  std::vector<char> t(2 * 13 * 8 * h->esz);
   for (int r = 0; r < 13; ++r)
     for (int m = 0; m < 8; ++m) {
-      if (h->esz == 4) ((float*)t.data())[r * 8 + m] = (float)w[r][m];
-      else ((double*)t.data())[r * 8 + m] = w[r][m];
+      if (h->esz == 4) {
+        ((float*)t.data())[r * 8 + m] = (float)w[r][m];
+        ((float*)t.data())[104 + r * 8 + m] = (float)c[r][m];
+      } else {
+        ((double*)t.data())[r * 8 + m] = w[r][m];
+        ((double*)t.data())[104 + r * 8 + m] = c[r][m];
+      }
     }
   VI_TRY(wait_external(h));
   VI_HIP(hipStreamSynchronize(h->stream));
@@ -1809,9 +1837,6 @@ static int gradient_checks(vi_handle h, int nt, const void* dobs, int checkpoint
   VI_TRY(check_ready(h));
   if (h->nrec < 1 || h->rec_nt < nt)
     return fail(VI_ESTATE, std::string(who) + ": receivers with capacity >= nt are required");
-  if (h->fs == 2)
-    return fail(VI_ESTATE, std::string(who) + ": the adjoint step has no H-AFDA surface; "
-                                              "use the imaging free surface (mode 1)");
   VI_TRY(wait_external(h));
   VI_HIP(hipStreamSynchronize(h->stream));
   return 0;

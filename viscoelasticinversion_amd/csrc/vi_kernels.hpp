@@ -373,7 +373,6 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
                                                      const int* __restrict__ tile_shot,
                                                      int kstep, const RecSlot rs) {
   static_assert(!REC || RZ <= 2, "recording needs the prefetched CPML memory");
-  static_assert(!(REC && AFDA), "the adjoint has no H-AFDA surface");
   constexpr int TZ = NW * RZ;
   constexpr int WX = TX + 8;
   constexpr unsigned E = sizeof(R);
@@ -526,7 +525,6 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void
                                                    const int* __restrict__ tile_shot,
                                                    int kstep, const RecSlot rs) {
   static_assert(!REC || RZ <= 2, "recording needs the prefetched CPML memory");
-  static_assert(!(REC && AFDA), "the adjoint has no H-AFDA surface");
   constexpr int TZ = NW * RZ;
   constexpr int WX = TX + 8;
   constexpr unsigned E = sizeof(R);
