@@ -3,7 +3,7 @@
 Each check is chosen so that a plausible mistake in the oracle (a wrong operator offset,
 sign, dropped term, swapped memory term, wrong a/b formula) fails it. None of them
 retypes the oracle's formulas: they use exact rational algebra, polynomial exactness,
-conservation laws, solved recursions, and wave speeds from closed-form physics.
+conservation laws, reciprocity, solved recursions, and wave speeds from closed-form physics.
 """
 import math
 from fractions import Fraction
@@ -285,6 +285,42 @@ def test_p_wave_speed(oracle_mod):
     lag = (k - (len(a) - 1)) + 0.5 * (y0 - y2) / (y0 - 2 * y1 + y2)
     c = 80 * h / (lag * dt)
     assert abs(c / vp - 1) < 0.02, c
+
+
+@pytest.mark.parametrize("pair", [((0, 1), (0, 1)), ((2, 0), (2, 0)), ((0, 0), (2, 1))])
+def test_elastic_step_is_reciprocal(oracle_mod, pair):
+    """Source-receiver reciprocity, G_ij(B, A) = G_ji(A, B). For the elastic leapfrog with
+    diagonal buoyancy and (D+)^T = -D-, the force-to-velocity transfer after any number of
+    steps is a polynomial in M^-1 K times M^-1 with K symmetric, hence symmetric: swapping
+    source and receiver gives the same trace to round-off (heterogeneous medium, Dirichlet
+    walls). A velocity-half operator that is not the negative transpose of its stress-half
+    partner (a D+/D- swap, an offset or sign error) breaks it; the buoyancy and modulus
+    averages are diagonal and do not (they are pinned elsewhere)."""
+    nx, nz, nt, dt = 60, 50, 400, 4e-4
+    rng = np.random.default_rng(2)
+    vp = 2000 + 800 * rng.random((nz, nx))
+    vs = vp / (1.6 + 0.4 * rng.random((nz, nx)))
+    rho = 1800 + 600 * rng.random((nz, nx))
+    mu = rho * vs ** 2
+    model = dict(rho=rho, lam=rho * vp ** 2 - 2 * mu, mu=mu)
+    t = np.arange(nt) * dt
+    w = (np.sin(2 * np.pi * 25 * t) * np.exp(-((t - 0.01) / 0.004) ** 2))[None, :]
+    A, B = (17, 12), (41, 33)
+    kinds = {0: 0, 2: 2}  # force_z -> vz (component 1), force_x -> vx (component 0)
+
+    def trace(src, kind, rec, comp):
+        o = oracle_mod.Oracle(nx, nz, 4.0, 5.0, dt, model, dtype=F64)
+        o.set_sources([src[0]], [src[1]], [kinds[kind]], w)
+        o.set_receivers([rec[0]], [rec[1]], [comp])
+        tr = o.run(nt)[:, 0]
+        o.close()
+        return tr
+
+    (k1, c1), (k2, c2) = pair
+    ab = trace(A, k1, B, c1)
+    ba = trace(B, k2, A, c2)
+    assert np.abs(ab).max() > 0
+    assert np.linalg.norm(ab - ba) <= 1e-13 * np.linalg.norm(ab)
 
 
 @pytest.mark.parametrize("nrelax", [0, 3])
