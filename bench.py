@@ -99,13 +99,15 @@ def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True):
     # per-kernel durations of the batched launch (events recorded by the launches)
     for sim in sims:
         sim.reset()
+    prof_steps = 50
     b.profile(1)
-    b.step(50)
+    b.step(prof_steps)
     ms_v, n_v = b.profile_read(0)
     ms_s, n_s = b.profile_read(1)
     b.profile(0)
-    bytes_s = sum(sim.algorithmic_bytes(1) for sim in sims)
+    bytes_s = sum(sim.algorithmic_bytes(1) for sim in sims)  # all shots, one step
     bytes_v = sum(sim.algorithmic_bytes(0) for sim in sims)
+    block = b.cache_block
     b.close()
     # plain stepper, one shot at a time
     done = []
@@ -123,18 +125,22 @@ def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True):
     for sim in sims:
         sim.close()
     model.close()
-    avg_s = ms_s / max(n_s, 1) * 1e-3
-    avg_v = ms_v / max(n_v, 1) * 1e-3
+    # kernel time per step (one launch per group of shots and half step)
+    step_s = ms_s / prof_steps * 1e-3
+    step_v = ms_v / prof_steps * 1e-3
+    launches = n_s / prof_steps
     return {"workload": f"{nshots} shots, widths U{{512..1536}} x 512, L=3, CPML + free surface, "
                         f"{nt} steps, FP32", "cells_per_step": cells,
             "batched": {"wall_s": round(wall_b, 4), "GPt_per_s": round(cells * nt / wall_b / 1e9, 3),
                         "shots_per_s": round(nshots / wall_b, 3),
                         "latency_p50_s": round(wall_b, 4), "latency_p99_s": round(wall_b, 4),
                         "rows_per_wave": rows, "graph_steps": graph_steps,
-                        "k_stress_avg_us": round(avg_s * 1e6, 2),
-                        "k_stress_hbm_frac": round(bytes_s / avg_s / 1e9 / HBM_PEAK_GBS, 4),
-                        "k_velocity_avg_us": round(avg_v * 1e6, 2),
-                        "k_velocity_hbm_frac": round(bytes_v / avg_v / 1e9 / HBM_PEAK_GBS, 4),
+                        "cache_block_MB": block >> 20,
+                        "launch_groups": launches,
+                        "k_stress_us_per_step": round(step_s * 1e6, 2),
+                        "k_stress_hbm_frac": round(bytes_s / step_s / 1e9 / HBM_PEAK_GBS, 4),
+                        "k_velocity_us_per_step": round(step_v * 1e6, 2),
+                        "k_velocity_hbm_frac": round(bytes_v / step_v / 1e9 / HBM_PEAK_GBS, 4),
                         "autotune_us_per_step": {str(k): round(v, 1) for k, v in table.items()},
                         "traces_finite": finite},
             "serial_plain_stepper": None if not serial else {
@@ -258,6 +264,7 @@ def pmc_probe(nshots, steps, copy_bytes=1 << 30):
     model, sims = build_multishot(mcfg, "float32")
     b = Batch(sims)
     b.set_tiling(int(os.environ.get("VI_ROWS", "2")))
+    b.set_cache_block(0)  # one launch over all shots: per-launch counters of the whole batch
     b.step(steps)
     b.sync()
     hip = ctypes.CDLL("libamdhip64.so")

@@ -236,6 +236,14 @@ int vi_batch_destroy(vi_batch b);
 int vi_batch_set(vi_batch b, const vi_handle* hs, int n);
 int vi_batch_set_tiling(vi_batch b, int rows_per_wave);
 int vi_batch_get_tiling(vi_batch b);
+/* cache blocking over shots (no counterpart in the reference): the shots of a precision/L
+ * class are cut into consecutive groups whose working set (state, memory variables, CPML
+ * memory, model columns) is at most `bytes`, and each group runs all steps of a block (a
+ * graph block, or the steps of one vi_batch_step call) before the next starts, so its data
+ * stays in the Infinity Cache. Results are unchanged. 0 = every shot in every launch; the
+ * default is 224 MB (of the 256 MB Infinity Cache). */
+int vi_batch_set_cache_block(vi_batch b, size_t bytes);
+size_t vi_batch_get_cache_block(vi_batch b);
 int vi_batch_autotune(vi_batch b, int trial_steps, int* chosen, double* us_per_step);
 /* replay a captured hipGraph of steps_per_graph steps (0 = direct launches) */
 int vi_batch_set_graph(vi_batch b, int steps_per_graph);

@@ -50,6 +50,35 @@ def test_batch_equals_oracle_and_alone(dtype):
     model.close()
 
 
+@pytest.mark.parametrize("block", [0, 1, 3 << 20])
+def test_cache_blocked_batch_equals_alone(block):
+    """Cache blocking over shots (groups of at most `block` bytes of working set run each
+    block of steps in turn; 1 byte = one shot per group) changes nothing in the results,
+    eager or with graph blocks."""
+    from viscoelasticinversion_amd import Batch
+    from viscoelasticinversion_amd.configs import build_multishot, shot_config
+    mcfg = _small_multishot(nshots=6, nt=45)
+    nt = mcfg["nt"]
+    print(f"[case] cache-blocked batch, block {block} bytes", flush=True)
+    model, sims = build_multishot(mcfg, "float32")
+    b = Batch(sims)
+    b.set_cache_block(block)
+    assert b.cache_block == block
+    b.set_graph(10)
+    b.step(17)
+    b.step(28)
+    b.sync()
+    for s, sim in enumerate(sims):
+        got = _collect(sim, nt, mcfg["nrelax"])
+        alone = run_gpu(shot_config(mcfg, s), "float32")
+        for k in got:
+            assert np.array_equal(got[k], alone[k]), (s, k)
+    b.close()
+    for sim in sims:
+        sim.close()
+    model.close()
+
+
 @pytest.mark.parametrize("rows", [1, 2, 4, 8])
 def test_batch_tilings(rows):
     from viscoelasticinversion_amd import Batch

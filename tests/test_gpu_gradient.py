@@ -93,6 +93,29 @@ def test_gradient_matches_finite_differences(nrelax, fs, cpml):
                 (key, j, i, ad, fd)
 
 
+def test_gradient_tau_to_zero_is_the_elastic_gradient():
+    """With tau = 0 the SLS terms vanish from the stress update (X1 = X tau = 0), so the
+    viscoelastic step is the elastic one and its gradient with respect to rho, lam and mu is
+    the elastic gradient (the memory variables still evolve but feed nothing back)."""
+    cfg0, dobs = _setup(0, True, True, seed=21)
+    cfg3 = small_cfg(nx=72, nz=56, nrelax=3, cpml=True, fs=True, seed=21, nt=cfg0["nt"])
+    for k in ("src_ix", "src_iz", "src_kind", "wavelets", "rec_ix", "rec_iz", "rec_comp"):
+        cfg3[k] = cfg0[k]
+    m3 = dict(cfg0["model"])
+    m3["tau"] = np.zeros_like(m3["rho"])
+    print("[case] gradient tau -> 0", flush=True)
+    s0 = _sim(cfg0, cfg0["model"])
+    J0, g0 = s0.gradient(dobs, cfg0["nt"], checkpoint_every=0)
+    s0.close()
+    s3 = _sim(cfg3, m3)
+    J3, g3 = s3.gradient(dobs, cfg0["nt"], checkpoint_every=0)
+    s3.close()
+    assert abs(J3 - J0) <= 1e-13 * J0
+    for k in ("rho", "lam", "mu"):
+        assert np.linalg.norm(g3[k] - g0[k]) <= 1e-12 * np.linalg.norm(g0[k]), k
+    assert np.abs(g3["tau"]).max() > 0  # the data are still sensitive to attenuation
+
+
 def test_gradient_checkpoint_interval_invariant():
     """The gradient does not depend on the checkpoint interval (1, 7, sqrt(nt), and nt or more:
     one segment, recorded during the forward pass without recomputation)."""

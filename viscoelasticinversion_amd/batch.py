@@ -99,6 +99,15 @@ class Batch:
     def set_graph(self, steps_per_graph):
         L.check(L.lib().vi_batch_set_graph(self._h, int(steps_per_graph)))
 
+    def set_cache_block(self, nbytes):
+        """Advance groups of shots with at most ``nbytes`` of working set one after another
+        through each block of steps (0: every shot in every launch); results are unchanged."""
+        L.check(L.lib().vi_batch_set_cache_block(self._h, int(nbytes)))
+
+    @property
+    def cache_block(self):
+        return int(L.lib().vi_batch_get_cache_block(self._h))
+
     def step(self, nsteps=1):
         L.check(L.lib().vi_batch_step(self._h, int(nsteps)))
 

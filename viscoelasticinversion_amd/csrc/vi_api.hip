@@ -222,6 +222,11 @@ struct vi_batch_s {
   bool built = false;
   int rz = 8;
   int graph_steps = 0;
+  // cache blocking over shots (0: every shot in every launch); default: groups of at most
+  // 224 MB of working set, which stay in the 256 MB Infinity Cache through a block of steps
+  // (64 shots x U{512..1536} x 512, L = 3, FP32: 38.5 GPt/s against 32.4 in lockstep, and
+  // 37.0 / 38.2 / 37.0 at 160 / 192 / 256 MB)
+  size_t block_bytes = (size_t)224 << 20;
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
   hipStream_t stream = nullptr;
@@ -630,21 +635,42 @@ void invalidate_graph(vi_batch_s* b) {
   b->graph_def = nullptr;
 }
 
+// Bytes one step of a shot streams through: its state, memory variables, CPML memory and
+// the columns of the model it reads.
+size_t working_set_bytes(vi_sim_s* h) {
+  size_t t = 0;
+  for (auto* b : mutable_arrays(h))
+    if (b != &h->traces) t += b->bytes;
+  const ModelStore& m = *h->model;
+  for (auto& a : m.arr)
+    if (a.p) t += (size_t)m.nzp * (size_t)(h->g.nx + 8) * m.esz;
+  return t;
+}
+
 int batch_build(vi_batch_s* b) {
   invalidate_graph(b);
   b->groups.clear();
-  std::map<std::pair<int, int>, size_t> gi;
+  // one launch group per (precision, L) class; with cache blocking, each class is cut into
+  // consecutive groups of at most block_bytes of working set (at least one shot each), and
+  // every group runs all steps of a block before the next starts, so a group's data stays in
+  // the Infinity Cache across its launches (shots are independent, nothing else changes)
+  std::map<std::pair<int, int>, size_t> gi;  // class -> its current (last) group
+  std::vector<size_t> filled;
   for (auto* h : b->members) {
     VI_TRY(check_ready(h));
     if (h->rz != b->rz) VI_TRY(apply_tiling(h, b->rz));
     auto key = std::make_pair((int)h->esz, h->L);
-    if (!gi.count(key)) {
+    const size_t ws = working_set_bytes(h);
+    if (!gi.count(key) ||
+        (b->block_bytes && filled[gi[key]] > 0 && filled[gi[key]] + ws > b->block_bytes)) {
       gi[key] = b->groups.size();
       b->groups.emplace_back(new BatchGroup());
       b->groups.back()->esz = (int)h->esz;
       b->groups.back()->L = h->L;
+      filled.push_back(0);
     }
     b->groups[gi[key]]->members.push_back(h);
+    filled[gi[key]] += ws;
   }
   for (auto& gp : b->groups) {
     BatchGroup& g = *gp;
@@ -1958,6 +1984,17 @@ int vi_batch_set_tiling(vi_batch b, int rz) {
 }
 
 int vi_batch_get_tiling(vi_batch b) { return b ? b->rz : -1; }
+
+size_t vi_batch_get_cache_block(vi_batch b) { return b ? b->block_bytes : 0; }
+
+int vi_batch_set_cache_block(vi_batch b, size_t bytes) {
+  if (!b) return fail(VI_EINVAL, "null batch");
+  VI_HIP(hipStreamSynchronize(b->stream));
+  b->block_bytes = bytes;
+  b->built = false;
+  invalidate_graph(b);
+  return 0;
+}
 
 int vi_batch_set_graph(vi_batch b, int steps_per_graph) {
   if (!b || steps_per_graph < 0) return fail(VI_EINVAL, "vi_batch_set_graph: bad arguments");
