@@ -72,7 +72,7 @@ def run_cpu_baseline(cfg, budget_s=12.0):
                       f"{el:.1f} s on {threads} threads"}
 
 
-def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True):
+def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True, cache_block_mb=None):
     """Batched multi-shot forward (uneven windows, one launch per half step for all shots)
     against the plain single-shot stepper run serially on the same GPU, same work."""
     from viscoelasticinversion_amd import Batch
@@ -81,13 +81,15 @@ def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True):
     model, sims = build_multishot(mcfg, "float32")
     cells = sum(sh["nx"] for sh in mcfg["shots"]) * mcfg["nz"]
     b = Batch(sims)
+    if cache_block_mb is not None:
+        b.set_cache_block(int(cache_block_mb) << 20)
     if rows:
         b.set_tiling(rows)
         table = {}
     else:
         rows, table = b.autotune(trial_steps=20)
     b.set_graph(graph_steps)
-    b.step(graph_steps)  # capture + warm up
+    b.step(graph_steps or 100)  # capture + warm up
     b.sync()
     for sim in sims:
         sim.reset()
@@ -314,6 +316,10 @@ def main():
                     help="fix the kernel variant (rows per wavefront) instead of autotuning")
     ap.add_argument("--no-serial", action="store_true",
                     help="multi-shot leg: skip the serial plain-stepper comparison")
+    ap.add_argument("--graph-steps", type=int, default=100,
+                    help="steps per hipGraph block of the batched leg (0: eager launches)")
+    ap.add_argument("--cache-block-mb", type=int, default=None,
+                    help="batch cache block in MB (default: the library's; 0 = lockstep)")
     ap.add_argument("--only-multishot", action="store_true",
                     help="run only the batched multi-shot leg and print its JSON")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -332,8 +338,9 @@ def main():
         return
     if args.only_multishot:
         print(json.dumps({"multishot": bench_multishot(
-            args.multishot_shots, args.multishot_steps, rows=args.rows,
-            serial=not args.no_serial)}), flush=True)
+            args.multishot_shots, args.multishot_steps, graph_steps=args.graph_steps,
+            rows=args.rows, serial=not args.no_serial,
+            cache_block_mb=args.cache_block_mb)}), flush=True)
         return
     if args.serving_rates:
         for r in (float(x) for x in args.serving_rates.split(",")):
@@ -397,7 +404,9 @@ def main():
              "autotune_us_per_step": {str(k): round(v, 2) for k, v in tune_table.items()}}
     if not args.no_multishot:
         extra["multishot"] = bench_multishot(args.multishot_shots, args.multishot_steps,
-                                             serial=not args.no_serial)
+                                             graph_steps=args.graph_steps,
+                                             serial=not args.no_serial,
+                                             cache_block_mb=args.cache_block_mb)
     if not args.no_gradient:
         extra["gradient"] = bench_gradient(args.gradient_shots)
     if not args.no_serving:
