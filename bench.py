@@ -207,7 +207,7 @@ def bench_gradient(nshots=16, nt=3000, checkpoint_every=0):
             "misfit": J, "gradient_finite": bool(all(np.isfinite(v).all() for v in total.values()))}
 
 
-def bench_serving(njobs=24, rate=20.0, seed=0, block_steps=100, max_gradients=4):
+def bench_serving(njobs=24, rate=20.0, seed=0, block_steps=100, max_gradients=1):
     """Serving workload: a seeded Poisson stream of ``njobs`` shot jobs at ``rate`` per
     second (70 % forward, 30 % gradient; grids 256^2, 512x1024, 1024x2048, 2048^2; isotropic
     or VTI; L=3; FP32; 1000-4000 steps) served with continuous batching under an HBM budget,

@@ -157,7 +157,7 @@ class Server:
     ``max_forward``: forward jobs in one batch; ``max_head_wait``: seconds a job that does
     not fit may be overtaken by later, smaller jobs."""
 
-    def __init__(self, precision="float32", hbm_budget=None, block_steps=100, max_gradients=4,
+    def __init__(self, precision="float32", hbm_budget=None, block_steps=100, max_gradients=1,
                  max_forward=128, graph=True, max_head_wait=1.0, keep_outputs=True):
         from .batch import mem_info
         self.precision = precision
