@@ -244,6 +244,8 @@ int vi_batch_get_tiling(vi_batch b);
  * default is 224 MB (of the 256 MB Infinity Cache). */
 int vi_batch_set_cache_block(vi_batch b, size_t bytes);
 size_t vi_batch_get_cache_block(vi_batch b);
+/* times trial_steps of every tiling (1, 2, 4, 8 rows per wavefront; us_per_step[4]) and, when a
+ * cache block is set, lockstep at the fastest tiling; keeps the fastest and restores the state */
 int vi_batch_autotune(vi_batch b, int trial_steps, int* chosen, double* us_per_step);
 /* replay a captured hipGraph of steps_per_graph steps (0 = direct launches) */
 int vi_batch_set_graph(vi_batch b, int steps_per_graph);

@@ -171,16 +171,23 @@ def test_member_stepped_alone_between_batch_steps(graph):
     model.close()
 
 
-def test_batch_autotune_keeps_state():
+@pytest.mark.parametrize("block", [None, 1])
+def test_batch_autotune_keeps_state(block):
+    """The autotuner's trials (every tiling, and lockstep against the cache block when one
+    is set; 1 byte = one shot per group) leave the state and step index as they were."""
     from viscoelasticinversion_amd import Batch
     from viscoelasticinversion_amd.configs import build_multishot, shot_config
     mcfg = _small_multishot(nshots=3, nt=30, precision_seed=5)
-    print("[case] batch autotune", flush=True)
+    print(f"[case] batch autotune, block {block}", flush=True)
     model, sims = build_multishot(mcfg, "float32")
     b = Batch(sims)
+    if block is not None:
+        b.set_cache_block(block)
     b.step(12)
     chosen, table = b.autotune(trial_steps=4)
     assert chosen in (1, 2, 4, 8) and b.tiling == chosen
+    if block is not None:
+        assert b.cache_block in (0, block)
     b.step(18)
     for s, sim in enumerate(sims):
         got = _collect(sim, 30, 3)

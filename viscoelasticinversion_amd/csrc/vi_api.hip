@@ -2079,28 +2079,42 @@ int vi_batch_autotune(vi_batch b, int trial_steps, int* chosen, double* us_per_s
   hipEvent_t e0, e1;
   VI_HIP(hipEventCreate(&e0));
   VI_HIP(hipEventCreate(&e1));
+  // one trial of the current tiling and cache block: 2 steps untimed, then trial_steps timed
+  auto trial = [&](double* us) -> int {
+    b->built = false;
+    VI_TRY(batch_prepare(b));
+    VI_TRY(batch_launch(b, 2));
+    VI_HIP(hipEventRecord(e0, b->stream));
+    VI_TRY(batch_launch(b, trial_steps));
+    VI_HIP(hipEventRecord(e1, b->stream));
+    VI_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *us = 1e3 * ms / trial_steps;
+    return restore();
+  };
   const int cands[4] = {1, 2, 4, 8};
   int best = b->rz;
   double best_us = 1e300;
   int rc = 0;
   for (int c = 0; c < 4 && !rc; ++c) {
     b->rz = cands[c];
-    b->built = false;
-    if ((rc = batch_prepare(b))) break;
-    if ((rc = batch_launch(b, 2))) break;
-    if ((rc = hipEventRecord(e0, b->stream) == hipSuccess ? 0 : VI_EDEVICE)) break;
-    if ((rc = batch_launch(b, trial_steps))) break;
-    if ((rc = hipEventRecord(e1, b->stream) == hipSuccess ? 0 : VI_EDEVICE)) break;
-    if ((rc = hipEventSynchronize(e1) == hipSuccess ? 0 : VI_EDEVICE)) break;
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    const double us = 1e3 * ms / trial_steps;
+    double us = 0;
+    if ((rc = trial(&us))) break;
     if (us_per_step) us_per_step[c] = us;
     if (us < best_us) {
       best_us = us;
       best = cands[c];
     }
-    if ((rc = restore())) break;
+  }
+  // cache blocking over shots against lockstep, at the chosen tiling
+  const size_t blk = b->block_bytes;
+  if (!rc && blk) {
+    b->rz = best;
+    b->block_bytes = 0;
+    double us = 0;
+    rc = trial(&us);
+    if (rc || us >= best_us) b->block_bytes = blk;
   }
   (void)hipStreamSynchronize(b->stream);
   if (rc) (void)restore();

@@ -26,7 +26,8 @@
 // bit. References: operators src/diff-funcs.jl:3-32; velocity src/visco-kernels.jl:3-20;
 // constitutive + memory src/visco-kernels.jl:22-85; VTI stiffnesses
 // src/sciml_solver/Models.jl:27-33 in the form of ViscoelasticProblem.jl:21-35; CPML
-// src/sciml_solver/CPML.jl:291-328; imaging src/sciml_solver/CPML.jl:372-380; source
+// src/sciml_solver/CPML.jl:291-328; imaging src/sciml_solver/CPML.jl:372-380; H-AFDA free
+// surface src/free-surface-kernels.jl:16-214 with src/common/Stencil.jl:52-105; source
 // src/sciml_solver/Source.jl:35-40.
 #pragma once
 #include <hip/hip_runtime.h>
