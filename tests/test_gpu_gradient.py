@@ -93,6 +93,42 @@ def test_gradient_matches_finite_differences(nrelax, fs, cpml):
                 (key, j, i, ad, fd)
 
 
+@pytest.mark.parametrize("nrelax,fs", [(0, True), (3, True), (3, "afda")])
+def test_adjoint_dot_product(nrelax, fs):
+    """<J dm, dd> = <dm, J^T dd> for the Jacobian J of the receiver traces with respect to the
+    model: J^T dd is the gradient with the residual set to dd (observed data d - dd), J dm
+    the central difference of the traces along dm (FP64), for random dm and dd."""
+    cfg, _ = _setup(nrelax, fs, True, seed=31 + nrelax)
+    nt = cfg["nt"]
+    rng = np.random.default_rng(77)
+    sim = _sim(cfg, cfg["model"])
+    sim.step(nt)
+    d0 = sim.traces(nt)
+    sim.close()
+    dd = rng.standard_normal(d0.shape) * np.abs(d0).max()
+    print(f"[case] adjoint dot product L={nrelax} fs={fs}", flush=True)
+    sim = _sim(cfg, cfg["model"])
+    _, g = sim.gradient(d0 - dd, nt, checkpoint_every=0)
+    sim.close()
+    nz, nx = cfg["nz"], cfg["nx"]
+    for key in ["rho", "lam", "mu"] + (["tau"] if nrelax else []):
+        dm = np.zeros((nz, nx))
+        dm[4:-4, 4:-4] = rng.standard_normal((nz - 8, nx - 8))
+        base = cfg["model"][key]
+        eps = 1e-6
+        tr = []
+        for sgn in (1, -1):
+            m = dict(cfg["model"]); m[key] = base * (1 + sgn * eps * dm)
+            s_ = _sim(cfg, m)
+            s_.step(nt)
+            tr.append(s_.traces(nt))
+            s_.close()
+        jdm = (tr[0] - tr[1]) / (2 * eps)          # J (base * dm)
+        lhs = float(np.sum(jdm * dd))
+        rhs = float(np.sum(g[key] * base * dm))
+        assert abs(lhs - rhs) <= 1e-6 * max(abs(lhs), abs(rhs)), (key, lhs, rhs)
+
+
 def test_gradient_tau_to_zero_is_the_elastic_gradient():
     """With tau = 0 the SLS terms vanish from the stress update (X1 = X tau = 0), so the
     viscoelastic step is the elastic one and its gradient with respect to rho, lam and mu is
