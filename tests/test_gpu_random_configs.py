@@ -54,3 +54,40 @@ def test_random_configuration(seed):
         ref.pop("traces")
         gpu.pop("traces")
     compare(gpu, ref, dtype, f"random seed {seed}")
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_batch_equals_alone(seed):
+    """A random mix of the configurations above (different sizes, surfaces, L, VTI and
+    precision) in one batch, with a random cache block and graph block: every shot equals the
+    same shot stepped alone, bit for bit."""
+    from viscoelasticinversion_amd import Batch
+    from viscoelasticinversion_amd.configs import build_simulation
+    rng = np.random.default_rng(500 + seed)
+    picks = [_random_cfg(int(s)) for s in rng.integers(0, 1000, 5)]
+    nt = 24
+    sims, cfgs = [], []
+    for cfg, _, dtype in picks:
+        cfg["nt"] = nt
+        prec = "float64" if dtype == np.float64 else "float32"
+        sims.append(build_simulation(cfg, prec))
+        cfgs.append((cfg, prec))
+    block = int(rng.choice([0, 1, 1 << 20, 224 << 20]))
+    graph = int(rng.choice([0, 5, 10]))
+    print(f"[case] random batch {seed}: block {block}, graph {graph}", flush=True)
+    b = Batch(sims)
+    b.set_cache_block(block)
+    b.set_graph(graph)
+    b.step(11)
+    b.step(13)
+    b.sync()
+    for (cfg, prec), sim in zip(cfgs, sims):
+        got = {f: sim.get(f) for f in ("vx", "vz", "sxx", "szz", "sxz")}
+        alone = run_gpu(cfg, prec)
+        for f in got:
+            assert np.array_equal(got[f], alone[f]), f
+        if len(cfg["rec_ix"]):
+            assert np.array_equal(sim.traces(nt), alone["traces"])
+    b.close()
+    for s in sims:
+        s.close()
