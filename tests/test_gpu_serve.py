@@ -9,17 +9,20 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_served_results_equal_alone_under_budget():
+@pytest.mark.parametrize("max_gradients", [1, 2])
+def test_served_results_equal_alone_under_budget(max_gradients):
     from viscoelasticinversion_amd.configs import serving_workload
     from viscoelasticinversion_amd.serve import Server, job_bytes, run_serial
     grids = ((96, 64), (160, 96), (200, 120), (120, 150))
     jobs = serving_workload(16, rate=400.0, seed=5, grids=grids, steps=(60, 180))
     kinds = [j.kind for j in jobs]
-    print(f"[case] serve 16 jobs: {kinds.count('gradient')} gradient", flush=True)
+    print(f"[case] serve 16 jobs: {kinds.count('gradient')} gradient, "
+          f"max_gradients {max_gradients}", flush=True)
     assert "gradient" in kinds and "forward" in kinds
     need = [job_bytes(j) for j in jobs]
     budget = int(2.5 * max(need))
-    srv = Server(hbm_budget=budget, block_steps=25, max_gradients=2, max_head_wait=0.05)
+    srv = Server(hbm_budget=budget, block_steps=25, max_gradients=max_gradients,
+                 max_head_wait=0.05)
     events = []
     res = srv.run(jobs, on_event=lambda w, j, t: events.append((t, w, j)))
     assert all(r.error is None for r in res), [r.error for r in res]
