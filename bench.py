@@ -139,6 +139,11 @@ def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True, cache_bloc
                         "rows_per_wave": rows, "graph_steps": graph_steps,
                         "cache_block_MB": block >> 20,
                         "launch_groups": launches,
+                        # both kernels' algorithmic bytes over the wall time of the timed run
+                        # (launch groups of the two streams overlap, so per-kernel sums below
+                        # count concurrent time twice)
+                        "algorithmic_TBps_wall": round((bytes_s + bytes_v) * nt / wall_b / 1e12, 3),
+                        "hbm_frac_wall": round((bytes_s + bytes_v) * nt / wall_b / 1e9 / HBM_PEAK_GBS, 4),
                         "k_stress_us_per_step": round(step_s * 1e6, 2),
                         "k_stress_hbm_frac": round(bytes_s / step_s / 1e9 / HBM_PEAK_GBS, 4),
                         "k_velocity_us_per_step": round(step_v * 1e6, 2),

@@ -50,18 +50,21 @@ def test_batch_equals_oracle_and_alone(dtype):
     model.close()
 
 
+@pytest.mark.parametrize("streams", [1, 2])
 @pytest.mark.parametrize("block", [0, 1, 3 << 20])
-def test_cache_blocked_batch_equals_alone(block):
+def test_cache_blocked_batch_equals_alone(block, streams):
     """Cache blocking over shots (groups of at most `block` bytes of working set run each
-    block of steps in turn; 1 byte = one shot per group) changes nothing in the results,
-    eager or with graph blocks."""
+    block of steps in turn; 1 byte = one shot per group), on one stream or with consecutive
+    groups alternating between two, changes nothing in the results, eager or with graph
+    blocks."""
     from viscoelasticinversion_amd import Batch
     from viscoelasticinversion_amd.configs import build_multishot, shot_config
     mcfg = _small_multishot(nshots=6, nt=45)
     nt = mcfg["nt"]
-    print(f"[case] cache-blocked batch, block {block} bytes", flush=True)
+    print(f"[case] cache-blocked batch, block {block} bytes, {streams} streams", flush=True)
     model, sims = build_multishot(mcfg, "float32")
     b = Batch(sims)
+    b.set_streams(streams)
     b.set_cache_block(block)
     assert b.cache_block == block
     b.set_graph(10)

@@ -69,6 +69,7 @@ SIGNATURES = {
     "vi_batch_get_tiling": (_I, [_H]),
     "vi_batch_set_cache_block": (_I, [_H, ctypes.c_size_t]),
     "vi_batch_get_cache_block": (ctypes.c_size_t, [_H]),
+    "vi_batch_set_streams": (_I, [_H, _I]),
     "vi_batch_autotune": (_I, [_H, _I, _IP, _DP]),
     "vi_batch_set_graph": (_I, [_H, _I]),
     "vi_batch_step": (_I, [_H, _I]),

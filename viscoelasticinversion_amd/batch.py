@@ -108,6 +108,10 @@ class Batch:
     def cache_block(self):
         return int(L.lib().vi_batch_get_cache_block(self._h))
 
+    def set_streams(self, n):
+        """2 (default): two launch groups at once on two streams; 1: one at a time."""
+        L.check(L.lib().vi_batch_set_streams(self._h, int(n)))
+
     def step(self, nsteps=1):
         L.check(L.lib().vi_batch_step(self._h, int(nsteps)))
 

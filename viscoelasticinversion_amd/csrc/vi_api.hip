@@ -230,6 +230,12 @@ struct vi_batch_s {
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
   hipStream_t stream = nullptr;
+  // second stream: with two streams, consecutive launch groups alternate between them and two
+  // groups (each at most half the cache block) run at once, overlapping each other's launch
+  // ramp and drain; side joins stream at the end of every block (fork/join events)
+  int nstreams = 2;
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   Prof prof;
 };
 
@@ -656,13 +662,14 @@ int batch_build(vi_batch_s* b) {
   // the Infinity Cache across its launches (shots are independent, nothing else changes)
   std::map<std::pair<int, int>, size_t> gi;  // class -> its current (last) group
   std::vector<size_t> filled;
+  // two groups run at once on two streams: each gets half of the block
+  const size_t limit = b->nstreams == 2 ? b->block_bytes / 2 : b->block_bytes;
   for (auto* h : b->members) {
     VI_TRY(check_ready(h));
     if (h->rz != b->rz) VI_TRY(apply_tiling(h, b->rz));
     auto key = std::make_pair((int)h->esz, h->L);
     const size_t ws = working_set_bytes(h);
-    if (!gi.count(key) ||
-        (b->block_bytes && filled[gi[key]] > 0 && filled[gi[key]] + ws > b->block_bytes)) {
+    if (!gi.count(key) || (limit && filled[gi[key]] > 0 && filled[gi[key]] + ws > limit)) {
       gi[key] = b->groups.size();
       b->groups.emplace_back(new BatchGroup());
       b->groups.back()->esz = (int)h->esz;
@@ -705,13 +712,23 @@ int batch_build(vi_batch_s* b) {
 }
 
 int batch_launch(vi_batch_s* b, int nsteps) {
-  for (auto& gp : b->groups) {
-    BatchGroup& g = *gp;
-    VI_TRY(launch_steps(b->stream, g.esz, b->rz, g.L, g.desc_dev.p,
-                        (const int*)g.tile_shot_dev.p, g.ntiles, (int)g.members.size(), nsteps,
-                        &b->prof, g.afda));
+  const bool two = b->nstreams == 2 && b->groups.size() > 1;
+  if (two) {  // fork: the side stream starts after everything queued on the batch stream
+    VI_HIP(hipEventRecord(b->fork_ev, b->stream));
+    VI_HIP(hipStreamWaitEvent(b->side, b->fork_ev, 0));
   }
-  return 0;
+  int rc = 0;
+  for (size_t q = 0; q < b->groups.size() && !rc; ++q) {
+    BatchGroup& g = *b->groups[q];
+    rc = launch_steps(two && (q & 1) ? b->side : b->stream, g.esz, b->rz, g.L, g.desc_dev.p,
+                      (const int*)g.tile_shot_dev.p, g.ntiles, (int)g.members.size(), nsteps,
+                      &b->prof, g.afda);
+  }
+  if (two) {  // join, also after an error, so no work is left behind on the side stream
+    VI_HIP(hipEventRecord(b->join_ev, b->side));
+    VI_HIP(hipStreamWaitEvent(b->stream, b->join_ev, 0));
+  }
+  return rc;
 }
 
 int batch_prepare(vi_batch_s* b) {
@@ -1944,6 +1961,9 @@ int vi_batch_create(vi_batch* out) {
   if (vi_device_count() < 1) return fail(VI_EDEVICE, "vi_batch_create: no HIP device visible");
   std::unique_ptr<vi_batch_s> b(new vi_batch_s());
   VI_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  VI_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+  VI_HIP(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
+  VI_HIP(hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming));
   *out = b.release();
   return 0;
 }
@@ -1951,10 +1971,24 @@ int vi_batch_create(vi_batch* out) {
 int vi_batch_destroy(vi_batch b) {
   if (!b) return 0;
   (void)hipStreamSynchronize(b->stream);
+  (void)hipStreamSynchronize(b->side);
   invalidate_graph(b);
-  hipStream_t s = b->stream;
+  hipStream_t s = b->stream, s2 = b->side;
+  hipEvent_t e0 = b->fork_ev, e1 = b->join_ev;
   delete b;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s2) (void)hipStreamDestroy(s2);
   if (s) (void)hipStreamDestroy(s);
+  return 0;
+}
+
+int vi_batch_set_streams(vi_batch b, int n) {
+  if (!b || (n != 1 && n != 2)) return fail(VI_EINVAL, "vi_batch_set_streams: n must be 1 or 2");
+  VI_HIP(hipStreamSynchronize(b->stream));
+  b->nstreams = n;
+  b->built = false;
+  invalidate_graph(b);
   return 0;
 }
 
