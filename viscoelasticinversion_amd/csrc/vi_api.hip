@@ -969,9 +969,13 @@ struct GradRun : vi_grad_s {
     VI_HIP(hipHostMalloc((void**)&host_tr, (size_t)nt * nrec * E, 0));
     VI_HIP(hipEventCreateWithFlags(&ev_tr, hipEventDisableTiming));
     VI_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    {  // short tiles when the grid is small (enough waves in flight)
+    {  // 2-row tiles only when their launch holds at least four rounds of resident
+       // workgroups (3 per CU at the 2-row register count); shorter launches lose more
+       // to the last round's tail than the taller tiles save in halo rows (1600x600: 1875
+       // 2-row workgroups, 2 % slower per gradient than 1-row tiles; 2048x1024: equal)
       const long ntx = (d.nx - 8 + vi::TX - 1) / vi::TX;
-      arz = ntx * ((d.nz - 8 + 1) / 2) >= 256L * 16 ? 2 : 1;
+      const long tiles2 = ntx * ((d.nz - 8 + 2 * vi::NW - 1) / (2 * vi::NW));
+      arz = tiles2 >= 4L * 3 * 256 ? 2 : 1;
     }
     VI_HIP(hipStreamSynchronize(st));
     h->own_pending = true;
