@@ -50,13 +50,13 @@ def test_batch_equals_oracle_and_alone(dtype):
     model.close()
 
 
-@pytest.mark.parametrize("streams", [1, 2])
+@pytest.mark.parametrize("streams", [1, 2, 3, 4])
 @pytest.mark.parametrize("block", [0, 1, 3 << 20])
 def test_cache_blocked_batch_equals_alone(block, streams):
-    """Cache blocking over shots (groups of at most `block` bytes of working set run each
-    block of steps in turn; 1 byte = one shot per group), on one stream or with consecutive
-    groups alternating between two, changes nothing in the results, eager or with graph
-    blocks."""
+    """Cache blocking over shots (groups of at most `block` / `streams` bytes of working set
+    run each block of steps in turn; 1 byte = one shot per group), on one stream or with
+    consecutive groups taking 2 to 4 streams in turn, changes nothing in the results, eager
+    or with graph blocks."""
     from viscoelasticinversion_amd import Batch
     from viscoelasticinversion_amd.configs import build_multishot, shot_config
     mcfg = _small_multishot(nshots=6, nt=45)
@@ -80,6 +80,18 @@ def test_cache_blocked_batch_equals_alone(block, streams):
     for sim in sims:
         sim.close()
     model.close()
+
+
+def test_set_streams_bounds():
+    from viscoelasticinversion_amd import Batch, _lib
+    print("[case] set_streams bounds", flush=True)
+    b = Batch([])
+    for n in (0, 5, -1):
+        with pytest.raises(_lib.ViError):
+            b.set_streams(n)
+    for n in (1, 4, 2):
+        b.set_streams(n)
+    b.close()
 
 
 @pytest.mark.parametrize("rows", [1, 2, 4, 8])

@@ -109,7 +109,8 @@ class Batch:
         return int(L.lib().vi_batch_get_cache_block(self._h))
 
     def set_streams(self, n):
-        """2 (default): two launch groups at once on two streams; 1: one at a time."""
+        """n = 1..4 (default 2): n launch groups at once on n streams, each at most 1/n of
+        the cache block; 1: one at a time."""
         L.check(L.lib().vi_batch_set_streams(self._h, int(n)))
 
     def step(self, nsteps=1):

@@ -230,12 +230,14 @@ struct vi_batch_s {
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
   hipStream_t stream = nullptr;
-  // second stream: with two streams, consecutive launch groups alternate between them and two
-  // groups (each at most half the cache block) run at once, overlapping each other's launch
-  // ramp and drain; side joins stream at the end of every block (fork/join events)
+  // side streams: with n streams, consecutive launch groups take the streams in turn and n
+  // groups (each at most 1/n of the cache block) run at once, overlapping each other's launch
+  // ramp and drain; the side streams fork from stream before a block and join it after
+  // (events)
+  static constexpr int MAX_STREAMS = 4;
   int nstreams = 2;
-  hipStream_t side = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  hipStream_t side[MAX_STREAMS - 1] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[MAX_STREAMS - 1] = {};
   Prof prof;
 };
 
@@ -662,8 +664,8 @@ int batch_build(vi_batch_s* b) {
   // the Infinity Cache across its launches (shots are independent, nothing else changes)
   std::map<std::pair<int, int>, size_t> gi;  // class -> its current (last) group
   std::vector<size_t> filled;
-  // two groups run at once on two streams: each gets half of the block
-  const size_t limit = b->nstreams == 2 ? b->block_bytes / 2 : b->block_bytes;
+  // n groups run at once on n streams: each gets 1/n of the block
+  const size_t limit = b->block_bytes / (size_t)b->nstreams;
   for (auto* h : b->members) {
     VI_TRY(check_ready(h));
     if (h->rz != b->rz) VI_TRY(apply_tiling(h, b->rz));
@@ -712,21 +714,22 @@ int batch_build(vi_batch_s* b) {
 }
 
 int batch_launch(vi_batch_s* b, int nsteps) {
-  const bool two = b->nstreams == 2 && b->groups.size() > 1;
-  if (two) {  // fork: the side stream starts after everything queued on the batch stream
+  const int ns = (int)std::min<size_t>((size_t)b->nstreams, b->groups.size());
+  if (ns > 1) {  // fork: the side streams start after everything queued on the batch stream
     VI_HIP(hipEventRecord(b->fork_ev, b->stream));
-    VI_HIP(hipStreamWaitEvent(b->side, b->fork_ev, 0));
+    for (int k = 0; k < ns - 1; ++k) VI_HIP(hipStreamWaitEvent(b->side[k], b->fork_ev, 0));
   }
   int rc = 0;
   for (size_t q = 0; q < b->groups.size() && !rc; ++q) {
     BatchGroup& g = *b->groups[q];
-    rc = launch_steps(two && (q & 1) ? b->side : b->stream, g.esz, b->rz, g.L, g.desc_dev.p,
+    const int k = (int)(q % (size_t)ns);
+    rc = launch_steps(k ? b->side[k - 1] : b->stream, g.esz, b->rz, g.L, g.desc_dev.p,
                       (const int*)g.tile_shot_dev.p, g.ntiles, (int)g.members.size(), nsteps,
                       &b->prof, g.afda);
   }
-  if (two) {  // join, also after an error, so no work is left behind on the side stream
-    VI_HIP(hipEventRecord(b->join_ev, b->side));
-    VI_HIP(hipStreamWaitEvent(b->stream, b->join_ev, 0));
+  for (int k = 0; k < ns - 1; ++k) {  // join, also after an error: no work is left behind
+    VI_HIP(hipEventRecord(b->join_ev[k], b->side[k]));
+    VI_HIP(hipStreamWaitEvent(b->stream, b->join_ev[k], 0));
   }
   return rc;
 }
@@ -1965,9 +1968,11 @@ int vi_batch_create(vi_batch* out) {
   if (vi_device_count() < 1) return fail(VI_EDEVICE, "vi_batch_create: no HIP device visible");
   std::unique_ptr<vi_batch_s> b(new vi_batch_s());
   VI_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  VI_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
   VI_HIP(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
-  VI_HIP(hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming));
+  for (int k = 0; k < vi_batch_s::MAX_STREAMS - 1; ++k) {
+    VI_HIP(hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking));
+    VI_HIP(hipEventCreateWithFlags(&b->join_ev[k], hipEventDisableTiming));
+  }
   *out = b.release();
   return 0;
 }
@@ -1975,20 +1980,26 @@ int vi_batch_create(vi_batch* out) {
 int vi_batch_destroy(vi_batch b) {
   if (!b) return 0;
   (void)hipStreamSynchronize(b->stream);
-  (void)hipStreamSynchronize(b->side);
+  for (auto s : b->side)
+    if (s) (void)hipStreamSynchronize(s);
   invalidate_graph(b);
-  hipStream_t s = b->stream, s2 = b->side;
-  hipEvent_t e0 = b->fork_ev, e1 = b->join_ev;
+  constexpr int NS = vi_batch_s::MAX_STREAMS - 1;
+  hipStream_t s = b->stream, side[NS];
+  hipEvent_t e0 = b->fork_ev, ej[NS];
+  for (int k = 0; k < NS; ++k) side[k] = b->side[k], ej[k] = b->join_ev[k];
   delete b;
   if (e0) (void)hipEventDestroy(e0);
-  if (e1) (void)hipEventDestroy(e1);
-  if (s2) (void)hipStreamDestroy(s2);
+  for (int k = 0; k < NS; ++k) {
+    if (ej[k]) (void)hipEventDestroy(ej[k]);
+    if (side[k]) (void)hipStreamDestroy(side[k]);
+  }
   if (s) (void)hipStreamDestroy(s);
   return 0;
 }
 
 int vi_batch_set_streams(vi_batch b, int n) {
-  if (!b || (n != 1 && n != 2)) return fail(VI_EINVAL, "vi_batch_set_streams: n must be 1 or 2");
+  if (!b || n < 1 || n > vi_batch_s::MAX_STREAMS)
+    return fail(VI_EINVAL, "vi_batch_set_streams: n must be 1 to 4");
   VI_HIP(hipStreamSynchronize(b->stream));
   b->nstreams = n;
   b->built = false;
