@@ -243,11 +243,11 @@ int vi_batch_get_tiling(vi_batch b);
  * stays in the Infinity Cache. Results are unchanged. 0 = every shot in every launch; the
  * default is 224 MB (of the 256 MB Infinity Cache). */
 int vi_batch_set_cache_block(vi_batch b, size_t bytes);
+size_t vi_batch_get_cache_block(vi_batch b);
 /* n = 1..4 streams (default 2): consecutive launch groups take the streams in turn, so n
  * groups run at once (each at most 1/n of the cache block), overlapping each other's launch
  * ramp and drain; 1: one group at a time. Results are unchanged. */
 int vi_batch_set_streams(vi_batch b, int n);
-size_t vi_batch_get_cache_block(vi_batch b);
 /* times trial_steps of every tiling (1, 2, 4, 8 rows per wavefront; us_per_step[4]) and, when a
  * cache block is set, lockstep at the fastest tiling; keeps the fastest and restores the state */
 int vi_batch_autotune(vi_batch b, int trial_steps, int* chosen, double* us_per_step);
