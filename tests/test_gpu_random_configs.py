@@ -40,7 +40,7 @@ def _random_cfg(seed):
     return cfg, rows, dtype
 
 
-@pytest.mark.parametrize("seed", range(24))
+@pytest.mark.parametrize("seed", range(96))
 def test_random_configuration(seed):
     cfg, rows, dtype = _random_cfg(seed)
     prec = "float64" if dtype == np.float64 else "float32"
