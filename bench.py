@@ -6,6 +6,10 @@ FP32, L = 3 SLS mechanisms, 20-cell CPML on the sides and bottom, free surface, 
 Ricker force source, 201 receivers, dt = 0.4 ms. One "step" is one full time step (the
 velocity kernel then the stress kernel) over the whole grid, inputs resident in HBM.
 
+The timed region is bracketed on both sides by a synchronize of the library's HIP stream,
+which carries every launch of the step (the library does not use torch streams). There is
+one process, so there is no barrier.
+
 Prints ONE JSON line: value = grid-point updates per second (Nx*Nz*steps / wall time),
 ``roofline`` for the dominant (stress) kernel from HIP events recorded around every launch
 on the library's stream, and ``cpu_baseline`` = the reference-structured CPU restatement
