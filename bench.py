@@ -8,12 +8,15 @@ velocity kernel then the stress kernel) over the whole grid, inputs resident in 
 
 The timed region is bracketed on both sides by a synchronize of the library's HIP stream,
 which carries every launch of the step (the library does not use torch streams). There is
-one process, so there is no barrier.
+one process, so there is no barrier. No events are recorded inside the timed region.
 
-Prints ONE JSON line: value = grid-point updates per second (Nx*Nz*steps / wall time),
-``roofline`` for the dominant (stress) kernel from HIP events recorded around every launch
-on the library's stream, and ``cpu_baseline`` = the reference-structured CPU restatement
-(multi-pass, OpenMP) timed on this host on a bounded sample of the same workload.
+Prints ONE JSON line: value = grid-point updates per second (Nx*Nz*steps / wall time of the
+timed region); ``roofline`` for the dominant (stress) kernel, its average launch duration
+taken from HIP events that the launches themselves record on the library's stream in a
+separate pass of at least 100 steps; ``fp32_vs_fp64`` = relative L2 of the FP32 result
+against an FP64 run of the full workload; ``cpu_baseline`` = the reference-structured CPU
+restatement (multi-pass, OpenMP) timed on this host on a bounded sample of the same workload,
+FP32 and FP64.
 """
 from __future__ import annotations
 
@@ -53,27 +56,66 @@ def pmc_traffic(kernel_name):
         return None
 
 
-def run_cpu_baseline(cfg, budget_s=12.0):
-    """Time the oracle's reference-structured multi-pass path (FP32) on host cores."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_cpu_baseline(cfg, budget_s=8.0):
+    """Time the oracle's reference-structured multi-pass path on the host cores, FP32 (the
+    GPU's arithmetic) and FP64 (the reference's CPU precision), each on at most ``budget_s``
+    seconds of steps of the same workload (at least 2 s or the whole workload)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from helpers import make_oracle
     threads = host_threads()
-    o = make_oracle(cfg, np.float32)
-    o.run(5, mode=1, nthreads=threads)  # warm up page faults
-    t0 = time.perf_counter()
-    n_probe = 10
-    o.run(n_probe, mode=1, nthreads=threads)
-    dt_probe = (time.perf_counter() - t0) / n_probe
-    n = int(max(10, min(cfg["nt"], budget_s / max(dt_probe, 1e-9))))
-    t0 = time.perf_counter()
-    o.run(n, mode=1, nthreads=threads)
-    el = time.perf_counter() - t0
-    o.close()
     cells = cfg["nx"] * cfg["nz"]
-    return {"value": cells * n / el / 1e9, "unit": "GPt/s", "cores": threads, "kind": "port",
-            "sample": f"{n} of {cfg['nt']} steps of the same {cfg['nx']}x{cfg['nz']} L=3 FP32 "
-                      f"workload, reference-structured multi-pass C restatement, "
-                      f"{el:.1f} s on {threads} threads"}
+    res = {}
+    for name, dt in (("f32", np.float32), ("f64", np.float64)):
+        o = make_oracle(cfg, dt)
+        o.run(15, mode=1, nthreads=threads)  # page faults, thread start-up
+        # chunks of steps until the budget is spent: the cost per step grows as the wave
+        # spreads (subnormal values ahead of the wavefront are slow on x86), so a short
+        # probe would not predict it
+        n, el, chunk = 0, 0.0, 10
+        while el < max(2.0, budget_s) and 15 + n + chunk <= cfg["nt"]:
+            t0 = time.perf_counter()
+            o.run(chunk, mode=1, nthreads=threads)
+            el += time.perf_counter() - t0
+            n += chunk
+        o.close()
+        res[name] = (cells * n / el / 1e9, n, el)
+    v32, n32, e32 = res["f32"]
+    v64, n64, e64 = res["f64"]
+    return {"value": round(v32, 4), "unit": "GPt/s", "cores": threads, "kind": "port",
+            "value_f64": round(v64, 4), "cpu_model": cpu_model(),
+            "sample": f"steps 16..{15 + n32} of the {cfg['nt']}-step {cfg['nx']}x{cfg['nz']} "
+                      f"L=3 workload in FP32 ({e32:.1f} s) and steps 16..{15 + n64} in FP64 "
+                      f"({e64:.1f} s), reference-structured multi-pass C restatement, OpenMP "
+                      f"on {threads} threads of {cpu_model()}"}
+
+
+def precision_error(cfg):
+    """Relative L2 of the FP32 result against an FP64 run of the same workload (all steps),
+    per field and for the receiver traces."""
+    from viscoelasticinversion_amd.configs import build_simulation
+    out = {}
+    for prec in ("float32", "float64"):
+        sim = build_simulation(cfg, prec)
+        sim.step(cfg["nt"])
+        out[prec] = {k: sim.get(k).astype(np.float64) for k in ("vx", "vz", "sxx", "szz", "sxz")}
+        out[prec]["traces"] = sim.traces(cfg["nt"]).astype(np.float64)
+        sim.close()
+    err = {}
+    for k, b in out["float64"].items():
+        nb = np.linalg.norm(b)
+        err[k] = float(np.linalg.norm(out["float32"][k] - b) / nb) if nb else 0.0
+    return {"steps": cfg["nt"], "relative_l2": {k: float(f"{v:.3e}") for k, v in err.items()}}
 
 
 def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True, cache_block_mb=None):
@@ -114,7 +156,27 @@ def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True, cache_bloc
     bytes_s = sum(sim.algorithmic_bytes(1) for sim in sims)  # all shots, one step
     bytes_v = sum(sim.algorithmic_bytes(0) for sim in sims)
     block = b.cache_block
+    # lockstep (no cache blocking): one launch per half step streams the whole 2.2 GB
+    # working set from HBM, the DRAM-bound case; events on every launch, eager launches
+    b.set_cache_block(0)
+    b.set_graph(0)
+    b.step(2)
+    lock_steps = 40
+    b.profile(1)
+    b.step(lock_steps)
+    ms_lv, n_lv = b.profile_read(0)
+    ms_ls, n_ls = b.profile_read(1)
+    b.profile(0)
     b.close()
+    avg_ls = ms_ls / max(n_ls, 1) * 1e-3
+    avg_lv = ms_lv / max(n_lv, 1) * 1e-3
+    lockstep = {"launches_timed": n_ls, "k_stress_avg_launch_us": round(avg_ls * 1e6, 2),
+                "k_stress_achieved_GBs": round(bytes_s / avg_ls / 1e9, 1),
+                "k_stress_hbm_frac": round(bytes_s / avg_ls / 1e9 / HBM_PEAK_GBS, 4),
+                "k_velocity_avg_launch_us": round(avg_lv * 1e6, 2),
+                "k_velocity_achieved_GBs": round(bytes_v / avg_lv / 1e9, 1),
+                "k_velocity_hbm_frac": round(bytes_v / avg_lv / 1e9 / HBM_PEAK_GBS, 4),
+                "k_stress_algorithmic_bytes": bytes_s, "k_velocity_algorithmic_bytes": bytes_v}
     # plain stepper, one shot at a time
     done = []
     for sim in sims:
@@ -154,6 +216,7 @@ def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True, cache_bloc
                         "k_velocity_hbm_frac": round(bytes_v / step_v / 1e9 / HBM_PEAK_GBS, 4),
                         "autotune_us_per_step": {str(k): round(v, 1) for k, v in table.items()},
                         "traces_finite": finite},
+            "lockstep_dram_bound": lockstep,
             "serial_plain_stepper": None if not serial else {
                 "wall_s": round(wall_s, 4),
                 "GPt_per_s": round(cells * nt / wall_s / 1e9, 3),
@@ -305,7 +368,8 @@ def main():
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--nz", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--cpu-budget", type=float, default=8.0,
+                    help="seconds of CPU-baseline steps per precision (at least 2)")
     ap.add_argument("--multishot-shots", type=int, default=64)
     ap.add_argument("--multishot-steps", type=int, default=3000)
     ap.add_argument("--no-multishot", action="store_true")
@@ -333,8 +397,6 @@ def main():
                     help="run only the batched multi-shot leg and print its JSON")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short batched run + 1 GiB copy for rocprofv3 --pmc passes")
-    ap.add_argument("--profile-every", type=int, default=50,
-                    help="time the launches of every N-th step with dispatch events")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the stepper has no multi-GPU path in this build "
@@ -361,8 +423,9 @@ def main():
               flush=True)
         return
     from viscoelasticinversion_amd.configs import build_simulation, config_layered_visco
-    nt_total = args.warmup + 2 * args.steps
-    cfg = config_layered_visco(nx=args.nx, nz=args.nz, nt=nt_total)
+    prof_steps = max(100, args.steps)  # kernel-timing pass: at least 100 launches per kernel
+    nt_total = args.warmup + args.steps + prof_steps
+    cfg = config_layered_visco(nx=args.nx, nz=args.nz, nt=max(4000, nt_total))
     sim = build_simulation(cfg, "float32")
     cells = cfg["nx"] * cfg["nz"]
     if args.rows:
@@ -373,22 +436,22 @@ def main():
 
     sim.step(args.warmup)
     sim.sync()
-    # timed region: K steps; the launches of every N-th step carry HIP events recorded by
-    # the dispatch itself on the library's stream (per-kernel durations)
-    sim.profile(args.profile_every)
+    # timed region: K steps, plain launches (no events)
     t0 = time.perf_counter()
     sim.step(args.steps)
     sim.sync()
     wall = time.perf_counter() - t0
+    # kernel durations: a separate pass whose every launch records a HIP event pair on the
+    # library's stream (hipExtLaunchKernelGGL), so they time the kernels themselves
+    sim.profile(1)
+    t0 = time.perf_counter()
+    sim.step(prof_steps)
+    sim.sync()
+    wall_ev = time.perf_counter() - t0
     ms_v, n_v = sim.profile_read(0)
     ms_s, n_s = sim.profile_read(1)
     sim.profile(0)
-    # same K steps again without events, reported beside it
-    t0 = time.perf_counter()
-    sim.step(args.steps)
-    sim.sync()
-    wall_noev = time.perf_counter() - t0
-    tr = sim.traces(nt_total)
+    tr = sim.traces(args.warmup + args.steps + prof_steps)
     finite = bool(np.isfinite(tr).all())
     bytes_v = sim.algorithmic_bytes(0)
     bytes_s = sim.algorithmic_bytes(1)
@@ -403,14 +466,19 @@ def main():
             "frac": round(ach_s / HBM_PEAK_GBS, 4),
             "traffic": pmc_traffic("k_stress"),
             "kernel": "k_stress<float,3>", "avg_launch_us": round(avg_s * 1e6, 3),
-            "algorithmic_bytes_per_launch": bytes_s}
+            "launches_timed": n_s, "algorithmic_bytes_per_launch": bytes_s,
+            "residency": "the 1024x512 working set (~60 MB) fits the 256 MB Infinity Cache, so "
+                         "this rate is served from the cache, not DRAM; the DRAM-bound rate is "
+                         "extra.multishot.lockstep_dram_bound (2.2 GB working set)"}
     extra = {"k_velocity": {"avg_launch_us": round(avg_v * 1e6, 3), "achieved_GBs": round(ach_v, 1),
+                            "hbm_frac": round(ach_v / HBM_PEAK_GBS, 4), "launches_timed": n_v,
                             "algorithmic_bytes_per_launch": bytes_v,
                             "traffic": pmc_traffic("k_velocity")},
              "step_kernel_us": round((avg_s + avg_v) * 1e6, 3),
-             "value_without_events": round(cells * args.steps / wall_noev / 1e9, 3),
+             "value_with_events": round(cells * prof_steps / wall_ev / 1e9, 3),
              "traces_finite": finite, "rows_per_wave": rows,
              "autotune_us_per_step": {str(k): round(v, 2) for k, v in tune_table.items()}}
+    fp64 = precision_error(config_layered_visco(nx=args.nx, nz=args.nz, nt=4000))
     if not args.no_multishot:
         extra["multishot"] = bench_multishot(args.multishot_shots, args.multishot_steps,
                                              graph_steps=args.graph_steps,
@@ -422,17 +490,17 @@ def main():
         extra["serving"] = bench_serving(args.serving_jobs, args.serving_rate)
     cpu = None
     if not args.no_cpu:
-        cfg_cpu = config_layered_visco(nx=args.nx, nz=args.nz, nt=args.steps)
+        cfg_cpu = config_layered_visco(nx=args.nx, nz=args.nz, nt=4000)
         cpu = run_cpu_baseline(cfg_cpu, args.cpu_budget)
     line = {"metric": METRIC, "value": round(value, 3), "unit": "GPt/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(wall / args.steps * 1e3, 5), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-            "data": "synthetic (seeded layered model)",
+            "data": "synthetic (seeded layered model; no datasets or checkpoints)",
             "config": {"workload": f"layered viscoelastic single shot {cfg['nx']}x{cfg['nz']}, "
                                    "L=3 SLS, CPML 20 + free surface, dt 0.4 ms",
                        "nx": cfg["nx"], "nz": cfg["nz"], "nrelax": 3, "nrec": 201},
-            "roofline": roof, "cpu_baseline": cpu, "extra": extra}
+            "roofline": roof, "cpu_baseline": cpu, "fp32_vs_fp64": fp64, "extra": extra}
     print(json.dumps(line), flush=True)
 
 

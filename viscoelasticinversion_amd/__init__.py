@@ -5,7 +5,8 @@ The compute path is the HIP library ``libvi_mi355x.so`` behind the C ABI of
 """
 from . import segy
 from .api import (CPMLConfig, DirichletBC, Fields, FreeSurfaceBC, IsotropicModel, K,
-                  Parameters, Problem, QModel, RickerSource, Solution, VTIModel, _ricker, dK_dw,
+                  Parameters, Problem, QModel, RickerSource, Solution, StepParameters, VTIModel,
+                  _ricker, apply_parameters, dK_dw,
                   find_optimal_taun, get_bc, get_Q, get_tau, gradient_to_thomsen_params,
                   gradient_to_velocity_params,
                   leapfrog_step, make_problem, misfit_and_pullback, solve_problem,
@@ -19,5 +20,6 @@ __all__ = [
     "find_optimal_taun", "get_bc", "get_Q", "get_tau", "leapfrog_step", "make_problem",
     "solve_problem", "Simulation", "FIELDS", "COMPONENTS", "SOURCE_KINDS", "Batch", "Model",
     "mem_info", "gradient_to_velocity_params", "misfit_and_pullback", "VTIModel",
-    "thomsen_to_stiffness", "segy", "gradient_to_thomsen_params",
+    "thomsen_to_stiffness", "segy", "gradient_to_thomsen_params", "StepParameters",
+    "apply_parameters",
 ]

@@ -388,13 +388,69 @@ class Fields:
     sim: Simulation
 
 
+@dataclass
+class StepParameters:
+    """The fused step's Parameters(λs, μs, τs, τns) of visco-kernels.jl:108-113 (arrays
+    (Nz, Nx); τns the L relaxation times). ``rho`` is the density the velocity kernel reads
+    (visco-kernels.jl:10-11); None keeps the density already loaded."""
+    lam: object
+    mu: object
+    tau: object = None
+    taun: object = None
+    rho: object = None
+
+
+def apply_parameters(sim, parameters):
+    """Load the model of ``parameters`` into ``sim`` where it differs from what the
+    simulation holds: a StepParameters (λs, μs, τs, τns[, ρ]) or a Parameters (its
+    IsotropicModel / VTIModel and QModel). Unchanged arrays are not uploaded again."""
+    if isinstance(parameters, StepParameters):
+        p = parameters
+        if sim.vti:
+            raise ValueError("this simulation has a VTI model: pass a Parameters with a VTIModel")
+        if sim.nrelax and p.tau is None and (sim.model_host is None or sim.model_host.get("tau") is None):
+            raise ValueError("τs are required when the simulation has memory variables")
+        if sim.model_differs(rho=p.rho, lam=p.lam, mu=p.mu, tau=p.tau if sim.nrelax else None):
+            if p.rho is None and (sim.model_host is None or sim.model_host.get("rho") is None):
+                raise ValueError("rho is required: the simulation holds no density to keep")
+            rho = p.rho if p.rho is not None else sim.model_host["rho"]
+            tau = p.tau if p.tau is not None else (sim.model_host or {}).get("tau")
+            sim.set_model(rho, p.lam, p.mu, tau if sim.nrelax else None)
+        taun = p.taun
+    elif isinstance(parameters, Parameters):
+        p = parameters
+        if (p.Nx, p.Nz) != (sim.nx, sim.nz):
+            raise ValueError("parameters grid differs from the simulation's grid")
+        if p.nrelax != sim.nrelax:
+            raise ValueError("parameters have another number of SLS mechanisms than the simulation")
+        m = p.model
+        tau = p.qmodel.tau if p.qmodel is not None else None
+        if isinstance(m, VTIModel):
+            if not sim.vti or sim.model_differs(rho=m.rho, c11=m.C11, c13=m.C13, c33=m.C33,
+                                                c55=m.C55, tau=tau):
+                sim.set_model_vti(m.rho, m.C11, m.C13, m.C33, m.C55, tau)
+        elif sim.vti or sim.model_differs(rho=m.rho, lam=m.lam, mu=m.mu, tau=tau):
+            sim.set_model(m.rho, m.lam, m.mu, tau)
+        taun = p.qmodel.taun if p.qmodel is not None else None
+    else:
+        raise TypeError("parameters must be a StepParameters or a Parameters")
+    if sim.nrelax and taun is not None:
+        t = np.asarray(taun, dtype=np.float64).ravel()
+        if sim.taun is None or not np.array_equal(sim.taun, t):
+            sim.set_relaxation(t)
+
+
 def leapfrog_step(fields, parameters=None, dt=None, nsteps=1):
     """leapfrog_step!(fields, parameters, dt) (visco-kernels.jl:87-100): advance the device
-    state by full steps (velocity kernel, then stress kernel). ``dt`` must equal the
-    simulation's dt when given."""
+    state by full steps (velocity kernel, then stress kernel) with the model of
+    ``parameters`` (StepParameters or Parameters; loaded where it differs from the
+    simulation's, see ``apply_parameters``; None keeps the loaded model). ``dt`` must equal
+    the simulation's dt when given."""
     sim = fields.sim if isinstance(fields, Fields) else fields
     if dt is not None and not math.isclose(dt, sim.dt, rel_tol=0, abs_tol=0):
         raise ValueError("dt differs from the simulation's dt")
+    if parameters is not None:
+        apply_parameters(sim, parameters)
     sim.step(nsteps)
     sim.sync()
     return None

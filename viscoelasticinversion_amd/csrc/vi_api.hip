@@ -104,7 +104,8 @@ struct DevBuf {
 // HIP events around every launch of one stream, summed per kernel kind.
 struct Prof {
   bool on = false;
-  int every = 1;  // time launches of every `every`-th step of a launch sequence
+  int every = 1;   // time the launches of every `every`-th step
+  long count = 0;  // steps seen since enable, counted across calls
   std::vector<hipEvent_t> ev;
   std::vector<int> kind;
   size_t used = 0;
@@ -145,7 +146,10 @@ struct Prof {
   void reset() {
     ms[0] = ms[1] = 0;
     n[0] = n[1] = 0;
+    count = 0;
   }
+  // called once per step: whether this step's launches carry events
+  bool tick() { return on && (count++ % every) == 0; }
 };
 
 }  // namespace
@@ -558,9 +562,10 @@ int launch_steps(hipStream_t s, size_t esz, int rz, int L, void* descs, const in
                  int ntiles, int nshots, int nsteps, Prof* prof, bool afda) {
   if (nshots == 1) tile_shot = nullptr;  // every tile belongs to shot 0
   for (int q = 0; q < nsteps; ++q) {
+    const bool timed = prof && prof->tick();
     for (int k = 0; k < 2; ++k) {
       hipEvent_t e0 = nullptr, e1 = nullptr;
-      if (prof && prof->on && q % prof->every == 0) VI_TRY(prof->begin(k, &e0, &e1));
+      if (timed) VI_TRY(prof->begin(k, &e0, &e1));
       VI_TRY(esz == 4 ? launch_one<float>(s, rz, L, k, descs, tile_shot, ntiles, q, e0, e1, afda)
                       : launch_one<double>(s, rz, L, k, descs, tile_shot, ntiles, q, e0, e1, afda));
     }
@@ -735,6 +740,8 @@ int batch_launch(vi_batch_s* b, int nsteps) {
 }
 
 int batch_prepare(vi_batch_s* b) {
+  for (auto* h : b->members)
+    if (h->grad_busy) return fail(VI_ESTATE, "batch member in use by a gradient job");
   bool stale = !b->built;
   // a member stepped on its own since the last batch step has another step index than the
   // batch descriptors (sources and receivers would use the wrong time sample)
@@ -2011,6 +2018,8 @@ int vi_batch_set(vi_batch b, const vi_handle* hs, int n) {
   if (!b || n < 0 || (n && !hs)) return fail(VI_EINVAL, "vi_batch_set: bad arguments");
   for (int q = 0; q < n; ++q) {
     if (!hs[q]) return fail(VI_EINVAL, "vi_batch_set: null handle");
+    if (hs[q]->grad_busy)
+      return fail(VI_ESTATE, "vi_batch_set: handle in use by a gradient job");
     for (int p = 0; p < q; ++p)
       if (hs[p] == hs[q]) return fail(VI_EINVAL, "vi_batch_set: handle listed twice");
   }

@@ -57,6 +57,25 @@ def afda_default_points():
     return list(out)
 
 
+def observed_traces(dobs, nrec, nt, dtype):
+    """Validate observed traces for a gradient of ``nt`` steps: ``dobs`` must be
+    (nrec, nt_obs) with nt_obs >= nt (receiver-major, the layout ``traces()`` returns).
+    Returns ``nt`` and the first ``nt`` columns as a contiguous array of ``dtype``. A short,
+    transposed or flat array raises ValueError instead of being reinterpreted."""
+    d = np.asarray(dobs)
+    if d.ndim != 2:
+        raise ValueError(f"dobs must be a 2-D (nrec, nt) array, got shape {d.shape}")
+    if d.shape[0] != nrec:
+        raise ValueError(f"dobs has {d.shape[0]} rows but the simulation has {nrec} receivers "
+                         "(expected (nrec, nt))")
+    nt = d.shape[1] if nt is None else int(nt)
+    if nt < 1:
+        raise ValueError("nt must be >= 1")
+    if d.shape[1] < nt:
+        raise ValueError(f"dobs holds {d.shape[1]} time samples, fewer than nt = {nt}")
+    return nt, np.ascontiguousarray(d[:, :nt], dtype=dtype)
+
+
 class Simulation:
     """One simulation on the GPU.
 
@@ -76,6 +95,8 @@ class Simulation:
         self.vti = False
         self.nrec = 0
         self.rec_nt = 0
+        self.model_host = None  # the model arrays as last uploaded (None: a shared window)
+        self.taun = None
 
     # -- helpers ---------------------------------------------------------------------
     def _arr(self, a, shape=None):
@@ -105,6 +126,7 @@ class Simulation:
         L.check(L.lib().vi_set_model(self._h, *(self._ptr(a) for a in arrs),
                                      self._ptr(t) if t is not None else None))
         self.vti = False
+        self.model_host = dict(zip(("rho", "lam", "mu"), arrs), tau=t)
 
     def set_model_vti(self, rho, c11, c13, c33, c55, tau=None):
         """VTI medium: density and stiffnesses C11, C13, C33, C55 (VTIModel, Models.jl:27-33)."""
@@ -118,6 +140,7 @@ class Simulation:
         L.check(L.lib().vi_set_model_vti(self._h, *(self._ptr(a) for a in arrs),
                                          self._ptr(t) if t is not None else None))
         self.vti = True
+        self.model_host = dict(zip(("rho", "c11", "c13", "c33", "c55"), arrs), tau=t)
 
     def set_model_dict(self, m):
         """Model from a dict: rho, lam, mu (isotropic) or rho, c11, c13, c33, c55 (VTI),
@@ -132,6 +155,7 @@ class Simulation:
         L.check(L.lib().vi_set_model_window(self._h, model.handle, int(x0)))
         self._model_ref = model
         self.vti = bool(getattr(model, "vti", False))
+        self.model_host = None
 
     def device_bytes(self):
         b = ctypes.c_size_t()
@@ -149,6 +173,25 @@ class Simulation:
         if t.size != self.nrelax:
             raise ValueError(f"expected {self.nrelax} relaxation times")
         L.check(L.lib().vi_set_relaxation(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        self.taun = t.copy()
+
+    def model_differs(self, **arrays):
+        """Whether any given model array (rho, lam, mu, c11, c13, c33, c55, tau; None = not
+        given) differs from what was last uploaded, in this simulation's precision. A model
+        read from a shared device Model always counts as different."""
+        if self.model_host is None:
+            return True
+        shp = (self.nz, self.nx)
+        for k, v in arrays.items():
+            if v is None:
+                continue
+            cur = self.model_host.get(k)
+            if cur is None:
+                return True
+            new = np.broadcast_to(np.asarray(v, dtype=self.dtype), shp)
+            if not np.array_equal(cur, new):
+                return True
+        return False
 
     def set_cpml(self, thickness=0, d0=0.0, alpha_max=0.0, kappa_max=1.0, power=2.0,
                  left=True, right=True, top=False, bottom=True):
@@ -240,9 +283,7 @@ class Simulation:
         adjoint of ``nt`` steps from a zero state. ``dobs`` is (nrec, nt). Leaves the
         simulation reset. Returns (J, g) with g = {"rho", "lam", "mu"[, "tau"]} for an
         isotropic model and {"rho", "c11", "c13", "c33", "c55"[, "tau"]} for a VTI model."""
-        d = self._arr(dobs)
-        nt = d.shape[1] if nt is None else int(nt)
-        d = np.ascontiguousarray(d.reshape(self.nrec, -1)[:, :nt])
+        nt, d = observed_traces(dobs, self.nrec, nt, self.dtype)
         shp = (self.nz, self.nx)
         keys = ("rho", "c11", "c13", "c33", "c55", "tau") if self.vti else ("rho", "lam", "mu", "tau")
         g = {k: np.zeros(shp, dtype=self.dtype) for k in keys}
@@ -312,9 +353,7 @@ class GradientJob:
     end() or cancel()."""
 
     def __init__(self, sim, dobs, nt=None, checkpoint_every=0):
-        d = sim._arr(dobs)
-        nt = d.shape[1] if nt is None else int(nt)
-        d = np.ascontiguousarray(d.reshape(sim.nrec, -1)[:, :nt])
+        nt, d = observed_traces(dobs, sim.nrec, nt, sim.dtype)
         self.sim, self.nt = sim, nt
         h = ctypes.c_void_p()
         L.check(L.lib().vi_gradient_begin(sim.handle, nt, sim._ptr(d), int(checkpoint_every),
