@@ -104,9 +104,11 @@ int vi_afda_default_points(int out[13]);
 /* d0 = -(power+1) vmax ln(R) / (2 thickness dx)   (CPML.jl:60) */
 double vi_cpml_auto_d0(int thickness, double vmax, double dx, double reflection_coef,
                        double power);
-/* a, b, kappa of CPMLCoefficients along one axis (CPML.jl:109-189), in double */
+/* a, b, kappa of CPMLCoefficients along one axis (CPML.jl:109-189), computed in the given
+ * precision as the reference computes them for its Float32 / Float64 arrays (the values the
+ * kernels use), returned as double */
 int vi_cpml_profile(int n, const vi_cpml_desc* c, double dt, int axis /*0=x,1=z*/,
-                    double* a, double* b, double* kappa);
+                    int precision, double* a, double* b, double* kappa);
 /* number of visible HIP devices (0 when none); never fails */
 int vi_device_count(void);
 /* free and total device memory of the current device */
@@ -166,6 +168,19 @@ int vi_reset(vi_handle h);
 int vi_step(vi_handle h, int nsteps);
 int vi_sync(vi_handle h);
 long vi_step_index(vi_handle h);
+/* Right-hand side at the handle's current state, without changing it: the analogue of
+ * stress_eq!(ds, s, v, p, t) and velocity_eq!(dv, s, v, p, t)
+ * (src/sciml_solver/ViscoelasticProblem.jl:62-174) for the fused scheme. dv [2][nz][nx] =
+ * (vx, vz) rates of the velocity half step from the stored stresses (force sources added
+ * before the buoyancy); ds [3][nz][nx] = (sxx, szz, sxz) rates of the stress half step from
+ * the stored velocities and memory variables (explosive sources added; dszz = 0 on the free-
+ * surface row; with stress imaging the image rows carry the negated rates of their mirror
+ * rows); dm [3][L][nz][nx] = memory-variable rates (NULL to skip). src_values [nsrc] = the
+ * sample of each source as given to vi_set_sources (NULL = no source term). CPML strips use
+ * psi' = b psi + a d as the step does (psi is not changed), so a half step of vi_step is
+ * x + dt * rate in the handle's precision. 0 outside the interior. The device workspace
+ * (rate planes) is allocated on the first call and kept by the handle. */
+int vi_rates(vi_handle h, const void* src_values, void* dv, void* ds, void* dm);
 /* traces recorded so far: out[nrec][nt] for steps [0, nt) since vi_set_receivers */
 int vi_get_traces(vi_handle h, void* out, int nt);
 /* device bytes owned by the handle (state, CPML memory, traces, tables, private model) */
@@ -261,7 +276,8 @@ int vi_batch_profile_read(vi_batch b, int kernel, double* ms_total, long* launch
 
 /* ---- kernel timing: HIP events recorded by the launches themselves
  * (hipExtLaunchKernelGGL) on the handle's stream, for the launches of every `every`-th
- * step (1 = all); 0 disables. Enabling resets the sums. ----------------------------- */
+ * step (1 = all; steps counted across calls since enabling); 0 disables. Enabling resets
+ * the sums. -------------------------------------------------------------------------- */
 int vi_profile_enable(vi_handle h, int every);
 /* kernel 0 = velocity, 1 = stress: summed milliseconds and launch count since enable */
 int vi_profile_read(vi_handle h, int kernel, double* ms_total, long* launches);

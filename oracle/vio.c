@@ -6,6 +6,7 @@
  */
 #include "vio.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -23,37 +24,47 @@ void vio_fd_coeffs(double h, double out[4]) {
   out[3] = (-5.0 / 7168.0) / h;
 }
 
-/* CPMLCoefficients (src/sciml_solver/CPML.jl:109-189) along one axis, in double:
- *   lo strip i in [0,T):       p = (T - i - 0.5)/T
- *   hi strip i in [n-T, n):    p = (i - (n-T) + 0.5)/T
- *   d = d0 p^power, alpha = alpha_max (1-p), kappa = 1 + (kappa_max-1) p
- *   b = exp(-(d/kappa + alpha) dt)                 where d > 0 or alpha > 0, else 0
- *   a = d/(kappa (d + kappa alpha)) (b - 1)        where |d| > 0, else 0            */
-void vio_cpml_axis(int n, int T, double d0, double alpha_max, double kappa_max, double power,
-                   double dt, int lo, int hi, double* a, double* b, double* kappa, double* dd,
-                   double* al) {
-  for (int i = 0; i < n; ++i) {
-    double p = -1.0;
-    if (lo && i < T) p = (T - i - 0.5) / T;
-    if (hi && i >= n - T) p = (i - (n - T) + 0.5) / T;
-    double d = 0.0, alpha = 0.0, k = 1.0;
-    if (p >= 0.0) {
-      d = d0 * pow(p, power);
-      alpha = alpha_max * (1.0 - p);
-      k = 1.0 + (kappa_max - 1.0) * p;
-    }
-    double bb = 0.0, aa = 0.0;
-    if (d > 0.0 || alpha > 0.0) {
-      bb = exp(-(d / k + alpha) * dt);
-      if (fabs(d) > 0.0) aa = d / (k * (d + k * alpha)) * (bb - 1.0);
-    }
-    a[i] = aa;
-    b[i] = bb;
-    kappa[i] = k;
-    if (dd) dd[i] = d;
-    if (al) al[i] = alpha;
+/* CPMLCoefficients (src/sciml_solver/CPML.jl:109-189) along one axis, in the working
+ * precision F of the arrays, as the reference computes them:
+ *   lo strip i in [0,T):       p = (T - i - 0.5)/T                      (CPML.jl:124)
+ *   hi strip i in [n-T, n):    p = (i - (n-T) + 0.5)/T                  (CPML.jl:134)
+ *   d = F(d0 p^power), alpha = F(alpha_max (1-p)), kappa = F(1 + (kappa_max-1) p), the
+ *   profiles evaluated in double and stored in F                        (CPML.jl:91-161)
+ *   where d > 0 or alpha > 0:
+ *     b = F(exp(-(d/kappa + alpha) dt)), d/kappa + alpha summed in F, the product with
+ *     the double dt and the exponential in double                       (CPML.jl:172-174)
+ *     a = d/(kappa (d + kappa alpha)) (b - 1) in F, where |d| > eps(F)  (CPML.jl:175-177)
+ * vio_cpml_axis gives F = double, vio_cpml_axis_f32 F = float. */
+#define VIO_CPML_AXIS(NAME_, F, EPS, ABS)                                                     \
+  void NAME_(int n, int T, double d0, double alpha_max, double kappa_max, double power,       \
+             double dt, int lo, int hi, F* a, F* b, F* kappa, F* dd, F* al) {                 \
+    for (int i = 0; i < n; ++i) {                                                          \
+      double p = -1.0;                                                                     \
+      if (lo && i < T) p = (T - i - 0.5) / T;                                              \
+      if (hi && i >= n - T) p = (i - (n - T) + 0.5) / T;                                   \
+      F d = 0, alpha = 0, k = 1;                                                           \
+      if (p >= 0.0) {                                                                      \
+        d = (F)(d0 * pow(p, power));                                                       \
+        alpha = (F)(alpha_max * (1.0 - p));                                                \
+        k = (F)(1.0 + (kappa_max - 1.0) * p);                                              \
+      }                                                                                    \
+      F bb = 0, aa = 0;                                                                    \
+      if (d > 0 || alpha > 0) {                                                            \
+        const F dk = d / k;                                                                \
+        const F s = dk + alpha;                                                            \
+        bb = (F)exp(-(double)s * dt);                                                      \
+        if (ABS(d) > EPS) aa = d / (k * (d + k * alpha)) * (bb - (F)1);                    \
+      }                                                                                    \
+      a[i] = aa;                                                                           \
+      b[i] = bb;                                                                           \
+      kappa[i] = k;                                                                        \
+      if (dd) dd[i] = d;                                                                   \
+      if (al) al[i] = alpha;                                                               \
+    }                                                                                      \
   }
-}
+VIO_CPML_AXIS(vio_cpml_axis, double, DBL_EPSILON, fabs)
+VIO_CPML_AXIS(vio_cpml_axis_f32, float, FLT_EPSILON, fabsf)
+#undef VIO_CPML_AXIS
 
 /* H-AFDA free surface (free_surface = 2): adjusted one-sided FD approximations after Kristek,
  * Moczo & Archuleta (2002, Stud. Geophys. Geod. 46), the scheme of the reference's draft
@@ -121,12 +132,16 @@ int vio_afda_weights(double h, const int* counts, double out[13][8]) {
 
 #define REAL float
 #define SUF _f32
+#define VIO_CPML_AXIS_REAL vio_cpml_axis_f32
 #include "vio_body.h"
+#undef VIO_CPML_AXIS_REAL
 #undef REAL
 #undef SUF
 
 #define REAL double
 #define SUF _f64
+#define VIO_CPML_AXIS_REAL vio_cpml_axis
 #include "vio_body.h"
+#undef VIO_CPML_AXIS_REAL
 #undef REAL
 #undef SUF

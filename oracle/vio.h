@@ -65,10 +65,15 @@ void vio_fd_coeffs(double h, double out[4]);
 int vio_afda_weights(double h, const int* counts, double out[13][8]);
 void vio_afda_default_points(int out[13]);
 
-/* CPML a, b, kappa along one axis of n cells (CPML.jl:109-189); lo/hi = strip enabled. */
+/* CPML a, b, kappa (and the profiles d, alpha; NULL to skip) along one axis of n cells
+ * (CPML.jl:109-189) in double and in float, each computed in its own precision as the
+ * reference computes them for Float64 and Float32 arrays; lo/hi = strip enabled. */
 void vio_cpml_axis(int n, int thickness, double d0, double alpha_max, double kappa_max,
                    double power, double dt, int lo, int hi,
                    double* a, double* b, double* kappa, double* d, double* alpha);
+void vio_cpml_axis_f32(int n, int thickness, double d0, double alpha_max, double kappa_max,
+                       double power, double dt, int lo, int hi,
+                       float* a, float* b, float* kappa, float* d, float* alpha);
 
 /* float instance */
 void* vio_create_f32(const vio_desc* d, const float* rho, const float* lam, const float* mu,
@@ -82,6 +87,9 @@ int vio_run_f32(void* h, int nsteps, int mode, int nthreads, float* traces);
 int vio_get_f32(void* h, int field, float* out);
 int vio_set_f32(void* h, int field, const float* in);
 long vio_step_index_f32(void* h);
+/* rates of the next half steps at the current state (see vio_body.h): dv[2][nz][nx],
+ * ds[3][nz][nx], dm[3][L][nz][nx] (may be NULL); src_values[nsrc] (may be NULL) */
+int vio_rates_f32(void* h, const float* src_values, float* dv, float* ds, float* dm);
 void vio_destroy_f32(void* h);
 
 /* double instance */
@@ -97,6 +105,7 @@ int vio_run_f64(void* h, int nsteps, int mode, int nthreads, double* traces);
 int vio_get_f64(void* h, int field, double* out);
 int vio_set_f64(void* h, int field, const double* in);
 long vio_step_index_f64(void* h);
+int vio_rates_f64(void* h, const double* src_values, double* dv, double* ds, double* dm);
 void vio_destroy_f64(void* h);
 
 #ifdef __cplusplus

@@ -100,15 +100,22 @@ def lib():
             f = getattr(_lib, "vio_step_index" + suf)
             f.restype = ctypes.c_long
             f.argtypes = [vp]
+            f = getattr(_lib, "vio_rates" + suf)
+            f.restype = ctypes.c_int
+            f.argtypes = [vp, rp, rp, rp, rp]
             f = getattr(_lib, "vio_destroy" + suf)
             f.restype = None
             f.argtypes = [vp]
         _lib.vio_fd_coeffs.restype = None
         _lib.vio_fd_coeffs.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         dp = ctypes.POINTER(ctypes.c_double)
+        fp = ctypes.POINTER(ctypes.c_float)
         _lib.vio_cpml_axis.restype = None
         _lib.vio_cpml_axis.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 5 + [
             ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp]
+        _lib.vio_cpml_axis_f32.restype = None
+        _lib.vio_cpml_axis_f32.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 5 + [
+            ctypes.c_int, ctypes.c_int, fp, fp, fp, fp, fp]
     return _lib
 
 
@@ -118,12 +125,16 @@ def fd_coeffs(h: float) -> np.ndarray:
     return out
 
 
-def cpml_axis(n, thickness, d0, alpha_max, kappa_max, power, dt, lo, hi):
-    """Return dict of a, b, kappa, d, alpha (float64 arrays of length n)."""
-    arrs = {k: np.zeros(n) for k in ("a", "b", "kappa", "d", "alpha")}
-    P = ctypes.POINTER(ctypes.c_double)
-    lib().vio_cpml_axis(n, thickness, d0, alpha_max, kappa_max, power, dt, int(lo), int(hi),
-                        *(arrs[k].ctypes.data_as(P) for k in ("a", "b", "kappa", "d", "alpha")))
+def cpml_axis(n, thickness, d0, alpha_max, kappa_max, power, dt, lo, hi, dtype=np.float64):
+    """Return dict of a, b, kappa, d, alpha (arrays of length n), computed in ``dtype``
+    (float64 or float32) as the reference computes its CPML coefficients."""
+    dt_ = np.dtype(dtype)
+    arrs = {k: np.zeros(n, dtype=dt_) for k in ("a", "b", "kappa", "d", "alpha")}
+    ct = ctypes.c_float if dt_ == np.float32 else ctypes.c_double
+    fn = lib().vio_cpml_axis_f32 if dt_ == np.float32 else lib().vio_cpml_axis
+    P = ctypes.POINTER(ct)
+    fn(n, thickness, d0, alpha_max, kappa_max, power, dt, int(lo), int(hi),
+       *(arrs[k].ctypes.data_as(P) for k in ("a", "b", "kappa", "d", "alpha")))
     return arrs
 
 
@@ -235,6 +246,23 @@ class Oracle:
         a = np.ascontiguousarray(values, dtype=self.dtype).reshape(self._shape(field))
         if getattr(lib(), "vio_set" + self.suf)(self.h, field, self._p(a)):
             raise ValueError("field not present")
+
+    def rates(self, src_values=None):
+        """Rates of the next half steps at the current state (the state is not changed):
+        dv (2, nz, nx) = (vx, vz), ds (3, nz, nx) = (sxx, szz, sxz), dm (3, L, nz, nx) or
+        None; ``src_values`` = one sample per source (None: no source term)."""
+        dv = np.zeros((2, self.nz, self.nx), dtype=self.dtype)
+        ds = np.zeros((3, self.nz, self.nx), dtype=self.dtype)
+        dm = np.zeros((3, self.L, self.nz, self.nx), dtype=self.dtype) if self.L else None
+        sv = None
+        if src_values is not None:
+            sv = np.ascontiguousarray(src_values, dtype=self.dtype).ravel()
+        rc = getattr(lib(), "vio_rates" + self.suf)(
+            self.h, self._p(sv) if sv is not None else None, self._p(dv), self._p(ds),
+            self._p(dm) if dm is not None else None)
+        if rc:
+            raise RuntimeError("oracle rates failed")
+        return dv, ds, dm
 
     @property
     def step_index(self):

@@ -27,7 +27,8 @@
  *     s += dt*ds (old M), then M_n += dt*(a1_n e + a2_n M_n),
  *     a1_n = -1/(L taun_n), a2_n = -1/taun_n                        visco-kernels.jl:49-57
  *     explosive sources add their sample to dsxx and dszz.
- *   CPML (CPML.jl:91-189, ViscoelasticProblem.jl:208-216), per derivative d in a strip:
+ *   CPML (CPML.jl:91-189, ViscoelasticProblem.jl:208-216; a, b, kappa computed in REAL as
+ *   the reference does, see vio.c), per derivative d in a strip:
  *     psi = b*psi + a*d;  d = d/kappa + psi   (profile index = array index of the cell)
  *   Free surface at row js = 4 (stress imaging, CPML.jl:372-380, generalised to 4 image
  *   rows for the 8th-order stencil): on row js, ezz is replaced by the value that makes
@@ -145,16 +146,11 @@ void* NAME(vio_create)(const vio_desc* d, const REAL* rho, const REAL* lam, cons
   if (s->T) {
     s->xlo = d->cpml_left; s->xhi = d->cpml_right;
     s->zlo = d->cpml_top; s->zhi = d->cpml_bottom;
-    double* a = (double*)calloc(nx > nz ? nx : nz, sizeof(double));
-    double* b = (double*)calloc(nx > nz ? nx : nz, sizeof(double));
-    double* k = (double*)calloc(nx > nz ? nx : nz, sizeof(double));
-    vio_cpml_axis(nx, s->T, d->cpml_d0, d->cpml_alpha_max, d->cpml_kappa_max, d->cpml_power,
-                  d->dt, s->xlo, s->xhi, a, b, k, NULL, NULL);
-    for (int i = 0; i < nx; ++i) { s->ax[i] = (REAL)a[i]; s->bx[i] = (REAL)b[i]; s->kx[i] = (REAL)k[i]; }
-    vio_cpml_axis(nz, s->T, d->cpml_d0, d->cpml_alpha_max, d->cpml_kappa_max, d->cpml_power,
-                  d->dt, s->zlo, s->zhi, a, b, k, NULL, NULL);
-    for (int j = 0; j < nz; ++j) { s->az[j] = (REAL)a[j]; s->bz[j] = (REAL)b[j]; s->kz[j] = (REAL)k[j]; }
-    free(a); free(b); free(k);
+    /* coefficients in the working precision (CPML.jl:109-189) */
+    VIO_CPML_AXIS_REAL(nx, s->T, d->cpml_d0, d->cpml_alpha_max, d->cpml_kappa_max,
+                       d->cpml_power, d->dt, s->xlo, s->xhi, s->ax, s->bx, s->kx, NULL, NULL);
+    VIO_CPML_AXIS_REAL(nz, s->T, d->cpml_d0, d->cpml_alpha_max, d->cpml_kappa_max,
+                       d->cpml_power, d->dt, s->zlo, s->zhi, s->az, s->bz, s->kz, NULL, NULL);
     for (int q = 0; q < 4; ++q) s->psi[q] = NAME(vio_alloc)((size_t)nz * 2 * s->T);
     for (int q = 4; q < 8; ++q) s->psi[q] = NAME(vio_alloc)((size_t)2 * s->T * nx);
   }
@@ -687,6 +683,150 @@ int NAME(vio_run)(void* h, int nsteps, int mode, int nthreads, REAL* traces) {
     if (traces && s->nrec) NAME(sample)(s, traces + (size_t)k * s->nrec);
     s->step = n + 1;
   }
+  return 0;
+}
+
+/* ---- right-hand side of the step (stress_eq! / velocity_eq!, ViscoelasticProblem.jl:62-174) --
+ * The rates the next half steps apply, from the current state, written without changing it:
+ *   dv[2][nz][nx] = (vx, vz) rates of the velocity half step from the stored stresses (image
+ *     rows as stored), force sources from src_values[q] (the sample of source q; NULL = 0);
+ *   ds[3][nz][nx] = (sxx, szz, sxz) rates of the stress half step from the stored velocities
+ *     and memory variables, explosive sources from src_values; on the free-surface row the
+ *     ezz that makes dszz = 0 is used and dszz is 0; with stress imaging the image rows carry
+ *     the negated rates of their mirror rows;
+ *   dm[3][L][nz][nx] (may be NULL) = a1_n e + a2_n M_n per mechanism.
+ * In CPML strips each derivative uses psi' = b psi + a d (psi itself is not changed), as the
+ * step does. Cells outside the interior get 0. So the step's update is x + dt * rate in REAL,
+ * operation for operation. */
+static inline REAL NAME(cpml_rate)(const REAL* psi, REAL a, REAL b, REAL k, REAL d) {
+  const REAL p = b * (*psi) + a * d;
+  return d / k + p;
+}
+
+int NAME(vio_rates)(void* h, const REAL* src_values, REAL* dv, REAL* ds, REAL* dm) {
+  SIM* s = (SIM*)h;
+  if (!s || !dv || !ds) return -1;
+  const int nx = s->d.nx, nz = s->d.nz, T = s->T, L = s->L;
+  const size_t nc = s->ncell;
+  const int fs = s->d.free_surface, afda = fs == 2;
+  const REAL two = 2, quarter = (REAL)0.25;
+  const REAL *Dx = s->Dx, *Dz = s->Dz;
+  memset(dv, 0, 2 * nc * sizeof(REAL));
+  memset(ds, 0, 3 * nc * sizeof(REAL));
+  if (dm && L) memset(dm, 0, (size_t)3 * L * nc * sizeof(REAL));
+  REAL *dvx = dv, *dvz = dv + nc, *dsxx = ds, *dszz = ds + nc, *dsxz = ds + 2 * nc;
+  for (int j = 4; j < nz - 4; ++j) {
+    const int zk = NAME(zk)(s, j);
+    for (int i = 4; i < nx - 4; ++i) {
+      const size_t c = (size_t)j * nx + i;
+      const int xk = NAME(xk)(s, i);
+      /* velocity rates */
+      REAL dsxx_x = OP_P(s->sxx, c, 1, Dx);
+      REAL dsxz_z = afda && j < 8 ? NAME(afda_dz)(s->sxz, i, nx, s->afda[j - 4])
+                                  : OP_M(s->sxz, c, (size_t)nx, Dz);
+      REAL dsxz_x = OP_M(s->sxz, c, 1, Dx);
+      REAL dszz_z = afda && j < 7 ? NAME(afda_dz)(s->szz, i, nx, s->afda[4 + j - 4])
+                                  : OP_P(s->szz, c, (size_t)nx, Dz);
+      if (xk >= 0) {
+        const size_t p = (size_t)j * 2 * T + xk;
+        dsxx_x = NAME(cpml_rate)(&s->psi[2][p], s->ax[i], s->bx[i], s->kx[i], dsxx_x);
+        dsxz_x = NAME(cpml_rate)(&s->psi[3][p], s->ax[i], s->bx[i], s->kx[i], dsxz_x);
+      }
+      if (zk >= 0) {
+        const size_t p = (size_t)zk * nx + i;
+        dsxz_z = NAME(cpml_rate)(&s->psi[6][p], s->az[j], s->bz[j], s->kz[j], dsxz_z);
+        dszz_z = NAME(cpml_rate)(&s->psi[7][p], s->az[j], s->bz[j], s->kz[j], dszz_z);
+      }
+      REAL fx = dsxx_x + dsxz_z;
+      REAL fz = dszz_z + dsxz_x;
+      int has_x = 0; /* an explosive source on this cell */
+      if (s->srcmark[c]) {
+        for (int q = 0; q < s->nsrc; ++q) {
+          if ((size_t)s->siz[q] * nx + s->six[q] != c) continue;
+          const REAL w = src_values ? src_values[q] : (REAL)0;
+          if (s->skind[q] == VIO_SRC_FORCE_Z) fz = fz + w;
+          else if (s->skind[q] == VIO_SRC_FORCE_X) fx = fx + w;
+          else has_x = 1;
+        }
+      }
+      dvx[c] = (two / (s->rho[c] + s->rho[c + 1])) * fx;
+      dvz[c] = (two / (s->rho[c] + s->rho[c + nx])) * fz;
+      /* stress rates */
+      REAL exx = OP_M(s->vx, c, 1, Dx);
+      REAL ezz = afda && j >= 5 && j < 8 ? NAME(afda_dz)(s->vz, i, nx, s->afda[7 + j - 5])
+                                         : OP_M(s->vz, c, (size_t)nx, Dz);
+      REAL ezx = afda && j < 7 ? NAME(afda_dz)(s->vx, i, nx, s->afda[10 + j - 4])
+                               : OP_P(s->vx, c, (size_t)nx, Dz);
+      REAL exz = OP_P(s->vz, c, 1, Dx);
+      if (xk >= 0) {
+        const size_t p = (size_t)j * 2 * T + xk;
+        exx = NAME(cpml_rate)(&s->psi[0][p], s->ax[i], s->bx[i], s->kx[i], exx);
+        exz = NAME(cpml_rate)(&s->psi[1][p], s->ax[i], s->bx[i], s->kx[i], exz);
+      }
+      if (zk >= 0) {
+        const size_t p = (size_t)zk * nx + i;
+        ezx = NAME(cpml_rate)(&s->psi[4][p], s->az[j], s->bz[j], s->kz[j], ezx);
+        ezz = NAME(cpml_rate)(&s->psi[5][p], s->az[j], s->bz[j], s->kz[j], ezz);
+      }
+      const REAL gxz = ezx + exz;
+      const REAL lr = s->lam[c], mr = s->mu[c];
+      const REAL mxz = quarter * (((s->mu[c] + s->mu[c + 1]) + s->mu[c + nx]) + s->mu[c + nx + 1]);
+      const REAL px = s->c11 ? s->c11[c] : lr + two * mr;
+      const REAL pz = s->c11 ? s->c33[c] : px;
+      const int fs_row = fs && j == 4;
+      REAL r_xx, r_zz, r_xz;
+      if (L > 0) {
+        const REAL t = s->tau[c];
+        const REAL px1 = px * t, px0 = px1 + px, pz1 = pz * t, pz0 = pz1 + pz;
+        const REAL l1 = lr * t, l0 = l1 + lr, m1 = mxz * t, m0 = m1 + mxz;
+        const REAL* Mxx = s->M;
+        const REAL* Mzz = s->M + (size_t)L * nc;
+        const REAL* Mxz = s->M + (size_t)2 * L * nc;
+        REAL SMxx = Mxx[c], SMzz = Mzz[c], SMxz = Mxz[c];
+        for (int k = 1; k < L; ++k) {
+          SMxx = SMxx + Mxx[(size_t)k * nc + c];
+          SMzz = SMzz + Mzz[(size_t)k * nc + c];
+          SMxz = SMxz + Mxz[(size_t)k * nc + c];
+        }
+        if (fs_row) ezz = -(((l0 * exx) + l1 * SMxx) + pz1 * SMzz) / pz0;
+        r_xx = px0 * exx + l0 * ezz + px1 * SMxx + l1 * SMzz;
+        r_zz = l0 * exx + pz0 * ezz + l1 * SMxx + pz1 * SMzz;
+        r_xz = m0 * gxz + m1 * SMxz;
+        if (dm) {
+          REAL* o = dm;
+          for (int k = 0; k < L; ++k) {
+            const size_t q = (size_t)k * nc + c;
+            o[q] = s->a1[k] * exx + s->a2[k] * Mxx[q];
+            o[(size_t)L * nc + q] = s->a1[k] * ezz + s->a2[k] * Mzz[q];
+            o[(size_t)2 * L * nc + q] = s->a1[k] * gxz + s->a2[k] * Mxz[q];
+          }
+        }
+      } else {
+        if (fs_row) ezz = -(lr * exx) / pz;
+        r_xx = px * exx + lr * ezz;
+        r_zz = lr * exx + pz * ezz;
+        r_xz = mxz * gxz;
+      }
+      if (has_x) {
+        /* explosive sources, each added in turn as the step does */
+        for (int q = 0; q < s->nsrc; ++q) {
+          if ((size_t)s->siz[q] * nx + s->six[q] != c || s->skind[q] != VIO_SRC_EXPLOSIVE) continue;
+          const REAL w = src_values ? src_values[q] : (REAL)0;
+          r_xx = r_xx + w;
+          r_zz = r_zz + w;
+        }
+      }
+      dsxx[c] = r_xx;
+      dszz[c] = fs_row ? (REAL)0 : r_zz;
+      dsxz[c] = r_xz;
+    }
+  }
+  if (fs == 1)
+    for (int m = 1; m <= 4; ++m)
+      for (int i = 4; i < nx - 4; ++i) {
+        dszz[(size_t)(4 - m) * nx + i] = -dszz[(size_t)(4 + m) * nx + i];
+        dsxz[(size_t)(4 - m) * nx + i] = -dsxz[(size_t)(4 + m - 1) * nx + i];
+      }
   return 0;
 }
 

@@ -40,19 +40,25 @@ def test_fd_coeffs_match_oracle(oracle_mod):
         assert np.array_equal(np.array(out[:]), oracle_mod.fd_coeffs(h))
 
 
+@pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("axis", [0, 1])
-def test_cpml_profile_matches_oracle_bitwise(oracle_mod, axis):
+def test_cpml_profile_matches_oracle_bitwise(oracle_mod, axis, precision):
+    """The coefficients the kernels use (computed in the handle's precision) equal the
+    oracle's in both precisions."""
     from viscoelasticinversion_amd import _lib
     n = 120
     c = _lib.CpmlDesc(20, 1234.5, 31.4, 1.7, 2.0, 1, 0, 1, 1)
     a, b, k = (np.zeros(n) for _ in range(3))
     P = ctypes.POINTER(ctypes.c_double)
-    _lib.check(_lib.lib().vi_cpml_profile(n, ctypes.byref(c), 4e-4, axis, a.ctypes.data_as(P),
-                                          b.ctypes.data_as(P), k.ctypes.data_as(P)))
+    _lib.check(_lib.lib().vi_cpml_profile(n, ctypes.byref(c), 4e-4, axis, precision,
+                                          a.ctypes.data_as(P), b.ctypes.data_as(P),
+                                          k.ctypes.data_as(P)))
     lo, hi = (1, 0) if axis == 0 else (1, 1)
-    ref = oracle_mod.cpml_axis(n, 20, 1234.5, 31.4, 1.7, 2.0, 4e-4, lo, hi)
-    assert np.array_equal(a, ref["a"]) and np.array_equal(b, ref["b"])
-    assert np.array_equal(k, ref["kappa"])
+    dt = np.float32 if precision == 0 else np.float64
+    ref = oracle_mod.cpml_axis(n, 20, 1234.5, 31.4, 1.7, 2.0, 4e-4, lo, hi, dtype=dt)
+    assert np.array_equal(a, ref["a"].astype(np.float64))
+    assert np.array_equal(b, ref["b"].astype(np.float64))
+    assert np.array_equal(k, ref["kappa"].astype(np.float64))
 
 
 @pytest.mark.parametrize("counts", [None, [8] * 13, [1, 2, 3, 4, 2, 3, 4, 2, 3, 4, 2, 3, 4]])

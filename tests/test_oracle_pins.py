@@ -187,9 +187,12 @@ def _ricker(f, t):
     return (1 - 2 * a) * np.exp(-a)
 
 
-def test_cpml_absorbs_outgoing_waves(oracle_mod):
+@pytest.mark.parametrize("kappa_max", [1.0, 2.0])
+def test_cpml_absorbs_outgoing_waves(oracle_mod, kappa_max):
     """A pulse in a box with CPML on all four sides leaves: the energy left after the waves
-    have crossed the boundary is < 1e-3 of the peak; with a rigid box it stays."""
+    have crossed the boundary is < 1e-3 of the peak; with a rigid box it stays. Also with
+    coordinate stretching (kappa_max = 2), where the stepper divides each derivative by
+    kappa."""
     n, h, dt, f = 160, 5.0, 5e-4, 15.0
     m = _uniform_model(n, n)
     nt = 900
@@ -199,7 +202,7 @@ def test_cpml_absorbs_outgoing_waves(oracle_mod):
         cp = None
         if T:
             d0 = -(2 + 1) * 2000.0 * np.log(1e-5) / (2 * T * h)
-            cp = dict(thickness=T, d0=d0, alpha_max=np.pi * f, kappa_max=1.0, power=2.0,
+            cp = dict(thickness=T, d0=d0, alpha_max=np.pi * f, kappa_max=kappa_max, power=2.0,
                       left=True, right=True, top=True, bottom=True)
         o = oracle_mod.Oracle(n, n, h, h, dt, m, dtype=F64, cpml=cp)
         o.set_sources([n // 2], [n // 2], [1], w)
@@ -228,6 +231,78 @@ def test_cpml_coefficients_match_cfs_stretching(oracle_mod):
     assert np.all(c["kappa"][20:80] == 1) and np.all(c["b"][20:80] == 0)
     # profiles grow towards the outer edge
     assert np.all(np.diff(c["d"][:20]) < 0) and np.all(np.diff(c["d"][80:]) > 0)
+
+
+def test_cpml_discrete_response_matches_cfs_with_kappa(oracle_mod):
+    """The recursion psi_n = b psi_{n-1} + a d_n, applied as d/kappa + psi, answers a harmonic
+    derivative d_n = exp(i w n dt) with H(w) = 1/kappa + a / (1 - b exp(-i w dt)). The
+    continuous CFS-PML stretches d/dx by 1/s(w), s = kappa + d/(alpha + i w) (Komatitsch &
+    Martin 2007), with d, alpha, kappa the profiles of CPMLConfig at the cell positions
+    (CPML.jl:91-134). With kappa_max = 2 the two agree to O((w + d/kappa + alpha) dt) at every
+    strip cell, which fixes kappa's place in both a and b (the zero-frequency identity above
+    does not see b); b without the 1/kappa in its exponent misses by more than 10 %."""
+    T, n, d0, amax, kmax, pw, dt = 20, 100, 500.0, 30.0, 2.0, 2.0, 2e-5
+    c = oracle_mod.cpml_axis(n, T, d0, amax, kmax, pw, dt, 1, 1)
+    i = np.arange(n)
+    pos = np.full(n, -1.0)
+    pos[:T] = (T - i[:T] - 0.5) / T            # left strip, CPML.jl:124 (0-based)
+    pos[n - T:] = (i[n - T:] - (n - T) + 0.5) / T  # right strip, CPML.jl:134
+    strip = pos >= 0
+    d = d0 * pos[strip] ** pw
+    al = amax * (1 - pos[strip])
+    k = 1 + (kmax - 1) * pos[strip]
+    np.testing.assert_allclose(c["d"][strip], d, rtol=1e-14)
+    np.testing.assert_allclose(c["alpha"][strip], al, rtol=1e-14)
+    np.testing.assert_allclose(c["kappa"][strip], k, rtol=1e-14)
+    assert np.all(c["d"][~strip] == 0) and np.all(c["kappa"][~strip] == 1)
+    a, b = c["a"][strip], c["b"][strip]
+    beta = d / k + al
+    b_slip = np.exp(-(d + al) * dt)
+    a_slip = d / (k * (d + k * al)) * (b_slip - 1)
+    for f in (1.0, 5.0, 20.0, 80.0, 300.0):
+        w = 2 * np.pi * f
+        z = np.exp(-1j * w * dt)
+        h_cont = 1 / (k + d / (al + 1j * w))
+        err = np.abs(1 / k + a / (1 - b * z) - h_cont) / np.abs(h_cont)
+        assert np.all(err <= (beta + w) * dt), (f, float(err.max()))
+        err_slip = np.abs(1 / k + a_slip / (1 - b_slip * z) - h_cont) / np.abs(h_cont)
+        assert err_slip.max() > 0.1, f
+
+
+def test_cpml_fp32_coefficients_follow_the_reference_rounding(oracle_mod):
+    """For Float32 arrays the reference stores the profiles in Float32, sums d/kappa + alpha
+    in Float32, takes the product with the Float64 dt and the exponential in Float64, rounds b
+    to Float32, forms a in Float32 and leaves a = 0 where |d| <= eps(Float32)
+    (CPML.jl:109-189). The oracle's FP32 coefficients are those values bit for bit; its FP64
+    coefficients are the same recipe in Float64."""
+    n, T, d0, amax, kmax, pw, dt = 90, 20, 1234.5, 31.4, 1.7, 2.0, 4e-4
+    for lo, hi in ((1, 1), (1, 0), (0, 1)):
+        c32 = oracle_mod.cpml_axis(n, T, d0, amax, kmax, pw, dt, lo, hi, dtype=np.float32)
+        c64 = oracle_mod.cpml_axis(n, T, d0, amax, kmax, pw, dt, lo, hi)
+        i = np.arange(n)
+        pos = np.full(n, -1.0)
+        if lo:
+            pos[:T] = (T - i[:T] - 0.5) / T
+        if hi:
+            pos[n - T:] = (i[n - T:] - (n - T) + 0.5) / T
+        on = pos >= 0
+        # double-precision elementary functions of the C library (as the reference's)
+        powf = np.array([math.pow(abs(q), pw) for q in pos])
+        for F, c in ((np.float32, c32), (np.float64, c64)):
+            d = np.where(on, d0 * powf, 0.0).astype(F)
+            al = np.where(on, amax * (1 - pos), 0.0).astype(F)
+            k = np.where(on, 1 + (kmax - 1) * pos, 1.0).astype(F)
+            act = (d > 0) | (al > 0)
+            s_ = (d / k + al).astype(F)
+            b = np.array([math.exp(-float(q) * dt) if on_ else 0.0
+                          for q, on_ in zip(s_, act)]).astype(F)
+            with np.errstate(invalid="ignore", divide="ignore"):
+                a = np.where(act & (np.abs(d) > np.finfo(F).eps),
+                             d / (k * (d + k * al)) * (b - F(1)), F(0)).astype(F)
+            for name, want in (("d", d), ("alpha", al), ("kappa", k), ("b", b), ("a", a)):
+                assert np.array_equal(c[name], want), (F, lo, hi, name)
+        # FP32 differs from FP64 rounded to FP32 somewhere (it is its own recipe)
+        assert not np.array_equal(c32["a"], c64["a"].astype(np.float32))
 
 
 def _rayleigh_speed(vp, vs):

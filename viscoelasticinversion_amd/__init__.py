@@ -10,7 +10,7 @@ from .api import (CPMLConfig, DirichletBC, Fields, FreeSurfaceBC, IsotropicModel
                   find_optimal_taun, get_bc, get_Q, get_tau, gradient_to_thomsen_params,
                   gradient_to_velocity_params,
                   leapfrog_step, make_problem, misfit_and_pullback, solve_problem,
-                  thomsen_to_stiffness)
+                  stress_eq, thomsen_to_stiffness, velocity_eq)
 from .batch import Batch, Model, mem_info
 from .stepper import COMPONENTS, FIELDS, SOURCE_KINDS, Simulation
 
@@ -21,5 +21,5 @@ __all__ = [
     "solve_problem", "Simulation", "FIELDS", "COMPONENTS", "SOURCE_KINDS", "Batch", "Model",
     "mem_info", "gradient_to_velocity_params", "misfit_and_pullback", "VTIModel",
     "thomsen_to_stiffness", "segy", "gradient_to_thomsen_params", "StepParameters",
-    "apply_parameters",
+    "apply_parameters", "stress_eq", "velocity_eq",
 ]

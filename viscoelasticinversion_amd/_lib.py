@@ -54,7 +54,7 @@ SIGNATURES = {
     "vi_afda_weights": (_I, [_D, _IP, _DP]),
     "vi_afda_default_points": (_I, [_IP]),
     "vi_cpml_auto_d0": (_D, [_I, _D, _D, _D, _D]),
-    "vi_cpml_profile": (_I, [_I, ctypes.POINTER(CpmlDesc), _D, _I, _DP, _DP, _DP]),
+    "vi_cpml_profile": (_I, [_I, ctypes.POINTER(CpmlDesc), _D, _I, _I, _DP, _DP, _DP]),
     "vi_device_count": (_I, []),
     "vi_mem_info": (_I, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "vi_model_create": (_I, [_I, _I, _I, _V, _V, _V, _V, ctypes.POINTER(_H)]),
@@ -77,6 +77,7 @@ SIGNATURES = {
     "vi_batch_profile_enable": (_I, [_H, _I]),
     "vi_batch_profile_read": (_I, [_H, _I, _DP, ctypes.POINTER(_L)]),
     "vi_create": (_I, [ctypes.POINTER(GridDesc), ctypes.POINTER(_H)]),
+    "vi_rates": (_I, [_H, _V, _V, _V, _V]),
     "vi_destroy": (_I, [_H]),
     "vi_set_model": (_I, [_H, _V, _V, _V, _V]),
     "vi_set_model_vti": (_I, [_H, _V, _V, _V, _V, _V, _V]),

@@ -9,6 +9,7 @@ Names, keyword arguments, defaults and error behaviour follow the reference:
   K, dK_dω, get_τ, get_Q, QModel   src/sciml_solver/Models.jl:65-107
   Parameters                       src/sciml_solver/Parameters.jl:31-86
   make_problem, solve_problem      src/sciml_solver/ViscoelasticProblem.jl:273-309
+  stress_eq, velocity_eq           src/sciml_solver/ViscoelasticProblem.jl:62-174
   Fields, leapfrog_step!           src/visco-kernels.jl:87-113
 The time integrator is the fused staggered leapfrog of visco-kernels.jl (the adaptive
 SciML Tsit5 path of the reference is replaced by it; ``solver`` only accepts "leapfrog").
@@ -379,6 +380,74 @@ def solve_problem(problem, solver="leapfrog", saveat=None, **kwargs):
         return Solution(t=t0 + np.arange(1, nt + 1) * p.dt, s=s, v=v, traces=tr, saved=saved)
     finally:
         sim.close()
+
+
+# ---------------------------------------------------------------------------- right-hand side
+def _rhs_simulation(p):
+    """The device simulation that evaluates the right-hand side for the Parameters ``p``:
+    created on first use and kept on ``p`` (model, CPML, free surface and source positions of
+    ``p``; CPML memory zero, as the reference's RHS never advances it)."""
+    sim = getattr(p, "_rhs_sim", None)
+    if sim is None:
+        sim = p.build()
+        if p.sources:
+            sim.set_sources([s.grid[0] for s in p.sources], [s.grid[1] for s in p.sources],
+                            [getattr(s, "kind", "force_z") for s in p.sources],
+                            np.zeros((len(p.sources), 1)))
+        p._rhs_sim = sim
+    return sim
+
+
+def _apply_images(s):
+    """Stress imaging on the state (the reference's apply_free_surface!(s), CPML.jl:382-393,
+    for the 8th-order stencil's 4 image rows above the surface row 4): szz[4-m] = -szz[4+m],
+    sxz[4-m] = -sxz[3+m] on the interior columns."""
+    nx = s.shape[-1]
+    for m in range(1, 5):
+        s[1, 4 - m, 4:nx - 4] = -s[1, 4 + m, 4:nx - 4]
+        s[2, 4 - m, 4:nx - 4] = -s[2, 3 + m, 4:nx - 4]
+
+
+def _rates(s, v, p, t, M):
+    sim = _rhs_simulation(p)
+    s = np.asarray(s)
+    v = np.asarray(v)
+    if s.shape != (3, p.Nz, p.Nx) or v.shape != (2, p.Nz, p.Nx):
+        raise ValueError("s must be (3, Nz, Nx) and v (2, Nz, Nx)")
+    for k, name in enumerate(("sxx", "szz", "sxz")):
+        sim.set(name, s[k])
+    for k, name in enumerate(("vx", "vz")):
+        sim.set(name, v[k])
+    if sim.nrelax:
+        m = np.zeros((3, sim.nrelax, p.Nz, p.Nx)) if M is None else np.asarray(M)
+        for k, name in enumerate(("mxx", "mzz", "mxz")):
+            sim.set(name, m[k])
+    vals = [float(src.value(t)) for src in p.sources] if p.sources else None
+    return sim.rates(vals, memory=True)
+
+
+def stress_eq(ds, s, v, p, t, M=None, dM=None):
+    """stress_eq!(ds, s, v, p, t) (ViscoelasticProblem.jl:62-102): ds (3, Nz, Nx) <- the
+    stress rates (sxx, szz, sxz) of the state s (3, Nz, Nx), v (2, Nz, Nx) at time t, on the
+    GPU. ``M`` (3, L, Nz, Nx) are the memory variables (None: zero) and ``dM`` receives their
+    rates when given. Explosive sources add their sample at t."""
+    _, r, dm = _rates(s, v, p, t, M)
+    ds[...] = r
+    if dM is not None and dm is not None:
+        dM[...] = dm
+    return None
+
+
+def velocity_eq(dv, s, v, p, t, M=None):
+    """velocity_eq!(dv, s, v, p, t) (ViscoelasticProblem.jl:110-174): dv (2, Nz, Nx) <- the
+    velocity rates (vx, vz) at time t, force sources added before the buoyancy. With the
+    stress-imaging free surface the images are first written into ``s``, as the reference's
+    apply_free_surface!(s) does inside velocity_eq!."""
+    if p.free_surface and p.free_surface != "afda":
+        _apply_images(s)
+    r, _, _ = _rates(s, v, p, t, M)
+    dv[...] = r
+    return None
 
 
 # ---------------------------------------------------------------------------- fused step

@@ -17,7 +17,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libvi_mi355x.so")
 SOURCES = [os.path.join(CSRC, "vi_api.hip")]
-HEADERS = [os.path.join(CSRC, "vi_kernels.hpp"), os.path.join(CSRC, "vi_adjoint.hpp"), os.path.join(ROOT, "include", "vi_mi355x.h")]
+HEADERS = [os.path.join(CSRC, "vi_kernels.hpp"), os.path.join(CSRC, "vi_adjoint.hpp"),
+           os.path.join(CSRC, "vi_rates.hpp"), os.path.join(ROOT, "include", "vi_mi355x.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

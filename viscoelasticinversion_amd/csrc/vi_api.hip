@@ -8,6 +8,7 @@
 #include "vi_mi355x.h"
 #include "vi_kernels.hpp"
 #include "vi_adjoint.hpp"
+#include "vi_rates.hpp"
 
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
@@ -16,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <string>
@@ -46,23 +48,32 @@ int fail(int code, const std::string& msg) {
 
 size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
 
-// CPMLCoefficients along one axis (CPML.jl:109-189), in double.
+// CPMLCoefficients along one axis (CPML.jl:109-189) in the working precision F, as the
+// reference computes them for its F arrays: the profiles d, alpha, kappa are evaluated in
+// double and stored in F (CPML.jl:91-161); where d > 0 or alpha > 0, b = F(exp(-(d/kappa +
+// alpha) dt)) with d/kappa + alpha summed in F and the product with the double dt and the
+// exponential in double (CPML.jl:172-174), and a = d/(kappa (d + kappa alpha)) (b - 1) in F
+// where |d| > eps(F) (CPML.jl:175-177).
+template <typename F>
 void cpml_axis(int n, int T, double d0, double alpha_max, double kappa_max, double power,
-               double dt, int lo, int hi, double* a, double* b, double* kappa) {
+               double dt, int lo, int hi, F* a, F* b, F* kappa) {
   for (int i = 0; i < n; ++i) {
     double p = -1.0;
     if (lo && i < T) p = (T - i - 0.5) / T;             // CPML.jl:124
     if (hi && i >= n - T) p = (i - (n - T) + 0.5) / T;  // CPML.jl:134
-    double d = 0.0, alpha = 0.0, k = 1.0;
+    F d = 0, alpha = 0, k = 1;
     if (p >= 0.0) {
-      d = d0 * std::pow(p, power);          // CPML.jl:91-93
-      alpha = alpha_max * (1.0 - p);        // CPML.jl:95-98
-      k = 1.0 + (kappa_max - 1.0) * p;      // CPML.jl:100-102
+      d = (F)(d0 * std::pow(p, power));          // CPML.jl:91-93
+      alpha = (F)(alpha_max * (1.0 - p));        // CPML.jl:95-98
+      k = (F)(1.0 + (kappa_max - 1.0) * p);      // CPML.jl:100-102
     }
-    double bb = 0.0, aa = 0.0;
-    if (d > 0.0 || alpha > 0.0) {
-      bb = std::exp(-(d / k + alpha) * dt);                                 // CPML.jl:174
-      if (std::fabs(d) > 0.0) aa = d / (k * (d + k * alpha)) * (bb - 1.0);  // CPML.jl:176
+    F bb = 0, aa = 0;
+    if (d > 0 || alpha > 0) {
+      const F dk = d / k;
+      const F sum = dk + alpha;
+      bb = (F)std::exp(-(double)sum * dt);                                            // :174
+      if (std::fabs(d) > std::numeric_limits<F>::epsilon())
+        aa = d / (k * (d + k * alpha)) * (bb - F(1));                                   // :176
     }
     a[i] = aa;
     b[i] = bb;
@@ -200,6 +211,7 @@ struct vi_sim_s {
   // device descriptor for launches of this handle alone
   std::vector<char> desc_host;
   DevBuf desc_dev, tile_shot_dev;
+  DevBuf rates_ws;  // vi_rates workspace (rate planes and the source list), grown on demand
   bool desc_dirty = true;
   unsigned long version = 1;  // bumped on every change that invalidates a descriptor
   // ordering with batches
@@ -780,6 +792,79 @@ int batch_finish(vi_batch_s* b, int nsteps) {
 }
 
 
+// ---------------------------------------------------------------- rates (right-hand side)
+// Workspace of one vi_rates call: the 5 + 3 L rate planes (state pitch) and the source list
+// (ix, iz, kind and value per source).
+size_t rates_bytes(const vi_sim_s* h) {
+  const size_t planes = (size_t)(5 + 3 * h->L) * h->plane * h->esz;
+  return planes + (size_t)h->nsrc * (3 * sizeof(int) + h->esz) + 256;
+}
+
+template <typename R, int L>
+void launch_rates_l(const vi::ShotDesc<R>& d, const int* six, const int* siz, const int* sk,
+                    const R* sv, int n, R* out, hipStream_t st) {
+  const dim3 grid((d.nx + 63) / 64, (d.nz + 3) / 4), block(64, 4);
+  hipLaunchKernelGGL((vi::k_rates<R, L>), grid, block, 0, st, d, six, siz, sk, sv, n, out);
+}
+
+// The rates of the next half steps at the handle's state into the host arrays dv [2][nz][nx],
+// ds [3][nz][nx] and dm [3][L][nz][nx] (optional), with ws (>= rates_bytes, 256-byte
+// aligned) as the device workspace. Allocates nothing: the source list is copied from the
+// handle's own host arrays and the caller's values.
+template <typename R>
+int rates_run(vi_sim_s* h, const void* src_values, void* dv, void* ds, void* dm, char* ws) {
+  hipStream_t st = h->stream;
+  vi::ShotDesc<R> d;
+  build_desc<R>(h, 0, &d);
+  const size_t E = sizeof(R), P = h->plane;
+  R* out = (R*)ws;
+  const size_t planes_b = (size_t)(5 + 3 * h->L) * P * E;
+  const int n = h->nsrc;
+  int* six = (int*)(ws + planes_b);
+  int* siz = six + n;
+  int* sk = siz + n;
+  R* sv = (R*)(ws + ((planes_b + 3 * (size_t)n * sizeof(int) + 63) & ~(size_t)63));
+  if (n) {
+    VI_HIP(hipMemcpyAsync(six, h->src_ix.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
+    VI_HIP(hipMemcpyAsync(siz, h->src_iz.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
+    VI_HIP(hipMemcpyAsync(sk, h->src_kind.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
+    if (src_values) VI_HIP(hipMemcpyAsync(sv, src_values, n * E, hipMemcpyHostToDevice, st));
+  }
+  const R* svp = src_values ? sv : nullptr;
+  switch (h->L) {
+#define VI_RL(l) case l: launch_rates_l<R, l>(d, six, siz, sk, svp, n, out, st); break;
+    VI_RL(0) VI_RL(1) VI_RL(2) VI_RL(3) VI_RL(4) VI_RL(5) VI_RL(6)
+#undef VI_RL
+    default: return fail(VI_EINVAL, "nrelax out of range");
+  }
+  VI_HIP(hipGetLastError());
+  if (h->fs == 1) {
+    hipLaunchKernelGGL(vi::k_rates_images<R>, dim3((h->g.nx - 8 + 63) / 64), dim3(64, 4), 0, st,
+                       h->g.nx, h->ldx, P, out);
+    VI_HIP(hipGetLastError());
+  }
+  const int nx = h->g.nx, nz = h->g.nz;
+  const size_t nc = (size_t)nx * nz;
+  for (int q = 0; q < 2; ++q)
+    VI_TRY(download_plane(st, (char*)dv + q * nc * E, out + q * P, h->ldx, nx, nz, E));
+  for (int q = 0; q < 3; ++q)
+    VI_TRY(download_plane(st, (char*)ds + q * nc * E, out + (2 + q) * P, h->ldx, nx, nz, E));
+  if (dm)
+    for (int q = 0; q < 3 * h->L; ++q)
+      VI_TRY(download_plane(st, (char*)dm + q * nc * E, out + (5 + q) * P, h->ldx, nx, nz, E));
+  VI_HIP(hipStreamSynchronize(st));
+  h->own_pending = false;
+  return 0;
+}
+
+int rates_checks(vi_sim_s* h, const void* dv, const void* ds, const char* who) {
+  VI_TRY(check_handle(h));
+  if (!dv || !ds) return fail(VI_EINVAL, std::string(who) + ": dv and ds are required");
+  VI_TRY(check_ready(h));
+  VI_TRY(wait_external(h));
+  return 0;
+}
+
 // ---------------------------------------------------------------- gradient (adjoint)
 // Device buffers of one gradient call: checkpoints (forward state every K steps), the
 // segment slots (per step: s, v^{n+1/2}, memory-variable sums, psi), the adjoint state with
@@ -1258,14 +1343,26 @@ double vi_cpml_auto_d0(int thickness, double vmax, double dx, double R, double p
   return -(power + 1.0) * vmax * std::log(R) / (2.0 * thickness * dx);  // CPML.jl:60
 }
 
-int vi_cpml_profile(int n, const vi_cpml_desc* c, double dt, int axis, double* a, double* b,
-                    double* kappa) {
-  if (!c || n <= 0 || !a || !b || !kappa || c->thickness < 0 || 2 * c->thickness > n)
+int vi_cpml_profile(int n, const vi_cpml_desc* c, double dt, int axis, int precision,
+                    double* a, double* b, double* kappa) {
+  if (!c || n <= 0 || !a || !b || !kappa || c->thickness < 0 || 2 * c->thickness > n ||
+      (precision != VI_F32 && precision != VI_F64))
     return fail(VI_EINVAL, "vi_cpml_profile: bad arguments");
   const int lo = axis == 0 ? c->left : c->top;
   const int hi = axis == 0 ? c->right : c->bottom;
-  cpml_axis(n, c->thickness, c->d0, c->alpha_max, c->kappa_max, c->power, dt, lo, hi, a, b,
-            kappa);
+  if (precision == VI_F64) {
+    cpml_axis(n, c->thickness, c->d0, c->alpha_max, c->kappa_max, c->power, dt, lo, hi, a, b,
+              kappa);
+    return 0;
+  }
+  std::vector<float> f(3 * (size_t)n);
+  cpml_axis(n, c->thickness, c->d0, c->alpha_max, c->kappa_max, c->power, dt, lo, hi,
+            f.data(), f.data() + n, f.data() + 2 * n);
+  for (int i = 0; i < n; ++i) {
+    a[i] = f[i];
+    b[i] = f[n + i];
+    kappa[i] = f[2 * n + i];
+  }
   return 0;
 }
 
@@ -1454,24 +1551,16 @@ int vi_set_cpml(vi_handle h, const vi_cpml_desc* c) {
   if (c->top && h->fs)
     return fail(VI_EINVAL, "vi_set_cpml: top strip conflicts with the free surface");
   const int nx = h->g.nx, nz = h->g.nz, T = c->thickness;
-  std::vector<double> a(std::max(nx, nz)), b(std::max(nx, nz)), k(std::max(nx, nz));
+  // [ax bx kx (nx each) | az bz kz (nz each)] in the handle's precision
   std::vector<char> hostc((size_t)(3 * nx + 3 * nz) * h->esz);
-  auto put = [&](size_t off, const std::vector<double>& v, int n) {
-    for (int q = 0; q < n; ++q) {
-      if (h->esz == 4) ((float*)hostc.data())[off + q] = (float)v[q];
-      else ((double*)hostc.data())[off + q] = v[q];
-    }
+  auto fill = [&](auto* f) {
+    cpml_axis(nx, T, c->d0, c->alpha_max, c->kappa_max, c->power, h->g.dt, c->left, c->right,
+              f, f + nx, f + 2 * nx);
+    cpml_axis(nz, T, c->d0, c->alpha_max, c->kappa_max, c->power, h->g.dt, c->top, c->bottom,
+              f + 3 * nx, f + 3 * nx + nz, f + 3 * nx + 2 * nz);
   };
-  cpml_axis(nx, T, c->d0, c->alpha_max, c->kappa_max, c->power, h->g.dt, c->left, c->right,
-            a.data(), b.data(), k.data());
-  put(0, a, nx);
-  put(nx, b, nx);
-  put(2 * nx, k, nx);
-  cpml_axis(nz, T, c->d0, c->alpha_max, c->kappa_max, c->power, h->g.dt, c->top, c->bottom,
-            a.data(), b.data(), k.data());
-  put(3 * nx, a, nz);
-  put(3 * nx + nz, b, nz);
-  put(3 * nx + 2 * nz, k, nz);
+  if (h->esz == 4) fill((float*)hostc.data());
+  else fill((double*)hostc.data());
   VI_HIP(hipStreamSynchronize(h->stream));
   VI_TRY(h->coef.alloc(hostc.size(), h->stream));
   VI_HIP(hipMemcpyAsync(h->coef.p, hostc.data(), hostc.size(), hipMemcpyHostToDevice,
@@ -1695,6 +1784,13 @@ int vi_sync(vi_handle h) {
   return 0;
 }
 
+int vi_rates(vi_handle h, const void* src_values, void* dv, void* ds, void* dm) {
+  VI_TRY(rates_checks(h, dv, ds, "vi_rates"));
+  VI_TRY(h->rates_ws.reserve(rates_bytes(h)));
+  return h->esz == 4 ? rates_run<float>(h, src_values, dv, ds, dm, (char*)h->rates_ws.p)
+                     : rates_run<double>(h, src_values, dv, ds, dm, (char*)h->rates_ws.p);
+}
+
 long vi_step_index(vi_handle h) { return h ? h->step : -1; }
 
 int vi_get_traces(vi_handle h, void* out, int nt) {
@@ -1720,7 +1816,7 @@ int vi_device_bytes(vi_handle h, size_t* bytes) {
   size_t t = 0;
   for (auto* b : mutable_arrays(h)) t += b->bytes;
   t += h->coef.bytes + h->src_tab.bytes + h->src_w.bytes + h->src_wc.bytes + h->rec_tab.bytes;
-  t += h->desc_dev.bytes + h->tile_shot_dev.bytes;
+  t += h->desc_dev.bytes + h->tile_shot_dev.bytes + h->rates_ws.bytes;
   if (h->model && h->model.use_count() == 1)
     for (auto& b : h->model->arr) t += b.bytes;
   *bytes = t;

@@ -271,6 +271,23 @@ class Simulation:
     def step_index(self):
         return int(L.lib().vi_step_index(self._h))
 
+    def rates(self, src_values=None, memory=True):
+        """Right-hand side at the current state (the state is not changed): dv (2, nz, nx) =
+        (vx, vz), ds (3, nz, nx) = (sxx, szz, sxz) and, when ``memory`` and nrelax > 0,
+        dm (3, L, nz, nx) (else None). ``src_values``: one sample per source in the order of
+        set_sources (None: no source term). One half step of ``step`` is x + dt * rate."""
+        dv = np.empty((2, self.nz, self.nx), dtype=self.dtype)
+        ds = np.empty((3, self.nz, self.nx), dtype=self.dtype)
+        dm = np.empty((3, self.nrelax, self.nz, self.nx), dtype=self.dtype) \
+            if memory and self.nrelax else None
+        sv = None
+        if src_values is not None:
+            sv = np.ascontiguousarray(np.asarray(src_values, dtype=self.dtype).ravel())
+        L.check(L.lib().vi_rates(self._h, self._ptr(sv) if sv is not None else None,
+                                 self._ptr(dv), self._ptr(ds),
+                                 self._ptr(dm) if dm is not None else None))
+        return dv, ds, dm
+
     def traces(self, nt=None):
         nt = min(self.step_index, self.rec_nt) if nt is None else int(nt)
         out = np.empty((self.nrec, nt), dtype=self.dtype)
