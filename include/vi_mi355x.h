@@ -181,6 +181,14 @@ long vi_step_index(vi_handle h);
  * x + dt * rate in the handle's precision. 0 outside the interior. The device workspace
  * (rate planes) is allocated on the first call and kept by the handle. */
 int vi_rates(vi_handle h, const void* src_values, void* dv, void* ds, void* dm);
+/* vi_rates without any device allocation (no counterpart in the reference). For callers that
+ * cannot allocate device memory: an external time integrator evaluating the right-hand side
+ * in a tight loop, serving loops that keep the device footprint fixed. The workspace is a
+ * static device array of 128 MiB (5 + 3 L rate planes of the padded grid plus the source
+ * list; larger handles are refused with VI_EINVAL), one per GPU, held until the process
+ * ends. Calls that overlap on one GPU share it and race; the results of a call do not depend
+ * on earlier calls. */
+int vi_rates_noalloc(vi_handle h, const void* src_values, void* dv, void* ds, void* dm);
 /* traces recorded so far: out[nrec][nt] for steps [0, nt) since vi_set_receivers */
 int vi_get_traces(vi_handle h, void* out, int nt);
 /* device bytes owned by the handle (state, CPML memory, traces, tables, private model) */

@@ -78,6 +78,7 @@ SIGNATURES = {
     "vi_batch_profile_read": (_I, [_H, _I, _DP, ctypes.POINTER(_L)]),
     "vi_create": (_I, [ctypes.POINTER(GridDesc), ctypes.POINTER(_H)]),
     "vi_rates": (_I, [_H, _V, _V, _V, _V]),
+    "vi_rates_noalloc": (_I, [_H, _V, _V, _V, _V]),
     "vi_destroy": (_I, [_H]),
     "vi_set_model": (_I, [_H, _V, _V, _V, _V]),
     "vi_set_model_vti": (_I, [_H, _V, _V, _V, _V, _V, _V]),

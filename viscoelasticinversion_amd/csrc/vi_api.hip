@@ -20,6 +20,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -854,6 +855,36 @@ int rates_run(vi_sim_s* h, const void* src_values, void* dv, void* ds, void* dm,
       VI_TRY(download_plane(st, (char*)dm + q * nc * E, out + (5 + q) * P, h->ldx, nx, nz, E));
   VI_HIP(hipStreamSynchronize(st));
   h->own_pending = false;
+  return 0;
+}
+
+// Device workspace of vi_rates_noalloc: a fixed array in the library's code object, so the
+// entry point never allocates. Each GPU that loads the library holds its own copy until the
+// process ends; host code must use its per-device address from hipGetSymbolAddress, never
+// the array's name. Calls that overlap on one GPU share it and race.
+constexpr size_t kRatesScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) char scratch[kRatesScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch[kMaxDevices] = {};
+
+// The address of `scratch` on `device`, looked up once per device with it current.
+int scratch_on(int device, char** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) return fail(VI_EDEVICE, "device index out of range");
+  if (!g_scratch[device]) {
+    int cur = 0;
+    VI_HIP(hipGetDevice(&cur));
+    if (cur != device) VI_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return fail(VI_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = (char*)p;
+  }
+  *out = g_scratch[device];
   return 0;
 }
 
@@ -1789,6 +1820,19 @@ int vi_rates(vi_handle h, const void* src_values, void* dv, void* ds, void* dm) 
   VI_TRY(h->rates_ws.reserve(rates_bytes(h)));
   return h->esz == 4 ? rates_run<float>(h, src_values, dv, ds, dm, (char*)h->rates_ws.p)
                      : rates_run<double>(h, src_values, dv, ds, dm, (char*)h->rates_ws.p);
+}
+
+int vi_rates_noalloc(vi_handle h, const void* src_values, void* dv, void* ds, void* dm) {
+  VI_TRY(rates_checks(h, dv, ds, "vi_rates_noalloc"));
+  const size_t need = rates_bytes(h);
+  if (need > kRatesScratchBytes)
+    return fail(VI_EINVAL, "vi_rates_noalloc: this handle needs " + std::to_string(need) +
+                               " bytes of workspace, more than the " +
+                               std::to_string(kRatesScratchBytes) + " of the static one");
+  char* ws = nullptr;
+  VI_TRY(scratch_on(h->device, &ws));
+  return h->esz == 4 ? rates_run<float>(h, src_values, dv, ds, dm, ws)
+                     : rates_run<double>(h, src_values, dv, ds, dm, ws);
 }
 
 long vi_step_index(vi_handle h) { return h ? h->step : -1; }
