@@ -191,8 +191,13 @@ int vi_rates(vi_handle h, const void* src_values, void* dv, void* ds, void* dm);
 int vi_rates_noalloc(vi_handle h, const void* src_values, void* dv, void* ds, void* dm);
 /* traces recorded so far: out[nrec][nt] for steps [0, nt) since vi_set_receivers */
 int vi_get_traces(vi_handle h, void* out, int nt);
-/* device bytes owned by the handle (state, CPML memory, traces, tables, private model) */
+/* device bytes owned by the handle (state, CPML memory, traces, tables, private model and
+ * the workspaces it keeps: see vi_trim) */
 int vi_device_bytes(vi_handle h, size_t* bytes);
+/* free the workspaces a handle keeps for its next call (no counterpart in the reference): a
+ * gradient leaves its checkpoint, slot and adjoint buffers with the handle (vi_gradient_bytes
+ * of its largest call) so that repeated gradients reuse them, vi_rates its rate planes */
+int vi_trim(vi_handle h);
 
 /* ---- gradient by the discrete adjoint (no counterpart in the reference: Enzyme is a
  * dependency but never called; the ENGD optimiser's grad_fn, EnergyNaturalGradient.jl:246,
@@ -204,7 +209,7 @@ int vi_device_bytes(vi_handle h, size_t* bytes);
  * 2 sqrt(nt) states. checkpoint_every >= nt stores every step's forward state during the
  * forward pass and recomputes nothing (memory for nt steps; vi_gradient_bytes tells the
  * size). The results do not depend on checkpoint_every. The handle is left reset (zero
- * state, step 0). */
+ * state, step 0) and keeps the workspace for its next gradient (vi_trim frees it). */
 int vi_gradient(vi_handle h, int nt, const void* dobs /*[nrec][nt]*/, int checkpoint_every,
                 double* misfit, void* g_rho, void* g_lam, void* g_mu, void* g_tau);
 /* the same for a VTI model (VTIModel, Models.jl:27-33): gradients with respect to rho, C11,
@@ -217,9 +222,8 @@ int vi_gradient_vti(vi_handle h, int nt, const void* dobs /*[nrec][nt]*/, int ch
  * begin copies dobs and allocates the job's buffers (vi_gradient_bytes); the handle then
  * belongs to the job (other calls on it fail with VI_ESTATE) until end or cancel.
  * advance enqueues at most max_steps forward, recomputed or transposed steps on the handle's
- * stream and returns without waiting (with nothing enqueued while the forward traces are
- * still being copied to the host for the residual); *finished = 1 once all work is
- * enqueued. idle returns 1 when everything enqueued so far has completed, 0 otherwise.
+ * stream and returns without waiting (the residual and the misfit are computed on the
+ * device); *finished = 1 once all work is enqueued. idle returns 1 when everything enqueued so far has completed, 0 otherwise.
  * end finishes any remaining work, waits, writes the misfit and the gradients (isotropic:
  * rho, lam, mu, tau; VTI: rho, C13, C55, tau, C11, C33; any may be NULL), resets the handle
  * and frees the job; cancel frees it without results. */

@@ -79,6 +79,7 @@ SIGNATURES = {
     "vi_create": (_I, [ctypes.POINTER(GridDesc), ctypes.POINTER(_H)]),
     "vi_rates": (_I, [_H, _V, _V, _V, _V]),
     "vi_rates_noalloc": (_I, [_H, _V, _V, _V, _V]),
+    "vi_trim": (_I, [_H]),
     "vi_destroy": (_I, [_H]),
     "vi_set_model": (_I, [_H, _V, _V, _V, _V]),
     "vi_set_model_vti": (_I, [_H, _V, _V, _V, _V, _V, _V]),

@@ -10,8 +10,9 @@
 // CPML / memory-variable arithmetic of one cell, which also writes the adjoint of every raw
 // derivative) followed by a gather kernel that applies the transposed 8th-order operators:
 // (D+)^T = -D- and (D-)^T = -D+ on the padded interior, so s_hat += -D-(d_hat) etc.
-// The forward quantities each adjoint kernel needs (strain rates, stress divergence, CPML
-// state, memory variables) are recomputed from the stored forward state of that step.
+// The forward quantities each adjoint kernel needs (the CPML-stretched strain rates, the stress
+// divergence and the memory-variable sums of that step) are stored by the recording forward
+// kernels, so the transposed pointwise kernels read them per cell without halos.
 // The forward scheme is the one of src/visco-kernels.jl:3-85 with the CPML of
 // src/sciml_solver/CPML.jl:291-328 and the stress imaging of CPML.jl:372-380 (fs = 1) or the
 // H-AFDA rows of src/free-surface-kernels.jl (fs = 2). With H-AFDA the raw z-derivative G of
@@ -38,16 +39,11 @@ struct AdjDesc {
   const R *c11, *c33;             // VTI only
   int vti;
   const R *ax, *bx, *kx, *az, *bz, *kz;
-  // forward state of the current step (pitch ldx): stress s^n (pre-step) with images, the
-  // velocity v^{n+1/2} after the velocity half, the memory-variable sums sum_k M^n_k of the
-  // xx, zz and xz components (three planes), CPML memory psi^n
-  const R *fsxx, *fszz, *fsxz, *fvx, *fvz, *fM;
-  const R* fpsi[8];
-  // sources of the shot (flat list) and the sample index of this step
-  int nsrc, snt;
-  const int *src_cell, *src_kind;
-  const R* src_w;
-  long step;
+  // forward quantities of the current step (pitch ldx, see RecSlot): the velocity half's
+  // stress divergence fx, fz (stretched, sources included, before the buoyancy), the stress
+  // half's stretched strain rates exx, ezz, gxz and the memory-variable sums sum_k M^n_k of
+  // the xx, zz and xz components (three planes)
+  const R *ffx, *ffz, *fexx, *fezz, *fgxz, *fM;
   // adjoint state (pitch ldx) and per-step work arrays
   R *avx, *avz, *asxx, *aszz, *asxz, *aM;
   R* apsi[8];
@@ -56,6 +52,12 @@ struct AdjDesc {
   // gradient accumulators (pitch ldx): lam, mu (through C11 and C33), mu_xz, tau, bx, bz;
   // VTI: g_lam = C13, g_mu = C11, g_c33 = C33, g_muxz = C55 at the sxz node
   R *g_lam, *g_mu, *g_muxz, *g_tau, *g_bx, *g_bz, *g_c33;
+  // residual injection into the velocity adjoint, done by k_adj_stress (which does not touch
+  // v_hat): the first ntgt_v targets of the CSR target list (components vx, vz), and the
+  // residual row of this step
+  const int *tcell, *tcomp, *tptr, *trec;
+  int ntgt_v;
+  const R* resid_row;
 };
 
 template <typename R>
@@ -75,13 +77,6 @@ __device__ __forceinline__ R gm(const R* g, unsigned cb, int stb, const R* D) {
   return v;
 }
 
-// forward CPML on a raw derivative d with memory p (pre-step): returns the stretched value
-__device__ __forceinline__ float cpml_fwd(float d, float p, float a, float b, float k) {
-  return d / k + (b * p + a * d);
-}
-__device__ __forceinline__ double cpml_fwd(double d, double p, double a, double b, double k) {
-  return d / k + (b * p + a * d);
-}
 // adjoint of (p' = b p + a d; dt = d/k + p'): given the adjoint of the stretched value (h)
 // and of p' (ph), returns the adjoint of d and writes the adjoint of p into *ph
 template <typename R>
@@ -103,7 +98,8 @@ template <typename R, int L>
 struct AdjStressIn {
   static constexpr int LM = L > 0 ? L : 1;
   R axv = 0, bxv = 0, kxv = 1, azv = 0, bzv = 0, kzv = 1;
-  R fp0 = 0, fp1 = 0, fp4 = 0, fp5 = 0, p0h = 0, p1h = 0, p4h = 0, p5h = 0;
+  R p0h = 0, p1h = 0, p4h = 0, p5h = 0;
+  R exx, ezz, gxz;  // the forward step's stretched strain rates
   R lr, mr, mu1, mu2, mu3, c11v, c33v, t, SMxx = 0, SMzz = 0, SMxz = 0;
   R sxxh, szz_c, sxz_c, szz_im, sxz_im;
   R mh[3 * LM];
@@ -118,14 +114,15 @@ struct AdjStressIn {
     const unsigned qx = (unsigned)(j * 2 * d.T + xk) * E, qz = (unsigned)(zk * ldx + i) * E;
     if (xk >= 0) {
       axv = gld(d.ax, i * E); bxv = gld(d.bx, i * E); kxv = gld(d.kx, i * E);
-      fp0 = gld(d.fpsi[0], qx); fp1 = gld(d.fpsi[1], qx);
       p0h = gld((const R*)d.apsi[0], qx); p1h = gld((const R*)d.apsi[1], qx);
     }
     if (zk >= 0) {
       azv = d.az[j]; bzv = d.bz[j]; kzv = d.kz[j];
-      fp4 = gld(d.fpsi[4], qz); fp5 = gld(d.fpsi[5], qz);
       p4h = gld((const R*)d.apsi[4], qz); p5h = gld((const R*)d.apsi[5], qz);
     }
+    exx = gld(d.fexx, cb);
+    ezz = gld(d.fezz, cb);
+    gxz = gld(d.fgxz, cb);
     lr = gld(d.lam, mb);
     mr = gld(d.mu, mb);
     mu1 = gld(d.mu, mb + E);
@@ -156,7 +153,7 @@ struct AdjStressIn {
 
 template <typename R, int L>
 __device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, const AdjStressIn<R, L>& in,
-                                                int i, int j, R exx, R ezx, R ezz, R exz) {
+                                                int i, int j) {
   constexpr unsigned E = sizeof(R);
   constexpr int LM = L > 0 ? L : 1;
   const int ldx = d.ldx;
@@ -175,16 +172,8 @@ __device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, const AdjSt
   const unsigned ibz = (unsigned)((8 - j) * ldx + i) * E, ibx = (unsigned)((7 - j) * ldx + i) * E;
   R* Ma[3 * LM];
   for (int k = 0; k < 3 * L; ++k) Ma[k] = d.aM + (size_t)k * d.plane;
-  // ---- forward strain rates of this step (CPML-stretched)
-  if (xk >= 0) {
-    exx = cpml_fwd(exx, in.fp0, axv, bxv, kxv);
-    exz = cpml_fwd(exz, in.fp1, axv, bxv, kxv);
-  }
-  if (zk >= 0) {
-    ezx = cpml_fwd(ezx, in.fp4, azv, bzv, kzv);
-    ezz = cpml_fwd(ezz, in.fp5, azv, bzv, kzv);
-  }
-  const R gxz = ezx + exz;
+  // ---- forward strain rates of this step (CPML-stretched), stored by the recording kernel
+  const R exx = in.exx, ezz = in.ezz, gxz = in.gxz;
   const R mxz = quarter * (((mr + in.mu1) + in.mu2) + in.mu3);
   const R px = d.vti ? in.c11v : lr + two * mr;
   const R pz = d.vti ? in.c33v : px;
@@ -305,7 +294,8 @@ __device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, const AdjSt
 template <typename R>
 struct AdjVelIn {
   R axv = 0, bxv = 0, kxv = 1, azv = 0, bzv = 0, kzv = 1;
-  R fp2 = 0, fp3 = 0, fp6 = 0, fp7 = 0, p2h = 0, p3h = 0, p6h = 0, p7h = 0;
+  R p2h = 0, p3h = 0, p6h = 0, p7h = 0;
+  R fx, fz;  // the forward step's stress divergence (stretched, sources included)
   R rho, rho1, rho2, acc_bx, acc_bz, avx, avz;
   __device__ __forceinline__ void load(const AdjDesc<R>& d, int i, int j) {
     constexpr unsigned E = sizeof(R);
@@ -317,14 +307,14 @@ struct AdjVelIn {
     const unsigned qx = (unsigned)(j * 2 * d.T + xk) * E, qz = (unsigned)(zk * ldx + i) * E;
     if (xk >= 0) {
       axv = gld(d.ax, i * E); bxv = gld(d.bx, i * E); kxv = gld(d.kx, i * E);
-      fp2 = gld(d.fpsi[2], qx); fp3 = gld(d.fpsi[3], qx);
       p2h = gld((const R*)d.apsi[2], qx); p3h = gld((const R*)d.apsi[3], qx);
     }
     if (zk >= 0) {
       azv = d.az[j]; bzv = d.bz[j]; kzv = d.kz[j];
-      fp6 = gld(d.fpsi[6], qz); fp7 = gld(d.fpsi[7], qz);
       p6h = gld((const R*)d.apsi[6], qz); p7h = gld((const R*)d.apsi[7], qz);
     }
+    fx = gld(d.ffx, cb);
+    fz = gld(d.ffz, cb);
     rho = gld(d.rho, mb);
     rho1 = gld(d.rho, mb + E);
     rho2 = gld(d.rho, mb + d.ldm * E);
@@ -337,8 +327,7 @@ struct AdjVelIn {
 
 template <typename R>
 __device__ __forceinline__ void adj_vel_cell(const AdjDesc<R>& d, const AdjVelIn<R>& in, int i,
-                                             int j, R dxx, R dxzz, R dxzx, R dzz, R vxh,
-                                             R vzh) {
+                                             int j, R vxh, R vzh) {
   constexpr unsigned E = sizeof(R);
   const int ldx = d.ldx;
   const unsigned cb = (unsigned)(j * ldx + i) * E;
@@ -347,26 +336,10 @@ __device__ __forceinline__ void adj_vel_cell(const AdjDesc<R>& d, const AdjVelIn
   const int zk = d.T ? strip_x(j, d.nz, d.T, d.zlo, d.zhi) : -1;
   const unsigned qx = (unsigned)(j * 2 * d.T + xk) * E, qz = (unsigned)(zk * ldx + i) * E;
   const R axv = in.axv, bxv = in.bxv, kxv = in.kxv, azv = in.azv, bzv = in.bzv, kzv = in.kzv;
-  const R fp2 = in.fp2, fp3 = in.fp3, fp6 = in.fp6, fp7 = in.fp7;
   R p2h = in.p2h, p3h = in.p3h, p6h = in.p6h, p7h = in.p7h;
   const R rho = in.rho, rho1 = in.rho1, rho2 = in.rho2;
   const R acc_bx = in.acc_bx, acc_bz = in.acc_bz;
-  if (xk >= 0) {
-    dxx = cpml_fwd(dxx, fp2, axv, bxv, kxv);
-    dxzx = cpml_fwd(dxzx, fp3, axv, bxv, kxv);
-  }
-  if (zk >= 0) {
-    dxzz = cpml_fwd(dxzz, fp6, azv, bzv, kzv);
-    dzz = cpml_fwd(dzz, fp7, azv, bzv, kzv);
-  }
-  R fx = dxx + dxzz, fz = dzz + dxzx;
-  for (int s = 0; s < d.nsrc; ++s) {  // source list: wave-uniform scalar loads
-    if ((unsigned)sld(d.src_cell, s) * E != cb) continue;
-    const R val = d.step < d.snt ? d.src_w[(size_t)s * d.snt + d.step] : R(0);
-    const int kind = sld(d.src_kind, s);
-    if (kind == 0) fz = fz + val;
-    else if (kind == 2) fx = fx + val;
-  }
+  const R fx = in.fx, fz = in.fz;  // stored by the recording kernel
   const R bxb = two / (rho + rho1);
   const R bzb = two / (rho + rho2);
   // gradient with respect to the buoyancies (scattered to rho at the end)
@@ -410,48 +383,43 @@ __device__ __forceinline__ R afda_corr(const R* q, int st, const R* tab, int t0,
 // XCD-aware tile order, every stenciled field staged once in LDS with its halo (TileLoad).
 // The sums are the ones of the one-cell formulas, in the same order.
 
-// Transposed stress half, pointwise: stencils of the stored v^{n+1/2} from LDS.
+// Transposed stress half, pointwise: every forward quantity is stored per cell, so the tile
+// needs no LDS and no barrier; the tile shape only keeps the loads of a wave coalesced.
 template <typename R, int L, int RZ>
 __global__ __launch_bounds__(TX* NW) void k_adj_stress(const AdjDesc<R> d) {
-  constexpr int TZ = NW * RZ, WX = TX + 8;
-  __shared__ R s_vx[(TZ + 8) * WX];
-  __shared__ R s_vz[(TZ + 8) * WX];
+  constexpr int TZ = NW * RZ;
   const int lane = threadIdx.x, w = threadIdx.y;
+  // u_hat += R^T r^n for the velocity receivers (k_adj_inject's sums, in the same order):
+  // no other thread of this kernel reads or writes v_hat, and k_adj_vel runs after it
+  for (int t = (int)blockIdx.x * (TX * NW) + w * TX + lane; t < d.ntgt_v;
+       t += (int)gridDim.x * (TX * NW)) {
+    R s = 0;
+    for (int k = d.tptr[t]; k < d.tptr[t + 1]; ++k) s = s + d.resid_row[d.trec[k]];
+    R* p = (d.tcomp[t] == 0 ? d.avx : d.avz) + d.tcell[t];
+    *p = *p + s;
+  }
   const int ntx = (d.nx - 8 + TX - 1) / TX;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int i0 = 4 + (tile % ntx) * TX, j0 = 4 + (tile / ntx) * TZ;
-  TileLoad<R, TZ + 8, 4> t_vx, t_vz;
-  t_vx.fetch(d.fvx, j0 - 4, i0, d.ldx, lane, w);
-  t_vz.fetch(d.fvz, j0 - 4, i0, d.ldx, lane, w);
   const int i = i0 + lane;
+  if (i >= d.nx - 4) return;
   AdjStressIn<R, L> in[RZ];
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     const int j = j0 + w * RZ + r;
-    if (i < d.nx - 4 && j < d.nz - 4) in[r].load(d, i, j);
+    if (j < d.nz - 4) in[r].load(d, i, j);
   }
-  t_vx.store(s_vx, lane, w);
-  t_vz.store(s_vz, lane, w);
-  __syncthreads();
-  if (i >= d.nx - 4) return;
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     const int j = j0 + w * RZ + r;
     if (j >= d.nz - 4) return;
-    const R* pvx = &s_vx[(w * RZ + r + 4) * WX + lane + 4];
-    const R* pvz = &s_vz[(w * RZ + r + 4) * WX + lane + 4];
-    R ezx = op_p(pvx, WX, d.Dz), ezz = op_m(pvz, WX, d.Dz);
-    const int ju = __builtin_amdgcn_readfirstlane(j0 + w * RZ) + r;  // j, wave-uniform
-    if (d.fs == 2 && ju < 8) {  // H-AFDA rows (first tile row: LDS row = grid row)
-      if (ju < 7) ezx = afda_dz(&s_vx[4 * WX + lane + 4], WX, d.afda, ju + 6);   // D_{j-4}
-      if (ju >= 5) ezz = afda_dz(&s_vz[4 * WX + lane + 4], WX, d.afda, ju + 2);  // C_{j-4}
-    }
-    adj_stress_cell<R, L>(d, in[r], i, j, op_m(pvx, 1, d.Dx), ezx, ezz, op_p(pvz, 1, d.Dx));
+    adj_stress_cell<R, L>(d, in[r], i, j);
   }
 }
 
 // Transposed stress half, gather (v_hat -= D^T e_hat), fused with the transposed velocity
-// half, pointwise (which needs only the cell's new v_hat): e_hat and the stored s^n from LDS.
+// half, pointwise (which needs only the cell's new v_hat and the stored stress divergence):
+// e_hat from LDS.
 template <typename R, int RZ>
 __global__ __launch_bounds__(TX* NW) void k_adj_vel(const AdjDesc<R> d) {
   constexpr int TZ = NW * RZ, WX = TX + 8;
@@ -460,24 +428,17 @@ __global__ __launch_bounds__(TX* NW) void k_adj_vel(const AdjDesc<R> d) {
   __shared__ R s_e3[TZ * WX];        // e_hat exz: x halo
   __shared__ R s_e1[(TZ + 8) * TX];  // e_hat ezx: z halo
   __shared__ R s_e2[(TZ + 8) * TX];  // e_hat ezz: z halo
-  __shared__ R s_xx[TZ * WX];        // sxx: x halo
-  __shared__ R s_zz[(TZ + 8) * TX];  // szz: z halo
-  __shared__ R s_xz[(TZ + 8) * WX];  // sxz: x and z halo
   const int lane = threadIdx.x, w = threadIdx.y;
   const int ntx = (d.nx - 8 + TX - 1) / TX;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int i0 = 4 + (tile % ntx) * TX, j0 = 4 + (tile / ntx) * TZ;
   const int ldx = d.ldx;
-  TileLoad<R, TZ, 4> t_e0, t_e3, t_xx;
-  TileLoad<R, TZ + 8, 0> t_e1, t_e2, t_zz;
-  TileLoad<R, TZ + 8, 4> t_xz;
+  TileLoad<R, TZ, 4> t_e0, t_e3;
+  TileLoad<R, TZ + 8, 0> t_e1, t_e2;
   t_e0.fetch(d.ehat[0], j0, i0, ldx, lane, w);
   t_e3.fetch(d.ehat[3], j0, i0, ldx, lane, w);
   t_e1.fetch(d.ehat[1], j0 - 4, i0, ldx, lane, w);
   t_e2.fetch(d.ehat[2], j0 - 4, i0, ldx, lane, w);
-  t_xx.fetch(d.fsxx, j0, i0, ldx, lane, w);
-  t_zz.fetch(d.fszz, j0 - 4, i0, ldx, lane, w);
-  t_xz.fetch(d.fsxz, j0 - 4, i0, ldx, lane, w);
   const int i = i0 + lane;
   AdjVelIn<R> in[RZ];
 #pragma unroll
@@ -489,9 +450,6 @@ __global__ __launch_bounds__(TX* NW) void k_adj_vel(const AdjDesc<R> d) {
   t_e3.store(s_e3, lane, w);
   t_e1.store(s_e1, lane, w);
   t_e2.store(s_e2, lane, w);
-  t_xx.store(s_xx, lane, w);
-  t_zz.store(s_zz, lane, w);
-  t_xz.store(s_xz, lane, w);
   __syncthreads();
   if (i >= d.nx - 4) return;
 #pragma unroll
@@ -505,24 +463,17 @@ __global__ __launch_bounds__(TX* NW) void k_adj_vel(const AdjDesc<R> d) {
             op_m(&s_e1[(tr + 4) * TX + lane], TX, d.Dz);
     R gvz = op_p(&s_e2[(tr + 4) * TX + lane], TX, d.Dz) +
             op_m(&s_e3[tr * WX + lane + 4], 1, d.Dx);
-    const R* pxz = &s_xz[(tr + 4) * WX + lane + 4];
-    R dsxz_z = op_m(pxz, WX, d.Dz), dszz_z = op_p(&s_zz[(tr + 4) * TX + lane], TX, d.Dz);
     const int ju = __builtin_amdgcn_readfirstlane(j0 + w * RZ) + r;  // j, wave-uniform
     if (d.fs == 2 && ju < 12) {  // H-AFDA: rows 4..7 hold the source rows (LDS row 8 - j0 + k)
       const int s0 = 8 - j0;
       gvx = gvx - afda_corr(&s_e1[s0 * TX + lane], TX, d.afda, 10, 3, ju);      // D0..2
       gvz = gvz - afda_corr(&s_e2[(s0 + 1) * TX + lane], TX, d.afda, 7, 3, ju); // C1..3
-      if (ju < 8) {  // recomputed forward divergence (first tile row: LDS row = grid row)
-        dsxz_z = afda_dz(&s_xz[4 * WX + lane + 4], WX, d.afda, ju - 4);       // A_{j-4}
-        if (ju < 7) dszz_z = afda_dz(&s_zz[4 * TX + lane], TX, d.afda, ju);   // B_{j-4}
-      }
     }
     const R vxh = in[r].avx - gvx;
     const R vzh = in[r].avz - gvz;
     gst(d.avx, cb, vxh);
     gst(d.avz, cb, vzh);
-    adj_vel_cell<R>(d, in[r], i, j, op_p(&s_xx[tr * WX + lane + 4], 1, d.Dx), dsxz_z,
-                    op_m(pxz, 1, d.Dx), dszz_z, vxh, vzh);
+    adj_vel_cell<R>(d, in[r], i, j, vxh, vzh);
   }
 }
 
@@ -614,6 +565,39 @@ __global__ __launch_bounds__(256) void k_adj_inject(R* f0, R* f1, R* f2, R* f3, 
   }
   R* p = f + tcell[t];
   *p = *p + s;
+}
+
+// ---- residual and misfit: r = d - d_obs with the traces and the residual [nt][nrec] and
+// the observed data [nrec][nt]; J = 1/2 sum r^2 accumulated in double, per receiver in time
+// order and then over the receivers in order (a fixed order: the same J on every run)
+template <typename R>
+__global__ __launch_bounds__(256) void k_residual(const R* __restrict__ traces,
+                                                  const R* __restrict__ dobs,
+                                                  R* __restrict__ resid, int nt, int nrec) {
+  const size_t n = (size_t)nt * nrec;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const size_t t = k / (size_t)nrec, r = k % (size_t)nrec;
+    resid[k] = traces[k] - dobs[r * nt + t];
+  }
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_misfit_rec(const R* __restrict__ resid, int nt, int nrec,
+                                                    double* __restrict__ part) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  double s = 0.0;
+  for (int t = 0; t < nt; ++t) {
+    const double x = (double)resid[(size_t)t * nrec + r];
+    s += 0.5 * x * x;
+  }
+  part[r] = s;
+}
+__global__ void k_misfit_sum(const double* __restrict__ part, int nrec, double* __restrict__ J) {
+  if (blockIdx.x * blockDim.x + threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int r = 0; r < nrec; ++r) s += part[r];
+  *J = s;
 }
 
 // ---- batched device-to-device copy: up to 16 regions per launch (blockIdx.y = region).

@@ -82,6 +82,22 @@ void cpml_axis(int n, int T, double d0, double alpha_max, double kappa_max, doub
   }
 }
 
+// Pinned host memory, grown on demand.
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  ~HostBuf() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int reserve(size_t nbytes);
+};
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -96,22 +112,31 @@ struct DevBuf {
   }
   int alloc(size_t nbytes, hipStream_t s) {
     release();
-    bytes = nbytes;
     if (!nbytes) return 0;
     VI_HIP(hipMalloc(&p, nbytes));
+    bytes = nbytes;
     VI_HIP(hipMemsetAsync(p, 0, nbytes, s));
     return 0;
   }
-  // grow-only allocation without clearing (tables that are fully rewritten)
+  // grow-only allocation without clearing (buffers that are written before they are read)
   int reserve(size_t nbytes) {
     if (nbytes <= bytes && p) return 0;
     release();
-    bytes = nbytes;
     if (!nbytes) return 0;
     VI_HIP(hipMalloc(&p, nbytes));
+    bytes = nbytes;
     return 0;
   }
 };
+
+int HostBuf::reserve(size_t nbytes) {
+  if (nbytes <= bytes && p) return 0;
+  release();
+  if (!nbytes) return 0;
+  VI_HIP(hipHostMalloc(&p, nbytes, 0));
+  bytes = nbytes;
+  return 0;
+}
 
 // HIP events around every launch of one stream, summed per kernel kind.
 struct Prof {
@@ -213,6 +238,10 @@ struct vi_sim_s {
   std::vector<char> desc_host;
   DevBuf desc_dev, tile_shot_dev;
   DevBuf rates_ws;  // vi_rates workspace (rate planes and the source list), grown on demand
+  // gradient workspace kept between gradient calls (checkpoints, segment slots, adjoint
+  // state), so that repeated gradients do not allocate and free it each time; vi_trim frees it
+  DevBuf grad_ck, grad_seg, grad_adj;
+  HostBuf grad_host;  // pinned staging of the gradient outputs
   bool desc_dirty = true;
   unsigned long version = 1;  // bumped on every change that invalidates a descriptor
   // ordering with batches
@@ -898,14 +927,14 @@ int rates_checks(vi_sim_s* h, const void* dv, const void* ds, const char* who) {
 
 // ---------------------------------------------------------------- gradient (adjoint)
 // Device buffers of one gradient call: checkpoints (forward state every K steps), the
-// segment slots (per step: s, v^{n+1/2}, memory-variable sums, psi), the adjoint state with
-// work planes and accumulators, the residual and the injection/source lists. Regions inside
-// the checkpoint and slot buffers start on 256-byte boundaries.
+// segment slots (per step what the adjoint needs from the forward pass: the stress divergence
+// fx, fz, the strain rates exx, ezz, gxz and the memory-variable sums; see vi::RecSlot), the
+// adjoint state with work planes and accumulators, the residual and the injection lists.
+// Regions inside the checkpoint buffer start on 256-byte boundaries.
 struct GradLayout {
   int K = 1, nck = 1;
   size_t state_bytes = 0, slot_bytes = 0, psi_total = 0, guard = 0, adj_bytes = 0;
   size_t resid_bytes = 0, lists_bytes = 0;
-  size_t psi_off[8] = {};
   size_t total() const {
     return state_bytes * nck + slot_bytes * K + adj_bytes + resid_bytes + lists_bytes;
   }
@@ -913,8 +942,13 @@ struct GradLayout {
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+void swap_buf(DevBuf& a, DevBuf& b) {
+  std::swap(a.p, b.p);
+  std::swap(a.bytes, b.bytes);
+}
+
 // live[] = byte sizes of the checkpointed arrays (5 state planes, M, 8 psi strips), in the
-// order the gradient copies them; lists_ints bounds the injection + source lists
+// order the gradient copies them; lists_ints bounds the injection lists
 GradLayout grad_layout(int nt, int K, int L, size_t pb, size_t ldx, size_t esz,
                        const size_t live[14], int nrec, size_t lists_ints) {
   GradLayout g;
@@ -923,52 +957,51 @@ GradLayout grad_layout(int nt, int K, int L, size_t pb, size_t ldx, size_t esz,
   g.nck = (nt + g.K - 1) / g.K;
   for (int q = 0; q < 14; ++q) g.state_bytes += align256(live[q]);
   g.slot_bytes = (L ? 8 : 5) * pb;
-  for (int q = 0; q < 8; ++q) {
-    g.psi_total += live[6 + q];
-    g.psi_off[q] = g.slot_bytes;
-    g.slot_bytes += align256(live[6 + q]);
-  }
+  for (int q = 0; q < 8; ++q) g.psi_total += live[6 + q];
   g.guard = 4 * ldx * esz;
   const size_t nplanes_adj = 5 + 3 * (size_t)L + 4 + 4 + 7 + 2;
   g.adj_bytes = nplanes_adj * pb + 8 * g.guard + g.psi_total;
-  g.resid_bytes = (size_t)nt * nrec * esz;
+  g.resid_bytes = 2 * align256((size_t)nt * nrec * esz) + ((size_t)nrec + 1) * sizeof(double);
   g.lists_bytes = lists_ints * sizeof(int);
   return g;
 }
 
 // A gradient computation as a sequence of asynchronous chunks on the handle's stream:
-//   forward pass storing a checkpoint every K steps -> traces to pinned host memory ->
-//   residual and misfit on the host -> for each segment, last first: restore its
-//   checkpoint, recompute it storing every step's forward state in the slots, then the
-//   transposed steps backwards -> finalisation of the accumulators.
+//   forward pass storing a checkpoint every K steps -> residual and misfit on the device ->
+//   for each segment, last first: restore its checkpoint, recompute it storing every step's
+//   forward quantities in the slots, then the transposed steps backwards -> finalisation of
+//   the accumulators.
 // The last segment is recorded into the slots while the forward pass computes it, so it is
 // never recomputed; with one segment (K >= nt) nothing is recomputed at all.
 // advance() enqueues at most max_steps forward, recomputed or transposed steps and never
-// waits for the device (it returns without work while the traces are still in flight), so
-// one host thread can interleave many gradients with other work on the GPU.
+// waits for the device, so one host thread can interleave many gradients with other work on
+// the GPU.
 template <typename R>
 struct GradRun : vi_grad_s {
-  int nt = 0, K = 1, nck = 1, L = 0, nrec = 0, nsrc = 0, ntgt = 0;
+  int nt = 0, K = 1, nck = 1, L = 0, nrec = 0, nsrc = 0, ntgt = 0, ntgt_v = 0;
   size_t E = sizeof(R), pb = 0;
   GradLayout lay;
   size_t psi_bytes[8] = {};
   std::vector<DevBuf*> live;
   DevBuf ck, seg, adj, resid, lists;
-  R* host_tr = nullptr;     // pinned: traces, then the residual [nt][nrec]
-  std::vector<R> dobs;      // [nrec][nt]
   double J = 0.0;
+  R* dobs_dev = nullptr;    // in resid, after the residual: the observed data [nrec][nt]
+  double* Jdev = nullptr;   // in resid, after the observed data: J, then nrec partial sums
   vi::AdjDesc<R> d{};
   R* g_rho_out = nullptr;
   R* g_mu_out = nullptr;
   const int *t_cell = nullptr, *t_comp = nullptr, *t_ptr = nullptr, *t_rec = nullptr;
-  hipEvent_t ev_tr = nullptr;
-  enum Phase { FWD, RESID, RESTORE, RECOMP, BACK, DONE } ph = FWD;
+  enum Phase { FWD, RESTORE, RECOMP, BACK, DONE } ph = FWD;
   int rz0 = 0;  // the handle's tiling before the job (restored at the end)
   int s = 0, n = 0;  // segment, step (FWD: steps done in segment s)
 
   ~GradRun() override {
-    if (host_tr) (void)hipHostFree(host_tr);
-    if (ev_tr) (void)hipEventDestroy(ev_tr);
+    if (h) {  // the workspace goes back to the handle for its next gradient
+      (void)hipStreamSynchronize(h->stream);
+      swap_buf(ck, h->grad_ck);
+      swap_buf(seg, h->grad_seg);
+      swap_buf(adj, h->grad_adj);
+    }
   }
 
   struct Cp { const void* s; void* d; size_t b; };
@@ -1020,21 +1053,29 @@ struct GradRun : vi_grad_s {
     for (int q = 0; q < 14; ++q) live_bytes[q] = live[q]->bytes;
     for (int q = 0; q < 8; ++q) psi_bytes[q] = live_bytes[6 + q];
     // injection targets: distinct (component, cell) pairs with their receivers in ascending
-    // order; layout [tcell | tcomp | tptr (ntgt+1) | trec (nrec) | src_cell | src_kind]
+    // order; layout [tcell | tcomp | tptr (ntgt+1) | trec (nrec)]
     std::map<std::pair<int, int>, std::vector<int>> targets;
     for (int r = 0; r < nrec; ++r)
       targets[{h->rec_comp[r], h->rec_iz[r] * h->ldx + h->rec_ix[r]}].push_back(r);
     ntgt = (int)targets.size();
-    std::vector<int> lst(3 * (size_t)ntgt + 1 + (size_t)nrec + 2 * (size_t)nsrc);
+    ntgt_v = 0;  // targets of the velocity components come first (the map orders by component)
+    for (auto& kv : targets) ntgt_v += kv.first.first <= 1;
+    std::vector<int> lst(3 * (size_t)ntgt + 1 + (size_t)nrec);
     lay = grad_layout(nt, K_, L, pb, h->ldx, E, live_bytes, nrec, lst.size());
     K = lay.K;
     nck = lay.nck;
-    VI_TRY(ck.alloc(lay.state_bytes * nck, st));
-    VI_TRY(seg.alloc(lay.slot_bytes * K, st));
+    // the handle's workspace from an earlier gradient, grown when too small; checkpoints
+    // and slots are written before they are read, so only the adjoint part is cleared
+    swap_buf(ck, h->grad_ck);
+    swap_buf(seg, h->grad_seg);
+    swap_buf(adj, h->grad_adj);
+    VI_TRY(ck.reserve(lay.state_bytes * nck));
+    VI_TRY(seg.reserve(lay.slot_bytes * K));
     // adjoint state, work arrays and accumulators; ehat/dhat planes get 4 zero guard rows in
     // front: the transposed z-operators of the free-surface image rows (rows 0..3) read rows
     // -4..-1, where the derivative adjoints are 0
-    VI_TRY(adj.alloc(lay.adj_bytes, st));
+    VI_TRY(adj.reserve(lay.adj_bytes));
+    VI_HIP(hipMemsetAsync(adj.p, 0, lay.adj_bytes, st));
     char* ab = (char*)adj.p;
     auto take = [&](size_t bytes) { char* p = ab; ab += bytes; return (R*)p; };
     auto take_guarded = [&]() { ab += lay.guard; return take(pb); };
@@ -1047,7 +1088,11 @@ struct GradRun : vi_grad_s {
     g_rho_out = take(pb);
     g_mu_out = take(pb);
     for (int q = 0; q < 8; ++q) d.apsi[q] = take(psi_bytes[q]);
-    VI_TRY(resid.alloc(lay.resid_bytes, st));
+    // [residual [nt][nrec] | observed data [nrec][nt] | J, nrec partial sums]
+    VI_TRY(resid.reserve(lay.resid_bytes));
+    dobs_dev = (R*)((char*)resid.p + align256((size_t)nt * nrec * E));
+    VI_HIP(hipMemcpyAsync(dobs_dev, dobs_, (size_t)nt * nrec * E, hipMemcpyHostToDevice, st));
+    Jdev = (double*)((char*)dobs_dev + align256((size_t)nt * nrec * E));
     {
       int t = 0, k = 0;
       int* tptr = lst.data() + 2 * ntgt;
@@ -1061,17 +1106,17 @@ struct GradRun : vi_grad_s {
       }
       tptr[ntgt] = k;
     }
-    const size_t src0 = 3 * (size_t)ntgt + 1 + nrec;
-    for (int q = 0; q < nsrc; ++q) {
-      lst[src0 + q] = h->src_iz[q] * h->ldx + h->src_ix[q];
-      lst[src0 + nsrc + q] = h->src_kind[q];
-    }
     VI_TRY(lists.alloc(lst.size() * sizeof(int), st));
     VI_HIP(hipMemcpyAsync(lists.p, lst.data(), lst.size() * sizeof(int), hipMemcpyHostToDevice, st));
     t_cell = (const int*)lists.p;
     t_comp = t_cell + ntgt;
     t_ptr = t_comp + ntgt;
     t_rec = t_ptr + ntgt + 1;
+    d.tcell = t_cell;
+    d.tcomp = t_comp;
+    d.tptr = t_ptr;
+    d.trec = t_rec;
+    d.ntgt_v = ntgt_v;
     // static part of the adjoint descriptor
     d.nx = h->g.nx; d.nz = h->g.nz; d.ldx = h->ldx; d.ldm = h->model->ldm;
     {
@@ -1084,16 +1129,9 @@ struct GradRun : vi_grad_s {
       d.rho = sd->rho; d.lam = sd->lam; d.mu = sd->mu; d.tau = sd->tau;
       d.c11 = sd->c11; d.c33 = sd->c33; d.vti = sd->vti;
       d.ax = sd->ax; d.bx = sd->bx; d.kx = sd->kx; d.az = sd->az; d.bz = sd->bz; d.kz = sd->kz;
-      d.src_w = sd->src_w; d.snt = sd->snt;
     }
     d.L = L;
     d.plane = h->plane;
-    d.nsrc = nsrc;
-    d.src_cell = (const int*)lists.p + src0;
-    d.src_kind = d.src_cell + nsrc;
-    dobs.assign(dobs_, dobs_ + (size_t)nrec * nt);
-    VI_HIP(hipHostMalloc((void**)&host_tr, (size_t)nt * nrec * E, 0));
-    VI_HIP(hipEventCreateWithFlags(&ev_tr, hipEventDisableTiming));
     VI_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     {  // 2-row tiles only when their launch holds at least four rounds of resident
        // workgroups (3 per CU at the 2-row register count); shorter launches lose more
@@ -1137,18 +1175,20 @@ struct GradRun : vi_grad_s {
 
   int backward_step(int step, const char* slot) {
     hipStream_t st = h->stream;
-    d.fsxx = (const R*)(slot);
-    d.fszz = (const R*)(slot + pb);
-    d.fsxz = (const R*)(slot + 2 * pb);
-    d.fvx = (const R*)(slot + 3 * pb);
-    d.fvz = (const R*)(slot + 4 * pb);
+    d.ffx = (const R*)(slot);
+    d.ffz = (const R*)(slot + pb);
+    d.fexx = (const R*)(slot + 2 * pb);
+    d.fezz = (const R*)(slot + 3 * pb);
+    d.fgxz = (const R*)(slot + 4 * pb);
     d.fM = (const R*)(slot + 5 * pb);
-    for (int q = 0; q < 8; ++q) d.fpsi[q] = (const R*)(slot + lay.psi_off[q]);
-    d.step = step;
-    if (ntgt)
-      hipLaunchKernelGGL(vi::k_adj_inject<R>, dim3((ntgt + 255) / 256), dim3(256), 0, st, d.avx,
-                         d.avz, d.asxx, d.aszz, d.asxz, t_cell, t_comp, t_ptr, t_rec, ntgt,
-                         (const R*)resid.p + (size_t)step * nrec);
+    d.resid_row = (const R*)resid.p + (size_t)step * nrec;
+    // stress-component receivers: their own launch before k_adj_stress reads s_hat (the
+    // velocity-component ones are injected by k_adj_stress itself)
+    const int ns = ntgt - ntgt_v;
+    if (ns > 0)
+      hipLaunchKernelGGL(vi::k_adj_inject<R>, dim3((ns + 255) / 256), dim3(256), 0, st, d.avx,
+                         d.avz, d.asxx, d.aszz, d.asxz, t_cell + ntgt_v, t_comp + ntgt_v,
+                         t_ptr + ntgt_v, t_rec, ns, d.resid_row);
     return arz == 1 ? backward_kernels<1>() : backward_kernels<2>();
   }
 
@@ -1156,7 +1196,6 @@ struct GradRun : vi_grad_s {
     vi::RecSlot rs{};
     rs.base = (char*)seg.p + (size_t)k * lay.slot_bytes;
     rs.pb = pb;
-    for (int q = 0; q < 8; ++q) rs.psi_off[q] = lay.psi_off[q];
     return rs;
   }
 
@@ -1186,27 +1225,20 @@ struct GradRun : vi_grad_s {
           ++s;
           n = 0;
         }
-        if (s == nck) {
-          VI_HIP(hipMemcpyAsync(host_tr, h->traces.p, (size_t)nt * nrec * E,
-                                hipMemcpyDeviceToHost, st));
-          VI_HIP(hipEventRecord(ev_tr, st));
-          ph = RESID;
+        if (s == nck) {  // residual and misfit on the device, then the transposed steps
+          const size_t nn = (size_t)nt * nrec;
+          const unsigned gb = (unsigned)std::min<size_t>(4096, (nn + 255) / 256);
+          hipLaunchKernelGGL(vi::k_residual<R>, dim3(gb), dim3(256), 0, st,
+                             (const R*)h->traces.p, (const R*)dobs_dev, (R*)resid.p, nt, nrec);
+          hipLaunchKernelGGL(vi::k_misfit_rec<R>, dim3((nrec + 255) / 256), dim3(256), 0, st,
+                             (const R*)resid.p, nt, nrec, Jdev + 1);
+          hipLaunchKernelGGL(vi::k_misfit_sum, dim3(1), dim3(64), 0, st, (const double*)(Jdev + 1),
+                             nrec, Jdev);
+          VI_HIP(hipGetLastError());
+          s = nck - 1;  // its slots were filled by the forward pass
+          n = nt - 1;
+          ph = BACK;
         }
-      } else if (ph == RESID) {
-        const hipError_t q = hipEventQuery(ev_tr);
-        if (q == hipErrorNotReady) break;  // traces still on their way: nothing to enqueue
-        VI_HIP(q);
-        J = 0.0;  // residual and misfit, double accumulation
-        for (int r = 0; r < nrec; ++r)
-          for (int t = 0; t < nt; ++t) {
-            const R res = host_tr[(size_t)t * nrec + r] - dobs[(size_t)r * nt + t];
-            host_tr[(size_t)t * nrec + r] = res;
-            J += 0.5 * (double)res * (double)res;
-          }
-        VI_HIP(hipMemcpyAsync(resid.p, host_tr, (size_t)nt * nrec * E, hipMemcpyHostToDevice, st));
-        s = nck - 1;  // its slots were filled by the forward pass
-        n = nt - 1;
-        ph = BACK;
       } else if (ph == RESTORE) {
         VI_TRY(copy_state(s, false));
         hipLaunchKernelGGL(vi::k_set_step<R>, dim3(1), dim3(64), 0, st,
@@ -1250,18 +1282,26 @@ struct GradRun : vi_grad_s {
   }
 
   int finish(double* misfit, void* const out[6]) override {
-    while (ph != DONE) {  // drive the rest synchronously
-      VI_TRY(advance(1 << 30));
-      if (ph == RESID) VI_HIP(hipEventSynchronize(ev_tr));
-    }
+    while (ph != DONE) VI_TRY(advance(1 << 30));  // drive the rest synchronously
     hipStream_t st = h->stream;
     const int nx = h->g.nx, nz = h->g.nz;
     const R* planes[6] = {g_rho_out, d.g_lam, g_mu_out, L ? d.g_tau : nullptr,
                           d.vti ? d.g_mu : nullptr, d.vti ? d.g_c33 : nullptr};
+    // the gradients and J go through the handle's pinned staging buffer (one strided copy per
+    // plane at the full link rate; strided copies into pageable memory go row by row), then
+    // to the caller's arrays
+    const size_t pe = (size_t)nx * nz * E;
+    VI_TRY(h->grad_host.reserve(6 * pe + sizeof(double)));
+    char* hs = (char*)h->grad_host.p;
     for (int q = 0; q < 6; ++q)
       if (out && out[q] && planes[q])
-        VI_TRY(download_plane(st, out[q], planes[q], h->ldx, nx, nz, E));
+        VI_HIP(hipMemcpy2DAsync(hs + q * pe, nx * E, planes[q], h->ldx * E, nx * E, nz,
+                                hipMemcpyDeviceToHost, st));
+    VI_HIP(hipMemcpyAsync(hs + 6 * pe, Jdev, sizeof(double), hipMemcpyDeviceToHost, st));
     VI_HIP(hipStreamSynchronize(st));
+    for (int q = 0; q < 6; ++q)
+      if (out && out[q] && planes[q]) std::memcpy(out[q], hs + q * pe, pe);
+    std::memcpy(&J, hs + 6 * pe, sizeof(double));
     if (misfit) *misfit = J;
     // leave the handle at a zero state, step 0, with its own tiling
     h->grad_busy = false;
@@ -1835,6 +1875,18 @@ int vi_rates_noalloc(vi_handle h, const void* src_values, void* dv, void* ds, vo
                      : rates_run<double>(h, src_values, dv, ds, dm, ws);
 }
 
+int vi_trim(vi_handle h) {
+  VI_TRY(check_handle(h));
+  VI_TRY(wait_external(h));
+  VI_HIP(hipStreamSynchronize(h->stream));
+  h->grad_ck.release();
+  h->grad_seg.release();
+  h->grad_adj.release();
+  h->grad_host.release();
+  h->rates_ws.release();
+  return 0;
+}
+
 long vi_step_index(vi_handle h) { return h ? h->step : -1; }
 
 int vi_get_traces(vi_handle h, void* out, int nt) {
@@ -1861,6 +1913,7 @@ int vi_device_bytes(vi_handle h, size_t* bytes) {
   for (auto* b : mutable_arrays(h)) t += b->bytes;
   t += h->coef.bytes + h->src_tab.bytes + h->src_w.bytes + h->src_wc.bytes + h->rec_tab.bytes;
   t += h->desc_dev.bytes + h->tile_shot_dev.bytes + h->rates_ws.bytes;
+  t += h->grad_ck.bytes + h->grad_seg.bytes + h->grad_adj.bytes;
   if (h->model && h->model.use_count() == 1)
     for (auto& b : h->model->arr) t += b.bytes;
   *bytes = t;
@@ -1874,7 +1927,7 @@ int vi_gradient_bytes(vi_handle h, int nt, int checkpoint_every, size_t* bytes) 
   std::vector<DevBuf*> live = mutable_arrays(h);
   size_t lb[14];
   for (int q = 0; q < 14; ++q) lb[q] = live[q]->bytes;
-  const size_t ints = 4 * (size_t)h->nrec + 1 + 2 * (size_t)h->nsrc;  // bound: targets <= nrec
+  const size_t ints = 4 * (size_t)h->nrec + 1;  // bound: targets <= nrec
   *bytes = grad_layout(nt, checkpoint_every, h->L, h->plane * h->esz, h->ldx, h->esz, lb,
                        h->nrec, ints).total();
   return 0;
@@ -1908,7 +1961,7 @@ int vi_plan_bytes(const vi_grid_desc* g, int vti, int cpml_thickness, int nsrc, 
   t += 2 * (size_t)nsrc * src_nt * esz + (size_t)nrec * rec_nt * esz;  // wavelets twice
   if (grad_nt > 0)
     t += grad_layout(grad_nt, checkpoint_every, (int)L, pb, ldx, esz, live, nrec,
-                     4 * (size_t)nrec + 1 + 2 * (size_t)nsrc).total();
+                     4 * (size_t)nrec + 1).total();
   *bytes = t;
   return 0;
 }

@@ -70,15 +70,16 @@ struct ShotDesc {
   int nrec, rec_nt;
 };
 
-// Where a recording launch (REC) stores the forward state of its step for the adjoint:
-// planes (pitch ldx, pb bytes apart) 0..2 = sxx, szz, sxz before the step (with the
-// free-surface images), 3..4 = vx, vz after the velocity half, 5..7 = the memory-variable
-// sums of xx, zz, xz before the step; the 8 CPML memory arrays before the step at
-// base + psi_off[q]. Only written by the variants with REC = true.
+// Where a recording launch (REC) stores what the adjoint of its step needs from the forward
+// pass, for the interior cells (planes of pitch ldx, pb bytes apart): 0, 1 = fx, fz, the
+// velocity half's stress divergence (CPML-stretched, free-surface rows and force sources
+// included, before the buoyancy); 2, 3, 4 = exx, ezz, gxz, the stress half's CPML-stretched
+// strain rates (ezz before the free-surface replacement); 5..7 = the memory-variable sums of
+// xx, zz, xz before the step. The adjoint then needs neither the fields' halos nor the CPML
+// memory of the step. Only written by the variants with REC = true.
 struct RecSlot {
   char* base;
   unsigned long pb;
-  unsigned long psi_off[8];
 };
 
 template <typename R>
@@ -373,7 +374,7 @@ template <typename R, int RZ, bool REC = false, bool AFDA = false>
 __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restrict__ shots,
                                                      const int* __restrict__ tile_shot,
                                                      int kstep, const RecSlot rs) {
-  static_assert(!REC || RZ <= 2, "recording needs the prefetched CPML memory");
+  static_assert(!REC || RZ <= 2, "recording variants exist for 1 and 2 rows per wave");
   constexpr int TZ = NW * RZ;
   constexpr int WX = TX + 8;
   constexpr unsigned E = sizeof(R);
@@ -456,10 +457,6 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
       if (xk >= 0) {
         const unsigned q = (unsigned)(j * 2 * T + xk) * E;
         if constexpr (PRE) {
-          if constexpr (REC) {
-            gst((R*)(rs.base + rs.psi_off[2]), q, cp.px0[r]);
-            gst((R*)(rs.base + rs.psi_off[3]), q, cp.px1[r]);
-          }
           dsxx_x = cpml_apply_pre(d.psi[2], q, cp.px0[r], cp.ax, cp.bx, cp.kx, dsxx_x);
           dsxz_x = cpml_apply_pre(d.psi[3], q, cp.px1[r], cp.ax, cp.bx, cp.kx, dsxz_x);
         } else {
@@ -473,10 +470,6 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
         const unsigned q = (unsigned)(zk * ldx + i) * E;
         const R a = d.az[j], b = d.bz[j], k = d.kz[j];
         if constexpr (PRE) {
-          if constexpr (REC) {
-            gst((R*)(rs.base + rs.psi_off[6]), q, cp.pz0[r]);
-            gst((R*)(rs.base + rs.psi_off[7]), q, cp.pz1[r]);
-          }
           dsxz_z = cpml_apply_pre(d.psi[6], q, cp.pz0[r], a, b, k, dsxz_z);
           dszz_z = cpml_apply_pre(d.psi[7], q, cp.pz1[r], a, b, k, dszz_z);
         } else {
@@ -509,8 +502,8 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
       gst(vx, cb, nvx);
       gst(vz, cb, nvz);
       if constexpr (REC) {
-        gst(rec_plane<R>(rs, 3), cb, nvx);
-        gst(rec_plane<R>(rs, 4), cb, nvz);
+        gst(rec_plane<R>(rs, 0), cb, fx);
+        gst(rec_plane<R>(rs, 1), cb, fz);
       }
     }
   }
@@ -525,7 +518,7 @@ template <typename R, int L, int RZ, bool REC = false, bool AFDA = false>
 __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void k_stress(const ShotDesc<R>* __restrict__ shots,
                                                    const int* __restrict__ tile_shot,
                                                    int kstep, const RecSlot rs) {
-  static_assert(!REC || RZ <= 2, "recording needs the prefetched CPML memory");
+  static_assert(!REC || RZ <= 2, "recording variants exist for 1 and 2 rows per wave");
   constexpr int TZ = NW * RZ;
   constexpr int WX = TX + 8;
   constexpr unsigned E = sizeof(R);
@@ -564,8 +557,8 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void
   for (int k = 0; k < 3 * LM; ++k) Mp[k] = d.M + (size_t)k * d.plane;
   R wmu[RZ + 1], wmu1[RZ + 1];  // mu at columns i and i+1, rows jw .. jw+RZ
   // CPML memory before the barrier: pays for 2-row tiles (measured); the 1-row variant
-  // keeps its registers for occupancy unless it records
-  constexpr bool PRE = REC || RZ == 2;
+  // keeps its registers for occupancy
+  constexpr bool PRE = RZ == 2;
   CpmlRows<R, PRE ? RZ : 1> cp;
   if constexpr (PRE) cp.fetch(d, i, jw, i < d.nx - 4, 0, 1, 4, 5);
   const long step = d.step + kstep;
@@ -660,10 +653,6 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void
       if (xk >= 0) {
         const unsigned q = (unsigned)(j * 2 * T + xk) * E;
         if constexpr (PRE) {
-          if constexpr (REC) {
-            gst((R*)(rs.base + rs.psi_off[0]), q, cp.px0[r]);
-            gst((R*)(rs.base + rs.psi_off[1]), q, cp.px1[r]);
-          }
           exx = cpml_apply_pre(d.psi[0], q, cp.px0[r], cp.ax, cp.bx, cp.kx, exx);
           exz = cpml_apply_pre(d.psi[1], q, cp.px1[r], cp.ax, cp.bx, cp.kx, exz);
         } else {
@@ -677,10 +666,6 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void
         const unsigned q = (unsigned)(zk * ldx + i) * E;
         const R a = d.az[j], b = d.bz[j], k = d.kz[j];
         if constexpr (PRE) {
-          if constexpr (REC) {
-            gst((R*)(rs.base + rs.psi_off[4]), q, cp.pz0[r]);
-            gst((R*)(rs.base + rs.psi_off[5]), q, cp.pz1[r]);
-          }
           ezx = cpml_apply_pre(d.psi[4], q, cp.pz0[r], a, b, k, ezx);
           ezz = cpml_apply_pre(d.psi[5], q, cp.pz1[r], a, b, k, ezz);
         } else {
@@ -690,6 +675,11 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void
       }
     }
     const R gxz = ezx + exz;
+    if constexpr (REC) {
+      gst(rec_plane<R>(rs, 2), cb, exx);
+      gst(rec_plane<R>(rs, 3), cb, ezz);
+      gst(rec_plane<R>(rs, 4), cb, gxz);
+    }
     const R lr = olam[h];
     const R mxz = omxz[h];
     const R px = oc11[h];  // C11
@@ -745,15 +735,6 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void
       const R val = step < d.snt ? d.src_wc[(size_t)step * d.nsrc + s] : R(0);
       dsxx = dsxx + val;
       dszz = dszz + val;
-    }
-    if constexpr (REC) {  // the stress before the step, and its images (always -interior)
-      gst(rec_plane<R>(rs, 0), cb, osxx[h]);
-      gst(rec_plane<R>(rs, 1), cb, oszz[h]);
-      gst(rec_plane<R>(rs, 2), cb, osxz[h]);
-      if (fs == 1) {
-        if (j >= 5 && j <= 8) gst(rec_plane<R>(rs, 1), (unsigned)((8 - j) * ldx + i) * E, -oszz[h]);
-        if (j >= 4 && j <= 7) gst(rec_plane<R>(rs, 2), (unsigned)((7 - j) * ldx + i) * E, -osxz[h]);
-      }
     }
     const R nsxx = osxx[h] + dt * dsxx;
     R nszz = oszz[h] + dt * dszz;

@@ -162,6 +162,10 @@ class Simulation:
         L.check(L.lib().vi_device_bytes(self._h, ctypes.byref(b)))
         return b.value
 
+    def trim(self):
+        """Free the workspaces the handle keeps for its next gradient or rates call."""
+        L.check(L.lib().vi_trim(self._h))
+
     def gradient_bytes(self, nt, checkpoint_every=0):
         """Temporary device bytes of one gradient(…, nt, checkpoint_every) call."""
         b = ctypes.c_size_t()
@@ -303,7 +307,7 @@ class Simulation:
         nt, d = observed_traces(dobs, self.nrec, nt, self.dtype)
         shp = (self.nz, self.nx)
         keys = ("rho", "c11", "c13", "c33", "c55", "tau") if self.vti else ("rho", "lam", "mu", "tau")
-        g = {k: np.zeros(shp, dtype=self.dtype) for k in keys}
+        g = {k: np.empty(shp, dtype=self.dtype) for k in keys}  # every element is written
         J = ctypes.c_double()
         ptrs = [self._ptr(g[k]) for k in keys[:-1]] + [self._ptr(g["tau"]) if self.nrelax else None]
         fn = L.lib().vi_gradient_vti if self.vti else L.lib().vi_gradient
