@@ -189,6 +189,10 @@ int vi_rates(vi_handle h, const void* src_values, void* dv, void* ds, void* dm);
  * ends. Calls that overlap on one GPU share it and race; the results of a call do not depend
  * on earlier calls. */
 int vi_rates_noalloc(vi_handle h, const void* src_values, void* dv, void* ds, void* dm);
+/* *finite = 1 when every interior velocity value is finite, 0 after a blow-up (an unstable
+ * time step or model); waits for the handle's queued work. No counterpart in the reference:
+ * a serving loop fails a diverging job alone with it. */
+int vi_check_finite(vi_handle h, int* finite);
 /* traces recorded so far: out[nrec][nt] for steps [0, nt) since vi_set_receivers */
 int vi_get_traces(vi_handle h, void* out, int nt);
 /* device bytes owned by the handle (state, CPML memory, traces, tables, private model and

@@ -65,3 +65,45 @@ def test_plan_bounds_device_bytes(vti):
           f"plan {grad_plan - fwd_plan}", flush=True)
     assert base <= fwd_plan <= base + 64 * 1024
     assert grad_plan - fwd_plan == grad
+
+
+def test_blown_up_jobs_fail_alone():
+    """A forward job and a gradient job whose dt exceeds the stability limit (the pre-check
+    switched off) blow up: each fails alone with an error, every other job's result equals the
+    job run alone."""
+    from viscoelasticinversion_amd.api import stable_dt
+    from viscoelasticinversion_amd.configs import serving_job_config
+    from viscoelasticinversion_amd.serve import Job, Server, run_serial
+    jobs = []
+    for k, (kind, bad) in enumerate((("forward", False), ("forward", True), ("gradient", False),
+                                     ("gradient", True), ("forward", False))):
+        cfg = serving_job_config(96 + 16 * k, 72, 300, seed=40 + k)
+        if bad:
+            cfg["dt"] = 1.6 * stable_dt(cfg["model"], cfg["dx"], cfg["dz"])
+        dobs = np.zeros((len(cfg["rec_ix"]), 300), np.float32) if kind == "gradient" else None
+        jobs.append(Job(id=k, kind=kind, cfg=cfg, arrival=0.01 * k, dobs=dobs))
+    print("[case] serve 5 jobs, 2 unstable", flush=True)
+    res = Server(block_steps=25, check_cfl=False).run(jobs)
+    alone = run_serial(jobs, check_cfl=False)
+    for r, a in zip(res, alone):
+        if r.id in (1, 3):
+            assert r.error and "blew up" in r.error, r.error
+            assert a.error and "blew up" in a.error, a.error
+            continue
+        assert r.error is None and a.error is None
+        if r.kind == "forward":
+            assert np.array_equal(r.traces, a.traces)
+        else:
+            assert r.misfit == a.misfit and np.isfinite(r.misfit)
+
+
+def test_unstable_jobs_refused_on_arrival():
+    from viscoelasticinversion_amd.api import stable_dt
+    from viscoelasticinversion_amd.configs import serving_job_config
+    from viscoelasticinversion_amd.serve import Job, Server
+    good = serving_job_config(96, 72, 60, seed=1)
+    bad = serving_job_config(96, 72, 60, seed=2)
+    bad["dt"] = 1.05 * stable_dt(bad["model"], bad["dx"], bad["dz"])
+    res = Server(block_steps=20).run([Job(0, "forward", good), Job(1, "forward", bad, 0.001)])
+    assert res[0].error is None and np.abs(res[0].traces).max() > 0
+    assert res[1].error and "stability limit" in res[1].error and res[1].traces is None

@@ -80,6 +80,7 @@ SIGNATURES = {
     "vi_rates": (_I, [_H, _V, _V, _V, _V]),
     "vi_rates_noalloc": (_I, [_H, _V, _V, _V, _V]),
     "vi_trim": (_I, [_H]),
+    "vi_check_finite": (_I, [_H, ctypes.POINTER(_I)]),
     "vi_destroy": (_I, [_H]),
     "vi_set_model": (_I, [_H, _V, _V, _V, _V]),
     "vi_set_model_vti": (_I, [_H, _V, _V, _V, _V, _V, _V]),

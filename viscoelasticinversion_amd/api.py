@@ -382,6 +382,28 @@ def solve_problem(problem, solver="leapfrog", saveat=None, **kwargs):
         sim.close()
 
 
+# ---------------------------------------------------------------------------- stability
+# sum of |weights| x h of the staggered 8th-order operator (1225/1024, -245/3072, 49/5120,
+# -5/7168): the largest magnitude of its symbol
+FD_WEIGHT_SUM = 1225 / 1024 + 245 / 3072 + 49 / 5120 + 5 / 7168
+
+
+def stable_dt(model, dx, dz):
+    """Largest stable time step of the staggered 8th-order leapfrog for a model dict (rho and
+    lam, mu or c11, c33, and tau when viscoelastic): the von Neumann bound
+    dt <= 1 / (vmax * sum|c| * sqrt(1/dx^2 + 1/dz^2)) with vmax the largest unrelaxed
+    P-wave speed, sqrt(M (1 + tau) / rho) with M = lam + 2 mu (isotropic) or max(C11, C33)."""
+    rho = np.asarray(model["rho"], np.float64)
+    if "c11" in model:
+        m = np.maximum(np.asarray(model["c11"], np.float64), np.asarray(model["c33"], np.float64))
+    else:
+        m = np.asarray(model["lam"], np.float64) + 2 * np.asarray(model["mu"], np.float64)
+    if model.get("tau") is not None:
+        m = m * (1 + np.asarray(model["tau"], np.float64))
+    vmax = float(np.sqrt(m / rho).max())
+    return 1.0 / (vmax * FD_WEIGHT_SUM * math.sqrt(1.0 / dx ** 2 + 1.0 / dz ** 2))
+
+
 # ---------------------------------------------------------------------------- right-hand side
 def _rhs_simulation(p):
     """The device simulation that evaluates the right-hand side for the Parameters ``p``:

@@ -242,6 +242,7 @@ struct vi_sim_s {
   // state), so that repeated gradients do not allocate and free it each time; vi_trim frees it
   DevBuf grad_ck, grad_seg, grad_adj;
   HostBuf grad_host;  // pinned staging of the gradient outputs
+  DevBuf check_flag;  // vi_check_finite's flag
   bool desc_dirty = true;
   unsigned long version = 1;  // bumped on every change that invalidates a descriptor
   // ordering with batches
@@ -1887,6 +1888,31 @@ int vi_trim(vi_handle h) {
   return 0;
 }
 
+int vi_check_finite(vi_handle h, int* finite) {
+  VI_TRY(check_handle(h));
+  if (!finite) return fail(VI_EINVAL, "vi_check_finite: null argument");
+  VI_TRY(wait_external(h));
+  hipStream_t st = h->stream;
+  VI_TRY(h->check_flag.reserve(256));
+  int flag = 0;
+  VI_HIP(hipMemsetAsync(h->check_flag.p, 0, sizeof(int), st));
+  const dim3 grid((h->g.nx - 8 + 63) / 64, (h->g.nz - 8 + 3) / 4), block(64, 4);
+  if (h->esz == 4)
+    hipLaunchKernelGGL(vi::k_nonfinite<float>, grid, block, 0, st, (const float*)h->state[0].p,
+                       (const float*)h->state[1].p, h->g.nx, h->g.nz, h->ldx,
+                       (int*)h->check_flag.p);
+  else
+    hipLaunchKernelGGL(vi::k_nonfinite<double>, grid, block, 0, st, (const double*)h->state[0].p,
+                       (const double*)h->state[1].p, h->g.nx, h->g.nz, h->ldx,
+                       (int*)h->check_flag.p);
+  VI_HIP(hipGetLastError());
+  VI_HIP(hipMemcpyAsync(&flag, h->check_flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  VI_HIP(hipStreamSynchronize(st));
+  h->own_pending = false;
+  *finite = flag ? 0 : 1;
+  return 0;
+}
+
 long vi_step_index(vi_handle h) { return h ? h->step : -1; }
 
 int vi_get_traces(vi_handle h, void* out, int nt) {
@@ -1913,7 +1939,7 @@ int vi_device_bytes(vi_handle h, size_t* bytes) {
   for (auto* b : mutable_arrays(h)) t += b->bytes;
   t += h->coef.bytes + h->src_tab.bytes + h->src_w.bytes + h->src_wc.bytes + h->rec_tab.bytes;
   t += h->desc_dev.bytes + h->tile_shot_dev.bytes + h->rates_ws.bytes;
-  t += h->grad_ck.bytes + h->grad_seg.bytes + h->grad_adj.bytes;
+  t += h->grad_ck.bytes + h->grad_seg.bytes + h->grad_adj.bytes + h->check_flag.bytes;
   if (h->model && h->model.use_count() == 1)
     for (auto& b : h->model->arr) t += b.bytes;
   *bytes = t;

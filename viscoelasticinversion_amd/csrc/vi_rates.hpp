@@ -164,4 +164,17 @@ __global__ void k_rates_images(int nx, int ldx, size_t plane, R* __restrict__ ou
   dsxz[(4 - m) * ldx + i] = -dsxz[(4 + m - 1) * ldx + i];
 }
 
+// Whether every interior value of the planes is finite: a thread that finds a NaN or an
+// infinity stores 1 into *flag (cleared before; every writer writes the same value).
+template <typename R>
+__global__ __launch_bounds__(256) void k_nonfinite(const R* __restrict__ a,
+                                                   const R* __restrict__ b, int nx, int nz,
+                                                   int ldx, int* __restrict__ flag) {
+  const int i = 4 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 4 + blockIdx.y * 4 + threadIdx.y;
+  if (i >= nx - 4 || j >= nz - 4) return;
+  const size_t c = (size_t)j * ldx + i;
+  if (!isfinite(a[c]) || !isfinite(b[c])) *flag = 1;
+}
+
 }  // namespace vi

@@ -77,6 +77,34 @@ def job_bytes(job: Job, precision="float32") -> int:
                       c["nt"] if job.kind == "gradient" else 0, job.checkpoint_every)
 
 
+def stability_error(job):
+    """Why the job's time step cannot be stable (None when it can): dt above the von Neumann
+    bound of the staggered 8th-order leapfrog for the job's fastest unrelaxed P wave."""
+    from .api import stable_dt
+    c = job.cfg
+    lim = stable_dt(c["model"], c["dx"], c["dz"])
+    if c["dt"] > lim:
+        return (f"dt = {c['dt']:.4g} s exceeds the stability limit {lim:.4g} s of this model "
+                "and grid (8th-order staggered leapfrog)")
+    return None
+
+
+def gradient_checkpoint(job, budget, precision="float32", fraction=0.25):
+    """Checkpoint interval of a gradient job: nt (every step recorded in the forward pass,
+    nothing recomputed, ~25 % faster) when that plan needs at most ``fraction`` of the HBM
+    budget, else the job's own (0 = ceil(sqrt(nt))). The gradient does not depend on it."""
+    if job.kind != "gradient" or job.checkpoint_every:
+        return job.checkpoint_every
+    full = job_bytes(Job(job.id, job.kind, job.cfg, checkpoint_every=job.cfg["nt"]), precision)
+    return job.cfg["nt"] if full <= fraction * budget else 0
+
+
+def _with_checkpoint(job, K):
+    if K == job.checkpoint_every:
+        return job
+    return Job(job.id, job.kind, job.cfg, job.arrival, job.dobs, K)
+
+
 def _build(cfg, precision):
     from .configs import build_simulation
     return build_simulation(cfg, precision)
@@ -85,9 +113,12 @@ def _build(cfg, precision):
 def _run_gradient(job, precision):
     sim = _build(job.cfg, precision)
     try:
-        return sim.gradient(job.dobs, job.cfg["nt"], job.checkpoint_every)
+        J, g = sim.gradient(job.dobs, job.cfg["nt"], job.checkpoint_every)
     finally:
         sim.close()
+    if not np.isfinite(J):
+        raise FloatingPointError("the wavefield blew up: non-finite misfit")
+    return J, g
 
 
 def _start_gradient(job, precision):
@@ -155,10 +186,16 @@ class Server:
     memory when the server is created); ``block_steps``: steps per batched block (graph
     length) and per gradient chunk; ``max_gradients``: gradient jobs in flight at once;
     ``max_forward``: forward jobs in one batch; ``max_head_wait``: seconds a job that does
-    not fit may be overtaken by later, smaller jobs."""
+    not fit may be overtaken by later, smaller jobs; ``check_cfl``: refuse jobs whose dt
+    exceeds the stability limit when they arrive; ``store_fraction``: gradient jobs whose
+    every-step plan needs at most this fraction of the budget record every step instead of
+    checkpointing (see gradient_checkpoint). A forward job whose wavefield stops being finite
+    is failed alone after the block in which it happened; so is a gradient job whose misfit is
+    not finite."""
 
     def __init__(self, precision="float32", hbm_budget=None, block_steps=100, max_gradients=1,
-                 max_forward=128, graph=True, max_head_wait=1.0, keep_outputs=True):
+                 max_forward=128, graph=True, max_head_wait=1.0, keep_outputs=True,
+                 check_cfl=True, store_fraction=0.25):
         from .batch import mem_info
         self.precision = precision
         if hbm_budget is None:
@@ -171,16 +208,21 @@ class Server:
         self.graph = graph
         self.max_head_wait = float(max_head_wait)
         self.keep_outputs = keep_outputs
+        self.check_cfl = check_cfl
+        self.store_fraction = float(store_fraction)
         self.peak_bytes = 0
 
     def run(self, jobs, on_event=None):
         """Serve ``jobs`` in real time (arrival times relative to the call) and return their
         JobResults in job order."""
         from .batch import Batch
-        jobs = sorted(jobs, key=lambda j: j.arrival)
+        jobs = sorted((_with_checkpoint(j, gradient_checkpoint(j, self.budget, self.precision,
+                                                               self.store_fraction))
+                       for j in jobs), key=lambda j: j.arrival)
         res = {j.id: JobResult(j.id, j.kind, j.arrival, cells=j.cfg["nx"] * j.cfg["nz"],
                                steps=j.cfg["nt"]) for j in jobs}
         need = {j.id: job_bytes(j, self.precision) for j in jobs}
+        unstable = {j.id: stability_error(j) for j in jobs} if self.check_cfl else {}
         pending = deque(jobs)
         waiting = deque()
         active = {}  # forward: id -> [job, sim, remaining steps]
@@ -200,7 +242,12 @@ class Server:
         def arrivals():
             now = clock()
             while pending and pending[0].arrival <= now:
-                waiting.append(pending.popleft())
+                job = pending.popleft()
+                if unstable.get(job.id):
+                    res[job.id].error = unstable[job.id]
+                    res[job.id].done = clock()
+                    continue
+                waiting.append(job)
 
         def release(jid):
             nonlocal used
@@ -221,6 +268,9 @@ class Server:
                         del grads[jid]
                         r.misfit, r.gradient = gj.end()
                         sim.close()
+                        if not np.isfinite(r.misfit):
+                            r.error = "the wavefield blew up: non-finite misfit"
+                            r.gradient = None
                         if not self.keep_outputs:
                             r.gradient = None
                         release(jid)
@@ -280,7 +330,12 @@ class Server:
                     finished = []
                     for k in ids:
                         active[k][2] -= n
-                        if active[k][2] == 0:
+                        if not active[k][1].is_finite():  # failed alone, the others go on
+                            job, sim, left = active.pop(k)
+                            res[k].error = (f"the wavefield blew up (non-finite values by step "
+                                            f"{job.cfg['nt'] - left})")
+                            finished.append((job, sim, left))
+                        elif active[k][2] == 0:
                             finished.append(active.pop(k))
                     if finished:
                         # leave the batch before the handles are released
@@ -288,7 +343,8 @@ class Server:
                         batch.set([active[k][1] for k in members])
                         for job, sim, _ in finished:
                             r = res[job.id]
-                            r.traces = sim.traces(job.cfg["nt"])
+                            if r.error is None:
+                                r.traces = sim.traces(job.cfg["nt"])
                             sim.close()
                             if not self.keep_outputs:
                                 r.traces = None
@@ -311,10 +367,17 @@ class Server:
         return [res[j.id] for j in sorted(jobs, key=lambda j: j.id)]
 
 
-def run_serial(jobs, precision="float32", keep_outputs=True, on_event=None):
+def run_serial(jobs, precision="float32", keep_outputs=True, on_event=None, hbm_budget=None,
+               check_cfl=True, store_fraction=0.25):
     """Baseline: the plain single-shot stepper, one job at a time in arrival order (a job
-    starts when it has arrived and the previous one is done)."""
-    jobs = sorted(jobs, key=lambda j: j.arrival)
+    starts when it has arrived and the previous one is done), with the server's checks and
+    gradient checkpoint policy, so that the two differ only in how jobs share the GPU."""
+    from .batch import mem_info
+    if hbm_budget is None:
+        hbm_budget = int(0.85 * mem_info()[0])
+    jobs = sorted((_with_checkpoint(j, gradient_checkpoint(j, hbm_budget, precision,
+                                                           store_fraction))
+                   for j in jobs), key=lambda j: j.arrival)
     out = {}
     t0 = time.perf_counter()
     for job in jobs:
@@ -325,13 +388,20 @@ def run_serial(jobs, precision="float32", keep_outputs=True, on_event=None):
                       steps=job.cfg["nt"])
         r.admitted = time.perf_counter() - t0
         try:
+            err = stability_error(job) if check_cfl else None
+            if err:
+                raise FloatingPointError(err)
             if job.kind == "gradient":
                 r.misfit, r.gradient = _run_gradient(job, precision)
             else:
                 sim = _build(job.cfg, precision)
-                sim.step(job.cfg["nt"])
-                r.traces = sim.traces(job.cfg["nt"])
-                sim.close()
+                try:
+                    sim.step(job.cfg["nt"])
+                    if not sim.is_finite():
+                        raise FloatingPointError("the wavefield blew up (non-finite values)")
+                    r.traces = sim.traces(job.cfg["nt"])
+                finally:
+                    sim.close()
         except Exception as e:  # noqa: BLE001 - reported per job
             r.error = str(e)
         r.done = time.perf_counter() - t0

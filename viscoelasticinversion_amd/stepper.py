@@ -162,6 +162,12 @@ class Simulation:
         L.check(L.lib().vi_device_bytes(self._h, ctypes.byref(b)))
         return b.value
 
+    def is_finite(self):
+        """False once the wavefield holds a NaN or an infinity (an unstable step or model)."""
+        f = ctypes.c_int()
+        L.check(L.lib().vi_check_finite(self._h, ctypes.byref(f)))
+        return bool(f.value)
+
     def trim(self):
         """Free the workspaces the handle keeps for its next gradient or rates call."""
         L.check(L.lib().vi_trim(self._h))
