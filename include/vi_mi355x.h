@@ -16,6 +16,11 @@
  *   H-AFDA surface: FreeSurfaceStencil, Stencil.jl:52-105, and the row kernels of
  *                   free-surface-kernels.jl:16-214      -> vi_set_free_surface(h, 2),
  *                                                          vi_set_afda_points, vi_afda_weights
+ *   H-AFDA formula #3: _stencil_hermitian, Stencil.jl:119-189 (D3, D3_deriv of
+ *                   Stencil.jl:52-105), _stress_freesurf_j2! free-surface-kernels.jl:154-214
+ *                                                       -> vi_set_free_surface(h, 3),
+ *                                                          vi_set_afda_hermitian,
+ *                                                          vi_afda_hermitian_weights
  *   IsotropicModel / QModel         src/sciml_solver/Models.jl:3-25, 90-107 -> vi_set_model,
  *                                                                              vi_set_relaxation
  *   VTIModel                        src/sciml_solver/Models.jl:27-33   -> vi_set_model_vti
@@ -101,6 +106,13 @@ void vi_fd_coeffs(double h, double out[4]);
  * others 2..8); NULL = the default {3,4,6,8, 3,4,6, 2,4,6, 3,4,6}. Unused columns are 0. */
 int vi_afda_weights(double h, const int* counts, double out[13][8]);
 int vi_afda_default_points(int out[13]);
+/* H-AFDA formula #3 (the reference's _stencil_hermitian(pts_half, 0, 1, 1), Stencil.jl:119-189,
+ * as FreeSurfaceStencil's D3 / D3_deriv, Stencil.jl:52-105): d vz/dz at z = h from d vz/dz(0)
+ * and vz at the half points (m+1/2) h, m < points (2..8):
+ *   d vz/dz(h) = out[0] * d vz/dz(0) + sum_m out[1 + m] * vz((m+1/2) h);
+ * out[0] = alpha (dimensionless), out[1 + m] = beta_m / h, 0 for m >= points. Each weight is
+ * the correctly rounded exact rational (then divided by h). */
+int vi_afda_hermitian_weights(double h, int points, double out[9]);
 /* d0 = -(power+1) vmax ln(R) / (2 thickness dx)   (CPML.jl:60) */
 double vi_cpml_auto_d0(int thickness, double vmax, double dx, double reflection_coef,
                        double power);
@@ -147,12 +159,19 @@ int vi_set_cpml(vi_handle h, const vi_cpml_desc* c);
  * below). mode 0: none (Dirichlet ring); 1: stress imaging, szz and sxz mirrored with opposite
  * sign into rows 0..3 (CPML.jl:372-380); 2: H-AFDA, the vertical derivatives of rows 4..7 come
  * from one-sided stencils on rows 4..11 (vi_afda_weights) and nothing above the surface is
- * read or written. Needs nz >= 17 and no top CPML strip. The gradient entry points take every
- * mode (the adjoint transposes the one-sided rows). */
+ * read or written; 3: mode 2 with d vz/dz on row 5 (z = h) from the Hermitian formula #3
+ * (vi_afda_hermitian_weights) whose d vz/dz(0) is the surface row's value that keeps
+ * szz = 0 (-lam/(lam + 2 mu) d vx/dx when elastic, free-surface-kernels.jl:160-172; with the
+ * memory terms when L > 0; -C13/C33 d vx/dx for VTI). Needs nz >= 17 and no top CPML strip.
+ * The gradient entry points take modes 0..2 (the adjoint transposes the one-sided rows) and
+ * refuse mode 3 with VI_EINVAL. */
 int vi_set_free_surface(vi_handle h, int mode);
 /* points per H-AFDA stencil (see vi_afda_weights; NULL = default). All-8 counts, the full
  * order of the reference's FreeSurfaceStencil(8), diverge in heterogeneous media. */
 int vi_set_afda_points(vi_handle h, const int* counts);
+/* half points of formula #3 (2..8, default 5). The reference's FreeSurfaceStencil(8) uses 8;
+ * 6, 7 and 8 grow without bound in heterogeneous media (measured), 2 equals mode 2's row. */
+int vi_set_afda_hermitian(vi_handle h, int points);
 /* n sources at interior cells (ix, iz); wavelets [n][nt] in the handle's precision,
  * already scaled by the source coefficient; sample k is used by step k */
 int vi_set_sources(vi_handle h, int n, const int* ix, const int* iz, const int* kind, int nt,

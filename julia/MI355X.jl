@@ -165,13 +165,28 @@ function set_cpml!(s::Sim, c)
     return nothing
 end
 
-"Free surface: false (none), true / :image (stress imaging, CPML.jl:366-393) or :afda
-(H-AFDA one-sided stencils, free-surface-kernels.jl)."
+"""Free surface: false (none), true / :image (stress imaging, CPML.jl:366-393), :afda
+(H-AFDA one-sided stencils, free-surface-kernels.jl) or :hermitian (H-AFDA with the
+Hermitian formula #3 for vz,z on the first row below the surface,
+free-surface-kernels.jl:154-214)."""
 function set_free_surface!(s::Sim, mode)
     m = mode === false || mode === nothing || mode === :none ? 0 :
-        mode === :afda ? 2 : 1
+        mode === :afda ? 2 : mode === :hermitian ? 3 : 1
     check(ccall((:vi_set_free_surface, LIB), Cint, (Ptr{Cvoid}, Cint), s.h, m))
     return nothing
+end
+
+"Half points of formula #3 (2..8; the library's default is 5)."
+set_afda_hermitian!(s::Sim, points::Integer) =
+    check(ccall((:vi_set_afda_hermitian, LIB), Cint, (Ptr{Cvoid}, Cint), s.h, points))
+
+"""The library's formula #3 weights (_stencil_hermitian, Stencil.jl:119-189) as
+(alpha, beta) with vz,z(h) = alpha vz,z(0) + sum beta[m] vz((m - 1/2) h)."""
+function afda_hermitian_weights(h::Real, points::Integer)
+    out = zeros(Float64, 9)
+    check(ccall((:vi_afda_hermitian_weights, LIB), Cint, (Cdouble, Cint, Ptr{Cdouble}),
+                h, points, out))
+    return out[1], out[2:1+points]
 end
 
 """

@@ -130,6 +130,53 @@ int vio_afda_weights(double h, const int* counts, double out[13][8]) {
   return 0;
 }
 
+/* H-AFDA formula #3 (free_surface = 3): the Hermitian one-sided derivative of Stencil.jl:52-105,
+ * 119-189 (_stencil_hermitian), d vz/dz at z = h (row 5) from the surface derivative f'(0)
+ * and vz at the half points (m+1/2) h, m < n:
+ *   f'(h) = alpha f'(0) + sum_m beta_m f((m+1/2) h),
+ * exact for polynomials of degree <= n. With the nodes x_m = 2m+1 in units of h/2, w(x) =
+ * prod (x - x_l) and l_m the Lagrange basis, every polynomial p of degree <= n is
+ * sum p(x_m) l_m + c w, so p'(0) fixes c and
+ *   alpha = w'(2) / w'(0),   beta_m = l_m'(2) - alpha l_m'(0)
+ * (w'(0) != 0: all its terms have the sign (-1)^(n-1)). In integers, with W(x) = w'(x),
+ * N_m(x) = sum_{q != m} prod_{l != m, q} (x - x_l) and D_m = prod_{l != m} (x_m - x_l):
+ *   alpha = W(2) / W(0),   beta_m = 2 (N_m(2) W(0) - W(2) N_m(0)) / (D_m W(0)) per unit h,
+ * numerators and denominators exact in int64 and in double for n <= 8 (below 2^53), each
+ * rounded once by its division; out[0] = alpha, out[1 + m] = beta_m / h (0 for m >= n). */
+int vio_afda_hermitian(double h, int n, double out[9]) {
+  if (n < 2 || n > 8 || !(h > 0)) return -1;
+  long long X[8];
+  for (int m = 0; m < n; ++m) X[m] = 2 * m + 1;
+  long long W0 = 0, W2 = 0, N0[8], N2[8];
+  for (int k = 0; k < n; ++k) {
+    long long t0 = 1, t2 = 1;
+    for (int l = 0; l < n; ++l)
+      if (l != k) { t0 *= -X[l]; t2 *= 2 - X[l]; }
+    W0 += t0;
+    W2 += t2;
+    N0[k] = 0;
+    N2[k] = 0;
+    for (int q = 0; q < n; ++q) {
+      if (q == k) continue;
+      long long u0 = 1, u2 = 1;
+      for (int l = 0; l < n; ++l)
+        if (l != k && l != q) { u0 *= -X[l]; u2 *= 2 - X[l]; }
+      N0[k] += u0;
+      N2[k] += u2;
+    }
+  }
+  for (int m = 0; m < 9; ++m) out[m] = 0.0;
+  out[0] = (double)W2 / (double)W0;
+  for (int k = 0; k < n; ++k) {
+    long long D = 1;
+    for (int l = 0; l < n; ++l)
+      if (l != k) D *= X[k] - X[l];
+    const long long num = 2 * (N2[k] * W0 - W2 * N0[k]), den = D * W0;
+    out[1 + k] = ((double)num / (double)den) / h;
+  }
+  return 0;
+}
+
 #define REAL float
 #define SUF _f32
 #define VIO_CPML_AXIS_REAL vio_cpml_axis_f32

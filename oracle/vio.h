@@ -49,11 +49,13 @@ typedef struct {
   int nrelax;          /* L: number of SLS mechanisms, 0 = elastic */
   double dx, dz, dt;
   int free_surface;    /* traction-free surface at row 4: 0 none, 1 stress images in rows
-                          0..3, 2 H-AFDA one-sided stencils (nothing above the surface) */
+                          0..3, 2 H-AFDA one-sided stencils (nothing above the surface),
+                          3 H-AFDA with the Hermitian formula #3 for d vz/dz on row 5 */
   int cpml_thickness;  /* 0: no CPML */
   double cpml_d0, cpml_alpha_max, cpml_kappa_max, cpml_power;
   int cpml_left, cpml_right, cpml_top, cpml_bottom;
   int afda_points[13]; /* H-AFDA point counts per table row (see vio.c); all 0 = default */
+  int afda_hermitian_points; /* free_surface = 3: half points of formula #3; 0 = default */
 } vio_desc;
 
 /* Exact staggered 8th-order weights (1225/1024, -245/3072, 49/5120, -5/7168) / h. */
@@ -64,6 +66,14 @@ void vio_fd_coeffs(double h, double out[4]);
  * Returns -1 for a count out of range. */
 int vio_afda_weights(double h, const int* counts, double out[13][8]);
 void vio_afda_default_points(int out[13]);
+
+/* H-AFDA formula #3 (free_surface = 3): out[0] = alpha (multiplies d vz/dz at the surface),
+ * out[1 + m] = beta_m / h (multiplies vz at (m+1/2) h, m < n); d vz/dz at z = h. Returns -1
+ * for n outside [2, 8]. */
+int vio_afda_hermitian(double h, int n, double out[9]);
+/* default half-point count of formula #3: the largest that stays bounded in the measured
+ * heterogeneous media (6, 7 and the reference's 8 grow without bound) */
+#define VIO_HERMITIAN_DEFAULT 5
 
 /* CPML a, b, kappa (and the profiles d, alpha; NULL to skip) along one axis of n cells
  * (CPML.jl:109-189) in double and in float, each computed in its own precision as the

@@ -18,13 +18,15 @@ _lib = None
 VX, VZ, SXX, SZZ, SXZ, MXX, MZZ, MXZ = range(8)
 PSI_VX_X, PSI_VZ_X, PSI_SXX_X, PSI_SXZ_X, PSI_VX_Z, PSI_VZ_Z, PSI_SXZ_Z, PSI_SZZ_Z = range(8, 16)
 FUSED, MULTIPASS = 0, 1
-FS_NONE, FS_IMAGE, FS_AFDA = 0, 1, 2
+FS_NONE, FS_IMAGE, FS_AFDA, FS_HERMITIAN = 0, 1, 2, 3
 
 
 def fs_mode(free_surface) -> int:
-    """False/None/"none" -> 0, True/"image" -> 1 (stress imaging), "afda" -> 2 (H-AFDA)."""
+    """False/None/"none" -> 0, True/"image" -> 1 (stress imaging), "afda" -> 2 (H-AFDA),
+    "hermitian" -> 3 (H-AFDA with formula #3 for d vz/dz on row 5)."""
     if isinstance(free_surface, str):
-        return {"none": FS_NONE, "image": FS_IMAGE, "afda": FS_AFDA}[free_surface]
+        return {"none": FS_NONE, "image": FS_IMAGE, "afda": FS_AFDA,
+                "hermitian": FS_HERMITIAN}[free_surface]
     return int(free_surface or 0)
 
 
@@ -38,6 +40,15 @@ def afda_weights(h, counts=None):
         cp = (ctypes.c_int * 13)(*[int(c) for c in counts])
     if lib().vio_afda_weights(float(h), cp, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
         raise ValueError(f"afda point counts out of range: {counts}")
+    return out
+
+
+def afda_hermitian(h, n=5):
+    """Formula #3 weights (vio.c): [alpha, beta_0 / h, ..., beta_7 / h] for d vz/dz at z = h
+    from d vz/dz(0) and vz at (m+1/2) h, m < n."""
+    out = np.zeros(9)
+    if lib().vio_afda_hermitian(float(h), int(n), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
+        raise ValueError(f"hermitian point count out of range: {n}")
     return out
 
 
@@ -57,6 +68,7 @@ class Desc(ctypes.Structure):
         ("cpml_left", ctypes.c_int), ("cpml_right", ctypes.c_int),
         ("cpml_top", ctypes.c_int), ("cpml_bottom", ctypes.c_int),
         ("afda_points", ctypes.c_int * 13),
+        ("afda_hermitian_points", ctypes.c_int),
     ]
 
 
@@ -74,6 +86,9 @@ def lib():
         vp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)
         _lib.vio_afda_weights.restype = ctypes.c_int
         _lib.vio_afda_weights.argtypes = [ctypes.c_double, ip, ctypes.POINTER(ctypes.c_double)]
+        _lib.vio_afda_hermitian.restype = ctypes.c_int
+        _lib.vio_afda_hermitian.argtypes = [ctypes.c_double, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_double)]
         _lib.vio_afda_default_points.restype = None
         _lib.vio_afda_default_points.argtypes = [ip]
         for suf, ct in (("_f32", ctypes.c_float), ("_f64", ctypes.c_double)):
@@ -148,7 +163,7 @@ class Oracle:
     """
 
     def __init__(self, nx, nz, dx, dz, dt, model, dtype=np.float64, nrelax=0, taun=None,
-                 free_surface=False, cpml=None, afda_points=None):
+                 free_surface=False, cpml=None, afda_points=None, hermitian_points=None):
         self.dtype = np.dtype(dtype)
         self.suf = "_f32" if self.dtype == np.float32 else "_f64"
         self.ct = ctypes.c_float if self.suf == "_f32" else ctypes.c_double
@@ -160,6 +175,8 @@ class Oracle:
         if afda_points is not None:
             for k, c in enumerate(afda_points):
                 d.afda_points[k] = int(c)
+        if hermitian_points is not None:
+            d.afda_hermitian_points = int(hermitian_points)
         if cpml:
             d.cpml_thickness = int(cpml["thickness"])
             d.cpml_d0 = float(cpml["d0"])

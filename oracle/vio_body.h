@@ -76,7 +76,9 @@ typedef struct {
   REAL *c11, *c33; /* VTI only (NULL: isotropic, pi = lam + 2 mu) */
   REAL taun[VIO_MAXL], a1[VIO_MAXL], a2[VIO_MAXL];
   REAL Dx[4], Dz[4];
-  REAL afda[13][8]; /* H-AFDA one-sided weights (free_surface = 2) */
+  REAL afda[13][8]; /* H-AFDA one-sided weights (free_surface = 2, 3) */
+  REAL herm[9];     /* free_surface = 3: alpha, beta_m / h of formula #3 (d vz/dz on row 5) */
+  REAL* es;         /* free_surface = 3: the surface's d vz/dz per column (fused path) */
   REAL *ax, *bx, *kx, *az, *bz, *kz;
   int xlo, xhi, zlo, zhi;
   REAL* psi[8];
@@ -98,10 +100,15 @@ static REAL* NAME(vio_alloc)(size_t n) {
 void* NAME(vio_create)(const vio_desc* d, const REAL* rho, const REAL* lam, const REAL* mu,
                        const REAL* tau, const REAL* taun) {
   if (!d || d->nx < 16 || d->nz < 16 || d->nrelax < 0 || d->nrelax > VIO_MAXL) return NULL;
-  if (d->free_surface == 2) {
-    double w[13][8];
+  if (d->free_surface >= 2) {
+    double w[13][8], hw[9];
     if (vio_afda_weights(d->dz, d->afda_points[0] ? d->afda_points : NULL, w)) return NULL;
+    if (d->free_surface == 3 &&
+        vio_afda_hermitian(d->dz, d->afda_hermitian_points ? d->afda_hermitian_points : VIO_HERMITIAN_DEFAULT, hw))
+      return NULL;
   }
+  /* the surface row must not lie in a top CPML strip */
+  if (d->free_surface && d->cpml_thickness > 0 && d->cpml_top) return NULL;
   SIM* s = (SIM*)calloc(1, sizeof(SIM));
   s->d = *d;
   s->L = d->nrelax;
@@ -132,11 +139,17 @@ void* NAME(vio_create)(const vio_desc* d, const REAL* rho, const REAL* lam, cons
   for (int l = 0; l < 4; ++l) s->Dx[l] = (REAL)c[l];
   vio_fd_coeffs(d->dz, c);
   for (int l = 0; l < 4; ++l) s->Dz[l] = (REAL)c[l];
-  if (d->free_surface == 2) {
+  if (d->free_surface >= 2) {
     double w[13][8];
     vio_afda_weights(d->dz, d->afda_points[0] ? d->afda_points : NULL, w); /* validated above */
     for (int r = 0; r < 13; ++r)
       for (int m = 0; m < 8; ++m) s->afda[r][m] = (REAL)w[r][m];
+  }
+  if (d->free_surface == 3) {
+    double hw[9];
+    vio_afda_hermitian(d->dz, d->afda_hermitian_points ? d->afda_hermitian_points : VIO_HERMITIAN_DEFAULT, hw);
+    for (int m = 0; m < 9; ++m) s->herm[m] = (REAL)hw[m];
+    s->es = NAME(vio_alloc)(nx);
   }
 
   s->ax = NAME(vio_alloc)(nx); s->bx = NAME(vio_alloc)(nx); s->kx = NAME(vio_alloc)(nx);
@@ -168,6 +181,7 @@ void NAME(vio_destroy)(void* h) {
   free(s->six); free(s->siz); free(s->skind); free(s->sw); free(s->srcmark);
   free(s->rix); free(s->riz); free(s->rcomp);
   for (int q = 0; q < 7; ++q) free(s->scr[q]);
+  free(s->es);
   free(s);
 }
 
@@ -305,6 +319,12 @@ static inline REAL NAME(cpml)(REAL* psi, REAL a, REAL b, REAL k, REAL d) {
   return d / k + p;
 }
 
+/* psi' = b psi + a d without storing it (the rates and the surface value of formula #3) */
+static inline REAL NAME(cpml_rate)(const REAL* psi, REAL a, REAL b, REAL k, REAL d) {
+  const REAL p = b * (*psi) + a * d;
+  return d / k + p;
+}
+
 static inline REAL NAME(src_sample)(const SIM* s, int q, long n) {
   return n < s->snt ? s->sw[(size_t)q * s->snt + n] : (REAL)0;
 }
@@ -316,10 +336,49 @@ static inline REAL NAME(afda_dz)(const REAL* u, int i, int nx, const REAL* w) {
   return v;
 }
 
+/* d vz/dz at the surface (row 4, column i) that keeps szz = 0 there: the free-surface row's
+ * substitute ezz (dszz = 0), from the current state and without changing psi or M. With
+ * L = 0 and isotropy it is -lam/(lam + 2 mu) d vx/dx, the w,z(0) of
+ * free-surface-kernels.jl:160-172; the memory terms keep dszz = 0 when L > 0. */
+static inline REAL NAME(surface_ezz)(const SIM* s, int i) {
+  const int nx = s->d.nx, L = s->L;
+  const size_t nc = s->ncell, c = (size_t)4 * nx + i;
+  const REAL two = 2;
+  REAL exx = OP_M(s->vx, c, 1, s->Dx);
+  const int xk = NAME(xk)(s, i);
+  if (xk >= 0)
+    exx = NAME(cpml_rate)(&s->psi[0][(size_t)4 * 2 * s->T + xk], s->ax[i], s->bx[i], s->kx[i],
+                          exx);
+  const REAL lr = s->lam[c];
+  const REAL pz = s->c11 ? s->c33[c] : lr + two * s->mu[c];
+  if (L > 0) {
+    const REAL t = s->tau[c];
+    const REAL pz1 = pz * t, pz0 = pz1 + pz;
+    const REAL l1 = lr * t, l0 = l1 + lr;
+    const REAL* Mxx = s->M;
+    const REAL* Mzz = s->M + (size_t)L * nc;
+    REAL SMxx = Mxx[c], SMzz = Mzz[c];
+    for (int k = 1; k < L; ++k) {
+      SMxx = SMxx + Mxx[(size_t)k * nc + c];
+      SMzz = SMzz + Mzz[(size_t)k * nc + c];
+    }
+    return -(((l0 * exx) + l1 * SMxx) + pz1 * SMzz) / pz0;
+  }
+  return -(lr * exx) / pz;
+}
+
+/* formula #3 on row 5: alpha vz,z(0) + sum_m beta_m vz(row 4 + m), in that order */
+static inline REAL NAME(herm_dz)(const SIM* s, int i, REAL es) {
+  const int nx = s->d.nx;
+  REAL v = s->herm[0] * es;
+  for (int m = 0; m < 8; ++m) v = v + s->herm[1 + m] * s->vz[(size_t)(4 + m) * nx + i];
+  return v;
+}
+
 /* ---- fused path ------------------------------------------------------------------- */
 static void NAME(vel_fused)(SIM* s, long n) {
   const int nx = s->d.nx, nz = s->d.nz, T = s->T;
-  const int afda = s->d.free_surface == 2;
+  const int afda = s->d.free_surface >= 2;
   const REAL dt = (REAL)s->d.dt, two = 2;
   const REAL *Dx = s->Dx, *Dz = s->Dz;
 #pragma omp parallel for schedule(static)
@@ -441,15 +500,19 @@ static void NAME(stress_fused)(SIM* s, long n) {
   const int nx = s->d.nx, nz = s->d.nz, T = s->T;
   const REAL *Dx = s->Dx, *Dz = s->Dz;
   const int fs = s->d.free_surface;
-  const int afda = fs == 2;
+  const int afda = fs >= 2, herm = fs == 3;
+  /* formula #3 needs the surface value before row 4 updates psi and M */
+  if (herm)
+    for (int i = 4; i < nx - 4; ++i) s->es[i] = NAME(surface_ezz)(s, i);
 #pragma omp parallel for schedule(static)
   for (int j = 4; j < nz - 4; ++j) {
     const int zk = NAME(zk)(s, j);
     for (int i = 4; i < nx - 4; ++i) {
       const size_t c = (size_t)j * nx + i;
       REAL exx = OP_M(s->vx, c, 1, Dx);
-      REAL ezz = afda && j >= 5 && j < 8 ? NAME(afda_dz)(s->vz, i, nx, s->afda[7 + j - 5])
-                                         : OP_M(s->vz, c, (size_t)nx, Dz);
+      REAL ezz = herm && j == 5                 ? NAME(herm_dz)(s, i, s->es[i])
+                 : afda && j >= 5 && j < 8 ? NAME(afda_dz)(s->vz, i, nx, s->afda[7 + j - 5])
+                                           : OP_M(s->vz, c, (size_t)nx, Dz);
       REAL ezx = afda && j < 7 ? NAME(afda_dz)(s->vx, i, nx, s->afda[10 + j - 4])
                                : OP_P(s->vx, c, (size_t)nx, Dz);
       REAL exz = OP_P(s->vz, c, 1, Dx);
@@ -495,7 +558,7 @@ static void NAME(deriv_pass)(SIM* s, REAL* out, const REAL* u, int plus, int alo
 /* H-AFDA rows: overwrite rows [j0, j1) of a d/dz scratch array with the one-sided stencils
  * whose table rows start at r0 */
 static void NAME(afda_pass)(SIM* s, REAL* out, const REAL* u, int r0, int j0, int j1) {
-  if (s->d.free_surface != 2) return;
+  if (s->d.free_surface < 2) return;
   const int nx = s->d.nx;
   for (int j = j0; j < j1; ++j)
     for (int i = 4; i < nx - 4; ++i)
@@ -580,6 +643,9 @@ static void NAME(stress_multipass)(SIM* s, long n) {
   NAME(deriv_pass)(s, e4, s->vz, 1, 0); /* dxvz */
   NAME(afda_pass)(s, e2, s->vz, 7, 5, 8);
   NAME(afda_pass)(s, e3, s->vx, 10, 4, 7);
+  if (fs == 3) /* formula #3 on row 5, before psi and M change */
+    for (int i = 4; i < nx - 4; ++i)
+      e2[(size_t)5 * nx + i] = NAME(herm_dz)(s, i, NAME(surface_ezz)(s, i));
   NAME(cpml_pass)(s, e1, s->psi[0], 0);
   NAME(cpml_pass)(s, e4, s->psi[1], 0);
   NAME(cpml_pass)(s, e3, s->psi[4], 1);
@@ -698,17 +764,12 @@ int NAME(vio_run)(void* h, int nsteps, int mode, int nthreads, REAL* traces) {
  * In CPML strips each derivative uses psi' = b psi + a d (psi itself is not changed), as the
  * step does. Cells outside the interior get 0. So the step's update is x + dt * rate in REAL,
  * operation for operation. */
-static inline REAL NAME(cpml_rate)(const REAL* psi, REAL a, REAL b, REAL k, REAL d) {
-  const REAL p = b * (*psi) + a * d;
-  return d / k + p;
-}
-
 int NAME(vio_rates)(void* h, const REAL* src_values, REAL* dv, REAL* ds, REAL* dm) {
   SIM* s = (SIM*)h;
   if (!s || !dv || !ds) return -1;
   const int nx = s->d.nx, nz = s->d.nz, T = s->T, L = s->L;
   const size_t nc = s->ncell;
-  const int fs = s->d.free_surface, afda = fs == 2;
+  const int fs = s->d.free_surface, afda = fs >= 2, herm = fs == 3;
   const REAL two = 2, quarter = (REAL)0.25;
   const REAL *Dx = s->Dx, *Dz = s->Dz;
   memset(dv, 0, 2 * nc * sizeof(REAL));
@@ -753,8 +814,9 @@ int NAME(vio_rates)(void* h, const REAL* src_values, REAL* dv, REAL* ds, REAL* d
       dvz[c] = (two / (s->rho[c] + s->rho[c + nx])) * fz;
       /* stress rates */
       REAL exx = OP_M(s->vx, c, 1, Dx);
-      REAL ezz = afda && j >= 5 && j < 8 ? NAME(afda_dz)(s->vz, i, nx, s->afda[7 + j - 5])
-                                         : OP_M(s->vz, c, (size_t)nx, Dz);
+      REAL ezz = herm && j == 5                 ? NAME(herm_dz)(s, i, NAME(surface_ezz)(s, i))
+                 : afda && j >= 5 && j < 8 ? NAME(afda_dz)(s->vz, i, nx, s->afda[7 + j - 5])
+                                           : OP_M(s->vz, c, (size_t)nx, Dz);
       REAL ezx = afda && j < 7 ? NAME(afda_dz)(s->vx, i, nx, s->afda[10 + j - 4])
                                : OP_P(s->vx, c, (size_t)nx, Dz);
       REAL exz = OP_P(s->vz, c, 1, Dx);

@@ -26,7 +26,8 @@ def make_oracle(cfg, dtype):
     o = vio.Oracle(cfg["nx"], cfg["nz"], cfg["dx"], cfg["dz"], cfg["dt"], cfg["model"],
                    dtype=dtype, nrelax=cfg["nrelax"], taun=cfg.get("taun"),
                    free_surface=cfg["free_surface"], cpml=cfg["cpml"],
-                   afda_points=cfg.get("afda_points"))
+                   afda_points=cfg.get("afda_points"),
+                   hermitian_points=cfg.get("hermitian_points"))
     if len(cfg["src_ix"]):
         o.set_sources(cfg["src_ix"], cfg["src_iz"], cfg["src_kind"], cfg["wavelets"])
     if len(cfg["rec_ix"]):

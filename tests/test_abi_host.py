@@ -32,6 +32,41 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
 
 
+def _julia_ccalls():
+    """(name, argument types) of every ccall in julia/MI355X.jl."""
+    text = open(os.path.join(ROOT, "julia", "MI355X.jl")).read()
+    calls = []
+    for m in re.finditer(r"ccall\(\(:(vi_[a-z0-9_]+),\s*LIB\),\s*[A-Za-z{}]+,\s*\(", text):
+        depth, j, cur, args = 1, m.end(), "", []
+        while depth:
+            c = text[j]
+            depth += c in "({"
+            depth -= c in ")}"
+            if depth == 1 and c == ",":
+                args.append(cur.strip())
+                cur = ""
+            elif depth:
+                cur += c
+            j += 1
+        if cur.strip():
+            args.append(cur.strip())
+        calls.append((m.group(1), args))
+    return calls
+
+
+def test_julia_shim_binds_declared_symbols():
+    """Julia is not in this image, so the shim's ccalls are checked statically: each names a
+    function the header declares and the library exports, with its number of arguments."""
+    from viscoelasticinversion_amd import _lib
+    calls = _julia_ccalls()
+    assert len(calls) >= 20
+    declared = set(_declared_functions())
+    lib = _lib.lib()
+    for name, args in calls:
+        assert name in declared and hasattr(lib, name), name
+        assert len(args) == len(_lib.SIGNATURES[name][1]), (name, args)
+
+
 def test_fd_coeffs_match_oracle(oracle_mod):
     from viscoelasticinversion_amd import _lib
     for h in (1.0, 5.0, 2.5, 0.1):
@@ -79,6 +114,24 @@ def test_afda_weights_reject_bad_counts():
     with pytest.raises(ValueError):
         stepper.free_surface_mode("lamb")
     assert [stepper.free_surface_mode(m) for m in (False, True, "none", "image", "afda")] == [0, 1, 0, 1, 2]
+
+
+@pytest.mark.parametrize("points", range(2, 9))
+def test_hermitian_weights_match_oracle_bitwise(oracle_mod, points):
+    """The library's formula #3 weights (host code, no device) equal the oracle's, which are
+    pinned against the exact solution of the reference's augmented Vandermonde system."""
+    from viscoelasticinversion_amd import stepper
+    for h in (1.0, 2.5, 5.0, 7.3):
+        assert np.array_equal(stepper.afda_hermitian_weights(h, points),
+                              oracle_mod.afda_hermitian(h, points))
+
+
+def test_hermitian_weights_reject_bad_counts():
+    from viscoelasticinversion_amd import _lib, stepper
+    for bad in (0, 1, 9):
+        with pytest.raises(_lib.ViError):
+            stepper.afda_hermitian_weights(1.0, bad)
+    assert stepper.free_surface_mode("hermitian") == 3
 
 
 def test_cpml_auto_d0_matches_config():

@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _cfg(case, seed):
-    fs = {"image": True, "afda": "afda", "none": False}[case.split("_")[-2]]
+    fs = {"image": True, "afda": "afda", "herm": "hermitian", "none": False}[case.split("_")[-2]]
     L = int(case[-1])
     cfg = small_cfg(nx=83, nz=57, nrelax=L, cpml=True, fs=bool(fs), seed=seed, nt=12,
                     vti=case.startswith("vti"))
@@ -44,7 +44,8 @@ def _random_state(cfg, o, seed):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("case", ["image_L3", "afda_L3", "none_L0", "vti_image_L0", "vti_afda_L3"])
+@pytest.mark.parametrize("case", ["image_L3", "afda_L3", "herm_L3", "none_L0",
+                                  "vti_image_L0", "vti_afda_L3", "vti_herm_L0"])
 def test_rates_equal_oracle(case, dtype):
     from viscoelasticinversion_amd.configs import build_simulation
     prec = "float32" if dtype == np.float32 else "float64"
@@ -69,7 +70,7 @@ def test_rates_equal_oracle(case, dtype):
     o.close()
 
 
-@pytest.mark.parametrize("case", ["image_L3", "afda_L3"])
+@pytest.mark.parametrize("case", ["image_L3", "afda_L3", "herm_L3"])
 def test_step_is_state_plus_dt_rate(case):
     from viscoelasticinversion_amd.configs import build_simulation
     cfg = _cfg(case, seed=11)

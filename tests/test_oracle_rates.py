@@ -2,7 +2,7 @@
 ViscoelasticProblem.jl:62-174) is pinned by the step it belongs to: from any state, the
 velocity half step equals v + dt * dv, and the stress half step from the new velocities equals
 s + dt * ds and M + dt * dM, bit for bit, in FP32 and FP64, with every feature (CPML with
-kappa and alpha, stress imaging or H-AFDA surface, VTI, L = 0 and 3, force and explosive
+kappa and alpha, stress imaging, H-AFDA or formula #3 surface, VTI, L = 0 and 3, force and explosive
 sources, two sources on one cell), on both oracle code paths (fused and multi-pass)."""
 import numpy as np
 import pytest
@@ -25,9 +25,10 @@ def _state(o, L, T):
 
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("case", ["image_L3", "afda_L3", "none_L0", "vti_image_L0", "vti_afda_L3"])
+@pytest.mark.parametrize("case", ["image_L3", "afda_L3", "herm_L3", "none_L0",
+                                  "vti_image_L0", "vti_afda_L3", "vti_herm_L0"])
 def test_rates_reproduce_the_step(case, dtype, mode):
-    fs = {"image": True, "afda": "afda", "none": False}[case.split("_")[-2]]
+    fs = {"image": True, "afda": "afda", "herm": "hermitian", "none": False}[case.split("_")[-2]]
     L = int(case[-1])
     cfg = small_cfg(nx=70, nz=48, nrelax=L, cpml=True, fs=bool(fs), seed=len(case) + L,
                     nt=12, vti=case.startswith("vti"))

@@ -53,6 +53,7 @@ SIGNATURES = {
     "vi_fd_coeffs": (None, [_D, _DP]),
     "vi_afda_weights": (_I, [_D, _IP, _DP]),
     "vi_afda_default_points": (_I, [_IP]),
+    "vi_afda_hermitian_weights": (_I, [_D, _I, _DP]),
     "vi_cpml_auto_d0": (_D, [_I, _D, _D, _D, _D]),
     "vi_cpml_profile": (_I, [_I, ctypes.POINTER(CpmlDesc), _D, _I, _I, _DP, _DP, _DP]),
     "vi_device_count": (_I, []),
@@ -88,6 +89,7 @@ SIGNATURES = {
     "vi_set_cpml": (_I, [_H, ctypes.POINTER(CpmlDesc)]),
     "vi_set_free_surface": (_I, [_H, _I]),
     "vi_set_afda_points": (_I, [_H, _IP]),
+    "vi_set_afda_hermitian": (_I, [_H, _I]),
     "vi_set_sources": (_I, [_H, _I, _IP, _IP, _IP, _I, _V]),
     "vi_set_receivers": (_I, [_H, _I, _IP, _IP, _IP, _I]),
     "vi_set_state": (_I, [_H, _I, _V]),

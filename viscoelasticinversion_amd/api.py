@@ -22,7 +22,7 @@ from typing import Optional
 
 import numpy as np
 
-from .stepper import Simulation
+from .stepper import Simulation, free_surface_mode
 
 
 # ---------------------------------------------------------------------------- source
@@ -266,7 +266,9 @@ class Parameters:
     device: str = "mi355x"
     model: Optional[object] = None  # IsotropicModel or VTIModel
     cpml: Optional[CPMLConfig] = None
-    free_surface: object = False  # False, True / "image" (stress imaging) or "afda" (H-AFDA)
+    # False, True / "image" (stress imaging), "afda" (H-AFDA) or "hermitian" (H-AFDA with
+    # formula #3 on the first row below the surface)
+    free_surface: object = False
     precision: str = "float32"
     qmodel: Optional[QModel] = None
     receivers: Optional[tuple] = None
@@ -465,7 +467,7 @@ def velocity_eq(dv, s, v, p, t, M=None):
     velocity rates (vx, vz) at time t, force sources added before the buoyancy. With the
     stress-imaging free surface the images are first written into ``s``, as the reference's
     apply_free_surface!(s) does inside velocity_eq!."""
-    if p.free_surface and p.free_surface != "afda":
+    if free_surface_mode(p.free_surface) == 1:
         _apply_images(s)
     r, _, _ = _rates(s, v, p, t, M)
     dv[...] = r

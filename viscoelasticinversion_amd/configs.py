@@ -111,6 +111,8 @@ def build_simulation(cfg, precision="float32"):
     sim.set_free_surface(cfg["free_surface"])
     if cfg.get("afda_points") is not None:
         sim.set_afda_points(cfg["afda_points"])
+    if cfg.get("hermitian_points") is not None:
+        sim.set_afda_hermitian(cfg["hermitian_points"])
     sim.set_sources(cfg["src_ix"], cfg["src_iz"], cfg["src_kind"], cfg["wavelets"])
     sim.set_receivers(cfg["rec_ix"], cfg["rec_iz"], cfg["rec_comp"], cfg["nt"])
     return sim

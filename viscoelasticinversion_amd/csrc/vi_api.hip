@@ -227,9 +227,10 @@ struct vi_sim_s {
   bool relax_set = false;
   bool cpml_on = false;
   vi_cpml_desc cpml{};
-  int fs = 0;                 // 0 none, 1 stress imaging, 2 H-AFDA
+  int fs = 0;                 // 0 none, 1 stress imaging, 2 H-AFDA, 3 H-AFDA + formula #3
   int afda_pts[13] = {3, 4, 6, 8, 3, 4, 6, 2, 4, 6, 3, 4, 6};  // points per one-sided stencil
-  DevBuf afda;                // H-AFDA weights / dz, [13][8] in the handle's precision
+  int herm_pts = 5;           // half points of formula #3 (fs = 3)
+  DevBuf afda;                // H-AFDA tables (see upload_afda) in the handle's precision
   int nsrc = 0, snt = 0;
   int nrec = 0, rec_nt = 0;
   std::vector<int> src_ix, src_iz, src_kind, rec_ix, rec_iz, rec_comp;
@@ -465,7 +466,7 @@ void build_desc(const vi_sim_s* h, int tile_base, vi::ShotDesc<R>* d) {
     for (int q = 0; q < 8; ++q) d->psi[q] = (R*)h->psi[q].p;
   }
   d->fs = h->fs;
-  d->afda = h->fs == 2 ? (const R*)h->afda.p : nullptr;
+  d->afda = h->fs >= 2 ? (const R*)h->afda.p : nullptr;
   const int* st = (const int*)h->src_tab.p;
   d->src_off = st;
   d->src_cell = st + (h->ntiles + 1);
@@ -632,7 +633,7 @@ int step_single(vi_sim_s* h, int nsteps) {
   VI_TRY(refresh_own_desc(h));
   VI_TRY(launch_steps(h->stream, h->esz, h->rz, h->L, h->desc_dev.p,
                       (const int*)h->tile_shot_dev.p, h->ntiles, 1, nsteps, &h->prof,
-                      h->fs == 2));
+                      h->fs >= 2));
   h->step += nsteps;
   h->own_pending = true;
   return 0;
@@ -742,7 +743,7 @@ int batch_build(vi_batch_s* b) {
     }
     g.ntiles = base;
     g.afda = false;
-    for (auto* m : g.members) g.afda = g.afda || m->fs == 2;
+    for (auto* m : g.members) g.afda = g.afda || m->fs >= 2;
     VI_TRY(g.desc_dev.reserve(g.desc_host.size()));
     VI_TRY(g.tile_shot_dev.reserve(ts.size() * sizeof(int)));
     VI_HIP(hipMemcpyAsync(g.desc_dev.p, g.desc_host.data(), g.desc_host.size(),
@@ -1411,6 +1412,53 @@ int vi_afda_weights(double h, const int* counts, double out[13][8]) {
   return 0;
 }
 
+// H-AFDA formula #3, the reference's Hermitian stencil (Stencil.jl:52-105 D3/D3_deriv from
+// _stencil_hermitian, Stencil.jl:119-189): d vz/dz at z = h from d vz/dz(0) and vz at the half
+// points (m+1/2) h, m < n, exact for polynomials of degree <= n. With the nodes x_m = 2m+1
+// in units of h/2 and w(x) = prod (x - x_l), any such polynomial is sum p(x_m) l_m + c w, and
+// p'(0) fixes c:
+//   alpha = w'(2) / w'(0),   beta_m = l_m'(2) - alpha l_m'(0).
+// In integers, W = w', N_m(x) = sum_{q != m} prod_{l != m, q} (x - x_l), D_m = prod_{l != m}
+// (x_m - x_l): alpha = W(2)/W(0), beta_m = 2 (N_m(2) W(0) - W(2) N_m(0)) / (D_m W(0)) per unit
+// h. For n <= 8 every integer is below 2^53, so each weight is rounded once, by its division.
+int vi_afda_hermitian_weights(double h, int n, double out[9]) {
+  if (!out || !(h > 0) || n < 2 || n > 8)
+    return fail(VI_EINVAL, "vi_afda_hermitian_weights: need h > 0, 2 <= points <= 8, an output");
+  long long X[8], N0[8], N2[8], W0 = 0, W2 = 0;
+  for (int m = 0; m < n; ++m) X[m] = 2 * m + 1;
+  for (int m = 0; m < n; ++m) {
+    long long p0 = 1, p2 = 1;
+    N0[m] = N2[m] = 0;
+    for (int l = 0; l < n; ++l)
+      if (l != m) {
+        p0 *= -X[l];
+        p2 *= 2 - X[l];
+      }
+    W0 += p0;
+    W2 += p2;
+    for (int q = 0; q < n; ++q) {
+      if (q == m) continue;
+      long long t0 = 1, t2 = 1;
+      for (int l = 0; l < n; ++l)
+        if (l != m && l != q) {
+          t0 *= -X[l];
+          t2 *= 2 - X[l];
+        }
+      N0[m] += t0;
+      N2[m] += t2;
+    }
+  }
+  for (int m = 0; m < 9; ++m) out[m] = 0.0;
+  out[0] = (double)W2 / (double)W0;
+  for (int m = 0; m < n; ++m) {
+    long long D = 1;
+    for (int l = 0; l < n; ++l)
+      if (l != m) D *= X[m] - X[l];
+    out[1 + m] = ((double)(2 * (N2[m] * W0 - W2 * N0[m])) / (double)(D * W0)) / h;
+  }
+  return 0;
+}
+
 double vi_cpml_auto_d0(int thickness, double vmax, double dx, double R, double power) {
   return -(power + 1.0) * vmax * std::log(R) / (2.0 * thickness * dx);  // CPML.jl:60
 }
@@ -1649,10 +1697,11 @@ int vi_set_cpml(vi_handle h, const vi_cpml_desc* c) {
 // The H-AFDA table of a handle: the weights [13][8], then for the adjoint gathers the
 // difference G_a - G_c between each one-sided row j and the centred stencil it replaces,
 // over the target rows g = 4..11 (G_c of D- at j: D[g-j] for 0 <= g-j <= 3, -D[j-g-1] for
-// 0 <= j-g-1 <= 3; of D+: D[g-j-1], -D[j-g]).
+// 0 <= j-g-1 <= 3; of D+: D[g-j-1], -D[j-g]), then formula #3: beta_m / h (8), alpha.
 static int upload_afda(vi_sim_s* h) {
-  double w[13][8], D[4];
+  double w[13][8], D[4], hw[9];
   VI_TRY(vi_afda_weights(h->g.dz, h->afda_pts, w));
+  VI_TRY(vi_afda_hermitian_weights(h->g.dz, h->herm_pts, hw));
   vi_fd_coeffs(h->g.dz, D);
   auto gc = [&](bool minus, int j, int g) -> double {
     const int a = minus ? g - j : g - j - 1, b = minus ? j - g - 1 : j - g;
@@ -1668,17 +1717,18 @@ static int upload_afda(vi_sim_s* h) {
     const bool minus = r < 4 || (r >= 7 && r < 10);
     for (int m = 0; m < 8; ++m) c[r][m] = w[r][m] - gc(minus, j, 4 + m);
   }
-  std::vector<char> t(2 * 13 * 8 * h->esz);
+  std::vector<char> t(224 * h->esz);  // 208 + 8 + 1, rounded up
+  auto put = [&](int at, double v) {
+    if (h->esz == 4) ((float*)t.data())[at] = (float)v;
+    else ((double*)t.data())[at] = v;
+  };
   for (int r = 0; r < 13; ++r)
     for (int m = 0; m < 8; ++m) {
-      if (h->esz == 4) {
-        ((float*)t.data())[r * 8 + m] = (float)w[r][m];
-        ((float*)t.data())[104 + r * 8 + m] = (float)c[r][m];
-      } else {
-        ((double*)t.data())[r * 8 + m] = w[r][m];
-        ((double*)t.data())[104 + r * 8 + m] = c[r][m];
-      }
+      put(r * 8 + m, w[r][m]);
+      put(104 + r * 8 + m, c[r][m]);
     }
+  for (int m = 0; m < 8; ++m) put(208 + m, hw[1 + m]);
+  put(216, hw[0]);
   VI_TRY(wait_external(h));
   VI_HIP(hipStreamSynchronize(h->stream));
   VI_TRY(h->afda.alloc(t.size(), h->stream));
@@ -1689,12 +1739,13 @@ static int upload_afda(vi_sim_s* h) {
 
 int vi_set_free_surface(vi_handle h, int mode) {
   VI_TRY(check_handle(h));
-  if (mode < 0 || mode > 2)
-    return fail(VI_EINVAL, "vi_set_free_surface: mode must be 0 (none), 1 (imaging) or 2 (H-AFDA)");
+  if (mode < 0 || mode > 3)
+    return fail(VI_EINVAL, "vi_set_free_surface: mode must be 0 (none), 1 (imaging), 2 (H-AFDA) "
+                           "or 3 (H-AFDA with formula #3)");
   if (mode && h->cpml_on && h->cpml.top)
     return fail(VI_EINVAL, "vi_set_free_surface: conflicts with a top CPML strip");
   if (mode && h->g.nz < 17) return fail(VI_EINVAL, "vi_set_free_surface: nz must be >= 17");
-  if (mode == 2) VI_TRY(upload_afda(h));
+  if (mode >= 2) VI_TRY(upload_afda(h));
   h->fs = mode;
   touch(h);
   return 0;
@@ -1707,7 +1758,17 @@ int vi_set_afda_points(vi_handle h, const int* counts) {
   double w[13][8];
   VI_TRY(vi_afda_weights(1.0, pts, w));  // range check
   std::memcpy(h->afda_pts, pts, sizeof pts);
-  if (h->fs == 2) VI_TRY(upload_afda(h));
+  if (h->fs >= 2) VI_TRY(upload_afda(h));
+  touch(h);
+  return 0;
+}
+
+int vi_set_afda_hermitian(vi_handle h, int points) {
+  VI_TRY(check_handle(h));
+  double w[9];
+  VI_TRY(vi_afda_hermitian_weights(1.0, points, w));  // range check
+  h->herm_pts = points;
+  if (h->fs >= 2) VI_TRY(upload_afda(h));
   touch(h);
   return 0;
 }
@@ -2113,6 +2174,9 @@ static int gradient_checks(vi_handle h, int nt, const void* dobs, int checkpoint
   VI_TRY(check_ready(h));
   if (h->nrec < 1 || h->rec_nt < nt)
     return fail(VI_ESTATE, std::string(who) + ": receivers with capacity >= nt are required");
+  if (h->fs == 3)
+    return fail(VI_EINVAL, std::string(who) + ": free-surface mode 3 (formula #3) has no "
+                           "adjoint; use mode 2 (H-AFDA) or 1 (imaging)");
   VI_TRY(wait_external(h));
   VI_HIP(hipStreamSynchronize(h->stream));
   return 0;

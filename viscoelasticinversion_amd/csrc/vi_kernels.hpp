@@ -27,7 +27,7 @@
 // constitutive + memory src/visco-kernels.jl:22-85; VTI stiffnesses
 // src/sciml_solver/Models.jl:27-33 in the form of ViscoelasticProblem.jl:21-35; CPML
 // src/sciml_solver/CPML.jl:291-328; imaging src/sciml_solver/CPML.jl:372-380; H-AFDA free
-// surface src/free-surface-kernels.jl:16-214 with src/common/Stencil.jl:52-105; source
+// surface src/free-surface-kernels.jl:16-214 with src/common/Stencil.jl:52-189; source
 // src/sciml_solver/Source.jl:35-40.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -56,8 +56,11 @@ struct ShotDesc {
   const R *c11, *c33;
   int vti;
   int T, xlo, xhi, zlo, zhi;
-  int fs;          // free surface on row 4: 0 none, 1 stress imaging, 2 H-AFDA
-  const R* afda;   // fs = 2: one-sided d/dz weights [13][8] over rows 4..11 (A0..3 B0..2 C1..3 D0..2)
+  int fs;          // free surface on row 4: 0 none, 1 stress imaging, 2 H-AFDA, 3 H-AFDA with
+                   // the Hermitian formula #3 for d vz/dz on row 5
+  // fs >= 2: one-sided d/dz weights [13][8] over rows 4..11 (A0..3 B0..2 C1..3 D0..2), the
+  // adjoint's difference table [13][8], then formula #3: beta_m / h (m < 8), alpha
+  const R* afda;
   const R *ax, *bx, *kx, *az, *bz, *kz;
   R* psi[8];  // 0..3: [nz][2T] (vx_x, vz_x, sxx_x, sxz_x); 4..7: [2T][ldx] (vx_z, vz_z, sxz_z, szz_z)
   // sources / receivers grouped by tile (CSR over this shot's tiles)
@@ -127,6 +130,18 @@ __device__ __forceinline__ R afda_dz(const R* q, int st, const R* tab, int tr) {
   return v;
 }
 
+// H-AFDA formula #3 on row 5 (fs = 3): d vz/dz at z = h = alpha es + sum_m beta_m vz(row 4+m),
+// es = d vz/dz at the surface, accumulated in the oracle's order (alpha es first). The
+// weights sit at tab[208..215] (beta / h) and tab[216] (alpha).
+template <typename R>
+__device__ __forceinline__ R herm_dz(const R* q, int st, const R* tab, R es) {
+  const __attribute__((address_space(4))) R* w = (const __attribute__((address_space(4))) R*)tab;
+  R v = w[216] * es;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v = v + w[208 + m] * q[m * st];
+  return v;
+}
+
 // Global-memory access through a uniform base pointer (SGPR pair) plus a 32-bit per-lane
 // byte offset: compiles to global_load/store in saddr mode (no 64-bit per-lane address
 // arithmetic, no FLAT instructions that would also count against the LDS wait counter).
@@ -155,10 +170,51 @@ __device__ __forceinline__ R cpml_apply(R* psi, unsigned pb, R a, R b, R k, R d)
   return d / k + q;
 }
 
+// The same derivative without storing psi (what the next update would use).
+template <typename R>
+__device__ __forceinline__ R cpml_rate(const R* psi, unsigned pb, R a, R b, R k, R d) {
+  const R q = b * gld(psi, pb) + a * d;
+  return d / k + q;
+}
+
 __device__ __forceinline__ int strip_x(int i, int nx, int T, int lo, int hi) {
   if (lo && i < T) return i;
   if (hi && i >= nx - T) return i - (nx - 2 * T);
   return -1;
+}
+
+// d vz/dz at the surface (row 4, column i) that keeps szz = 0 there, from the current state
+// and without storing anything: the free-surface row's substitute ezz, the vz,z(0) that
+// formula #3 takes (with L = 0 and isotropy -lam/(lam + 2 mu) d vx/dx,
+// src/free-surface-kernels.jl:160-172). exx = the raw D- d vx/dx of row 4.
+template <typename R, int L>
+__device__ __forceinline__ R surface_ezz(const ShotDesc<R>& d, int i, R exx) {
+  constexpr unsigned E = sizeof(R);
+  const int T = d.T;
+  if (T) {
+    const int xk = strip_x(i, d.nx, T, d.xlo, d.xhi);
+    if (xk >= 0) {
+      const unsigned q = (unsigned)(4 * 2 * T + xk) * E;
+      exx = cpml_rate(d.psi[0], q, gld(d.ax, i * E), gld(d.bx, i * E), gld(d.kx, i * E), exx);
+    }
+  }
+  const unsigned mb = (unsigned)(4 * d.ldm + i) * E, cb = (unsigned)(4 * d.ldx + i) * E;
+  const R lr = gld(d.lam, mb);
+  const R pz = d.vti ? gld(d.c33, mb) : lr + R(2) * gld(d.mu, mb);
+  if constexpr (L > 0) {
+    const R t = gld(d.tau, mb);
+    const R pz1 = pz * t, pz0 = pz1 + pz;
+    const R l1 = lr * t, l0 = l1 + lr;
+    R SMxx = gld((const R*)d.M, cb), SMzz = gld((const R*)d.M + (size_t)L * d.plane, cb);
+#pragma unroll
+    for (int k = 1; k < L; ++k) {
+      SMxx = SMxx + gld((const R*)d.M + (size_t)k * d.plane, cb);
+      SMzz = SMzz + gld((const R*)d.M + (size_t)(L + k) * d.plane, cb);
+    }
+    return -(((l0 * exx) + l1 * SMxx) + pz1 * SMzz) / pz0;
+  } else {
+    return -(lr * exx) / pz;
+  }
 }
 
 // Cooperative load of NROWS rows (starting at global row jr0) of one field into an LDS
@@ -431,7 +487,7 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
 
   const int nx = d.nx, nz = d.nz, T = d.T;
   const bool col_ok = i < nx - 4;
-  const bool afda = AFDA && d.fs == 2;
+  const bool afda = AFDA && d.fs >= 2;
   const R dt = d.dt, two = R(2);
   const R Dx[4] = {d.Dx[0], d.Dx[1], d.Dx[2], d.Dx[3]};
   const R Dz[4] = {d.Dz[0], d.Dz[1], d.Dz[2], d.Dz[3]};
@@ -529,6 +585,7 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void
   constexpr int RH = HOIST ? RZ : 1;
   __shared__ R s_vx[(TZ + 8) * WX];  // vx rows j0-4 .. j0+TZ+3, columns i0-4 .. i0+67
   __shared__ R s_vz[(TZ + 8) * WX];  // vz, same window
+  __shared__ R s_es[AFDA ? TX : 1];  // formula #3: d vz/dz at the surface per column
   const int lane = threadIdx.x, w = threadIdx.y;
   const int gtile = xcd_tile(blockIdx.x, gridDim.x);
   const ShotDesc<R>& d = shots[tile_shot ? tile_shot[gtile] : 0];  // null: one shot
@@ -621,6 +678,15 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void
   const R Dz[4] = {d.Dz[0], d.Dz[1], d.Dz[2], d.Dz[3]};
   const int xk = T ? strip_x(i, nx, T, d.xlo, d.xhi) : -1;
   const int fs = d.fs;
+  if constexpr (AFDA) {
+    // formula #3: row 5 needs row 4's surface value, taken here before any wave of the tile
+    // updates row 4's psi and memory variables (fs and tz are uniform over the block)
+    if (fs == 3 && tz == 0) {
+      if (w == 0 && col_ok)
+        s_es[lane] = surface_ezz<R, L>(d, i, op_m(&s_vx[4 * WX + lane + 4], 1, Dx));
+      __syncthreads();
+    }
+  }
   R a1[LM], a2[LM];
 #pragma unroll
   for (int k = 0; k < L; ++k) {
@@ -642,8 +708,11 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void
     // H-AFDA rows (only in the first tile row, where LDS row = grid row); ju is j, known to
     // the compiler as wave-uniform, so this is a scalar branch
     const int ju = __builtin_amdgcn_readfirstlane(jw) + r;
-    if (AFDA && fs == 2 && ju < 8) {
-      if (ju >= 5) ezz = afda_dz(&s_vz[4 * WX + lane + 4], WX, d.afda, ju + 2);  // C_{j-4}
+    if (AFDA && fs >= 2 && ju < 8) {
+      if (fs == 3 && ju == 5)  // formula #3
+        ezz = herm_dz(&s_vz[4 * WX + lane + 4], WX, d.afda, s_es[lane]);
+      else if (ju >= 5)
+        ezz = afda_dz(&s_vz[4 * WX + lane + 4], WX, d.afda, ju + 2);  // C_{j-4}
       if (ju < 7) ezx = afda_dz(&s_vx[4 * WX + lane + 4], WX, d.afda, ju + 6);   // D_{j-4}
     }
     const bool ok = col_ok && j < nz - 4;

@@ -7,7 +7,8 @@
 //                   sources added before the buoyancy (values per source, not per step);
 //   planes 2, 3, 4  dsxx, dszz, dsxz: the stress half step's rates from the stored velocities
 //                   and memory variables, explosive sources added; on the free-surface row the
-//                   ezz that makes dszz = 0 is used and dszz is 0;
+//                   ezz that makes dszz = 0 is used and dszz is 0 (with formula #3, row 5
+//                   recomputes that value from row 4's operands, which k_rates leaves as is);
 //   planes 5 ..     dM (3 L planes: xx, zz, xz per mechanism): a1_n e + a2_n M_n.
 // In CPML strips a derivative uses psi' = b psi + a d without storing it, as the step does, so
 // the step's update is x + dt * rate operation for operation (checked against the oracle).
@@ -17,12 +18,6 @@
 #include "vi_kernels.hpp"
 
 namespace vi {
-
-template <typename R>
-__device__ __forceinline__ R cpml_rate(const R* psi, unsigned pb, R a, R b, R k, R d) {
-  const R q = b * gld(psi, pb) + a * d;
-  return d / k + q;
-}
 
 // out: rate planes of the state pitch ldx, `plane` elements apart; sources: cell (six, siz),
 // kind and value of each source in the caller's order (sval null: no source term)
@@ -49,7 +44,7 @@ __global__ __launch_bounds__(256) void k_rates(const ShotDesc<R> d, const int* _
   const R two = R(2), quarter = R(0.25);
   const R Dx[4] = {d.Dx[0], d.Dx[1], d.Dx[2], d.Dx[3]};
   const R Dz[4] = {d.Dz[0], d.Dz[1], d.Dz[2], d.Dz[3]};
-  const bool afda = d.fs == 2;
+  const bool afda = d.fs >= 2;
   const int xk = T ? strip_x(i, nx, T, d.xlo, d.xhi) : -1;
   const int zk = T ? strip_x(j, nz, T, d.zlo, d.zhi) : -1;
   const unsigned qx = (unsigned)(j * 2 * T + xk) * E, qz = (unsigned)(zk * ldx + i) * E;
@@ -85,8 +80,11 @@ __global__ __launch_bounds__(256) void k_rates(const ShotDesc<R> d, const int* _
   // ---- stress rates
   R exx = op_m(d.vx + c, 1, Dx);
   R exz = op_p(d.vz + c, 1, Dx);
-  R ezz = afda && j >= 5 && j < 8 ? afda_dz(d.vz + 4 * ldx + i, ldx, d.afda, j + 2)
-                                  : op_m(d.vz + c, ldx, Dz);
+  R ezz = d.fs == 3 && j == 5
+              ? herm_dz(d.vz + 4 * ldx + i, ldx, d.afda,
+                        surface_ezz<R, L>(d, i, op_m(d.vx + 4 * ldx + i, 1, Dx)))
+          : afda && j >= 5 && j < 8 ? afda_dz(d.vz + 4 * ldx + i, ldx, d.afda, j + 2)
+                                    : op_m(d.vz + c, ldx, Dz);
   R ezx = afda && j < 7 ? afda_dz(d.vx + 4 * ldx + i, ldx, d.afda, j + 6)
                         : op_p(d.vx + c, ldx, Dz);
   if (xk >= 0) {

@@ -22,7 +22,7 @@ SOURCE_KINDS = {"force_z": L.VI_SRC_FORCE_Z, "explosive": L.VI_SRC_EXPLOSIVE,
 COMPONENTS = {"vx": 0, "vz": 1, "sxx": 2, "szz": 3, "sxz": 4}
 
 
-FREE_SURFACE_MODES = {"none": 0, "image": 1, "imaging": 1, "afda": 2}
+FREE_SURFACE_MODES = {"none": 0, "image": 1, "imaging": 1, "afda": 2, "hermitian": 3}
 
 
 def _as_int32(a):
@@ -30,10 +30,11 @@ def _as_int32(a):
 
 
 def free_surface_mode(mode) -> int:
-    """False/None/"none" -> 0, True/"image" -> 1 (stress imaging), "afda" -> 2 (H-AFDA)."""
+    """False/None/"none" -> 0, True/"image" -> 1 (stress imaging), "afda" -> 2 (H-AFDA),
+    "hermitian" -> 3 (H-AFDA with the Hermitian formula #3 for d vz/dz at z = h)."""
     if isinstance(mode, str):
         if mode not in FREE_SURFACE_MODES:
-            raise ValueError(f"unknown free surface {mode!r}: none, image or afda")
+            raise ValueError(f"unknown free surface {mode!r}: none, image, afda or hermitian")
         return FREE_SURFACE_MODES[mode]
     return int(bool(mode))
 
@@ -48,6 +49,16 @@ def afda_weights(h, counts=None):
     if counts is not None:
         cp = (ctypes.c_int * 13)(*[int(c) for c in counts])
     L.check(L.lib().vi_afda_weights(float(h), cp, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def afda_hermitian_weights(h, points=5):
+    """Formula #3 as the library computes it (_stencil_hermitian, Stencil.jl:119-189):
+    [alpha, beta_0 / h, ..., beta_7 / h] with d vz/dz(h) = alpha d vz/dz(0) +
+    sum_m beta_m vz((m+1/2) h), m < points. Host-only; needs no GPU."""
+    out = np.zeros(9)
+    L.check(L.lib().vi_afda_hermitian_weights(float(h), int(points),
+                                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
     return out
 
 
@@ -216,7 +227,8 @@ class Simulation:
 
     def set_free_surface(self, mode=True):
         """False/"none": Dirichlet top; True/"image": stress imaging (FreeSurfaceBC,
-        CPML.jl:366-393); "afda": the H-AFDA one-sided surface (free-surface-kernels.jl)."""
+        CPML.jl:366-393); "afda": the H-AFDA one-sided surface (free-surface-kernels.jl);
+        "hermitian": H-AFDA with formula #3 on row 5 (free-surface-kernels.jl:154-214)."""
         L.check(L.lib().vi_set_free_surface(self._h, free_surface_mode(mode)))
 
     def set_afda_points(self, counts=None):
@@ -225,6 +237,10 @@ class Simulation:
         if counts is not None:
             cp = (ctypes.c_int * 13)(*[int(c) for c in counts])
         L.check(L.lib().vi_set_afda_points(self._h, cp))
+
+    def set_afda_hermitian(self, points=5):
+        """Half points of formula #3 (2..8); see ``afda_hermitian_weights``."""
+        L.check(L.lib().vi_set_afda_hermitian(self._h, int(points)))
 
     def set_sources(self, ix, iz, kind, wavelets):
         ix, iz = _as_int32(ix), _as_int32(iz)
