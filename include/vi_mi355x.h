@@ -75,13 +75,15 @@ enum { VI_SRC_FORCE_Z = 0, VI_SRC_EXPLOSIVE = 1, VI_SRC_FORCE_X = 2 };
 enum { VI_REC_VX = 0, VI_REC_VZ = 1, VI_REC_SXX = 2, VI_REC_SZZ = 3, VI_REC_SXZ = 4 };
 
 /* Grid and physics of one simulation (Parameters.jl:47-63: Nx, Nz, dx, dz, dt,
- * fd_order_x, fd_order_z). Only 8th-order staggered operators are implemented. */
+ * fd_order_x, fd_order_z). Staggered operators of order 2, 4, 6 or 8 per axis
+ * (Stencil(xorder, zorder, dx, dz; stagger), Stencil.jl:280-320); the Dirichlet ring is 4
+ * cells for every order (the reference pads by order/2). */
 typedef struct {
   int nx, nz;
   double dx, dz, dt;
   int precision;       /* VI_F32 or VI_F64 */
   int nrelax;          /* L standard linear solids, 0 = elastic, at most 6 */
-  int fd_order_x, fd_order_z; /* must be 8 */
+  int fd_order_x, fd_order_z; /* 2, 4, 6 or 8 */
 } vi_grid_desc;
 
 /* CPMLConfig (CPML.jl:28-63). d0 must be given explicitly; vi_cpml_auto_d0 computes the
@@ -97,6 +99,10 @@ const char* vi_last_error(void);
 const char* vi_version(void);
 /* staggered 8th-order weights / h, as _stencil (Stencil.jl:9-14) gives them */
 void vi_fd_coeffs(double h, double out[4]);
+/* staggered first-derivative weights / h of order 2, 4, 6 or 8 (_stencil at +-1/2 ..
+ * +-(order-1)/2, Stencil.jl:9-14 and 280-320): out[l] multiplies u[c+l] - u[c-l-1] (D+) or
+ * u[c+l-1] - u[c-l] (D-); entries beyond order/2 are 0 */
+int vi_fd_weights(double h, int order, double out[4]);
 /* H-AFDA one-sided d/dz weights / h (the FreeSurfaceStencil coefficients, Stencil.jl:52-105,
  * formulas #1, #2 and #4 of Kristek et al. 2002, carried to every row an 8th-order centred
  * stencil would take above the surface). Table rows: A0..A3 = d sxz/dz at z = k h (from

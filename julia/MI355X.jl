@@ -92,8 +92,9 @@ mutable struct Sim{T<:Union{Float32,Float64}}
     rec_nt::Int
     model::Any      # host copy of the arrays last uploaded (to detect changes)
     τns::Vector{Float64}
-    function Sim{T}(Nx, Nz, dx, dz, dt; nrelax=0) where {T<:Union{Float32,Float64}}
-        g = Ref(GridDesc(Nx, Nz, dx, dz, dt, precision_code(T), nrelax, 8, 8))
+    function Sim{T}(Nx, Nz, dx, dz, dt; nrelax=0, fd_order_x=8,
+                    fd_order_z=8) where {T<:Union{Float32,Float64}}
+        g = Ref(GridDesc(Nx, Nz, dx, dz, dt, precision_code(T), nrelax, fd_order_x, fd_order_z))
         h = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:vi_create, LIB), Cint, (Ref{GridDesc}, Ref{Ptr{Cvoid}}), g, h))
         s = new{T}(h[], Nx, Nz, nrelax, Float64(dt), 0, 0, 0, nothing, Float64[])
@@ -334,13 +335,14 @@ end
 
 The keyword constructor of src/sciml_solver/Parameters.jl:47-86 for the MI355X backend, with
 the same names and defaults (a 40 Hz Ricker source at the centre, ρ = 2000, λ = μ = 1e9).
-Only 8th-order operators and the Dirichlet ring with an optional CPML are available (the
-reference's periodic default is defective). `qmodel` adds L SLS mechanisms (τ and τns);
+Operators of order 2, 4, 6 or 8 per axis and the Dirichlet ring with an optional CPML are
+available (the reference's periodic default is defective). `qmodel` adds L SLS mechanisms (τ and τns);
 `receivers` is (ix, iz, components).
 """
 struct GPUParameters
     Nx::Int; Nz::Int
     dx::Float64; dz::Float64; dt::Float64
+    fd_order_x::Int; fd_order_z::Int
     model::Any
     qmodel::Any
     cpml::Any
@@ -354,16 +356,16 @@ function Parameters(; Nx=64, Nz=64, dx=10.0, dz=10.0, dt=0.001, fd_order_x=8, fd
                     bc=:dirichlet, source=nothing, device=MI355XBackend(), model=nothing,
                     cpml=nothing, free_surface=false, qmodel=nothing, receivers=nothing,
                     precision=Float32)
-    (fd_order_x == 8 && fd_order_z == 8) ||
-        error("only 8th-order staggered operators are implemented on MI355X")
+    (fd_order_x in (2, 4, 6, 8) && fd_order_z in (2, 4, 6, 8)) ||
+        error("staggered operators of order 2, 4, 6 or 8 are implemented on MI355X")
     bc === :dirichlet || error("boundary condition $bc is not implemented on MI355X; " *
                                "use :dirichlet with an optional CPMLConfig")
     device isa MI355XBackend || error("MI355X.Parameters needs device=MI355XBackend()")
     src = source === nothing ? RickerSource(40.0, 0.2, Nx ÷ 2, Nz ÷ 2) : source
     m = model === nothing ?
         (b=fill(1 / 2000.0, Nx, Nz), λ=fill(1e9, Nx, Nz), μ=fill(1e9, Nx, Nz)) : model
-    return GPUParameters(Nx, Nz, dx, dz, dt, m, qmodel, cpml, free_surface, src, receivers,
-                         precision)
+    return GPUParameters(Nx, Nz, dx, dz, dt, fd_order_x, fd_order_z, m, qmodel, cpml,
+                         free_surface, src, receivers, precision)
 end
 
 nrelax(p::GPUParameters) = p.qmodel === nothing ? 0 : length(p.qmodel.τns)
@@ -389,7 +391,8 @@ A device simulation configured from Parameters: model, relaxation, CPML, free su
 positions (samples are given per run) and receivers (capacity set by the run).
 """
 function Sim{T}(p::GPUParameters) where {T}
-    s = Sim{T}(p.Nx, p.Nz, p.dx, p.dz, p.dt; nrelax=nrelax(p))
+    s = Sim{T}(p.Nx, p.Nz, p.dx, p.dz, p.dt; nrelax=nrelax(p), fd_order_x=p.fd_order_x,
+               fd_order_z=p.fd_order_z)
     τ = p.qmodel === nothing ? nothing : p.qmodel.τ
     set_model!(s, p.model, τ)
     p.qmodel === nothing || set_relaxation!(s, Array(p.qmodel.τns))

@@ -24,6 +24,27 @@ void vio_fd_coeffs(double h, double out[4]) {
   out[3] = (-5.0 / 7168.0) / h;
 }
 
+/* Staggered first-derivative weights of order 2K (Stencil(order; stagger), Stencil.jl:280-320:
+ * _stencil at +-1/2 .. +-(2K-1)/2), by the closed form of the staggered-grid weights
+ *   c_l = (-1)^(l+1) / (2l-1) * prod_{i != l} (2i-1)^2 / |(2l-1)^2 - (2i-1)^2|,  l = 1..K
+ * (exact integers, one rounding by the division, then / h; unused entries 0). */
+int vio_fd_weights(double h, int order, double out[4]) {
+  if (order < 2 || order > 8 || order % 2 || !(h > 0)) return -1;
+  const int K = order / 2;
+  for (int l = 0; l < 4; ++l) out[l] = 0.0;
+  for (int l = 1; l <= K; ++l) {
+    long long num = 1, den = 2 * l - 1;
+    for (int i = 1; i <= K; ++i) {
+      if (i == l) continue;
+      const long long a = (2 * i - 1) * (2 * i - 1), b = (2 * l - 1) * (2 * l - 1) - a;
+      num *= a;
+      den *= b < 0 ? -b : b;
+    }
+    out[l - 1] = ((l % 2 ? 1.0 : -1.0) * (double)num / (double)den) / h;
+  }
+  return 0;
+}
+
 /* CPMLCoefficients (src/sciml_solver/CPML.jl:109-189) along one axis, in the working
  * precision F of the arrays, as the reference computes them:
  *   lo strip i in [0,T):       p = (T - i - 0.5)/T                      (CPML.jl:124)

@@ -56,10 +56,13 @@ typedef struct {
   int cpml_left, cpml_right, cpml_top, cpml_bottom;
   int afda_points[13]; /* H-AFDA point counts per table row (see vio.c); all 0 = default */
   int afda_hermitian_points; /* free_surface = 3: half points of formula #3; 0 = default */
+  int fd_order_x, fd_order_z; /* staggered operator orders 2, 4, 6, 8; 0 = 8 */
 } vio_desc;
 
 /* Exact staggered 8th-order weights (1225/1024, -245/3072, 49/5120, -5/7168) / h. */
 void vio_fd_coeffs(double h, double out[4]);
+/* Staggered weights / h of order 2, 4, 6 or 8 (entries beyond order/2 are 0); -1 if invalid. */
+int vio_fd_weights(double h, int order, double out[4]);
 
 /* H-AFDA one-sided d/dz weights / h (free_surface = 2): table rows A0..A3, B0..B2, C1..C3,
  * D0..D2 over field rows 4 .. 11 (see vio.c). counts NULL = the default point counts.

@@ -69,6 +69,7 @@ class Desc(ctypes.Structure):
         ("cpml_top", ctypes.c_int), ("cpml_bottom", ctypes.c_int),
         ("afda_points", ctypes.c_int * 13),
         ("afda_hermitian_points", ctypes.c_int),
+        ("fd_order_x", ctypes.c_int), ("fd_order_z", ctypes.c_int),
     ]
 
 
@@ -121,6 +122,9 @@ def lib():
             f = getattr(_lib, "vio_destroy" + suf)
             f.restype = None
             f.argtypes = [vp]
+        _lib.vio_fd_weights.restype = ctypes.c_int
+        _lib.vio_fd_weights.argtypes = [ctypes.c_double, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_double)]
         _lib.vio_fd_coeffs.restype = None
         _lib.vio_fd_coeffs.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         dp = ctypes.POINTER(ctypes.c_double)
@@ -132,6 +136,14 @@ def lib():
         _lib.vio_cpml_axis_f32.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 5 + [
             ctypes.c_int, ctypes.c_int, fp, fp, fp, fp, fp]
     return _lib
+
+
+def fd_weights(h: float, order: int) -> np.ndarray:
+    """Staggered weights / h of the given order (2, 4, 6, 8), padded to 4 with zeros."""
+    out = np.zeros(4)
+    if lib().vio_fd_weights(float(h), int(order), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
+        raise ValueError(f"unsupported order {order}")
+    return out
 
 
 def fd_coeffs(h: float) -> np.ndarray:
@@ -163,7 +175,8 @@ class Oracle:
     """
 
     def __init__(self, nx, nz, dx, dz, dt, model, dtype=np.float64, nrelax=0, taun=None,
-                 free_surface=False, cpml=None, afda_points=None, hermitian_points=None):
+                 free_surface=False, cpml=None, afda_points=None, hermitian_points=None,
+                 fd_order=(8, 8)):
         self.dtype = np.dtype(dtype)
         self.suf = "_f32" if self.dtype == np.float32 else "_f64"
         self.ct = ctypes.c_float if self.suf == "_f32" else ctypes.c_double
@@ -177,6 +190,7 @@ class Oracle:
                 d.afda_points[k] = int(c)
         if hermitian_points is not None:
             d.afda_hermitian_points = int(hermitian_points)
+        d.fd_order_x, d.fd_order_z = (int(o) for o in fd_order)
         if cpml:
             d.cpml_thickness = int(cpml["thickness"])
             d.cpml_d0 = float(cpml["d0"])

@@ -107,6 +107,12 @@ void* NAME(vio_create)(const vio_desc* d, const REAL* rho, const REAL* lam, cons
         vio_afda_hermitian(d->dz, d->afda_hermitian_points ? d->afda_hermitian_points : VIO_HERMITIAN_DEFAULT, hw))
       return NULL;
   }
+  {
+    double c[4];
+    const int ox = d->fd_order_x ? d->fd_order_x : 8, oz = d->fd_order_z ? d->fd_order_z : 8;
+    if (vio_fd_weights(1.0, ox, c) || vio_fd_weights(1.0, oz, c)) return NULL;
+    if (d->free_surface >= 2 && oz != 8) return NULL; /* the H-AFDA rows are 8th-order */
+  }
   /* the surface row must not lie in a top CPML strip */
   if (d->free_surface && d->cpml_thickness > 0 && d->cpml_top) return NULL;
   SIM* s = (SIM*)calloc(1, sizeof(SIM));
@@ -135,9 +141,9 @@ void* NAME(vio_create)(const vio_desc* d, const REAL* rho, const REAL* lam, cons
     }
   }
   double c[4];
-  vio_fd_coeffs(d->dx, c);
+  vio_fd_weights(d->dx, d->fd_order_x ? d->fd_order_x : 8, c); /* validated above */
   for (int l = 0; l < 4; ++l) s->Dx[l] = (REAL)c[l];
-  vio_fd_coeffs(d->dz, c);
+  vio_fd_weights(d->dz, d->fd_order_z ? d->fd_order_z : 8, c);
   for (int l = 0; l < 4; ++l) s->Dz[l] = (REAL)c[l];
   if (d->free_surface >= 2) {
     double w[13][8];

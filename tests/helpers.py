@@ -27,7 +27,8 @@ def make_oracle(cfg, dtype):
                    dtype=dtype, nrelax=cfg["nrelax"], taun=cfg.get("taun"),
                    free_surface=cfg["free_surface"], cpml=cfg["cpml"],
                    afda_points=cfg.get("afda_points"),
-                   hermitian_points=cfg.get("hermitian_points"))
+                   hermitian_points=cfg.get("hermitian_points"),
+                   fd_order=cfg.get("fd_order", (8, 8)))
     if len(cfg["src_ix"]):
         o.set_sources(cfg["src_ix"], cfg["src_iz"], cfg["src_kind"], cfg["wavelets"])
     if len(cfg["rec_ix"]):

@@ -151,8 +151,10 @@ def test_create_without_device_or_with_bad_args_fails_loudly():
     bad = _lib.GridDesc(8, 8, 1.0, 1.0, 1e-3, 0, 0, 8, 8)
     assert lib.vi_create(ctypes.byref(bad), ctypes.byref(h)) == _lib.VI_EINVAL
     assert b"nx and nz" in lib.vi_last_error()
-    bad = _lib.GridDesc(32, 32, 1.0, 1.0, 1e-3, 0, 0, 4, 8)
-    assert lib.vi_create(ctypes.byref(bad), ctypes.byref(h)) == _lib.VI_EINVAL
+    for ox, oz in ((3, 8), (8, 10), (0, 8), (8, -2)):  # orders must be 2, 4, 6 or 8
+        bad = _lib.GridDesc(32, 32, 1.0, 1.0, 1e-3, 0, 0, ox, oz)
+        assert lib.vi_create(ctypes.byref(bad), ctypes.byref(h)) == _lib.VI_EINVAL
+        assert b"fd_order" in lib.vi_last_error()
     bad = _lib.GridDesc(32, 32, 1.0, 1.0, 1e-3, 0, 7, 8, 8)
     assert lib.vi_create(ctypes.byref(bad), ctypes.byref(h)) == _lib.VI_EINVAL
     if ndev == 0:
@@ -236,8 +238,9 @@ def test_parameters_defaults_and_errors():
         Parameters(bc="periodic")
     with pytest.raises(ValueError):
         Parameters(bc="bogus")
+    assert Parameters(fd_order_x=4, fd_order_z=6).fd_order_z == 6
     with pytest.raises(NotImplementedError):
-        Parameters(fd_order_x=4)
+        Parameters(fd_order_x=10)
     with pytest.raises(ValueError):
         Parameters(device="cpu")
 

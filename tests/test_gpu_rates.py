@@ -5,9 +5,9 @@ ViscoelasticProblem.jl:62-174) on the GPU:
   kappa and alpha, VTI, L = 0 and 3, FP32 and FP64, with force and explosive sources;
 * one step of vi_step is x + dt * rate (velocity half, then stress half from the new v);
 * the assertions of the reference's own solver tests, tests/test_viscoelastic_problem.jl:49-246,
-  through the Python mirrors velocity_eq / stress_eq. Substitutions: 8th-order operators and
-  the Dirichlet ring (the reference's test uses fd_order 4 and the periodic BC, which this
-  library does not implement), 0-based grid indices.
+  through the Python mirrors velocity_eq / stress_eq, with the reference's fd_order 4.
+  Substitutions: the Dirichlet ring of 4 cells for the periodic BC (which this library does
+  not implement), 0-based grid indices.
 """
 import numpy as np
 import pytest
@@ -110,7 +110,8 @@ def _params(freq=FREQ, sx=NX // 2, sz=NZ // 2):
     from viscoelasticinversion_amd.api import IsotropicModel, Parameters, RickerSource
     model = IsotropicModel(2000.0, 1e9, 1e9, NX, NZ)
     return Parameters(Nx=NX, Nz=NZ, dx=10.0, dz=10.0, model=model,
-                      source=RickerSource(freq, TC, sx, sz), bc="dirichlet", precision="float32")
+                      source=RickerSource(freq, TC, sx, sz), bc="dirichlet", precision="float32",
+                      fd_order_x=4, fd_order_z=4)
 
 
 def _zero_state():

@@ -51,6 +51,7 @@ SIGNATURES = {
     "vi_last_error": (ctypes.c_char_p, []),
     "vi_version": (ctypes.c_char_p, []),
     "vi_fd_coeffs": (None, [_D, _DP]),
+    "vi_fd_weights": (_I, [_D, _I, _DP]),
     "vi_afda_weights": (_I, [_D, _IP, _DP]),
     "vi_afda_default_points": (_I, [_IP]),
     "vi_afda_hermitian_weights": (_I, [_D, _I, _DP]),

@@ -274,8 +274,8 @@ class Parameters:
     receivers: Optional[tuple] = None
 
     def __post_init__(self):
-        if self.fd_order_x != 8 or self.fd_order_z != 8:
-            raise NotImplementedError("only 8th-order staggered operators are implemented")
+        if self.fd_order_x not in (2, 4, 6, 8) or self.fd_order_z not in (2, 4, 6, 8):
+            raise NotImplementedError("staggered operators of order 2, 4, 6 or 8 are implemented")
         if str(self.device).lower() not in ("mi355x", "gpu", "hip"):
             raise ValueError(f"unsupported device {self.device!r}: this library runs on MI355X")
         self.bc_obj = get_bc(self.bc)
@@ -293,7 +293,7 @@ class Parameters:
     def build(self) -> Simulation:
         """Create and configure a device simulation for these parameters."""
         sim = Simulation(self.Nx, self.Nz, self.dx, self.dz, self.dt, self.precision,
-                         self.nrelax)
+                         self.nrelax, (self.fd_order_x, self.fd_order_z))
         m = self.model
         tau = self.qmodel.tau if self.qmodel is not None else None
         if isinstance(m, VTIModel):
@@ -385,16 +385,19 @@ def solve_problem(problem, solver="leapfrog", saveat=None, **kwargs):
 
 
 # ---------------------------------------------------------------------------- stability
-# sum of |weights| x h of the staggered 8th-order operator (1225/1024, -245/3072, 49/5120,
-# -5/7168): the largest magnitude of its symbol
-FD_WEIGHT_SUM = 1225 / 1024 + 245 / 3072 + 49 / 5120 + 5 / 7168
+# sum of |weights| x h of the staggered operator of each order (8: 1225/1024, -245/3072,
+# 49/5120, -5/7168): the largest magnitude of its symbol
+FD_WEIGHT_SUMS = {2: 1.0, 4: 9 / 8 + 1 / 24, 6: 75 / 64 + 25 / 384 + 3 / 640,
+                  8: 1225 / 1024 + 245 / 3072 + 49 / 5120 + 5 / 7168}
+FD_WEIGHT_SUM = FD_WEIGHT_SUMS[8]
 
 
-def stable_dt(model, dx, dz):
-    """Largest stable time step of the staggered 8th-order leapfrog for a model dict (rho and
-    lam, mu or c11, c33, and tau when viscoelastic): the von Neumann bound
-    dt <= 1 / (vmax * sum|c| * sqrt(1/dx^2 + 1/dz^2)) with vmax the largest unrelaxed
-    P-wave speed, sqrt(M (1 + tau) / rho) with M = lam + 2 mu (isotropic) or max(C11, C33)."""
+def stable_dt(model, dx, dz, fd_order=(8, 8)):
+    """Largest stable time step of the staggered leapfrog for a model dict (rho and lam, mu or
+    c11, c33, and tau when viscoelastic) and operator orders (x, z): the von Neumann bound
+    dt <= 1 / (vmax * sqrt((Sx/dx)^2 + (Sz/dz)^2)), S = sum|c| of the order's weights, with
+    vmax the largest unrelaxed P-wave speed, sqrt(M (1 + tau) / rho), M = lam + 2 mu
+    (isotropic) or max(C11, C33)."""
     rho = np.asarray(model["rho"], np.float64)
     if "c11" in model:
         m = np.maximum(np.asarray(model["c11"], np.float64), np.asarray(model["c33"], np.float64))
@@ -403,7 +406,8 @@ def stable_dt(model, dx, dz):
     if model.get("tau") is not None:
         m = m * (1 + np.asarray(model["tau"], np.float64))
     vmax = float(np.sqrt(m / rho).max())
-    return 1.0 / (vmax * FD_WEIGHT_SUM * math.sqrt(1.0 / dx ** 2 + 1.0 / dz ** 2))
+    sx, sz = (FD_WEIGHT_SUMS[int(o)] for o in fd_order)
+    return 1.0 / (vmax * math.sqrt((sx / dx) ** 2 + (sz / dz) ** 2))
 
 
 # ---------------------------------------------------------------------------- right-hand side

@@ -102,7 +102,7 @@ def build_simulation(cfg, precision="float32"):
     """Create a configured device Simulation for a config dict."""
     from .stepper import Simulation
     sim = Simulation(cfg["nx"], cfg["nz"], cfg["dx"], cfg["dz"], cfg["dt"], precision,
-                     cfg["nrelax"])
+                     cfg["nrelax"], cfg.get("fd_order", (8, 8)))
     sim.set_model_dict(cfg["model"])
     if cfg["nrelax"]:
         sim.set_relaxation(cfg["taun"])

@@ -426,9 +426,9 @@ void build_desc(const vi_sim_s* h, int tile_base, vi::ShotDesc<R>* d) {
   d->step = h->step;
   d->dt = (R)h->g.dt;
   double c[4];
-  vi_fd_coeffs(h->g.dx, c);
+  vi_fd_weights(h->g.dx, h->g.fd_order_x, c);  // orders validated by vi_create
   for (int l = 0; l < 4; ++l) d->Dx[l] = (R)c[l];
-  vi_fd_coeffs(h->g.dz, c);
+  vi_fd_weights(h->g.dz, h->g.fd_order_z, c);
   for (int l = 0; l < 4; ++l) d->Dz[l] = (R)c[l];
   for (int k = 0; k < h->L; ++k) {
     const R one = R(1), Lr = (R)h->L, tn = (R)h->taun[k];
@@ -1339,13 +1339,40 @@ extern "C" {
 const char* vi_last_error(void) { return g_err.c_str(); }
 const char* vi_version(void) { return "vi_mi355x 0.2 (gfx950)"; }
 
-void vi_fd_coeffs(double h, double out[4]) {
-  // _stencil(x, 0, 1) (Stencil.jl:9-14) at x = +-1/2 .. +-7/2, folded to the half stencil
-  // of diff-funcs.jl:3-32.
-  out[0] = (1225.0 / 1024.0) / h;
-  out[1] = (-245.0 / 3072.0) / h;
-  out[2] = (49.0 / 5120.0) / h;
-  out[3] = (-5.0 / 7168.0) / h;
+void vi_fd_coeffs(double h, double out[4]) { vi_fd_weights(h, 8, out); }
+
+// Staggered first-derivative weights of order 2K (K <= 4): _stencil(x, 0, 1)
+// (Stencil.jl:9-14) at the nodes x = +-1/2 .. +-(2K-1)/2, as Stencil(order; stagger)
+// builds them (Stencil.jl:280-320), folded to the half stencil of diff-funcs.jl:3-32 (the
+// weight at -x is minus the weight at +x). Each weight is the derivative of a Lagrange
+// basis polynomial at 0, w_m = sum_{q != m} prod_{l != m, q} (0 - x_l) / prod_{l != m}
+// (x_m - x_l), in integers of h/2 (every factor at most 14: exact in int64 and in double),
+// rounded once by the division and then divided by h. Order 8 gives 1225/1024, -245/3072,
+// 49/5120, -5/7168. Unused entries are 0.
+int vi_fd_weights(double h, int order, double out[4]) {
+  if (!out || !(h > 0) || order < 2 || order > 8 || order % 2)
+    return fail(VI_EINVAL, "vi_fd_weights: need h > 0, order 2, 4, 6 or 8 and an output");
+  const int K = order / 2, n = order;
+  long long X[8];
+  for (int l = 0; l < K; ++l) {
+    X[l] = 2 * l + 1;       // +(2l+1)/2
+    X[K + l] = -(2 * l + 1);  // -(2l+1)/2
+  }
+  for (int l = 0; l < 4; ++l) out[l] = 0.0;
+  for (int m = 0; m < K; ++m) {
+    long long num = 0, den = 1;
+    for (int l = 0; l < n; ++l)
+      if (l != m) den *= X[m] - X[l];
+    for (int q = 0; q < n; ++q) {
+      if (q == m) continue;
+      long long t = 1;
+      for (int l = 0; l < n; ++l)
+        if (l != m && l != q) t *= -X[l];
+      num += t;
+    }
+    out[m] = ((double)(2 * num) / (double)den) / h;
+  }
+  return 0;
 }
 
 // H-AFDA one-sided weights (Kristek, Moczo & Archuleta 2002; the reference's draft
@@ -1553,8 +1580,9 @@ int vi_create(const vi_grid_desc* g, vi_handle* out) {
     return fail(VI_EINVAL, "vi_create: precision must be VI_F32 or VI_F64");
   if (g->nrelax < 0 || g->nrelax > vi::MAXL)
     return fail(VI_EINVAL, "vi_create: nrelax must be in [0, 6]");
-  if (g->fd_order_x != 8 || g->fd_order_z != 8)
-    return fail(VI_EINVAL, "vi_create: only 8th-order staggered operators are implemented");
+  for (int o : {g->fd_order_x, g->fd_order_z})
+    if (o != 2 && o != 4 && o != 6 && o != 8)
+      return fail(VI_EINVAL, "vi_create: fd_order_x and fd_order_z must be 2, 4, 6 or 8");
   if (!(g->dx > 0) || !(g->dz > 0) || !(g->dt > 0))
     return fail(VI_EINVAL, "vi_create: dx, dz, dt must be positive");
   if (vi_device_count() < 1) return fail(VI_EDEVICE, "vi_create: no HIP device visible");
@@ -1745,6 +1773,8 @@ int vi_set_free_surface(vi_handle h, int mode) {
   if (mode && h->cpml_on && h->cpml.top)
     return fail(VI_EINVAL, "vi_set_free_surface: conflicts with a top CPML strip");
   if (mode && h->g.nz < 17) return fail(VI_EINVAL, "vi_set_free_surface: nz must be >= 17");
+  if (mode >= 2 && h->g.fd_order_z != 8)
+    return fail(VI_EINVAL, "vi_set_free_surface: the H-AFDA rows need fd_order_z = 8");
   if (mode >= 2) VI_TRY(upload_afda(h));
   h->fs = mode;
   touch(h);

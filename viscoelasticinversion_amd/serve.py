@@ -79,13 +79,15 @@ def job_bytes(job: Job, precision="float32") -> int:
 
 def stability_error(job):
     """Why the job's time step cannot be stable (None when it can): dt above the von Neumann
-    bound of the staggered 8th-order leapfrog for the job's fastest unrelaxed P wave."""
+    bound of the staggered leapfrog (the job's operator orders) for its fastest unrelaxed P
+    wave."""
     from .api import stable_dt
     c = job.cfg
-    lim = stable_dt(c["model"], c["dx"], c["dz"])
+    order = tuple(c.get("fd_order", (8, 8)))
+    lim = stable_dt(c["model"], c["dx"], c["dz"], order)
     if c["dt"] > lim:
         return (f"dt = {c['dt']:.4g} s exceeds the stability limit {lim:.4g} s of this model "
-                "and grid (8th-order staggered leapfrog)")
+                f"and grid (staggered leapfrog of orders {order[0]}, {order[1]})")
     return None
 
 

@@ -62,6 +62,15 @@ def afda_hermitian_weights(h, points=5):
     return out
 
 
+def fd_weights(h, order=8):
+    """Staggered first-derivative weights / h of order 2, 4, 6 or 8 as the library computes
+    them (Stencil.jl:280-320), padded to 4 with zeros. Host-only; needs no GPU."""
+    out = np.zeros(4)
+    L.check(L.lib().vi_fd_weights(float(h), int(order),
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
 def afda_default_points():
     out = (ctypes.c_int * 13)()
     L.check(L.lib().vi_afda_default_points(out))
@@ -93,12 +102,14 @@ class Simulation:
     ``precision`` is "float32" (the reference's GPU precision, utils.jl:4) or "float64".
     """
 
-    def __init__(self, nx, nz, dx, dz, dt, precision="float32", nrelax=0):
+    def __init__(self, nx, nz, dx, dz, dt, precision="float32", nrelax=0, fd_order=(8, 8)):
         self.nx, self.nz, self.dx, self.dz, self.dt = int(nx), int(nz), float(dx), float(dz), float(dt)
         self.dtype = np.dtype(np.float32 if precision in ("float32", np.float32, "f32") else np.float64)
         self.nrelax = int(nrelax)
+        self.fd_order = tuple(int(o) for o in fd_order)  # (x, z): 2, 4, 6 or 8
         g = L.GridDesc(self.nx, self.nz, self.dx, self.dz, self.dt,
-                       L.VI_F32 if self.dtype == np.float32 else L.VI_F64, self.nrelax, 8, 8)
+                       L.VI_F32 if self.dtype == np.float32 else L.VI_F64, self.nrelax,
+                       *self.fd_order)
         h = ctypes.c_void_p()
         L.check(L.lib().vi_create(ctypes.byref(g), ctypes.byref(h)))
         self._h = h
