@@ -248,8 +248,9 @@ int vi_gradient_vti(vi_handle h, int nt, const void* dobs /*[nrec][nt]*/, int ch
                     void* g_c55, void* g_tau);
 /* The same computation as a job that advances in chunks without blocking the caller, so one
  * host thread can interleave several gradients with other device work (serving).
- * begin copies dobs and allocates the job's buffers (vi_gradient_bytes); the handle then
- * belongs to the job (other calls on it fail with VI_ESTATE) until end or cancel.
+ * begin allocates the job's buffers (vi_gradient_bytes) and copies dobs to the device before
+ * it returns (the caller may free it then); the handle then belongs to the job (other calls
+ * on it fail with VI_ESTATE) until end or cancel.
  * advance enqueues at most max_steps forward, recomputed or transposed steps on the handle's
  * stream and returns without waiting (the residual and the misfit are computed on the
  * device); *finished = 1 once all work is enqueued. idle returns 1 when everything enqueued so far has completed, 0 otherwise.

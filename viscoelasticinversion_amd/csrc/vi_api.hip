@@ -1094,6 +1094,9 @@ struct GradRun : vi_grad_s {
     VI_TRY(resid.reserve(lay.resid_bytes));
     dobs_dev = (R*)((char*)resid.p + align256((size_t)nt * nrec * E));
     VI_HIP(hipMemcpyAsync(dobs_dev, dobs_, (size_t)nt * nrec * E, hipMemcpyHostToDevice, st));
+    // the caller may release dobs as soon as begin returns (a chunked job keeps no host
+    // pointer), so the copy completes here
+    VI_HIP(hipStreamSynchronize(st));
     Jdev = (double*)((char*)dobs_dev + align256((size_t)nt * nrec * E));
     {
       int t = 0, k = 0;
