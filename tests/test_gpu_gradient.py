@@ -257,3 +257,30 @@ def test_chunked_gradient_job_equals_gradient(vti, K):
         assert np.array_equal(g[k], g2[k]), k
     sim.step(2)  # the handle is free again
     sim.close()
+
+
+def test_chunked_gradient_job_after_the_observed_data_are_freed():
+    """begin copies the observed data before it returns: the caller's buffer (here a fresh
+    float64 array that the job converts and drops) can be freed and its memory reused while
+    the job runs, and the result still equals the one-call gradient bit for bit."""
+    import gc
+    from viscoelasticinversion_amd.configs import build_simulation
+    from viscoelasticinversion_amd.stepper import GradientJob
+    cfg = small_cfg(nx=80, nz=60, nrelax=2, seed=52, nt=60)
+    sim = build_simulation(cfg, "float32")
+    sim.step(60)
+    dobs = sim.traces(60).astype(np.float64) * 0.95
+    sim.reset()
+    print("[case] chunked gradient with the observed data freed after begin", flush=True)
+    J, g = sim.gradient(dobs, 60, checkpoint_every=6)
+    job = GradientJob(sim, dobs.copy(), 60, checkpoint_every=6)
+    gc.collect()
+    junk = [np.full(dobs.shape, np.nan, np.float32) for _ in range(8)]  # reuse freed memory
+    while not job.finished:
+        job.advance(7)
+    J2, g2 = job.end()
+    del junk
+    assert J2 == J
+    for k in g:
+        assert np.array_equal(g[k], g2[k]), k
+    sim.close()

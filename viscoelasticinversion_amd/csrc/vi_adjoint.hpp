@@ -568,8 +568,9 @@ __global__ __launch_bounds__(256) void k_adj_inject(R* f0, R* f1, R* f2, R* f3, 
 }
 
 // ---- residual and misfit: r = d - d_obs with the traces and the residual [nt][nrec] and
-// the observed data [nrec][nt]; J = 1/2 sum r^2 accumulated in double, per receiver in time
-// order and then over the receivers in order (a fixed order: the same J on every run)
+// the observed data [nrec][nt]; J = 1/2 sum r^2 accumulated in double, per receiver by one
+// workgroup (256 strided partial sums over time, then a fixed pairwise tree) and then over
+// the receivers in order (a fixed order: the same J on every run)
 template <typename R>
 __global__ __launch_bounds__(256) void k_residual(const R* __restrict__ traces,
                                                   const R* __restrict__ dobs,
@@ -581,17 +582,24 @@ __global__ __launch_bounds__(256) void k_residual(const R* __restrict__ traces,
     resid[k] = traces[k] - dobs[r * nt + t];
   }
 }
+// one workgroup of 256 threads per receiver (grid = nrec)
 template <typename R>
 __global__ __launch_bounds__(256) void k_misfit_rec(const R* __restrict__ resid, int nt, int nrec,
                                                     double* __restrict__ part) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrec) return;
+  __shared__ double s_part[256];
+  const int r = blockIdx.x, k = threadIdx.x;
   double s = 0.0;
-  for (int t = 0; t < nt; ++t) {
+  for (int t = k; t < nt; t += 256) {
     const double x = (double)resid[(size_t)t * nrec + r];
     s += 0.5 * x * x;
   }
-  part[r] = s;
+  s_part[k] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (k < h) s_part[k] += s_part[k + h];
+    __syncthreads();
+  }
+  if (k == 0) part[r] = s_part[0];
 }
 __global__ void k_misfit_sum(const double* __restrict__ part, int nrec, double* __restrict__ J) {
   if (blockIdx.x * blockDim.x + threadIdx.x != 0) return;

@@ -1235,7 +1235,7 @@ struct GradRun : vi_grad_s {
           const unsigned gb = (unsigned)std::min<size_t>(4096, (nn + 255) / 256);
           hipLaunchKernelGGL(vi::k_residual<R>, dim3(gb), dim3(256), 0, st,
                              (const R*)h->traces.p, (const R*)dobs_dev, (R*)resid.p, nt, nrec);
-          hipLaunchKernelGGL(vi::k_misfit_rec<R>, dim3((nrec + 255) / 256), dim3(256), 0, st,
+          hipLaunchKernelGGL(vi::k_misfit_rec<R>, dim3(nrec), dim3(256), 0, st,
                              (const R*)resid.p, nt, nrec, Jdev + 1);
           hipLaunchKernelGGL(vi::k_misfit_sum, dim3(1), dim3(64), 0, st, (const double*)(Jdev + 1),
                              nrec, Jdev);
