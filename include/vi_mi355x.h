@@ -169,8 +169,8 @@ int vi_set_cpml(vi_handle h, const vi_cpml_desc* c);
  * (vi_afda_hermitian_weights) whose d vz/dz(0) is the surface row's value that keeps
  * szz = 0 (-lam/(lam + 2 mu) d vx/dx when elastic, free-surface-kernels.jl:160-172; with the
  * memory terms when L > 0; -C13/C33 d vx/dx for VTI). Needs nz >= 17 and no top CPML strip.
- * The gradient entry points take modes 0..2 (the adjoint transposes the one-sided rows) and
- * refuse mode 3 with VI_EINVAL. */
+ * The gradient entry points take every mode: the adjoint transposes the one-sided rows, and
+ * with mode 3 row 5's dependence on the surface row's value. */
 int vi_set_free_surface(vi_handle h, int mode);
 /* points per H-AFDA stencil (see vi_afda_weights; NULL = default). All-8 counts, the full
  * order of the reference's FreeSurfaceStencil(8), diverge in heterogeneous media. */

@@ -1,7 +1,7 @@
 """Gradient of the L2 trace misfit by the hand-written discrete adjoint, checked against
 central finite differences of the forward map (FP64) at random grid points and along random
 directions, for elastic and viscoelastic (L = 1, 3) models with CPML and free surface (stress
-imaging and H-AFDA).
+imaging, H-AFDA, and H-AFDA with the Hermitian formula #3 on row 5).
 The forward map itself is pinned to the CPU oracle by test_gpu_parity.py."""
 import numpy as np
 import pytest
@@ -51,7 +51,8 @@ def _setup(nrelax, fs, cpml, seed, nt=120):
 
 @pytest.mark.parametrize("nrelax,fs,cpml", [(0, False, False), (0, True, True), (1, True, True),
                                             (3, True, True), (0, "afda", False),
-                                            (3, "afda", True)])
+                                            (3, "afda", True), (0, "hermitian", False),
+                                            (3, "hermitian", True)])
 def test_gradient_matches_finite_differences(nrelax, fs, cpml):
     cfg, dobs = _setup(nrelax, fs, cpml, seed=11 + nrelax)
     nt = cfg["nt"]
@@ -93,7 +94,7 @@ def test_gradient_matches_finite_differences(nrelax, fs, cpml):
                 (key, j, i, ad, fd)
 
 
-@pytest.mark.parametrize("nrelax,fs", [(0, True), (3, True), (3, "afda")])
+@pytest.mark.parametrize("nrelax,fs", [(0, True), (3, True), (3, "afda"), (3, "hermitian")])
 def test_adjoint_dot_product(nrelax, fs):
     """<J dm, dd> = <dm, J^T dd> for the Jacobian J of the receiver traces with respect to the
     model: J^T dd is the gradient with the residual set to dd (observed data d - dd), J dm

@@ -1,8 +1,8 @@
 """H-AFDA formula #3 (free surface "hermitian", mode 3) on the GPU: the HIP stepper equals the
 CPU oracle bit for bit on identical seeded inputs (the paths share the operation order) on
 every tiling, FP32 and FP64, elastic, viscoelastic and VTI, with CPML, for several Hermitian
-point counts, and in batches that mix every surface mode. The gradient entry points refuse
-the mode (it has no adjoint)."""
+point counts, and in batches that mix every surface mode. Its gradient is checked against
+finite differences and by the dot-product test in test_gpu_gradient.py and test_gpu_vti.py."""
 import numpy as np
 import pytest
 
@@ -77,10 +77,10 @@ def test_every_surface_mode_in_one_batch():
         s.close()
 
 
-def test_hermitian_mode_switches_and_gradient_refused():
-    """Points set before the mode are used; the handle switches between modes 2 and 3; the
-    gradient refuses mode 3 with a message and works again in mode 2."""
-    from viscoelasticinversion_amd import _lib
+def test_hermitian_mode_switches_and_gradients():
+    """Points set before the mode are used; the handle switches between modes 2 and 3, and the
+    gradient follows the mode: mode 3, then mode 2, then mode 3 again gives the first mode-3
+    gradient bit for bit (the adjoint's difference table is rebuilt on every switch)."""
     from viscoelasticinversion_amd.configs import build_simulation
     cfg = small_cfg(nx=80, nz=48, nrelax=1, nt=30, fs="hermitian")
     cfg["hermitian_points"] = 4
@@ -93,8 +93,9 @@ def test_hermitian_mode_switches_and_gradient_refused():
     _same({f: sim.get(f) for f in ("vx", "vz", "sxx", "szz", "sxz")}, 
           {f: ref[f] for f in ("vx", "vz", "sxx", "szz", "sxz")}, "points before mode")
     dobs = sim.traces(30) * 1.01
-    with pytest.raises(_lib.ViError, match="formula #3"):
-        sim.gradient(dobs, 30)
+    sim.reset()
+    J3, g3 = sim.gradient(dobs, 30)
+    assert np.isfinite(J3) and J3 > 0
     ref2 = run_oracle(dict(cfg, free_surface="afda"), np.float64)
     sim.reset()
     sim.set_free_surface("afda")
@@ -104,4 +105,11 @@ def test_hermitian_mode_switches_and_gradient_refused():
     sim.reset()
     J, g = sim.gradient(dobs, 30)
     assert np.isfinite(J) and J > 0
+    assert any(not np.array_equal(g[k], g3[k]) for k in g)
+    sim.reset()
+    sim.set_free_surface("hermitian")
+    J3b, g3b = sim.gradient(dobs, 30)
+    assert J3b == J3
+    for k in g3:
+        assert np.array_equal(g3b[k], g3[k]), k
     sim.close()

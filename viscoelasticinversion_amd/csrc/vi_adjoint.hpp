@@ -151,9 +151,13 @@ struct AdjStressIn {
   }
 };
 
+// esh: with formula #3 (fs = 3), the adjoint of the surface value es that row 5's d vz/dz
+// reads (alpha times row 5's raw-rate adjoint), added on the surface row (j = 4), whose
+// free-surface rate ezz* is that same value; zero otherwise. Returns the adjoint of the raw
+// d vz/dz of the cell (what formula #3's row 5 hands to the surface row).
 template <typename R, int L>
-__device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, const AdjStressIn<R, L>& in,
-                                                int i, int j) {
+__device__ __forceinline__ R adj_stress_cell(const AdjDesc<R>& d, const AdjStressIn<R, L>& in,
+                                             int i, int j, R esh = R(0)) {
   constexpr unsigned E = sizeof(R);
   constexpr int LM = L > 0 ? L : 1;
   const int ldx = d.ldx;
@@ -217,6 +221,7 @@ __device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, const AdjSt
     nm[2 * L + k] = (R(1) + dt * d.a2[k]) * hxz + SMxzh;
   }
   ezh = ezh + ezh_from_M;
+  if (fs_row && d.fs == 3) ezh = ezh + esh;  // es = ezz* of this row feeds row 5 (formula #3)
   R ezz_raw_h = ezh;
   if (fs_row) {  // ezz* = -(C13_0 exx + C13_1 SMxx + C33_1 SMzz)/C33_0 replaces the raw rate
     exh = exh - l0 / pz0 * ezh;
@@ -253,6 +258,11 @@ __device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, const AdjSt
       g33 = sxxh * dt * (l0 * dz33) + dt * Mzzh_sum * dz33;
       gtau = sxxh * dt * (px * exx + px * SMxx + lr * SMzz + lr * z + l0 * dzt) +
              dt * Mzzh_sum * dzt;
+      if (d.fs == 3) {  // z also enters row 5's d vz/dz
+        g13 = g13 + esh * dz13;
+        g33 = g33 + esh * dz33;
+        gtau = gtau + esh * dzt;
+      }
     }
     gtau = gtau + sxzh * dt * (mxz * gxz + mxz * SMxz);
     if (d.vti) {
@@ -284,6 +294,7 @@ __device__ __forceinline__ void adj_stress_cell(const AdjDesc<R>& d, const AdjSt
   gst(d.ehat[1], cb, ezx_r);
   gst(d.ehat[2], cb, ezz_r);
   gst(d.ehat[3], cb, exz_r);
+  return ezz_r;
 }
 
 // ---- adjoint of the velocity half step (pointwise part of one interior cell) ----------
@@ -402,18 +413,34 @@ __global__ __launch_bounds__(TX* NW) void k_adj_stress(const AdjDesc<R> d) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int i0 = 4 + (tile % ntx) * TX, j0 = 4 + (tile / ntx) * TZ;
   const int i = i0 + lane;
-  if (i >= d.nx - 4) return;
+  const bool col = i < d.nx - 4;
   AdjStressIn<R, L> in[RZ];
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     const int j = j0 + w * RZ + r;
-    if (j < d.nz - 4) in[r].load(d, i, j);
+    if (col && j < d.nz - 4) in[r].load(d, i, j);
   }
+  // formula #3 (fs = 3): row 5's d vz/dz reads the surface row's ezz*, so the surface row's
+  // cell needs row 5's raw-rate adjoint: row 5 goes first and hands it over through LDS (the
+  // first tile row holds rows 4..4+TZ-1; the condition is uniform over the workgroup)
+  __shared__ R s_ez[TX];
+  const bool herm = d.fs == 3 && j0 == 4 && 5 < d.nz - 4;
+  if (herm) {
+#pragma unroll
+    for (int r = 0; r < RZ; ++r)
+      if (col && j0 + w * RZ + r == 5) s_ez[lane] = adj_stress_cell<R, L>(d, in[r], i, 5);
+    __syncthreads();
+  }
+  if (!col) return;
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     const int j = j0 + w * RZ + r;
     if (j >= d.nz - 4) return;
-    adj_stress_cell<R, L>(d, in[r], i, j);
+    if (herm && j == 5) continue;
+    R esh = R(0);
+    if (herm && j == 4)
+      esh = ((const __attribute__((address_space(4))) R*)d.afda)[216] * s_ez[lane];  // alpha
+    adj_stress_cell<R, L>(d, in[r], i, j, esh);
   }
 }
 
@@ -464,7 +491,7 @@ __global__ __launch_bounds__(TX* NW) void k_adj_vel(const AdjDesc<R> d) {
     R gvz = op_p(&s_e2[(tr + 4) * TX + lane], TX, d.Dz) +
             op_m(&s_e3[tr * WX + lane + 4], 1, d.Dx);
     const int ju = __builtin_amdgcn_readfirstlane(j0 + w * RZ) + r;  // j, wave-uniform
-    if (d.fs == 2 && ju < 12) {  // H-AFDA: rows 4..7 hold the source rows (LDS row 8 - j0 + k)
+    if (d.fs >= 2 && ju < 12) {  // H-AFDA: rows 4..7 hold the source rows (LDS row 8 - j0 + k)
       const int s0 = 8 - j0;
       gvx = gvx - afda_corr(&s_e1[s0 * TX + lane], TX, d.afda, 10, 3, ju);      // D0..2
       gvz = gvz - afda_corr(&s_e2[(s0 + 1) * TX + lane], TX, d.afda, 7, 3, ju); // C1..3
@@ -531,7 +558,7 @@ __global__ __launch_bounds__(TX* NW) void k_adj_sgather(const AdjDesc<R> d, int 
              op_p(&s_d2[tr * WX + lane + 4], 1, d.Dx);
     R gszz = op_m(&s_d3[(tr + 4) * TX + lane], TX, d.Dz);
     const int ju = __builtin_amdgcn_readfirstlane(j0 + w * RZ) + r;  // j, wave-uniform
-    if (d.fs == 2 && ju < 12) {  // H-AFDA (row0 = 4): source rows 4..7 at LDS row 8 - j0 + k
+    if (d.fs >= 2 && ju < 12) {  // H-AFDA (row0 = 4): source rows 4..7 at LDS row 8 - j0 + k
       const int s0 = 8 - j0;
       gsxz = gsxz - afda_corr(&s_d1[s0 * TX + lane], TX, d.afda, 0, 4, ju);  // A0..3
       gszz = gszz - afda_corr(&s_d3[s0 * TX + lane], TX, d.afda, 4, 3, ju);  // B0..2

@@ -1215,14 +1215,14 @@ struct GradRun : vi_grad_s {
           // step is s K here, advanced by the plain launches of the earlier segments)
           for (; n < len && budget > 0; ++n, --budget) {
             const vi::RecSlot rs = slot_of(n);
-            VI_TRY(launch_rec<R>(st, h->rz, L, 0, h->desc_dev.p, h->ntiles, n, rs, h->fs == 2));
-            VI_TRY(launch_rec<R>(st, h->rz, L, 1, h->desc_dev.p, h->ntiles, n, rs, h->fs == 2));
+            VI_TRY(launch_rec<R>(st, h->rz, L, 0, h->desc_dev.p, h->ntiles, n, rs, h->fs >= 2));
+            VI_TRY(launch_rec<R>(st, h->rz, L, 1, h->desc_dev.p, h->ntiles, n, rs, h->fs >= 2));
           }
         } else {
           if (n == 0) VI_TRY(copy_state(s, true));
           const int m = std::min(len - n, budget);
           VI_TRY(launch_steps(st, E, h->rz, L, h->desc_dev.p, (const int*)h->tile_shot_dev.p,
-                              h->ntiles, 1, m, nullptr, h->fs == 2));
+                              h->ntiles, 1, m, nullptr, h->fs >= 2));
           n += m;
           budget -= m;
         }
@@ -1256,8 +1256,8 @@ struct GradRun : vi_grad_s {
         for (; n < n1 && budget > 0; ++n, --budget) {
           // both half steps record the step's forward state into its slot as they go
           const vi::RecSlot rs = slot_of(n - n0);
-          VI_TRY(launch_rec<R>(st, h->rz, L, 0, h->desc_dev.p, h->ntiles, n - n0, rs, h->fs == 2));
-          VI_TRY(launch_rec<R>(st, h->rz, L, 1, h->desc_dev.p, h->ntiles, n - n0, rs, h->fs == 2));
+          VI_TRY(launch_rec<R>(st, h->rz, L, 0, h->desc_dev.p, h->ntiles, n - n0, rs, h->fs >= 2));
+          VI_TRY(launch_rec<R>(st, h->rz, L, 1, h->desc_dev.p, h->ntiles, n - n0, rs, h->fs >= 2));
         }
         if (n == n1) {
           n = n1 - 1;
@@ -1729,7 +1729,9 @@ int vi_set_cpml(vi_handle h, const vi_cpml_desc* c) {
 // difference G_a - G_c between each one-sided row j and the centred stencil it replaces,
 // over the target rows g = 4..11 (G_c of D- at j: D[g-j] for 0 <= g-j <= 3, -D[j-g-1] for
 // 0 <= j-g-1 <= 3; of D+: D[g-j-1], -D[j-g]), then formula #3: beta_m / h (8), alpha.
-static int upload_afda(vi_sim_s* h) {
+// mode: the free-surface mode the table is for (2 or 3); with 3 the adjoint's difference row
+// of source row 5 is the Hermitian formula's vz weights minus the centred D-
+static int upload_afda(vi_sim_s* h, int mode) {
   double w[13][8], D[4], hw[9];
   VI_TRY(vi_afda_weights(h->g.dz, h->afda_pts, w));
   VI_TRY(vi_afda_hermitian_weights(h->g.dz, h->herm_pts, hw));
@@ -1748,6 +1750,8 @@ static int upload_afda(vi_sim_s* h) {
     const bool minus = r < 4 || (r >= 7 && r < 10);
     for (int m = 0; m < 8; ++m) c[r][m] = w[r][m] - gc(minus, j, 4 + m);
   }
+  if (mode == 3)  // row 5's d vz/dz = alpha es + sum_m beta_m vz(4 + m) (C1's place)
+    for (int m = 0; m < 8; ++m) c[7][m] = hw[1 + m] - gc(true, 5, 4 + m);
   std::vector<char> t(224 * h->esz);  // 208 + 8 + 1, rounded up
   auto put = [&](int at, double v) {
     if (h->esz == 4) ((float*)t.data())[at] = (float)v;
@@ -1778,7 +1782,7 @@ int vi_set_free_surface(vi_handle h, int mode) {
   if (mode && h->g.nz < 17) return fail(VI_EINVAL, "vi_set_free_surface: nz must be >= 17");
   if (mode >= 2 && h->g.fd_order_z != 8)
     return fail(VI_EINVAL, "vi_set_free_surface: the H-AFDA rows need fd_order_z = 8");
-  if (mode >= 2) VI_TRY(upload_afda(h));
+  if (mode >= 2) VI_TRY(upload_afda(h, mode));
   h->fs = mode;
   touch(h);
   return 0;
@@ -1791,7 +1795,7 @@ int vi_set_afda_points(vi_handle h, const int* counts) {
   double w[13][8];
   VI_TRY(vi_afda_weights(1.0, pts, w));  // range check
   std::memcpy(h->afda_pts, pts, sizeof pts);
-  if (h->fs >= 2) VI_TRY(upload_afda(h));
+  if (h->fs >= 2) VI_TRY(upload_afda(h, h->fs));
   touch(h);
   return 0;
 }
@@ -1801,7 +1805,7 @@ int vi_set_afda_hermitian(vi_handle h, int points) {
   double w[9];
   VI_TRY(vi_afda_hermitian_weights(1.0, points, w));  // range check
   h->herm_pts = points;
-  if (h->fs >= 2) VI_TRY(upload_afda(h));
+  if (h->fs >= 2) VI_TRY(upload_afda(h, h->fs));
   touch(h);
   return 0;
 }
@@ -2207,9 +2211,6 @@ static int gradient_checks(vi_handle h, int nt, const void* dobs, int checkpoint
   VI_TRY(check_ready(h));
   if (h->nrec < 1 || h->rec_nt < nt)
     return fail(VI_ESTATE, std::string(who) + ": receivers with capacity >= nt are required");
-  if (h->fs == 3)
-    return fail(VI_EINVAL, std::string(who) + ": free-surface mode 3 (formula #3) has no "
-                           "adjoint; use mode 2 (H-AFDA) or 1 (imaging)");
   VI_TRY(wait_external(h));
   VI_HIP(hipStreamSynchronize(h->stream));
   return 0;
