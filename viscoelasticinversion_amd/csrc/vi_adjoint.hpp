@@ -155,7 +155,8 @@ struct AdjStressIn {
 // reads (alpha times row 5's raw-rate adjoint), added on the surface row (j = 4), whose
 // free-surface rate ezz* is that same value; zero otherwise. Returns the adjoint of the raw
 // d vz/dz of the cell (what formula #3's row 5 hands to the surface row).
-template <typename R, int L>
+// HERM: the variant for formula #3 (fs = 3), the only one that reads esh.
+template <typename R, int L, bool HERM = false>
 __device__ __forceinline__ R adj_stress_cell(const AdjDesc<R>& d, const AdjStressIn<R, L>& in,
                                              int i, int j, R esh = R(0)) {
   constexpr unsigned E = sizeof(R);
@@ -221,7 +222,7 @@ __device__ __forceinline__ R adj_stress_cell(const AdjDesc<R>& d, const AdjStres
     nm[2 * L + k] = (R(1) + dt * d.a2[k]) * hxz + SMxzh;
   }
   ezh = ezh + ezh_from_M;
-  if (fs_row && d.fs == 3) ezh = ezh + esh;  // es = ezz* of this row feeds row 5 (formula #3)
+  if (HERM && fs_row) ezh = ezh + esh;  // es = ezz* of this row feeds row 5 (formula #3)
   R ezz_raw_h = ezh;
   if (fs_row) {  // ezz* = -(C13_0 exx + C13_1 SMxx + C33_1 SMzz)/C33_0 replaces the raw rate
     exh = exh - l0 / pz0 * ezh;
@@ -258,7 +259,7 @@ __device__ __forceinline__ R adj_stress_cell(const AdjDesc<R>& d, const AdjStres
       g33 = sxxh * dt * (l0 * dz33) + dt * Mzzh_sum * dz33;
       gtau = sxxh * dt * (px * exx + px * SMxx + lr * SMzz + lr * z + l0 * dzt) +
              dt * Mzzh_sum * dzt;
-      if (d.fs == 3) {  // z also enters row 5's d vz/dz
+      if (HERM) {  // z also enters row 5's d vz/dz
         g13 = g13 + esh * dz13;
         g33 = g33 + esh * dz33;
         gtau = gtau + esh * dzt;
@@ -394,14 +395,12 @@ __device__ __forceinline__ R afda_corr(const R* q, int st, const R* tab, int t0,
 // XCD-aware tile order, every stenciled field staged once in LDS with its halo (TileLoad).
 // The sums are the ones of the one-cell formulas, in the same order.
 
-// Transposed stress half, pointwise: every forward quantity is stored per cell, so the tile
-// needs no LDS and no barrier; the tile shape only keeps the loads of a wave coalesced.
-template <typename R, int L, int RZ>
-__global__ __launch_bounds__(TX* NW) void k_adj_stress(const AdjDesc<R> d) {
-  constexpr int TZ = NW * RZ;
+// u_hat += R^T r^n for the velocity receivers (k_adj_inject's sums, in the same order), by the
+// threads of a transposed-stress launch: no other thread of it reads or writes v_hat, and
+// k_adj_vel runs after it.
+template <typename R>
+__device__ __forceinline__ void inject_velocity_targets(const AdjDesc<R>& d) {
   const int lane = threadIdx.x, w = threadIdx.y;
-  // u_hat += R^T r^n for the velocity receivers (k_adj_inject's sums, in the same order):
-  // no other thread of this kernel reads or writes v_hat, and k_adj_vel runs after it
   for (int t = (int)blockIdx.x * (TX * NW) + w * TX + lane; t < d.ntgt_v;
        t += (int)gridDim.x * (TX * NW)) {
     R s = 0;
@@ -409,39 +408,70 @@ __global__ __launch_bounds__(TX* NW) void k_adj_stress(const AdjDesc<R> d) {
     R* p = (d.tcomp[t] == 0 ? d.avx : d.avz) + d.tcell[t];
     *p = *p + s;
   }
+}
+
+// The pointwise transposed stress half of a wave's RZ rows (loaded in `in`). Formula #3
+// (fs = 3): row 5's d vz/dz reads the surface row's ezz*, so the surface row's cell needs row
+// 5's raw-rate adjoint: row 5 goes first and hands it over through LDS (the first tile row
+// holds rows 4..4+TZ-1; the condition is uniform over the workgroup, and every wave of the
+// workgroup reaches the barrier).
+template <typename R, int L, int RZ, bool HERM>
+__device__ __forceinline__ void adj_stress_rows(const AdjDesc<R>& d, const AdjStressIn<R, L>* in,
+                                                int i, int j0, bool col) {
+  const int lane = threadIdx.x, w = threadIdx.y;
+  if constexpr (HERM) {
+    __shared__ R s_ez[TX];
+    const bool herm = j0 == 4 && 5 < d.nz - 4;
+    if (herm) {
+#pragma unroll
+      for (int r = 0; r < RZ; ++r)
+        if (col && j0 + w * RZ + r == 5)
+          s_ez[lane] = adj_stress_cell<R, L, true>(d, in[r], i, 5);
+      __syncthreads();
+    }
+    if (!col) return;
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      const int j = j0 + w * RZ + r;
+      if (j >= d.nz - 4) return;
+      if (herm && j == 5) continue;
+      R esh = R(0);
+      if (herm && j == 4)
+        esh = ((const __attribute__((address_space(4))) R*)d.afda)[216] * s_ez[lane];  // alpha
+      adj_stress_cell<R, L, true>(d, in[r], i, j, esh);
+    }
+  } else {
+    if (!col) return;
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+      const int j = j0 + w * RZ + r;
+      if (j >= d.nz - 4) return;
+      adj_stress_cell<R, L, false>(d, in[r], i, j);
+    }
+  }
+}
+
+// Transposed stress half, pointwise: every forward quantity is stored per cell, so the tile
+// needs no LDS (but for formula #3's hand-over); the tile shape keeps the loads coalesced.
+// HERM: the variant for formula #3 (fs = 3); the others carry none of its code.
+template <typename R, int L, int RZ, bool HERM>
+__global__ __launch_bounds__(TX* NW) void k_adj_stress(const AdjDesc<R> d) {
+  constexpr int TZ = NW * RZ;
+  const int lane = threadIdx.x, w = threadIdx.y;
+  inject_velocity_targets(d);
   const int ntx = (d.nx - 8 + TX - 1) / TX;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int i0 = 4 + (tile % ntx) * TX, j0 = 4 + (tile / ntx) * TZ;
   const int i = i0 + lane;
   const bool col = i < d.nx - 4;
+  if (!HERM && !col) return;  // (the formula #3 variant's lanes all reach its barrier)
   AdjStressIn<R, L> in[RZ];
 #pragma unroll
   for (int r = 0; r < RZ; ++r) {
     const int j = j0 + w * RZ + r;
     if (col && j < d.nz - 4) in[r].load(d, i, j);
   }
-  // formula #3 (fs = 3): row 5's d vz/dz reads the surface row's ezz*, so the surface row's
-  // cell needs row 5's raw-rate adjoint: row 5 goes first and hands it over through LDS (the
-  // first tile row holds rows 4..4+TZ-1; the condition is uniform over the workgroup)
-  __shared__ R s_ez[TX];
-  const bool herm = d.fs == 3 && j0 == 4 && 5 < d.nz - 4;
-  if (herm) {
-#pragma unroll
-    for (int r = 0; r < RZ; ++r)
-      if (col && j0 + w * RZ + r == 5) s_ez[lane] = adj_stress_cell<R, L>(d, in[r], i, 5);
-    __syncthreads();
-  }
-  if (!col) return;
-#pragma unroll
-  for (int r = 0; r < RZ; ++r) {
-    const int j = j0 + w * RZ + r;
-    if (j >= d.nz - 4) return;
-    if (herm && j == 5) continue;
-    R esh = R(0);
-    if (herm && j == 4)
-      esh = ((const __attribute__((address_space(4))) R*)d.afda)[216] * s_ez[lane];  // alpha
-    adj_stress_cell<R, L>(d, in[r], i, j, esh);
-  }
+  adj_stress_rows<R, L, RZ, HERM>(d, in, i, j0, col);
 }
 
 // Transposed stress half, gather (v_hat -= D^T e_hat), fused with the transposed velocity

@@ -1163,15 +1163,17 @@ struct GradRun : vi_grad_s {
     const dim3 gint(ntx * ((d.nz - 8 + TZ - 1) / TZ));
     const int row0 = d.fs == 1 ? 0 : 4;  // image rows only with imaging
     const dim3 gsg(ntx * ((d.nz - 4 - row0 + TZ - 1) / TZ));
+    const bool herm = d.fs == 3;  // formula #3 has kernels of its own
+#define VI_ADJ_STRESS(l)                                                                    \
+  case l:                                                                                   \
+    if (herm) hipLaunchKernelGGL((vi::k_adj_stress<R, l, RZ, true>), gint, blk, 0, st, d);  \
+    else hipLaunchKernelGGL((vi::k_adj_stress<R, l, RZ, false>), gint, blk, 0, st, d);      \
+    break;
     switch (L) {
-      case 0: hipLaunchKernelGGL((vi::k_adj_stress<R, 0, RZ>), gint, blk, 0, st, d); break;
-      case 1: hipLaunchKernelGGL((vi::k_adj_stress<R, 1, RZ>), gint, blk, 0, st, d); break;
-      case 2: hipLaunchKernelGGL((vi::k_adj_stress<R, 2, RZ>), gint, blk, 0, st, d); break;
-      case 3: hipLaunchKernelGGL((vi::k_adj_stress<R, 3, RZ>), gint, blk, 0, st, d); break;
-      case 4: hipLaunchKernelGGL((vi::k_adj_stress<R, 4, RZ>), gint, blk, 0, st, d); break;
-      case 5: hipLaunchKernelGGL((vi::k_adj_stress<R, 5, RZ>), gint, blk, 0, st, d); break;
-      case 6: hipLaunchKernelGGL((vi::k_adj_stress<R, 6, RZ>), gint, blk, 0, st, d); break;
+      VI_ADJ_STRESS(0) VI_ADJ_STRESS(1) VI_ADJ_STRESS(2) VI_ADJ_STRESS(3) VI_ADJ_STRESS(4)
+      VI_ADJ_STRESS(5) VI_ADJ_STRESS(6)
     }
+#undef VI_ADJ_STRESS
     hipLaunchKernelGGL((vi::k_adj_vel<R, RZ>), gint, blk, 0, st, d);
     hipLaunchKernelGGL((vi::k_adj_sgather<R, RZ>), gsg, blk, 0, st, d, row0);
     VI_HIP(hipGetLastError());
