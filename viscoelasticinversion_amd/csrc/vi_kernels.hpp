@@ -162,19 +162,32 @@ __device__ __forceinline__ int sld(const int* p, int idx) {
   return ((const __attribute__((address_space(4))) int*)p)[__builtin_amdgcn_readfirstlane(idx)];
 }
 
+// d / kappa, skipped by the whole wavefront when no lane has kappa != 1 (x / 1 = x exactly, so
+// the result is the same bit for bit): the usual kappa_max = 1 strips make no division. The
+// test is a wave-uniform (scalar) branch; the empty statement in it keeps the compiler from
+// computing the division anyway and selecting (which it does for a plain if).
+template <typename R>
+__device__ __forceinline__ R div_kappa(R d, R k) {
+  if (__builtin_amdgcn_ballot_w64(k != R(1)) != 0) {
+    asm volatile("");
+    d = d / k;
+  }
+  return d;
+}
+
 // CPML on one derivative d of a strip cell: psi = b psi + a d; return d/kappa + psi.
 template <typename R>
 __device__ __forceinline__ R cpml_apply(R* psi, unsigned pb, R a, R b, R k, R d) {
   const R q = b * gld(psi, pb) + a * d;
   gst(psi, pb, q);
-  return d / k + q;
+  return div_kappa(d, k) + q;
 }
 
 // The same derivative without storing psi (what the next update would use).
 template <typename R>
 __device__ __forceinline__ R cpml_rate(const R* psi, unsigned pb, R a, R b, R k, R d) {
   const R q = b * gld(psi, pb) + a * d;
-  return d / k + q;
+  return div_kappa(d, k) + q;
 }
 
 __device__ __forceinline__ int strip_x(int i, int nx, int T, int lo, int hi) {
@@ -349,7 +362,7 @@ template <typename R>
 __device__ __forceinline__ R cpml_apply_pre(R* psi, unsigned pb, R p, R a, R b, R k, R d) {
   const R q = b * p + a * d;
   gst(psi, pb, q);
-  return d / k + q;
+  return div_kappa(d, k) + q;
 }
 
 // The first 64 entries of a tile's source and receiver lists (CSR order), copied to LDS by
