@@ -265,7 +265,7 @@ def bench_gradient(nshots=16, nt=3000, modes=(0, None)):
                        f"CPML + free surface, {nt} steps, FP32; one simulation, the source "
                        f"moved between shots", "forward_s_per_shot": round(t_fwd, 4)}
     cells = acfg["nx"] * acfg["nz"]
-    headline = None
+    headline, J_one = None, None
     for mode in modes:
         K = nt if mode is None else mode
         name = "store_every_step" if K >= nt else "sqrt_nt_checkpoints"
@@ -301,8 +301,30 @@ def bench_gradient(nshots=16, nt=3000, modes=(0, None)):
                      "misfit": J,
                      "gradient_finite": bool(all(np.isfinite(v).all() for v in total.values()))}
         headline = out[name]["gradient_over_forward"]
+        J_one = J
     sim.close()
     out["gradient_over_forward"] = headline
+    # two shots at once, each a chunked job on a simulation (HIP stream) of its own, through
+    # configs.multishot_gradient: every step recorded when both workspaces fit in 80 % of the
+    # free HBM, else sqrt(nt) checkpoints; the workspaces are allocated before the timed run
+    from viscoelasticinversion_amd.configs import multishot_gradient
+    sims = [build_simulation(adjoint_shot(acfg, s, "start"), "float32") for s in range(2)]
+    free, _ = mem_info()
+    K2 = nt if 2 * sims[0].gradient_bytes(nt, nt) <= 0.8 * free else 0
+    for k, sm in enumerate(sims):
+        sm.gradient(dobs[k], nt, K2)
+    t0 = time.perf_counter()
+    J2, g2 = multishot_gradient(acfg, dobs, "float32", "start", checkpoint_every=K2, sims=sims)
+    wall2 = time.perf_counter() - t0
+    for sm in sims:
+        sm.close()
+    out["two_jobs"] = {"checkpoint_every": K2 or "ceil(sqrt(nt))", "wall_s": round(wall2, 4),
+                       "shots_per_s": round(nshots / wall2, 3),
+                       "s_per_shot": round(wall2 / nshots, 4),
+                       "throughput_over_forward": round(wall2 / nshots / t_fwd, 3),
+                       "forward_equivalent_GPt_per_s": round(cells * nt * nshots / wall2 / 1e9, 3),
+                       "misfit": J2, "misfit_equals_one_job": J2 == J_one,
+                       "gradient_finite": bool(all(np.isfinite(v).all() for v in g2.values()))}
     return out
 
 

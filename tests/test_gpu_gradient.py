@@ -232,6 +232,24 @@ def test_workload_gradient_vp_vs_rho_q_at_random_points():
         print(f"  {key}: 20 points, worst relative difference {worst:.2e}", flush=True)
 
 
+@pytest.mark.parametrize("concurrent,K", [(2, 0), (3, 40), (5, 200)])
+def test_multishot_gradient_concurrent_jobs_equal_one_at_a_time(concurrent, K):
+    """Shots computed as concurrent gradient jobs (each on a simulation and stream of its own,
+    sources moved from shot to shot) give the one-at-a-time misfit and gradients bit for bit,
+    for any number of jobs (more jobs than shots included) and checkpoint interval."""
+    from viscoelasticinversion_amd.configs import (config_adjoint, multishot_gradient,
+                                                   observed_data)
+    print(f"[case] multishot gradient, {concurrent} concurrent jobs, K={K}", flush=True)
+    acfg = config_adjoint(nx=200, nz=100, nshots=4, nrec=41, nt=200, seed=5, smooth_cells=8)
+    dobs = observed_data(acfg, "float32")
+    J1, g1 = multishot_gradient(acfg, dobs, "float32", "start", checkpoint_every=K)
+    Jc, gc = multishot_gradient(acfg, dobs, "float32", "start", checkpoint_every=K,
+                                concurrent=concurrent, chunk=37)
+    assert Jc == J1
+    for k in g1:
+        assert np.array_equal(gc[k], g1[k]), k
+
+
 @pytest.mark.parametrize("vti,K", [(False, 8), (True, 8), (False, 70)])
 def test_chunked_gradient_job_equals_gradient(vti, K):
     """The gradient as a chunked job (begin / advance by a few steps / end) is bit-identical

@@ -293,17 +293,56 @@ def multishot_misfit(acfg, model, dobs, precision="float32"):
     return float(sum(0.5 * np.sum((a.astype(np.float64) - b) ** 2) for a, b in zip(syn, dobs)))
 
 
-def multishot_gradient(acfg, dobs, precision="float32", model="start", checkpoint_every=0):
+def multishot_gradient(acfg, dobs, precision="float32", model="start", checkpoint_every=0,
+                       concurrent=1, chunk=100, sims=None):
     """Misfit and its gradient with respect to (vp, vs, rho, Q), summed over the shots,
-    from the discrete adjoint of each shot."""
+    from the discrete adjoint of each shot.
+
+    ``concurrent`` shots are computed at once, each as a chunked gradient job on a simulation
+    (and HIP stream) of its own whose source is moved from shot to shot; ``chunk`` is the
+    number of steps a job enqueues per turn. The launches of two jobs fill each other's
+    launch gaps: at 1600 x 600, 3000 steps, two jobs gave 10-12 % more shots per second than
+    one, four fewer (measured). The result does not depend on ``concurrent``: every shot's
+    gradient is computed alone and the sums run in shot order. ``sims``: simulations of this
+    workload's model to run the jobs on instead (one job each; their sources are moved and
+    they stay open, with their gradient workspaces)."""
+    import time
     from .api import gradient_to_velocity_params
+    from .stepper import GradientJob
     m = acfg[model] if isinstance(model, str) else model
+    nshots, nt = len(acfg["src_x"]), acfg["nt"]
+    own = sims is None
+    if own:
+        n = max(1, min(int(concurrent), nshots))
+        sims = [build_simulation(adjoint_shot(acfg, s, m), precision) for s in range(n)]
+    res = [None] * nshots
+    queue, active = list(range(nshots)), {}
+    try:
+        while queue or active:
+            for k, sim in enumerate(sims):
+                if k not in active and queue:
+                    s = queue.pop(0)
+                    c = adjoint_shot(acfg, s, m)
+                    sim.set_sources(c["src_ix"], c["src_iz"], c["src_kind"], c["wavelets"])
+                    active[k] = (s, GradientJob(sim, dobs[s], nt, checkpoint_every))
+            for k in list(active):
+                s, job = active[k]
+                if not job.finished:
+                    job.advance(chunk)
+                elif job.idle():
+                    res[s] = job.end()
+                    del active[k]
+            if active:
+                time.sleep(2e-4)  # the jobs' work is enqueued; wait for the device
+    finally:
+        for _, job in active.values():
+            job.cancel()
+        if own:
+            for sim in sims:
+                sim.close()
     J = 0.0
     total = None
-    for s in range(len(acfg["src_x"])):
-        sim = build_simulation(adjoint_shot(acfg, s, m), precision)
-        Js, g = sim.gradient(dobs[s], acfg["nt"], checkpoint_every)
-        sim.close()
+    for Js, g in res:
         gv = gradient_to_velocity_params(g, m["rho"], m["lam"], m["mu"], m["tau"], m["Q"])
         J += Js
         total = gv if total is None else {k: total[k] + gv[k] for k in gv}
