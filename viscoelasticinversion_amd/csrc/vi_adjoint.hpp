@@ -120,9 +120,9 @@ struct AdjStressIn {
       azv = d.az[j]; bzv = d.bz[j]; kzv = d.kz[j];
       p4h = gld((const R*)d.apsi[4], qz); p5h = gld((const R*)d.apsi[5], qz);
     }
-    exx = gld(d.fexx, cb);
-    ezz = gld(d.fezz, cb);
-    gxz = gld(d.fgxz, cb);
+    exx = gld_nt(d.fexx, cb);
+    ezz = gld_nt(d.fezz, cb);
+    gxz = gld_nt(d.fgxz, cb);
     lr = gld(d.lam, mb);
     mr = gld(d.mu, mb);
     mu1 = gld(d.mu, mb + E);
@@ -132,9 +132,9 @@ struct AdjStressIn {
     c33v = d.vti ? gld(d.c33, mb) : R(0);
     t = L > 0 ? gld(d.tau, mb) : R(0);
     if (L > 0) {
-      SMxx = gld(d.fM, cb);
-      SMzz = gld(d.fM + d.plane, cb);
-      SMxz = gld(d.fM + 2 * d.plane, cb);
+      SMxx = gld_nt(d.fM, cb);
+      SMzz = gld_nt(d.fM + d.plane, cb);
+      SMxz = gld_nt(d.fM + 2 * d.plane, cb);
     }
     sxxh = gld((const R*)d.asxx, cb);
     szz_c = gld((const R*)d.aszz, cb);
@@ -325,8 +325,8 @@ struct AdjVelIn {
       azv = d.az[j]; bzv = d.bz[j]; kzv = d.kz[j];
       p6h = gld((const R*)d.apsi[6], qz); p7h = gld((const R*)d.apsi[7], qz);
     }
-    fx = gld(d.ffx, cb);
-    fz = gld(d.ffz, cb);
+    fx = gld_nt(d.ffx, cb);
+    fz = gld_nt(d.ffz, cb);
     rho = gld(d.rho, mb);
     rho1 = gld(d.rho, mb + E);
     rho2 = gld(d.rho, mb + d.ldm * E);

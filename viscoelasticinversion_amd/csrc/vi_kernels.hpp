@@ -156,6 +156,20 @@ __device__ __forceinline__ void gst(T* base, unsigned boff, T v) {
   *(__attribute__((address_space(1))) T*)((__attribute__((address_space(1))) char*)base + boff) = v;
 }
 
+// The same with the non-temporal hint, for data touched once per step whose next use is far
+// away (the per-step slots of the gradient), so that streaming it does not evict the working
+// set that the next launches read again from the Infinity Cache.
+template <typename T>
+__device__ __forceinline__ T gld_nt(const T* base, unsigned boff) {
+  return __builtin_nontemporal_load((const __attribute__((address_space(1))) T*)(
+      (const __attribute__((address_space(1))) char*)base + boff));
+}
+template <typename T>
+__device__ __forceinline__ void gst_nt(T* base, unsigned boff, T v) {
+  __builtin_nontemporal_store(
+      v, (__attribute__((address_space(1))) T*)((__attribute__((address_space(1))) char*)base + boff));
+}
+
 // Load of a wave-uniform index through the scalar data cache (read-only data): waiting for it
 // counts on lgkmcnt only, so it does not drain the vector loads already in flight.
 __device__ __forceinline__ int sld(const int* p, int idx) {
@@ -571,8 +585,8 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
       gst(vx, cb, nvx);
       gst(vz, cb, nvz);
       if constexpr (REC) {
-        gst(rec_plane<R>(rs, 0), cb, fx);
-        gst(rec_plane<R>(rs, 1), cb, fz);
+        gst_nt(rec_plane<R>(rs, 0), cb, fx);
+        gst_nt(rec_plane<R>(rs, 1), cb, fz);
       }
     }
   }
@@ -758,9 +772,9 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void
     }
     const R gxz = ezx + exz;
     if constexpr (REC) {
-      gst(rec_plane<R>(rs, 2), cb, exx);
-      gst(rec_plane<R>(rs, 3), cb, ezz);
-      gst(rec_plane<R>(rs, 4), cb, gxz);
+      gst_nt(rec_plane<R>(rs, 2), cb, exx);
+      gst_nt(rec_plane<R>(rs, 3), cb, ezz);
+      gst_nt(rec_plane<R>(rs, 4), cb, gxz);
     }
     const R lr = olam[h];
     const R mxz = omxz[h];
@@ -782,9 +796,9 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void
         SMxz = SMxz + om[2 * L + k][h];
       }
       if constexpr (REC) {
-        gst(rec_plane<R>(rs, 5), cb, SMxx);
-        gst(rec_plane<R>(rs, 6), cb, SMzz);
-        gst(rec_plane<R>(rs, 7), cb, SMxz);
+        gst_nt(rec_plane<R>(rs, 5), cb, SMxx);
+        gst_nt(rec_plane<R>(rs, 6), cb, SMzz);
+        gst_nt(rec_plane<R>(rs, 7), cb, SMxz);
       }
       if (fs_row) ezz = -(((l0 * exx) + l1 * SMxx) + pz1 * SMzz) / pz0;
       dsxx = px0 * exx + l0 * ezz + px1 * SMxx + l1 * SMzz;
