@@ -233,13 +233,15 @@ def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True, cache_bloc
                 "latency_p99_s": round(float(np.percentile(done, 99)), 4)}}
 
 
-def bench_gradient(nshots=16, nt=3000, modes=(0, None)):
+def bench_gradient(nshots=16, nt=3000, modes=(0, "quarter", None)):
     """Adjoint gradient workload: 1600 x 600, L=3, CPML + free surface, ``nshots`` shots
     with 301 receivers, ``nt`` steps, FP32. Observed data come from the true model. One
     simulation (the starting model) serves every shot, its source moved between shots, so
     its gradient workspace is reused. Each mode is a checkpoint interval: 0 = ceil(sqrt(nt))
-    (checkpoints, segments recomputed), None = nt (every step recorded in the forward pass,
-    nothing recomputed; run when its memory is below half the free HBM)."""
+    (checkpoints, segments recomputed), "quarter" = the largest interval whose gradient needs
+    at most a quarter of the free HBM (fewer, longer segments: about nt - K steps recomputed),
+    None = nt (every step recorded in the forward pass, nothing recomputed; run when its
+    memory is below half the free HBM)."""
     from viscoelasticinversion_amd.api import gradient_to_velocity_params
     from viscoelasticinversion_amd.batch import mem_info
     from viscoelasticinversion_amd.configs import (adjoint_shot, build_simulation,
@@ -267,8 +269,14 @@ def bench_gradient(nshots=16, nt=3000, modes=(0, None)):
     cells = acfg["nx"] * acfg["nz"]
     headline, J_one = None, None
     for mode in modes:
-        K = nt if mode is None else mode
-        name = "store_every_step" if K >= nt else "sqrt_nt_checkpoints"
+        if mode == "quarter":  # the largest interval within a quarter of the free HBM
+            K = sim.checkpoint_for_memory(nt, 0.25 * mem_info()[0])
+            if K in (0, nt):
+                continue  # the same as one of the other modes
+            name = "largest_interval_in_quarter_hbm"
+        else:
+            K = nt if mode is None else mode
+            name = "store_every_step" if K >= nt else "sqrt_nt_checkpoints"
         need = sim.gradient_bytes(nt, K)
         free, _ = mem_info()
         if K >= nt and need > 0.5 * free:
@@ -467,7 +475,7 @@ def main():
             print(json.dumps({"serving": bench_serving(args.serving_jobs, r)}), flush=True)
         return
     if args.only_gradient:
-        modes = (0, None) if args.checkpoint_every < 0 else (args.checkpoint_every,)
+        modes = (0, "quarter", None) if args.checkpoint_every < 0 else (args.checkpoint_every,)
         print(json.dumps({"gradient": bench_gradient(args.gradient_shots, modes=modes)}),
               flush=True)
         return

@@ -92,13 +92,18 @@ def stability_error(job):
 
 
 def gradient_checkpoint(job, budget, precision="float32", fraction=0.25):
-    """Checkpoint interval of a gradient job: nt (every step recorded in the forward pass,
-    nothing recomputed, ~25 % faster) when that plan needs at most ``fraction`` of the HBM
-    budget, else the job's own (0 = ceil(sqrt(nt))). The gradient does not depend on it."""
+    """Checkpoint interval of a gradient job that does not set one: the largest interval
+    whose plan needs at most ``fraction`` of the HBM budget (nt when every step can be
+    recorded in the forward pass: nothing recomputed, ~25 % faster than ceil(sqrt(nt))
+    checkpoints; fewer, longer segments otherwise: less to recompute), else 0
+    (ceil(sqrt(nt)) checkpoints). The gradient does not depend on it."""
+    from .stepper import largest_checkpoint_interval
     if job.kind != "gradient" or job.checkpoint_every:
         return job.checkpoint_every
-    full = job_bytes(Job(job.id, job.kind, job.cfg, checkpoint_every=job.cfg["nt"]), precision)
-    return job.cfg["nt"] if full <= fraction * budget else 0
+    nt = job.cfg["nt"]
+    return largest_checkpoint_interval(
+        nt, lambda K: job_bytes(Job(job.id, job.kind, job.cfg, checkpoint_every=K), precision),
+        fraction * budget)
 
 
 def _with_checkpoint(job, K):

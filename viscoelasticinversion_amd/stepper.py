@@ -200,6 +200,12 @@ class Simulation:
         L.check(L.lib().vi_gradient_bytes(self._h, int(nt), int(checkpoint_every), ctypes.byref(b)))
         return b.value
 
+    def checkpoint_for_memory(self, nt, max_bytes):
+        """The largest checkpoint interval whose gradient of ``nt`` steps needs at most
+        ``max_bytes`` of device memory (0 = ceil(sqrt(nt)) when none does); see
+        ``largest_checkpoint_interval``."""
+        return largest_checkpoint_interval(nt, lambda K: self.gradient_bytes(nt, K), max_bytes)
+
     def set_relaxation(self, taun):
         t = np.ascontiguousarray(np.asarray(taun, dtype=np.float64).ravel())
         if t.size != self.nrelax:
@@ -444,6 +450,21 @@ class GradientJob:
         if self._h:
             h, self._h = self._h, None
             L.check(L.lib().vi_gradient_cancel(h))
+
+
+def largest_checkpoint_interval(nt, bytes_of, max_bytes):
+    """The largest checkpoint interval K whose gradient needs at most ``max_bytes`` according
+    to ``bytes_of(K)``, or 0 (= ceil(sqrt(nt)) checkpoints) when none of the intervals
+    ceil(nt / s), s = 1 .. ceil(sqrt(nt)), fits. A gradient with interval K recomputes
+    about nt - K forward steps (every segment but the last), so the largest interval that fits
+    is the fastest; the gradient itself does not depend on K."""
+    nt = int(nt)
+    smax = int(np.ceil(np.sqrt(nt)))
+    for s in range(1, smax + 1):
+        K = -(-nt // s)
+        if bytes_of(K) <= max_bytes:
+            return K
+    return 0
 
 
 def plan_bytes(nx, nz, precision="float32", nrelax=0, vti=False, cpml_thickness=0, nsrc=0,
