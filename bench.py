@@ -122,7 +122,7 @@ def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True, cache_bloc
     """Batched multi-shot forward (uneven windows, one launch per half step for all shots)
     against the plain single-shot stepper run serially on the same GPU, same work."""
     from viscoelasticinversion_amd import Batch
-    from viscoelasticinversion_amd.configs import build_multishot, config_multishot
+    from viscoelasticinversion_amd.configs import build_multishot, build_simulation, config_multishot
     mcfg = config_multishot(nshots=nshots, nt=nt)
     model, sims = build_multishot(mcfg, "float32")
     cells = sum(sh["nx"] for sh in mcfg["shots"]) * mcfg["nz"]
@@ -145,6 +145,10 @@ def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True, cache_bloc
     b.sync()
     wall_b = time.perf_counter() - t0
     finite = all(bool(np.isfinite(sim.traces(nt)).all()) for sim in sims[:4])
+    # error against FP64: four shots of the batch, each run alone in FP64
+    from viscoelasticinversion_amd.configs import shot_config
+    picks = [0, nshots // 3, (2 * nshots) // 3, nshots - 1]
+    f32 = {s: sims[s].traces(nt).astype(np.float64) for s in picks}
     # per-kernel durations of the batched launch (events recorded by the launches), with the
     # same launch groups run one at a time on one stream (each at most half the cache block,
     # as on two streams), so no launch shares the GPU with another and the per-kernel time
@@ -201,6 +205,13 @@ def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True, cache_bloc
     for sim in sims:
         sim.close()
     model.close()
+    fp64 = {}
+    for s_ in picks:
+        ref = build_simulation(shot_config(mcfg, s_), "float64")
+        ref.step(nt)
+        t64 = ref.traces(nt).astype(np.float64)
+        ref.close()
+        fp64[str(s_)] = float(f"{np.linalg.norm(f32[s_] - t64) / np.linalg.norm(t64):.3e}")
     # kernel time per step (one launch per group of shots and half step)
     step_s = ms_s / prof_steps * 1e-3
     step_v = ms_v / prof_steps * 1e-3
@@ -223,7 +234,9 @@ def bench_multishot(nshots, nt, graph_steps=100, rows=0, serial=True, cache_bloc
                         "k_velocity_us_per_step": round(step_v * 1e6, 2),
                         "k_velocity_hbm_frac": round(bytes_v / step_v / 1e9 / HBM_PEAK_GBS, 4),
                         "autotune_us_per_step": {str(k): round(v, 1) for k, v in table.items()},
-                        "traces_finite": finite},
+                        "traces_finite": finite,
+                        "fp32_vs_fp64_traces_relative_l2": {
+                            "shots": fp64, "reference": "the same shot run alone in FP64"}},
             "lockstep_dram_bound": lockstep,
             "serial_plain_stepper": None if not serial else {
                 "wall_s": round(wall_s, 4),
@@ -312,6 +325,24 @@ def bench_gradient(nshots=16, nt=3000, modes=(0, "quarter", None)):
         J_one = J
     sim.close()
     out["gradient_over_forward"] = headline
+    # error against FP64: shot 0's gradient (sqrt(nt) checkpoints) in FP32 and in FP64, both
+    # with respect to (vp, vs, rho, Q), and the misfit
+    errs = {}
+    gv_by = {}
+    for prec in ("float32", "float64"):
+        d0 = observed_data(dict(acfg, src_x=acfg["src_x"][:1]), prec)[0]
+        sm = build_simulation(adjoint_shot(acfg, 0, "start"), prec)
+        Jp, gp = sm.gradient(d0, nt, 0)
+        sm.close()
+        gv_by[prec] = (Jp, gradient_to_velocity_params(
+            {k: v.astype(np.float64) for k, v in gp.items()}, st["rho"], st["lam"], st["mu"],
+            st["tau"], st["Q"]))
+    (J32, g32), (J64, g64) = gv_by["float32"], gv_by["float64"]
+    for k in g64:
+        errs[k] = float(f"{np.linalg.norm(g32[k] - g64[k]) / np.linalg.norm(g64[k]):.3e}")
+    errs["misfit"] = float(f"{abs(J32 - J64) / abs(J64):.3e}")
+    out["fp32_vs_fp64_shot0"] = {"relative_l2": errs, "reference": "the same gradient in FP64 "
+                                 "(observed data and model in FP64)"}
     # two shots at once, each a chunked job on a simulation (HIP stream) of its own, through
     # configs.multishot_gradient: every step recorded when both workspaces fit in 80 % of the
     # free HBM, else sqrt(nt) checkpoints; the workspaces are allocated before the timed run
