@@ -39,7 +39,8 @@ import ViscoelasticInversion: make_problem, solve_problem, stress_eq!, velocity_
 import ChainRulesCore
 
 export MI355XBackend, Sim, Fields, StepParameters, leapfrog_step!, trace_misfit,
-       misfit_gradient, get_traces, set_receivers!, reset!
+       misfit_gradient, gradient_bytes, checkpoint_for_memory, get_traces, set_receivers!,
+       reset!
 
 const LIB = get(ENV, "VI_MI355X_LIB",
                 joinpath(@__DIR__, "..", "viscoelasticinversion_amd", "libvi_mi355x.so"))
@@ -325,6 +326,34 @@ function misfit_gradient(s::Sim{T}, dobs::AbstractMatrix; checkpoint_every=0) wh
                 (Ptr{Cvoid}, Cint, P, Cint, Ref{Cdouble}, P, P, P, P),
                 s.h, nt, d, checkpoint_every, J, g..., pτ))
     return J[], (ρ=g[1], λ=g[2], μ=g[3], τ=gτ)
+end
+
+"""
+    gradient_bytes(s, nt; checkpoint_every=0) -> Int
+
+Temporary device bytes of one `misfit_gradient` call of `nt` steps (vi_gradient_bytes).
+"""
+function gradient_bytes(s::Sim, nt::Integer; checkpoint_every::Integer=0)
+    b = Ref{Csize_t}(0)
+    check(ccall((:vi_gradient_bytes, LIB), Cint, (Ptr{Cvoid}, Cint, Cint, Ref{Csize_t}),
+                s.h, nt, checkpoint_every, b))
+    return Int(b[])
+end
+
+"""
+    checkpoint_for_memory(s, nt, max_bytes) -> Int
+
+The largest checkpoint interval K = cld(nt, k), k = 1 … ceil(√nt), whose gradient needs at most
+`max_bytes` of device memory, or 0 (checkpoints every ceil(√nt) steps) when none fits. A
+gradient recomputes about nt - K forward steps, so the largest interval that fits is the
+fastest; the gradient does not depend on K.
+"""
+function checkpoint_for_memory(s::Sim, nt::Integer, max_bytes::Integer)
+    for k in 1:ceil(Int, sqrt(nt))
+        K = cld(nt, k)
+        gradient_bytes(s, nt; checkpoint_every=K) <= max_bytes && return K
+    end
+    return 0
 end
 
 # ---- the reference's solver API on the device -----------------------------------------
