@@ -170,6 +170,24 @@ def test_gradient_checkpoint_interval_invariant():
             assert np.array_equal(g[k], res[0][1][k]), k
 
 
+def test_gradient_independent_of_tiling():
+    """The gradient does not depend on the handle's tiling (the recording launches use tiles
+    of at most two rows per wavefront), and the handle keeps its tiling afterwards."""
+    cfg, dobs = _setup(1, "afda", True, seed=9, nt=60)
+    print("[case] gradient tilings", flush=True)
+    res = []
+    for rows in (1, 2, 4, 8):
+        sim = _sim(cfg, cfg["model"])
+        sim.set_tiling(rows)
+        res.append(sim.gradient(dobs, 60, checkpoint_every=9))
+        assert sim.tiling == rows
+        sim.close()
+    for J, g in res[1:]:
+        assert J == res[0][0]
+        for k in g:
+            assert np.array_equal(g[k], res[0][1][k]), k
+
+
 def test_gradient_fp32_close_to_fp64():
     cfg, dobs = _setup(3, True, True, seed=7)
     nt = cfg["nt"]

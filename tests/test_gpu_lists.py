@@ -45,6 +45,27 @@ def test_dense_lists_every_tiling(rows):
     assert np.abs(ref["traces"]).max() > 0
 
 
+def test_retiling_between_steps_keeps_the_run():
+    """Changing the tiling between steps rebuilds the per-tile lists; the state, the step
+    index and the traces carry on."""
+    from viscoelasticinversion_amd.configs import build_simulation
+    cfg = _dense_cfg(nt=30, seed=8)
+    print("[case] re-tiling between steps", flush=True)
+    ref = run_oracle(cfg, np.float32)
+    sim = build_simulation(cfg, "float32")
+    done = 0
+    for rows, n in ((8, 6), (1, 5), (4, 9), (2, 10)):
+        sim.set_tiling(rows)
+        assert sim.tiling == rows
+        sim.step(n)
+        done += n
+    assert done == 30 and sim.step_index == 30
+    out = {k: sim.get(k) for k in ("vx", "vz", "sxx", "szz", "sxz", "mxx", "mzz", "mxz")}
+    out["traces"] = sim.traces(30)
+    sim.close()
+    compare(out, {k: ref[k] for k in out}, np.float32, "re-tiling")
+
+
 def test_dense_lists_fp64_short_wavelets():
     cfg = _dense_cfg(nt=26, seed=5, short_wavelets=True)
     print("[case] dense lists FP64, wavelets shorter than the run", flush=True)
