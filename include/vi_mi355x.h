@@ -294,14 +294,15 @@ int vi_batch_set(vi_batch b, const vi_handle* hs, int n);
 int vi_batch_set_tiling(vi_batch b, int rows_per_wave);
 int vi_batch_get_tiling(vi_batch b);
 /* cache blocking over shots (no counterpart in the reference): the shots of a precision/L
- * class are cut into consecutive groups whose working set (state, memory variables, CPML
- * memory, model columns) is at most `bytes`, and each group runs all steps of a block (a
+ * class are packed (first fit by decreasing working set) into groups whose working set
+ * (state, memory variables, CPML memory, model columns) is at most `bytes` per stream share
+ * (bytes / n streams), and each group runs all steps of a block (a
  * graph block, or the steps of one vi_batch_step call) before the next starts, so its data
  * stays in the Infinity Cache. Results are unchanged. 0 = every shot in every launch; the
  * default is 224 MB (of the 256 MB Infinity Cache). */
 int vi_batch_set_cache_block(vi_batch b, size_t bytes);
 size_t vi_batch_get_cache_block(vi_batch b);
-/* n = 1..4 streams (default 2): consecutive launch groups take the streams in turn, so n
+/* n = 1..4 streams (default 2): the launch groups take the streams in turn, so n
  * groups run at once (each at most 1/n of the cache block), overlapping each other's launch
  * ramp and drain; 1: one group at a time. Results are unchanged. */
 int vi_batch_set_streams(vi_batch b, int n);

@@ -707,28 +707,40 @@ size_t working_set_bytes(vi_sim_s* h) {
 int batch_build(vi_batch_s* b) {
   invalidate_graph(b);
   b->groups.clear();
-  // one launch group per (precision, L) class; with cache blocking, each class is cut into
-  // consecutive groups of at most block_bytes of working set (at least one shot each), and
-  // every group runs all steps of a block before the next starts, so a group's data stays in
-  // the Infinity Cache across its launches (shots are independent, nothing else changes)
-  std::map<std::pair<int, int>, size_t> gi;  // class -> its current (last) group
-  std::vector<size_t> filled;
-  // n groups run at once on n streams: each gets 1/n of the block
-  const size_t limit = b->block_bytes / (size_t)b->nstreams;
+  // one launch group per (precision, L) class; with cache blocking, each class is packed into
+  // groups of at most block_bytes of working set (at least one shot each), and every group
+  // runs all steps of a block before the next starts, so a group's data stays in the
+  // Infinity Cache across its launches (shots are independent, nothing else changes). The
+  // packing is first fit by decreasing working set: fewer, fuller groups than cutting the
+  // member list in order, so fewer launches per step.
+  std::map<std::pair<int, int>, std::vector<vi_sim_s*>> classes;
   for (auto* h : b->members) {
     VI_TRY(check_ready(h));
     if (h->rz != b->rz) VI_TRY(apply_tiling(h, b->rz));
-    auto key = std::make_pair((int)h->esz, h->L);
-    const size_t ws = working_set_bytes(h);
-    if (!gi.count(key) || (limit && filled[gi[key]] > 0 && filled[gi[key]] + ws > limit)) {
-      gi[key] = b->groups.size();
-      b->groups.emplace_back(new BatchGroup());
-      b->groups.back()->esz = (int)h->esz;
-      b->groups.back()->L = h->L;
-      filled.push_back(0);
+    classes[std::make_pair((int)h->esz, h->L)].push_back(h);
+  }
+  // n groups run at once on n streams: each gets 1/n of the block
+  const size_t limit = b->block_bytes / (size_t)b->nstreams;
+  for (auto& cl : classes) {
+    std::vector<std::pair<size_t, vi_sim_s*>> order;  // (working set, shot), largest first
+    for (auto* h : cl.second) order.emplace_back(working_set_bytes(h), h);
+    if (limit)
+      std::stable_sort(order.begin(), order.end(),
+                       [](const auto& x, const auto& y) { return x.first > y.first; });
+    const size_t first = b->groups.size();
+    std::vector<size_t> filled;
+    for (const auto& o : order) {
+      size_t q = 0;
+      while (q < filled.size() && limit && filled[q] + o.first > limit) ++q;
+      if (q == filled.size()) {
+        b->groups.emplace_back(new BatchGroup());
+        b->groups.back()->esz = cl.first.first;
+        b->groups.back()->L = cl.first.second;
+        filled.push_back(0);
+      }
+      b->groups[first + q]->members.push_back(o.second);
+      filled[q] += o.first;
     }
-    b->groups[gi[key]]->members.push_back(h);
-    filled[gi[key]] += ws;
   }
   for (auto& gp : b->groups) {
     BatchGroup& g = *gp;
