@@ -129,8 +129,17 @@ int vi_cpml_profile(int n, const vi_cpml_desc* c, double dt, int axis /*0=x,1=z*
                     int precision, double* a, double* b, double* kappa);
 /* number of visible HIP devices (0 when none); never fails */
 int vi_device_count(void);
-/* free and total device memory of the current device */
+/* free and total device memory of the current device; free includes the buffers the library
+ * keeps for reuse (below), which it frees when an allocation needs the memory */
 int vi_mem_info(size_t* free_bytes, size_t* total_bytes);
+/* reuse of large device buffers (no counterpart in the reference). Buffers of 64 MB or more
+ * that a handle releases (vi_destroy, vi_trim, a gradient workspace that is replaced) are
+ * kept, once their device is idle, for the next allocation on that device that they fit (at
+ * most 1/4 larger than asked), up to a third of the device memory; runs of jobs with fresh
+ * handles then do not hand tens of GB back to the runtime and ask for them again. An
+ * allocation that fails frees the kept buffers of its device and is tried again. */
+int vi_cached_bytes(size_t* bytes);  /* bytes kept on the current device */
+int vi_release_cached(void);         /* free them */
 
 /* ---- device-resident models (no counterpart in the reference: IsotropicModel arrays are
  * re-created per Parameters). A model is uploaded once and shared by any number of shots,

@@ -1,0 +1,90 @@
+"""Reuse of large device buffers: buffers of 64 MB or more that a simulation releases are
+kept by the library and handed to the next allocation they fit. A gradient that runs in a
+workspace left dirty by another job must equal the same gradient in fresh allocations, bit
+for bit, and the kept bytes must be visible (and freeable) through the memory queries."""
+import numpy as np
+import pytest
+
+from helpers import small_cfg
+
+pytestmark = pytest.mark.gpu
+
+NT = 120
+
+
+def _case(seed):
+    # 400 x 300 FP32 with every step recorded: the slot buffer is 120 steps x 8 planes x
+    # ~0.5 MB (~0.5 GB), well above the 64 MB threshold
+    cfg = small_cfg(nx=400, nz=300, nrelax=2, seed=seed, nt=NT)
+    cfg["rec_ix"] = np.arange(20, 380, 9).astype(np.int32)
+    cfg["rec_iz"] = np.full(len(cfg["rec_ix"]), 12, np.int32)
+    cfg["rec_comp"] = (np.arange(len(cfg["rec_ix"])) % 5).astype(np.int32)
+    return cfg
+
+
+def _gradient(cfg, dobs):
+    from viscoelasticinversion_amd.configs import build_simulation
+    sim = build_simulation(cfg, "float32")
+    try:
+        return sim.gradient(dobs, NT, NT)
+    finally:
+        sim.close()
+
+
+def _observed(cfg):
+    from viscoelasticinversion_amd.configs import build_simulation
+    c = dict(cfg)
+    c["model"] = {k: v * 1.02 for k, v in cfg["model"].items()}
+    sim = build_simulation(c, "float32")
+    sim.step(NT)
+    d = sim.traces(NT)
+    sim.close()
+    return d
+
+
+def test_gradient_in_a_reused_dirty_workspace_equals_fresh():
+    from viscoelasticinversion_amd import cached_bytes, mem_info, release_cached
+    print("[case] gradient in reused buffers", flush=True)
+    a, b = _case(11), _case(12)
+    da, db = _observed(a), _observed(b)
+    release_cached()
+    assert cached_bytes() == 0
+    J_fresh, g_fresh = _gradient(b, db)  # fresh allocations
+    kept = cached_bytes()
+    assert kept >= 64 << 20
+    free_with, _ = mem_info()
+    release_cached()
+    free_without, _ = mem_info()
+    assert cached_bytes() == 0
+    assert abs(free_with - free_without) < (64 << 20)  # kept bytes count as free
+    _gradient(b, db)
+    _gradient(a, da)  # another job writes its own values into the kept buffers
+    assert cached_bytes() >= 64 << 20
+    J, g = _gradient(b, db)  # runs in the buffers the other job left behind
+    assert J == J_fresh
+    for k in g_fresh:
+        assert np.array_equal(g[k], g_fresh[k]), k
+    release_cached()
+    assert cached_bytes() == 0
+
+
+def test_forward_after_reuse_equals_fresh():
+    from viscoelasticinversion_amd import cached_bytes, release_cached
+    from viscoelasticinversion_amd.configs import build_simulation
+    print("[case] forward in reused buffers", flush=True)
+    cfg = small_cfg(nx=2100, nz=1100, nrelax=3, seed=13, nt=30)  # ~9 MB planes, 64 MB M
+    release_cached()
+    outs = []
+    for rep in range(2):
+        _gradient(_case(14), _observed(_case(14)))  # leaves dirty kept buffers
+        sim = build_simulation(cfg, "float32")
+        sim.step(30)
+        outs.append({k: sim.get(k) for k in ("vx", "vz", "sxx", "szz", "sxz", "mxx")})
+        outs[-1]["traces"] = sim.traces(30)
+        sim.close()
+        if rep == 0:
+            release_cached()
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+    release_cached()
+    assert cached_bytes() == 0

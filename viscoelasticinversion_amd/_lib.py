@@ -59,6 +59,8 @@ SIGNATURES = {
     "vi_cpml_profile": (_I, [_I, ctypes.POINTER(CpmlDesc), _D, _I, _I, _DP, _DP, _DP]),
     "vi_device_count": (_I, []),
     "vi_mem_info": (_I, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "vi_cached_bytes": (_I, [ctypes.POINTER(ctypes.c_size_t)]),
+    "vi_release_cached": (_I, []),
     "vi_model_create": (_I, [_I, _I, _I, _V, _V, _V, _V, ctypes.POINTER(_H)]),
     "vi_model_create_vti": (_I, [_I, _I, _I, _V, _V, _V, _V, _V, _V, ctypes.POINTER(_H)]),
     "vi_model_destroy": (_I, [_H]),

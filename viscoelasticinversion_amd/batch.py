@@ -148,7 +148,21 @@ class Batch:
 
 
 def mem_info():
-    """(free, total) device bytes."""
+    """(free, total) device bytes; free includes the large buffers the library keeps for
+    reuse (freed when an allocation needs the memory)."""
     f, t = ctypes.c_size_t(), ctypes.c_size_t()
     L.check(L.lib().vi_mem_info(ctypes.byref(f), ctypes.byref(t)))
     return f.value, t.value
+
+
+def cached_bytes():
+    """Device bytes the library keeps for reuse: large buffers released by simulations,
+    handed to the next allocation they fit instead of back to the runtime."""
+    b = ctypes.c_size_t()
+    L.check(L.lib().vi_cached_bytes(ctypes.byref(b)))
+    return b.value
+
+
+def release_cached():
+    """Free the buffers the library keeps for reuse (for other users of the device)."""
+    L.check(L.lib().vi_release_cached())

@@ -98,22 +98,144 @@ struct HostBuf {
   int reserve(size_t nbytes);
 };
 
+// Large device buffers released by any handle are kept for the next large allocation on the
+// same device that they fit (at most a quarter larger than asked), instead of going back to
+// the runtime. Jobs run one after another with fresh handles (the serving loop, gradients
+// shot by shot) otherwise hand tens of GB back and forth, and such a hipMalloc or hipFree
+// sporadically took 0.1-1.5 s on MI355X (50-60 GB gradient workspaces). A kept buffer is
+// idle: its device is synchronised when the buffer is released, as hipFree would. At most a
+// third of the device's memory is kept (the oldest buffers are freed first); when an
+// allocation fails, every kept buffer of that device is freed and the allocation is tried
+// once more. vi_mem_info counts kept buffers as free; vi_release_cached frees them.
+namespace spare {
+constexpr size_t MIN_BYTES = size_t(64) << 20;
+struct Block {
+  int device;
+  size_t cap;
+  void* p;
+};
+struct Cache {
+  std::mutex mu;
+  std::vector<Block> blocks;  // oldest first
+};
+// never destroyed: nothing is handed back to the runtime during process exit
+Cache& cache() {
+  static Cache* c = new Cache;
+  return *c;
+}
+
+size_t bytes_on(int device) {
+  Cache& c = cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  size_t t = 0;
+  for (const Block& b : c.blocks)
+    if (b.device == device) t += b.cap;
+  return t;
+}
+
+// The smallest kept buffer of `device` that holds n bytes and is at most n + n/4 (false: none).
+bool take(int device, size_t n, void** p, size_t* cap) {
+  Cache& c = cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  size_t best = c.blocks.size();
+  for (size_t q = 0; q < c.blocks.size(); ++q) {
+    const Block& b = c.blocks[q];
+    if (b.device == device && b.cap >= n && b.cap - n <= n / 4 &&
+        (best == c.blocks.size() || b.cap < c.blocks[best].cap))
+      best = q;
+  }
+  if (best == c.blocks.size()) return false;
+  *p = c.blocks[best].p;
+  *cap = c.blocks[best].cap;
+  c.blocks.erase(c.blocks.begin() + (long)best);
+  return true;
+}
+
+// Free every kept buffer of `device`; true when there was one.
+bool drop(int device) {
+  Cache& c = cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  bool any = false;
+  for (size_t q = 0; q < c.blocks.size();) {
+    if (c.blocks[q].device == device) {
+      (void)hipFree(c.blocks[q].p);
+      c.blocks.erase(c.blocks.begin() + (long)q);
+      any = true;
+    } else {
+      ++q;
+    }
+  }
+  return any;
+}
+
+// Release a buffer: small ones to the runtime, large ones to the cache once the device is idle.
+void give(int device, size_t cap, void* p) {
+  if (cap < MIN_BYTES) {
+    (void)hipFree(p);
+    return;
+  }
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  if (cur != device) (void)hipSetDevice(device);
+  const bool idle = hipDeviceSynchronize() == hipSuccess;
+  size_t fr = 0, total = 0;
+  (void)hipMemGetInfo(&fr, &total);
+  if (cur != device) (void)hipSetDevice(cur);
+  if (!idle) {  // a failed device: nothing is kept
+    (void)hipFree(p);
+    return;
+  }
+  Cache& c = cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  c.blocks.push_back({device, cap, p});
+  size_t kept = 0;
+  for (const Block& b : c.blocks)
+    if (b.device == device) kept += b.cap;
+  for (size_t q = 0; q < c.blocks.size() && kept > total / 3;) {
+    if (c.blocks[q].device == device) {
+      kept -= c.blocks[q].cap;
+      (void)hipFree(c.blocks[q].p);
+      c.blocks.erase(c.blocks.begin() + (long)q);
+    } else {
+      ++q;
+    }
+  }
+}
+}  // namespace spare
+
 struct DevBuf {
   void* p = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;  // as asked
+  size_t cap = 0;    // as allocated (a reused buffer may be larger)
+  int device = 0;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) spare::give(device, cap, p);
     p = nullptr;
-    bytes = 0;
+    bytes = cap = 0;
+  }
+  // p = a kept buffer of the current device, else a new allocation (retried once after
+  // freeing the kept buffers when the device is out of memory)
+  int obtain(size_t nbytes) {
+    VI_HIP(hipGetDevice(&device));
+    if (nbytes >= spare::MIN_BYTES && spare::take(device, nbytes, &p, &cap)) return 0;
+    hipError_t e = hipMalloc(&p, nbytes);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      if (spare::drop(device)) e = hipMalloc(&p, nbytes);
+    }
+    if (e != hipSuccess) p = nullptr;
+    VI_HIP(e);
+    cap = nbytes;
+    return 0;
   }
   int alloc(size_t nbytes, hipStream_t s) {
     release();
     if (!nbytes) return 0;
-    VI_HIP(hipMalloc(&p, nbytes));
+    VI_TRY(obtain(nbytes));
     bytes = nbytes;
     VI_HIP(hipMemsetAsync(p, 0, nbytes, s));
     return 0;
@@ -123,7 +245,7 @@ struct DevBuf {
     if (nbytes <= bytes && p) return 0;
     release();
     if (!nbytes) return 0;
-    VI_HIP(hipMalloc(&p, nbytes));
+    VI_TRY(obtain(nbytes));
     bytes = nbytes;
     return 0;
   }
@@ -959,6 +1081,8 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 void swap_buf(DevBuf& a, DevBuf& b) {
   std::swap(a.p, b.p);
   std::swap(a.bytes, b.bytes);
+  std::swap(a.cap, b.cap);
+  std::swap(a.device, b.device);
 }
 
 // live[] = byte sizes of the checkpointed arrays (5 state planes, M, 8 psi strips), in the
@@ -1539,6 +1663,24 @@ int vi_device_count(void) {
 int vi_mem_info(size_t* free_bytes, size_t* total_bytes) {
   if (!free_bytes || !total_bytes) return fail(VI_EINVAL, "vi_mem_info: null argument");
   VI_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  int dev = 0;
+  VI_HIP(hipGetDevice(&dev));
+  *free_bytes += spare::bytes_on(dev);  // kept for reuse, freed when an allocation needs it
+  return 0;
+}
+
+int vi_cached_bytes(size_t* bytes) {
+  if (!bytes) return fail(VI_EINVAL, "vi_cached_bytes: null argument");
+  int dev = 0;
+  VI_HIP(hipGetDevice(&dev));
+  *bytes = spare::bytes_on(dev);
+  return 0;
+}
+
+int vi_release_cached(void) {
+  int dev = 0;
+  VI_HIP(hipGetDevice(&dev));
+  spare::drop(dev);
   return 0;
 }
 
