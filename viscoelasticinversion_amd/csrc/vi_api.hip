@@ -227,8 +227,11 @@ struct DevBuf {
       (void)hipGetLastError();
       if (spare::drop(device)) e = hipMalloc(&p, nbytes);
     }
-    if (e != hipSuccess) p = nullptr;
-    VI_HIP(e);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(e == hipErrorOutOfMemory ? VI_ENOMEM : VI_EDEVICE,
+                  "hipMalloc(" + std::to_string(nbytes) + " bytes): " + hipGetErrorString(e));
+    }
     cap = nbytes;
     return 0;
   }
