@@ -196,14 +196,19 @@ class Server:
     not fit may be overtaken by later, smaller jobs; ``check_cfl``: refuse jobs whose dt
     exceeds the stability limit when they arrive; ``store_fraction``: gradient jobs whose
     every-step plan needs at most this fraction of the budget record every step instead of
-    checkpointing (see gradient_checkpoint). A forward job whose wavefield stops being finite
-    is failed alone after the block in which it happened; so is a gradient job whose misfit is
-    not finite."""
+    checkpointing (see gradient_checkpoint); ``max_gradient_chunk``:
+    a gradient's steps per chunk start at ``block_steps`` and double each time the job is
+    found idle, up to this many (``block_steps`` keeps them fixed: gradients then run dry
+    between the loop's turns, which leaves the forward jobs more of the GPU, lower median
+    latency and lower throughput). A forward job whose wavefield stops being finite is failed
+    alone after the block in which it happened; so is a gradient job whose misfit is not
+    finite."""
 
     def __init__(self, precision="float32", hbm_budget=None, block_steps=100, max_gradients=1,
                  max_forward=128, graph=True, max_head_wait=1.0, keep_outputs=True,
-                 check_cfl=True, store_fraction=0.25):
+                 check_cfl=True, store_fraction=0.25, max_gradient_chunk=1 << 16):
         from .batch import mem_info
+        self.max_chunk = max(int(max_gradient_chunk), int(block_steps))
         self.precision = precision
         if hbm_budget is None:
             free, _ = mem_info()
@@ -234,6 +239,10 @@ class Server:
         waiting = deque()
         active = {}  # forward: id -> [job, sim, remaining steps]
         grads = {}  # gradient: id -> (sim, GradientJob)
+        # steps per gradient chunk: doubled each time a job is found idle, so that a chunk
+        # outlasts the loop's turn (a batched block) and the gradient does not run dry
+        # between two turns
+        chunk = {}
         used = 0
         batch = Batch([])
         if self.graph:
@@ -273,6 +282,7 @@ class Server:
                 try:
                     if gj.finished:
                         del grads[jid]
+                        chunk.pop(jid, None)
                         r.misfit, r.gradient = gj.end()
                         sim.close()
                         if not np.isfinite(r.misfit):
@@ -282,9 +292,11 @@ class Server:
                             r.gradient = None
                         release(jid)
                     else:
-                        gj.advance(self.block)
+                        chunk[jid] = min(2 * chunk[jid], self.max_chunk)
+                        gj.advance(chunk[jid])
                 except Exception as e:  # noqa: BLE001 - reported per job
                     grads.pop(jid, None)
+                    chunk.pop(jid, None)
                     gj.cancel()
                     sim.close()
                     r.error = str(e)
@@ -311,6 +323,7 @@ class Server:
                 note("admit", jid)
                 if job.kind == "gradient":
                     grads[jid] = _start_gradient(job, self.precision)
+                    chunk[jid] = self.block
                     grads[jid][1].advance(self.block)
                 else:
                     active[jid] = [job, _build(job.cfg, self.precision), job.cfg["nt"]]
