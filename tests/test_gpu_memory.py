@@ -88,3 +88,44 @@ def test_forward_after_reuse_equals_fresh():
         assert np.array_equal(outs[0][k], outs[1][k]), k
     release_cached()
     assert cached_bytes() == 0
+
+
+def test_failed_allocation_frees_kept_buffers_and_retries():
+    """A gradient workspace that does not fit next to the kept buffers, but fits once they are
+    freed: the allocation fails, the kept buffers of the device are freed, and the retry
+    succeeds."""
+    from viscoelasticinversion_amd import cached_bytes, mem_info, release_cached
+    from viscoelasticinversion_amd.configs import build_simulation
+
+    def make(nt):
+        return build_simulation(small_cfg(nx=1024, nz=1024, nrelax=0, seed=21, nt=nt), "float32")
+
+    print("[case] allocation retried after freeing kept buffers", flush=True)
+    release_cached()
+    sim = make(16)
+    per_step = sim.gradient_bytes(2000, 2000) / 2000  # every step recorded
+    sim.close()
+    free0, total = mem_info()
+    nt_a = int(min(total // 4, 60e9) / per_step)
+    sim = make(nt_a)
+    J_a, _ = sim.gradient(np.zeros((sim.nrec, nt_a), np.float32), nt_a, nt_a)
+    sim.close()
+    assert np.isfinite(J_a)
+    kept = cached_bytes()
+    assert kept >= 0.9 * nt_a * per_step
+    free_all, _ = mem_info()
+    raw = free_all - kept  # what the runtime reports free
+    nt_b = int((raw + kept / 2) / per_step)
+    sim = make(nt_b)
+    need = sim.gradient_bytes(nt_b, nt_b)
+    if not raw + 4e9 < need < free_all - 8e9:
+        sim.close()
+        pytest.skip(f"no window on this device: free {raw:.3g} + kept {kept:.3g}, need {need:.3g}")
+    print(f"  kept {kept / 1e9:.1f} GB, free without them {raw / 1e9:.1f} GB, "
+          f"gradient needs {need / 1e9:.1f} GB", flush=True)
+    J_b, g_b = sim.gradient(np.zeros((sim.nrec, nt_b), np.float32), nt_b, nt_b)
+    sim.close()
+    assert np.isfinite(J_b) and all(np.isfinite(v).all() for v in g_b.values())
+    assert cached_bytes() <= total / 3
+    release_cached()
+    assert cached_bytes() == 0
