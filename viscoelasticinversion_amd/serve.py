@@ -331,7 +331,8 @@ class Server:
         try:
             while pending or waiting or active or grads:
                 arrivals()
-                admit()
+                if not active:
+                    admit()
                 if active:
                     ids = sorted(active)
                     if ids != members:
@@ -343,9 +344,11 @@ class Server:
                         batch.set_tiling(rows)
                     n = min(self.block, min(active[k][2] for k in ids))
                     batch.step(n)
-                    # while the block runs: gradient chunks, arrivals
+                    # while the block runs: gradient chunks, arrivals, and admissions (the new
+                    # jobs' simulations are built on the host meanwhile and join the next block)
                     progress_gradients()
                     arrivals()
+                    admit()
                     batch.sync()
                     finished = []
                     for k in ids:
