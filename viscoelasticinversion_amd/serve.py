@@ -271,6 +271,25 @@ class Server:
             res[jid].done = clock()
             note("done", jid)
 
+        # Finished jobs whose simulation is closed later. Freeing device memory waits for all
+        # work queued on the device, so a simulation is closed only while no gradient work is
+        # queued; closing it beside a long gradient chunk would hold the loop (and the forward
+        # jobs) until that chunk had run. Its bytes stay accounted until it is closed.
+        closing = []  # (job id, simulation)
+
+        def finish(jid, sim):
+            res[jid].done = clock()
+            note("done", jid)
+            closing.append((jid, sim))
+
+        def close_finished():
+            nonlocal used
+            if closing and all(gj.idle() for _, gj in grads.values()):
+                for jid, sim in closing:
+                    sim.close()
+                    used -= need[jid]
+                closing.clear()
+
         def progress_gradients():
             """Enqueue the next chunk of every gradient whose last chunk has finished; collect
             the finished ones."""
@@ -284,13 +303,12 @@ class Server:
                         del grads[jid]
                         chunk.pop(jid, None)
                         r.misfit, r.gradient = gj.end()
-                        sim.close()
                         if not np.isfinite(r.misfit):
                             r.error = "the wavefield blew up: non-finite misfit"
                             r.gradient = None
                         if not self.keep_outputs:
                             r.gradient = None
-                        release(jid)
+                        finish(jid, sim)
                     else:
                         chunk[jid] = min(2 * chunk[jid], self.max_chunk)
                         gj.advance(chunk[jid])
@@ -329,7 +347,8 @@ class Server:
                     active[jid] = [job, _build(job.cfg, self.precision), job.cfg["nt"]]
 
         try:
-            while pending or waiting or active or grads:
+            while pending or waiting or active or grads or closing:
+                close_finished()
                 arrivals()
                 if not active:
                     admit()
@@ -368,17 +387,21 @@ class Server:
                             r = res[job.id]
                             if r.error is None:
                                 r.traces = sim.traces(job.cfg["nt"])
-                            sim.close()
                             if not self.keep_outputs:
                                 r.traces = None
-                            release(job.id)
+                            finish(job.id, sim)
                 elif grads:
                     progress_gradients()
                     time.sleep(0.0005)
+                elif closing:
+                    continue  # no gradient in flight: closed at the top of the loop
                 else:
                     nxt = pending[0].arrival - clock() if pending else 0.01
                     time.sleep(min(max(nxt, 0.0), 0.05))
         finally:
+            for jid, sim in closing:
+                sim.close()
+                used -= need[jid]
             for a in active.values():
                 a[1].close()
             for sim, gj in grads.values():
