@@ -19,7 +19,8 @@ __all__ = [
     "Parameters", "Problem", "QModel", "RickerSource", "Solution", "_ricker", "dK_dw",
     "find_optimal_taun", "get_bc", "get_Q", "get_tau", "leapfrog_step", "make_problem",
     "solve_problem", "Simulation", "FIELDS", "COMPONENTS", "SOURCE_KINDS", "Batch", "Model",
-    "mem_info", "cached_bytes", "release_cached", "gradient_to_velocity_params", "misfit_and_pullback", "VTIModel",
-    "thomsen_to_stiffness", "segy", "gradient_to_thomsen_params", "StepParameters",
-    "apply_parameters", "stress_eq", "velocity_eq",
+    "mem_info", "cached_bytes", "release_cached", "gradient_to_velocity_params",
+    "misfit_and_pullback", "VTIModel", "thomsen_to_stiffness", "segy",
+    "gradient_to_thomsen_params", "StepParameters", "apply_parameters", "stress_eq",
+    "velocity_eq",
 ]
