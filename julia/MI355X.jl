@@ -40,7 +40,7 @@ import ChainRulesCore
 
 export MI355XBackend, Sim, Fields, StepParameters, leapfrog_step!, trace_misfit,
        misfit_gradient, gradient_bytes, checkpoint_for_memory, get_traces, set_receivers!,
-       reset!
+       reset!, cached_bytes, release_cached!
 
 const LIB = get(ENV, "VI_MI355X_LIB",
                 joinpath(@__DIR__, "..", "viscoelasticinversion_amd", "libvi_mi355x.so"))
@@ -355,6 +355,22 @@ function checkpoint_for_memory(s::Sim, nt::Integer, max_bytes::Integer)
     end
     return 0
 end
+
+"""
+    cached_bytes() -> Int
+
+Device bytes the library keeps for reuse on the current device: buffers of 64 MB or more
+that simulations released, handed to the next allocation they fit instead of back to the
+runtime (an allocation that does not fit frees them first).
+"""
+function cached_bytes()
+    b = Ref{Csize_t}(0)
+    check(ccall((:vi_cached_bytes, LIB), Cint, (Ref{Csize_t},), b))
+    return Int(b[])
+end
+
+"Free the buffers the library keeps for reuse (e.g. before another library allocates)."
+release_cached!() = (check(ccall((:vi_release_cached, LIB), Cint, ())); nothing)
 
 # ---- the reference's solver API on the device -----------------------------------------
 """
