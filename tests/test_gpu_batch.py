@@ -4,7 +4,7 @@ relative L2 <= 1e-5 FP32 / 1e-12 FP64)."""
 import numpy as np
 import pytest
 
-from helpers import compare, run_gpu, run_oracle
+from helpers import compare, run_gpu, run_oracle, small_cfg
 
 pytestmark = pytest.mark.gpu
 
@@ -209,3 +209,33 @@ def test_batch_autotune_keeps_state(block):
         ref = run_oracle(shot_config(mcfg, s), np.float32)
         compare(got, {k: ref[k] for k in got}, np.float32, f"autotune shot {s}")
     b.close()
+
+
+def test_cache_blocked_batch_mixed_classes_equals_alone():
+    """Shots of different precisions, relaxation counts and sizes in one batch: each (precision,
+    L) class is packed into its own cache-block groups (largest working set first), and every
+    shot still equals itself stepped alone, bit for bit."""
+    from viscoelasticinversion_amd import Batch
+    from viscoelasticinversion_amd.configs import build_simulation
+    print("[case] cache-blocked batch, mixed precision and L", flush=True)
+    specs = [("float32", 2, 90, 60), ("float64", 2, 70, 50), ("float32", 0, 130, 48),
+             ("float32", 2, 160, 72), ("float64", 2, 100, 64), ("float32", 0, 60, 40),
+             ("float32", 2, 75, 45)]
+    cfgs = [small_cfg(nx=nx, nz=nz, nrelax=L, seed=30 + q, nt=30)
+            for q, (_, L, nx, nz) in enumerate(specs)]
+    sims = [build_simulation(c, p) for c, (p, _, _, _) in zip(cfgs, specs)]
+    b = Batch(sims)
+    b.set_streams(2)
+    b.set_cache_block(4 << 20)  # about two shots per group and stream
+    b.set_graph(10)
+    b.step(17)
+    b.step(13)
+    b.sync()
+    for q, (sim, c) in enumerate(zip(sims, cfgs)):
+        got = _collect(sim, 30, c["nrelax"])
+        alone = run_gpu(c, specs[q][0])
+        for k in got:
+            assert np.array_equal(got[k], alone[k]), (q, k)
+    b.close()
+    for sim in sims:
+        sim.close()
