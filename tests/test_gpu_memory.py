@@ -12,44 +12,46 @@ pytestmark = pytest.mark.gpu
 NT = 120
 
 
-def _case(seed):
-    # 400 x 300 FP32 with every step recorded: the slot buffer is 120 steps x 8 planes x
-    # ~0.5 MB (~0.5 GB), well above the 64 MB threshold
-    cfg = small_cfg(nx=400, nz=300, nrelax=2, seed=seed, nt=NT)
+def _case(seed, vti=False, fs=True):
+    # 400 x 300 with every step recorded: the slot buffer is 120 steps x 8 planes x ~0.7 MB
+    # (FP32), well above the 64 MB threshold
+    cfg = small_cfg(nx=400, nz=300, nrelax=2, seed=seed, nt=NT, vti=vti, fs=fs)
     cfg["rec_ix"] = np.arange(20, 380, 9).astype(np.int32)
     cfg["rec_iz"] = np.full(len(cfg["rec_ix"]), 12, np.int32)
     cfg["rec_comp"] = (np.arange(len(cfg["rec_ix"])) % 5).astype(np.int32)
     return cfg
 
 
-def _gradient(cfg, dobs):
+def _gradient(cfg, dobs, prec="float32"):
     from viscoelasticinversion_amd.configs import build_simulation
-    sim = build_simulation(cfg, "float32")
+    sim = build_simulation(cfg, prec)
     try:
         return sim.gradient(dobs, NT, NT)
     finally:
         sim.close()
 
 
-def _observed(cfg):
+def _observed(cfg, prec="float32"):
     from viscoelasticinversion_amd.configs import build_simulation
     c = dict(cfg)
     c["model"] = {k: v * 1.02 for k, v in cfg["model"].items()}
-    sim = build_simulation(c, "float32")
+    sim = build_simulation(c, prec)
     sim.step(NT)
     d = sim.traces(NT)
     sim.close()
     return d
 
 
-def test_gradient_in_a_reused_dirty_workspace_equals_fresh():
+@pytest.mark.parametrize("prec,vti,fs", [("float32", False, True), ("float64", True, "afda"),
+                                         ("float32", True, "hermitian")])
+def test_gradient_in_a_reused_dirty_workspace_equals_fresh(prec, vti, fs):
     from viscoelasticinversion_amd import cached_bytes, mem_info, release_cached
-    print("[case] gradient in reused buffers", flush=True)
-    a, b = _case(11), _case(12)
-    da, db = _observed(a), _observed(b)
+    print(f"[case] gradient in reused buffers: {prec}, vti {vti}, surface {fs}", flush=True)
+    a, b = _case(11, vti, fs), _case(12, vti, fs)
+    da, db = _observed(a, prec), _observed(b, prec)
     release_cached()
     assert cached_bytes() == 0
-    J_fresh, g_fresh = _gradient(b, db)  # fresh allocations
+    J_fresh, g_fresh = _gradient(b, db, prec)  # fresh allocations
     kept = cached_bytes()
     assert kept >= 64 << 20
     free_with, _ = mem_info()
@@ -57,10 +59,10 @@ def test_gradient_in_a_reused_dirty_workspace_equals_fresh():
     free_without, _ = mem_info()
     assert cached_bytes() == 0
     assert abs(free_with - free_without) < (64 << 20)  # kept bytes count as free
-    _gradient(b, db)
-    _gradient(a, da)  # another job writes its own values into the kept buffers
+    _gradient(b, db, prec)
+    _gradient(a, da, prec)  # another job writes its own values into the kept buffers
     assert cached_bytes() >= 64 << 20
-    J, g = _gradient(b, db)  # runs in the buffers the other job left behind
+    J, g = _gradient(b, db, prec)  # runs in the buffers the other job left behind
     assert J == J_fresh
     for k in g_fresh:
         assert np.array_equal(g[k], g_fresh[k]), k
