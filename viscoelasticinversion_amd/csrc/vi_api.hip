@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -530,12 +531,14 @@ int apply_tiling(vi_sim_s* h, int rz) {
   return 0;
 }
 
-// Default rows per wave (measured on MI355X over 256^2 ... 2048^2 grids): one-row tiles while
-// the grid has fewer than 4096 of them (the launch is then one round of workgroups and
-// latency decides), two-row tiles above that (halo re-reads amortised).
+// Default tiling (measured on MI355X, L = 3 FP32, us per step at 4-row / 8-row tiles, both
+// one row per wave): 256^2 19.9 / 9.1, 512^2 15.0 / 11.0, 1024x512 15.3 / 14.8, 1600x600
+// 26.2 / 27.1, 1024x2048 51.2 / 52.6, 2048^2 128.6 / 125.0. The 8-row tiles (half as many
+// workgroups of 8 waves) win where the launch is latency-bound and on the largest grids;
+// counted in 4-row tiles: 8-row tiles up to 2048 and from 12288 of them, 4-row in between.
 int default_rz(int nx, int nz) {
   const long tiles = (long)((nx - 8 + vi::TX - 1) / vi::TX) * ((nz - 8 + vi::NW - 1) / vi::NW);
-  return tiles >= 4096 ? 2 : 1;
+  return tiles <= 2048 || tiles >= 12288 ? 2 : 1;
 }
 
 template <typename R>
@@ -655,16 +658,24 @@ void launch(K kern, dim3 grid, dim3 block, hipStream_t s, hipEvent_t e0, hipEven
 template <typename R, int RZ, bool AFDA>
 int launch_kernel(hipStream_t s, int L, int k, const void* descs, const int* tile_shot,
                   int ntiles, int kstep, hipEvent_t e0, hipEvent_t e1) {
-  const dim3 block(vi::TX, vi::NW);
   const dim3 grid(ntiles);
   const vi::ShotDesc<R>* d = (const vi::ShotDesc<R>*)descs;
   const vi::RecSlot rs{};
+  // 8-row tiles (the 2-rows-per-wave tiling) run as 8 wavefronts of one row each: half as
+  // much work per wave as 4 waves of two rows, so each workgroup finishes sooner (1024x512:
+  // 14.5 against 17.8 us per step, and against 15.1 us with 4-row tiles)
+  constexpr int KRZ = RZ == 2 ? 1 : RZ, KNW = RZ == 2 ? 8 : vi::NW;
+  const dim3 kblock(vi::TX, KNW);
   if (k == 0) {
-    launch(vi::k_velocity<R, RZ, false, AFDA>, grid, block, s, e0, e1, d, tile_shot, kstep, rs);
+    launch(vi::k_velocity<R, KRZ, false, AFDA, KNW>, grid, kblock, s, e0, e1, d, tile_shot,
+           kstep, rs);
   } else {
     switch (L) {
-#define VI_STRESS(l) \
-  case l: launch(vi::k_stress<R, l, RZ, false, AFDA>, grid, block, s, e0, e1, d, tile_shot, kstep, rs); break;
+#define VI_STRESS(l)                                                                          \
+  case l:                                                                                     \
+    launch(vi::k_stress<R, l, KRZ, false, AFDA, KNW>, grid, kblock, s, e0, e1, d, tile_shot,  \
+           kstep, rs);                                                                        \
+    break;
       VI_STRESS(0) VI_STRESS(1) VI_STRESS(2) VI_STRESS(3) VI_STRESS(4) VI_STRESS(5) VI_STRESS(6)
 #undef VI_STRESS
       default: return fail(VI_EINVAL, "nrelax out of range");

@@ -250,10 +250,10 @@ __device__ __forceinline__ R surface_ezz(const ShotDesc<R>& d, int i, R exx) {
 // loads into registers and store() writes them to LDS, so a kernel can put every load of
 // all its tiles in flight before the first LDS write (one wait instead of a round trip
 // per tile). Rows past NROWS re-load the last row and are not written.
-template <typename R, int NROWS, int HALO>
+template <typename R, int NROWS, int HALO, int NWV = NW>
 struct TileLoad {
   static constexpr int W = TX + 2 * HALO;
-  static constexpr int K = (NROWS + NW - 1) / NW;
+  static constexpr int K = (NROWS + NWV - 1) / NWV;
   R v[K], hv[K];
   __device__ __forceinline__ void fetch(const R* __restrict__ g, int jr0, int i0, int ldx,
                                         int lane, int w) {
@@ -261,7 +261,7 @@ struct TileLoad {
     const int hg = lane < 4 ? lane - 4 : TX + lane - 4;  // halo column relative to i0
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int r = min(w + k * NW, NROWS - 1);
+      const int r = min(w + k * NWV, NROWS - 1);
       const unsigned rb = (unsigned)((jr0 + r) * ldx + i0) * E;
       v[k] = gld(g, rb + lane * E);
       if (HALO) hv[k] = lane < 8 ? gld(g, rb + hg * (int)E) : R(0);
@@ -271,7 +271,7 @@ struct TileLoad {
     const int hc = lane < 4 ? lane : lane + TX;  // LDS column of a halo value
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int r = w + k * NW;
+      const int r = w + k * NWV;
       if (r < NROWS) {
         sh[r * W + lane + HALO] = v[k];
         if (HALO && lane < 8) sh[r * W + hc] = hv[k];
@@ -293,11 +293,11 @@ __device__ __forceinline__ void dma_f32(const float* g, float* lds) {
   __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) float*)g,
                                    (__attribute__((address_space(3))) float*)lds, 4, 0, 0);
 }
-template <typename R, int NROWS, int HALO, bool DMA = true>
+template <typename R, int NROWS, int HALO, bool DMA = true, int NWV = NW>
 struct TileIn {
   static constexpr int W = TX + 2 * HALO;
-  static constexpr int K = (NROWS + NW - 1) / NW;
-  TileLoad<R, NROWS, HALO> t;  // FP64 path
+  static constexpr int K = (NROWS + NWV - 1) / NWV;
+  TileLoad<R, NROWS, HALO, NWV> t;  // FP64 path
   R hv[K];                     // FP32 path: halo values of lanes 0..7
   __device__ __forceinline__ void fetch(const R* __restrict__ g, int jr0, int i0, int ldx,
                                         int lane, int w, R* sh) {
@@ -306,7 +306,7 @@ struct TileIn {
       const int hg = lane < 4 ? lane - 4 : TX + lane - 4;  // halo column relative to i0
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const int r = w + k * NW;  // wave-uniform
+        const int r = w + k * NWV;  // wave-uniform
         if (r < NROWS) dma_f32(g + (size_t)(jr0 + r) * ldx + i0 + lane, &sh[r * W + HALO]);
         if (HALO) {
           const unsigned rb = (unsigned)((jr0 + min(r, NROWS - 1)) * ldx + i0) * E;
@@ -323,7 +323,7 @@ struct TileIn {
         const int hc = lane < 4 ? lane : lane + TX;  // LDS column of a halo value
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-          const int r = w + k * NW;
+          const int r = w + k * NWV;
           if (r < NROWS && lane < 8) sh[r * W + hc] = hv[k];
         }
       }
@@ -453,12 +453,12 @@ struct ListIn {
 
 // AFDA: the variant with the H-AFDA rows (any shot of the launch may use them; the others
 // run the same code as without)
-template <typename R, int RZ, bool REC = false, bool AFDA = false>
-__global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restrict__ shots,
+template <typename R, int RZ, bool REC = false, bool AFDA = false, int NWV = NW>
+__global__ __launch_bounds__(TX* NWV) void k_velocity(const ShotDesc<R>* __restrict__ shots,
                                                      const int* __restrict__ tile_shot,
                                                      int kstep, const RecSlot rs) {
   static_assert(!REC || RZ <= 2, "recording variants exist for 1 and 2 rows per wave");
-  constexpr int TZ = NW * RZ;
+  constexpr int TZ = NWV * RZ;
   constexpr int WX = TX + 8;
   constexpr unsigned E = sizeof(R);
   __shared__ R s_xz[(TZ + 8) * WX];  // sxz rows j0-4 .. j0+TZ+3, columns i0-4 .. i0+67
@@ -475,9 +475,9 @@ __global__ __launch_bounds__(TX* NW) void k_velocity(const ShotDesc<R>* __restri
   const int ldx = d.ldx, ldm = d.ldm;
   // register path (DMA = false): the LDS-direct rows measured 0.4 us slower in this kernel
   // at 1024x512 (same-box A/B), while they save 0.2-0.8 us in the stress kernel
-  TileIn<R, TZ + 8, 4, false> t_xz;
-  TileIn<R, TZ + 8, 0, false> t_zz;
-  TileIn<R, TZ, 4, false> t_xx;
+  TileIn<R, TZ + 8, 4, false, NWV> t_xz;
+  TileIn<R, TZ + 8, 0, false, NWV> t_zz;
+  TileIn<R, TZ, 4, false, NWV> t_xx;
   t_xz.fetch(d.sxz, j0 - 4, i0, ldx, lane, w, s_xz);
   t_zz.fetch(d.szz, j0 - 4, i0, ldx, lane, w, s_zz);
   t_xx.fetch(d.sxx, j0, i0, ldx, lane, w, s_xx);
@@ -597,12 +597,12 @@ constexpr int stress_waves() {
   return (sizeof(R) == 4 && RZ == 1 && L <= 3 && !REC && !AFDA) ? 8 : 1;
 }
 
-template <typename R, int L, int RZ, bool REC = false, bool AFDA = false>
-__global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void k_stress(const ShotDesc<R>* __restrict__ shots,
+template <typename R, int L, int RZ, bool REC = false, bool AFDA = false, int NWV = NW>
+__global__ __launch_bounds__(TX* NWV, (stress_waves<R, L, RZ, REC, AFDA>())) void k_stress(const ShotDesc<R>* __restrict__ shots,
                                                    const int* __restrict__ tile_shot,
                                                    int kstep, const RecSlot rs) {
   static_assert(!REC || RZ <= 2, "recording variants exist for 1 and 2 rows per wave");
-  constexpr int TZ = NW * RZ;
+  constexpr int TZ = NWV * RZ;
   constexpr int WX = TX + 8;
   constexpr unsigned E = sizeof(R);
   // Cell-local operands of all RZ rows are requested before the barrier (in flight with
@@ -622,7 +622,7 @@ __global__ __launch_bounds__(TX* NW, (stress_waves<R, L, RZ, REC, AFDA>())) void
   const int i0 = 4 + tx * TX, j0 = 4 + tz * TZ;
   const int i = i0 + lane;
   const int ldx = d.ldx, ldm = d.ldm;
-  TileIn<R, TZ + 8, 4> t_vx, t_vz;
+  TileIn<R, TZ + 8, 4, true, NWV> t_vx, t_vz;
   t_vx.fetch(d.vx, j0 - 4, i0, ldx, lane, w, s_vx);
   t_vz.fetch(d.vz, j0 - 4, i0, ldx, lane, w, s_vz);
 
