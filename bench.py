@@ -407,7 +407,7 @@ def bench_serving(njobs=24, rate=20.0, seed=0, block_steps=100, max_gradients=1)
 
 def pmc_probe(nshots, steps, copy_bytes=1 << 30):
     """Short run for rocprofv3 --pmc passes, three phases in order:
-    1. the single-shot 1024 x 512 L=3 workload, 20 steps, rows per wave from VI_ROWS1 (1);
+    1. the single-shot 1024 x 512 L=3 workload, 20 steps, rows per wave from VI_ROWS1 (2);
     2. the batched 64-shot workload, ``steps`` steps, rows per wave from VI_ROWS (2);
     3. one device-to-device copy of ``copy_bytes`` (a plain copy kernel of known size, used
        to scale FETCH_SIZE / WRITE_SIZE to bytes)."""
@@ -417,7 +417,7 @@ def pmc_probe(nshots, steps, copy_bytes=1 << 30):
                                                    config_layered_visco, config_multishot)
     cfg = config_layered_visco(nt=20)
     sim = build_simulation(cfg, "float32")
-    sim.set_tiling(int(os.environ.get("VI_ROWS1", "1")))
+    sim.set_tiling(int(os.environ.get("VI_ROWS1", "2")))
     sim.step(20)
     sim.sync()
     single = {k: sim.algorithmic_bytes(k) for k in (0, 1)}
