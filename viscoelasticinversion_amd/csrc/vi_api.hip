@@ -690,17 +690,21 @@ int launch_kernel(hipStream_t s, int L, int k, const void* descs, const int* til
 template <typename R, int RZ, bool AFDA>
 int launch_rec_kernel(hipStream_t s, int L, int k, const void* descs, int ntiles, int kstep,
                       const vi::RecSlot& rs) {
-  const dim3 block(vi::TX, vi::NW);
+  // 8-row tiles as 8 wavefronts of one row, as in the plain launches
+  constexpr int KRZ = RZ == 2 ? 1 : RZ, KNW = RZ == 2 ? 8 : vi::NW;
+  const dim3 block(vi::TX, KNW);
   const dim3 grid(ntiles);
   const vi::ShotDesc<R>* d = (const vi::ShotDesc<R>*)descs;
   const int* ts = nullptr;
   if (k == 0) {
-    hipLaunchKernelGGL((vi::k_velocity<R, RZ, true, AFDA>), grid, block, 0, s, d, ts, kstep, rs);
+    hipLaunchKernelGGL((vi::k_velocity<R, KRZ, true, AFDA, KNW>), grid, block, 0, s, d, ts, kstep,
+                       rs);
   } else {
     switch (L) {
-#define VI_STRESS(l)                                                                        \
-  case l:                                                                                   \
-    hipLaunchKernelGGL((vi::k_stress<R, l, RZ, true, AFDA>), grid, block, 0, s, d, ts, kstep, rs); \
+#define VI_STRESS(l)                                                                          \
+  case l:                                                                                     \
+    hipLaunchKernelGGL((vi::k_stress<R, l, KRZ, true, AFDA, KNW>), grid, block, 0, s, d, ts,   \
+                       kstep, rs);                                                            \
     break;
       VI_STRESS(0) VI_STRESS(1) VI_STRESS(2) VI_STRESS(3) VI_STRESS(4) VI_STRESS(5) VI_STRESS(6)
 #undef VI_STRESS
