@@ -286,8 +286,9 @@ int vi_plan_bytes(const vi_grid_desc* g, int vti, int cpml_thickness, int nsrc, 
 
 /* ---- tiling (no counterpart in the reference: KA picks workgroup shapes itself) ------ */
 /* tiling: tiles of 64 x (4 rows_per_wave) cells, rows_per_wave 1, 2, 4 or 8 (results are
- * identical); the 8-row tiles (2) run as 8 wavefronts of one row each. Default: 8-row tiles on
- * small grids (up to 2048 4-row tiles) and large ones (from 12288), 4-row tiles in between */
+ * identical); the 8- and 16-row tiles (2, 4) run as 8 wavefronts of one and two rows. Default:
+ * 8-row tiles up to 2048 4-row tiles, 16-row tiles from 12288, 4-row tiles in between; a
+ * batch defaults to 4-row tiles */
 int vi_set_tiling(vi_handle h, int rows_per_wave);
 int vi_get_tiling(vi_handle h);
 /* time every variant for trial_steps steps on a snapshot of the state, restore the state,

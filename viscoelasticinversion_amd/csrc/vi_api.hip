@@ -394,7 +394,9 @@ struct vi_batch_s {
   std::vector<long> steps;  // each member's step index as the batch descriptors hold it
   std::vector<std::unique_ptr<BatchGroup>> groups;
   bool built = false;
-  int rz = 8;
+  // rows per wave of the members' tiles; 4-row tiles by default (the 64-shot cache-blocked
+  // batch runs within 0.5 % of its fastest at 4- and 8-row tiles; vi_batch_autotune picks)
+  int rz = 1;
   int graph_steps = 0;
   // cache blocking over shots (0: every shot in every launch); default: groups of at most
   // 224 MB of working set, which stay in the 256 MB Infinity Cache through a block of steps
@@ -531,14 +533,15 @@ int apply_tiling(vi_sim_s* h, int rz) {
   return 0;
 }
 
-// Default tiling (measured on MI355X, L = 3 FP32, us per step at 4-row / 8-row tiles, both
-// one row per wave): 256^2 19.9 / 9.1, 512^2 15.0 / 11.0, 1024x512 15.3 / 14.8, 1600x600
-// 26.2 / 27.1, 1024x2048 51.2 / 52.6, 2048^2 128.6 / 125.0. The 8-row tiles (half as many
-// workgroups of 8 waves) win where the launch is latency-bound and on the largest grids;
-// counted in 4-row tiles: 8-row tiles up to 2048 and from 12288 of them, 4-row in between.
+// Default tiling (measured on MI355X, L = 3 FP32, us per step with 4-, 8- and 16-row tiles;
+// the 8- and 16-row tiles run as 8 wavefronts of one and two rows): 256^2 9.3 / 7.9 / 10.1,
+// 512^2 11.2 / 10.5 / 11.8, 1024x512 15.1 / 14.6 / 15.8, 1600x600 26.6 / 27.0 / 28.6,
+// 1024x2048 51.4 / 52.3 / 56.4, 2048^2 131.0 / 128.1 / 124.7. Counted in 4-row tiles: 8-row
+// tiles up to 2048 of them (the launch is latency-bound: fewer, shorter-lived workgroups),
+// 16-row tiles from 12288 (halo re-reads amortised), 4-row tiles in between.
 int default_rz(int nx, int nz) {
   const long tiles = (long)((nx - 8 + vi::TX - 1) / vi::TX) * ((nz - 8 + vi::NW - 1) / vi::NW);
-  return tiles <= 2048 || tiles >= 12288 ? 2 : 1;
+  return tiles <= 2048 ? 2 : tiles >= 12288 ? 4 : 1;
 }
 
 template <typename R>
@@ -661,10 +664,12 @@ int launch_kernel(hipStream_t s, int L, int k, const void* descs, const int* til
   const dim3 grid(ntiles);
   const vi::ShotDesc<R>* d = (const vi::ShotDesc<R>*)descs;
   const vi::RecSlot rs{};
-  // 8-row tiles (the 2-rows-per-wave tiling) run as 8 wavefronts of one row each: half as
-  // much work per wave as 4 waves of two rows, so each workgroup finishes sooner (1024x512:
-  // 14.5 against 17.8 us per step, and against 15.1 us with 4-row tiles)
-  constexpr int KRZ = RZ == 2 ? 1 : RZ, KNW = RZ == 2 ? 8 : vi::NW;
+  // 8- and 16-row tiles (the 2- and 4-rows-per-wave tilings) run as 8 wavefronts of one and
+  // two rows: half as much work per wave as 4 wavefronts, so each workgroup finishes sooner
+  // (1024x512: 14.5 against 17.8 us per step with 8-row tiles, and 15.1 us with 4-row tiles;
+  // 16-row tiles 15.8 against 24.7 us)
+  constexpr int KRZ = RZ == 2 ? 1 : RZ == 4 ? 2 : RZ;
+  constexpr int KNW = RZ == 2 || RZ == 4 ? 8 : vi::NW;
   const dim3 kblock(vi::TX, KNW);
   if (k == 0) {
     launch(vi::k_velocity<R, KRZ, false, AFDA, KNW>, grid, kblock, s, e0, e1, d, tile_shot,
